@@ -1,0 +1,280 @@
+#!/usr/bin/env python3
+"""Benchmark: marching cubes at 256^3 cells on one MI355X.
+
+Workload (one "step"): MarchingCubes over a 257 x 257 x 258 FP32 sample
+lattice already resident in HBM — the reference's MarchingCubes::new(257)
+layout, 256 x 256 x 257 = 16,842,752 cells — sampled from the union of 64
+seeded spheres minus a torus.  A step is one full extraction: sign
+classification, per-segment counts, scan, vertex and triangle emission into
+device buffers, and the read-back of the mesh sizes.  The output is
+bit-identical to the reference's CPU path (checked here against the golden
+digests of the oracle's mesh).
+
+Printed JSON line:
+  value         cells/s of the timed steps (wall clock, synchronised)
+  roofline      the dominant kernel (sign classification of the FP32 grid) —
+                algorithmic bytes per launch / its mean HIP-event duration
+                over the timed steps, against MI355X's 8 TB/s HBM peak
+  pipeline      every kernel of the step, and the step's algorithmic bytes
+                (4 B per sample + 12 B per vertex + 12 B per triangle)
+  program_path  the same extraction sampling the CSG program on the GPU
+                instead of reading the grid (VALU-bound, reported aside)
+  cpu_baseline  the oracle (a single-threaded restatement of the reference
+                CPU path) on the same lattice, median of a few runs
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import hashlib
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from isosurface_amd import Context, _abi as A  # noqa: E402
+from isosurface_amd import workloads as W  # noqa: E402
+
+METRIC = "MC cells/sec @256³ + % HBM roofline; learned-band speedup at 0 missed cells"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+class DeviceMesh:
+    def __init__(self, ctx: Context, nv: int, ni: int):
+        self.ctx = ctx
+        self.v = C.c_void_p()
+        self.i = C.c_void_p()
+        self.nv, self.ni = nv, ni
+        ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 12 * nv, C.byref(self.v)), "alloc")
+        ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 4 * ni, C.byref(self.i)), "alloc")
+
+    def out(self) -> A.IsoMeshOut:
+        return A.IsoMeshOut(self.v, self.i, self.nv, self.ni, 0, 0, A.ISO_MEMORY_DEVICE, 0)
+
+    def free(self):
+        self.ctx.lib.iso_device_free(self.ctx.handle, self.v)
+        self.ctx.lib.iso_device_free(self.ctx.handle, self.i)
+
+
+def extract(ctx: Context, desc, mesh: DeviceMesh) -> A.IsoMeshOut:
+    out = mesh.out()
+    ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out)), "iso_extract_mc")
+    return out
+
+
+def timed_steps(ctx, desc, mesh, steps):
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        extract(ctx, desc, mesh)
+    ctx.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_pass(ctx, desc, mesh, steps):
+    """Per-kernel mean durations (HIP events on the context's stream)."""
+    ctx.reset_kernel_stats()
+    ctx.set_timing(True)
+    for _ in range(steps):
+        extract(ctx, desc, mesh)
+    ctx.synchronize()
+    ctx.set_timing(False)
+    stats = ctx.kernel_stats()
+    ctx.reset_kernel_stats()
+    return {k: {"launches": n, "mean_us": 1e3 * ms / max(n, 1)} for k, (n, ms) in stats.items()}
+
+
+def digest(ctx, mesh: DeviceMesh, out) -> tuple:
+    nv, ni = int(out.vertex_count), int(out.index_count)
+    v = np.empty((nv, 3), np.float32)
+    i = np.empty(ni, np.uint32)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, v.ctypes.data, mesh.v, v.nbytes)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, i.ctypes.data, mesh.i, i.nbytes)
+    return (hashlib.sha256(v.astype("<f4").tobytes()).hexdigest(),
+            hashlib.sha256(i.astype("<u4").tobytes()).hexdigest())
+
+
+def cpu_baseline(grid: np.ndarray, size: int, budget_s: float) -> dict:
+    """The oracle (restated reference CPU path, one thread) on the same lattice."""
+    from oracle import oracle as O
+
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        pinned = True
+    except (AttributeError, OSError):
+        pinned = False
+    desc = O.reference_desc(size, source=1)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        O.extract_mc(grid=grid, desc=desc)
+        times.append(time.perf_counter() - t0)
+    cells = (size - 1) * (size - 1) * size
+    med = statistics.median(times)
+    return {
+        "value": cells / med,
+        "unit": "cells/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle MarchingCubes over the same resident {size}x{size}x{size + 1} grid, "
+                   f"{len(times)} runs, median {med:.3f} s, pinned={pinned}, "
+                   f"host {platform.processor() or platform.machine()}, "
+                   f"{os.cpu_count()} logical cpus"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=257)
+    ap.add_argument("--program-steps", type=int, default=5)
+    ap.add_argument("--cpu-budget", type=float, default=25.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the extraction path has no multi-GPU mode "
+              "(sharding is a design note only); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    size = args.size
+    ctx = Context(0)
+    src, _ = W.spheres_minus_torus()
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(size, A.ISO_SOURCE_GRID)
+    n = desc.nx * desc.ny * desc.nz
+    cells = (desc.nx - 1) * (desc.ny - 1) * (desc.nz - 1)
+    log(f"lattice {desc.nx}x{desc.ny}x{desc.nz}: {n} samples, {cells} cells")
+
+    # Resident FP32 input grid, sampled from the CSG program on the device.
+    dgrid = C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 4 * n, C.byref(dgrid)), "alloc grid")
+    ctx.check(ctx.lib.iso_program_sample_lattice(ctx.handle, C.byref(desc), dgrid,
+                                                 A.ISO_MEMORY_DEVICE), "sample_lattice")
+    ctx.check(ctx.lib.iso_grid_bind_device(ctx.handle, dgrid, desc.nx, desc.ny, desc.nz), "bind")
+
+    # Size device outputs from a first extraction.
+    mesh = DeviceMesh(ctx, 1, 3)
+    out = mesh.out()
+    st = ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out))
+    if st not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+        ctx.check(st, "iso_extract_mc")
+    nv, ni = int(out.vertex_count), int(out.index_count)
+    mesh.free()
+    mesh = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
+    out = extract(ctx, desc, mesh)
+    log(f"mesh: {nv} vertices, {ni // 3} triangles")
+
+    parity = None
+    golden = json.loads((ROOT / "tests" / "golden" / "mesh_digests.json").read_text())
+    key = f"spheres64_minus_torus_size{size}"
+    if key in golden:
+        dv, di = digest(ctx, mesh, out)
+        parity = dv == golden[key]["vertices_sha256"] and di == golden[key]["indices_sha256"]
+        log(f"parity vs oracle digests: {parity}")
+
+    for _ in range(args.warmup):
+        extract(ctx, desc, mesh)
+    elapsed = timed_steps(ctx, desc, mesh, args.steps)
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = cells * args.steps / elapsed
+    log(f"grid path: {ms_per_step:.4f} ms/step, {value / 1e9:.2f} G cells/s")
+
+    kernels = kernel_pass(ctx, desc, mesh, args.steps)
+    dom_name = "mc_signs_grid"
+    dom = kernels[dom_name]
+    dom_bytes = 4 * n + n // 8  # read every sample once, write one bit per sample
+    achieved = dom_bytes / (dom["mean_us"] * 1e-6) / 1e9
+    kernel_us = sum(k["mean_us"] for k in kernels.values())
+    pipe_bytes = 4 * n + 12 * nv + 4 * ni
+    log(f"kernels: {json.dumps(kernels)}")
+
+    # Program path: the CSG sampled by the extraction itself (no grid).
+    pdesc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
+    extract(ctx, pdesc, mesh)
+    p_elapsed = timed_steps(ctx, pdesc, mesh, args.program_steps)
+    p_kernels = kernel_pass(ctx, pdesc, mesh, max(1, args.program_steps // 2))
+    program_path = {
+        "value": cells * args.program_steps / p_elapsed,
+        "unit": "cells/s",
+        "ms_per_step": 1e3 * p_elapsed / args.program_steps,
+        "kernels_us": {k: round(v["mean_us"], 2) for k, v in p_kernels.items()},
+        "note": "64-sphere-minus-torus SDF evaluated on the GPU (VALU-bound, not an HBM stream)",
+    }
+    log(f"program path: {program_path['ms_per_step']:.3f} ms/step")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"MarchingCubes::new({size}) over a resident FP32 "
+                         f"{desc.nx}x{desc.ny}x{desc.nz} grid of the union of 64 seeded spheres "
+                         "minus a torus (R=0.3, r=0.05); IndexedVertices output"),
+            "cells": cells,
+            "samples": n,
+            "vertices": nv,
+            "triangles": ni // 3,
+            "parity_vs_oracle": parity,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes": dom_bytes,
+            "mean_us": dom["mean_us"],
+        },
+        "pipeline": {
+            "kernels_us": {k: round(v["mean_us"], 2) for k, v in kernels.items()},
+            "kernel_us_per_step": kernel_us,
+            "algorithmic_bytes": pipe_bytes,
+            "achieved_GBps": pipe_bytes / (kernel_us * 1e-6) / 1e9,
+            "frac_of_hbm_peak": pipe_bytes / (kernel_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+        },
+        "program_path": program_path,
+    }
+
+    if not args.no_cpu_baseline:
+        host_grid = np.empty((desc.nz, desc.ny, desc.nx), np.float32)
+        ctx.lib.iso_memcpy_to_host(ctx.handle, host_grid.ctypes.data, dgrid, host_grid.nbytes)
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(host_grid, size, args.cpu_budget)
+    else:
+        result["cpu_baseline"] = None
+
+    mesh.free()
+    ctx.lib.iso_device_free(ctx.handle, dgrid)
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

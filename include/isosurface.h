@@ -1,0 +1,201 @@
+/*
+ * isosurface_amd — C ABI of the MI355X (gfx950) isosurface extraction engine.
+ *
+ * The reference crate (swiftcoder/isosurface, Rust) has no FFI of its own: its
+ * callers use generic traits.  This header is the boundary a Rust caller would
+ * bind with `extern "C"` to replace the crate's CPU hot path:
+ *
+ *   MarchingCubes::<Signed>::new(size)            marching_cubes.rs:46-50
+ *   MarchingCubes::extract(&source, &mut extractor) marching_cubes.rs:59-82
+ *   ScalarSource::sample_scalar                   source.rs:21-28
+ *   Extractor::extract_vertex / extract_index     extractor.rs:17-20
+ *
+ * Conventions
+ *   - Every entry point returns an iso_status; nothing throws across the ABI.
+ *   - One context per host thread / stream.  A context owns its device
+ *     scratch buffers and reuses them across calls, as the reference's
+ *     algorithm objects reuse their layer buffers (primal_grid.rs:18-33).
+ *   - Sources: the crate's built-in implicit shapes and CSG operators
+ *     (src/implicit/) are described by an iso_node program evaluated on the
+ *     GPU; any other source is sampled by the caller on the host into a dense
+ *     FP32 grid and uploaded (iso_grid_upload / iso_grid_bind_device).
+ *   - Output matches the crate's IndexedVertices sink (extractor.rs:72-94):
+ *     xyz f32 per vertex, u32 indices, three per triangle, in the order the
+ *     reference emits them (first-encounter vertex order, mesh.rs:240-251;
+ *     triangles in cell order, then table order).
+ */
+#ifndef ISOSURFACE_AMD_H
+#define ISOSURFACE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISO_ABI_VERSION 1
+
+typedef enum iso_status {
+  ISO_OK = 0,
+  ISO_ERR_INVALID_ARGUMENT = 1,
+  /* Output buffers too small.  vertex_count / index_count hold the sizes
+   * the mesh needs; nothing beyond the capacities was written. */
+  ISO_ERR_CAPACITY = 2,
+  ISO_ERR_DEVICE = 3,        /* HIP runtime or kernel launch failure */
+  ISO_ERR_OUT_OF_MEMORY = 4,
+  ISO_ERR_PROGRAM = 5,       /* malformed iso_node program */
+  ISO_ERR_NO_DEVICE = 6,
+  ISO_ERR_NO_SOURCE = 7,     /* no program / grid bound for the requested source */
+  ISO_ERR_OVERFLOW = 8       /* mesh exceeds the u32 index range */
+} iso_status;
+
+typedef struct iso_ctx iso_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Creates a context on HIP device `device`.  `stream` is a hipStream_t to
+ * launch on, or NULL for a stream owned by the context. */
+int iso_ctx_create(int device, void* stream, iso_ctx** out);
+int iso_ctx_destroy(iso_ctx* ctx);
+int iso_ctx_synchronize(iso_ctx* ctx);
+/* Human-readable detail of the last failure on this context. */
+const char* iso_ctx_last_error(const iso_ctx* ctx);
+const char* iso_status_string(int status);
+int iso_abi_version(void);
+
+/* ---- sources: expression programs -------------------------------------- */
+
+/* Program opcodes.  A program is a postfix (post-order) list of nodes that
+ * leaves exactly one value, mirroring the crate's source types:
+ *   SPHERE       p[0]=radius                      implicit/sphere.rs:35-39
+ *   TORUS        p[0]=radius, p[1]=tube_radius    implicit/torus.rs:40-46
+ *   BOX          p[0..2]=half_extent              implicit/rectangular_prism.rs:36-41
+ *   CYLINDER     p[0]=radius, p[1]=half_length    implicit/cylinder.rs:41-48
+ *   UNION        min(a, b)                        implicit/csg.rs:35-39
+ *   INTERSECTION max(a, b)                        implicit/csg.rs:68-72
+ *   DIFFERENCE   max(b, -a)  (b minus a)          implicit/csg.rs:96-100
+ *   TRANSLATE    p[0..2]=offset: the nodes up to the matching END_TRANSLATE
+ *                are sampled at (p - offset), per component, as the examples'
+ *                DemoSource does (examples/common/sources.rs:38-43)
+ *   SMOOTH_UNION p[0]=k: polynomial smooth minimum of a and b (no reference
+ *                counterpart; used by the learned narrow-band workload)
+ * For binary operators, `a` is the earlier operand and `b` the later one,
+ * so Difference::new(a, b) is "a-nodes b-nodes DIFFERENCE". */
+enum {
+  ISO_OP_SPHERE = 1,
+  ISO_OP_TORUS = 2,
+  ISO_OP_BOX = 3,
+  ISO_OP_CYLINDER = 4,
+  ISO_OP_UNION = 16,
+  ISO_OP_INTERSECTION = 17,
+  ISO_OP_DIFFERENCE = 18,
+  ISO_OP_SMOOTH_UNION = 19,
+  ISO_OP_TRANSLATE = 32,
+  ISO_OP_END_TRANSLATE = 33
+};
+
+typedef struct iso_node {
+  uint32_t op;
+  uint32_t reserved; /* must be 0 */
+  float p[6];
+} iso_node;
+
+/* Checks that `nodes` is a well-formed program the device can evaluate
+ * (stack depths within limits) without touching a device.  On failure the
+ * reason is written to `message` (NUL-terminated, at most `message_len`). */
+int iso_program_validate(const iso_node* nodes, uint32_t n_nodes, char* message,
+                         size_t message_len);
+
+/* Validates `nodes` and makes it the context's current program. */
+int iso_program_upload(iso_ctx* ctx, const iso_node* nodes, uint32_t n_nodes);
+
+/* Samples the current program at n points (xyz interleaved, host memory):
+ * ScalarSource::sample_scalar evaluated on the GPU. */
+int iso_program_sample(iso_ctx* ctx, const float* points, uint64_t n, float* values);
+
+/* ---- sources: dense sample grids ---------------------------------------- */
+
+/* Copies a dense FP32 grid (x fastest, then y, then z) from host memory into
+ * the context's device buffer and makes it the current grid. */
+int iso_grid_upload(iso_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz);
+/* Uses caller-owned device memory as the current grid (no copy). */
+int iso_grid_bind_device(iso_ctx* ctx, const float* device_values, int64_t nx, int64_t ny,
+                         int64_t nz);
+
+/* ---- extraction ----------------------------------------------------------- */
+
+enum { ISO_SOURCE_PROGRAM = 0, ISO_SOURCE_GRID = 1 };
+enum { ISO_MEMORY_HOST = 0, ISO_MEMORY_DEVICE = 1 };
+
+/* Sample lattice.  Sample (i, j, k) sits at
+ *   (origin[0] + i*spacing[0], origin[1] + j*spacing[1], origin[2] + k*spacing[2])
+ * in f32, and cells are the (nx-1)(ny-1)(nz-1) cubes between samples,
+ * visited z-major (z, then y, then x) as PrimalGrid::traverse does
+ * (traversal/primal_grid.rs:59-85). */
+typedef struct iso_grid_desc {
+  int64_t nx, ny, nz;
+  float origin[3];
+  float spacing[3];
+  int32_t source; /* ISO_SOURCE_* */
+  int32_t flags;  /* reserved, 0 */
+} iso_grid_desc;
+
+/* Fills the lattice that MarchingCubes::new(size) traverses: size x size x
+ * (size+1) samples at i * (1/(size-1)) — including the reference's extra
+ * z layer (primal_grid.rs:44-69). */
+int iso_grid_desc_reference(int64_t size, int32_t source, iso_grid_desc* out);
+
+/* Samples the current program over a lattice into a dense FP32 grid (x
+ * fastest), in host or device memory (`memory` = ISO_MEMORY_*).  The values
+ * are bit-identical to the ones extraction classifies. */
+int iso_program_sample_lattice(iso_ctx* ctx, const iso_grid_desc* desc, float* out,
+                               int32_t memory);
+
+typedef struct iso_mesh_out {
+  float* vertices;          /* 3 floats per vertex */
+  uint32_t* indices;        /* 3 per triangle */
+  uint64_t vertex_capacity; /* in vertices */
+  uint64_t index_capacity;  /* in indices */
+  uint64_t vertex_count;    /* out */
+  uint64_t index_count;     /* out */
+  int32_t memory;           /* ISO_MEMORY_* of vertices / indices */
+  int32_t reserved;
+} iso_mesh_out;
+
+/* Marching cubes over the lattice: replaces MarchingCubes::extract with an
+ * IndexedVertices extractor (marching_cubes.rs:59-82, extractor.rs:72-94).
+ * Vertex order, positions and index lists are the reference's. */
+int iso_extract_mc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
+
+/* After an ISO_MEMORY_HOST extraction returned ISO_ERR_CAPACITY, the mesh is
+ * kept in the context; this copies it into larger buffers without
+ * re-extracting (the reference's Vec-growing sink needs no second pass). */
+int iso_mesh_fetch(iso_ctx* ctx, iso_mesh_out* out);
+
+/* ---- device memory helpers (for hosts without their own allocator) ------- */
+
+int iso_device_alloc(iso_ctx* ctx, uint64_t bytes, void** out);
+int iso_device_free(iso_ctx* ctx, void* ptr);
+int iso_memcpy_to_device(iso_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int iso_memcpy_to_host(iso_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- per-kernel timing ------------------------------------------------------ */
+
+/* When enabled, every kernel launch is bracketed by HIP events on the
+ * context's stream; durations accumulate per kernel name. */
+int iso_ctx_set_timing(iso_ctx* ctx, int enable);
+typedef struct iso_kernel_stat {
+  char name[48];
+  uint64_t launches;
+  double total_ms;
+} iso_kernel_stat;
+/* Writes up to `max` entries; *n_out receives the number available. */
+int iso_ctx_kernel_stats(iso_ctx* ctx, iso_kernel_stat* out, int max, int* n_out);
+int iso_ctx_reset_kernel_stats(iso_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISOSURFACE_AMD_H */

@@ -1,0 +1,542 @@
+// C ABI of isosurface_amd: contexts, sources, extraction, timing.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/isosurface.h"
+#include "mc_kernels.h"
+#include "program.h"
+
+namespace {
+
+// Device allocation that grows on demand and is reused across calls.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct PendingTiming {
+  const char* name;
+  hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct iso_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+
+  // current program
+  DevBuf prog;
+  int n_prog = 0;
+
+  // current grid
+  DevBuf grid_owned;
+  const float* grid = nullptr;
+  int64_t gnx = 0, gny = 0, gnz = 0;
+
+  // scratch
+  DevBuf signs, seg, blocks, totals, vout, iout, pts, vals;
+  unsigned long long* h_totals = nullptr;  // pinned
+
+  // mesh retained after a host-memory extraction whose buffers were too small
+  uint64_t kept_vertices = 0, kept_indices = 0;
+  bool kept = false;
+
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<PendingTiming> pending;
+  std::map<std::string, KernelStat> stats;
+};
+
+namespace {
+
+int fail(iso_ctx* c, int status, const std::string& msg) {
+  if (c) c->err = msg;
+  return status;
+}
+
+#define ISO_HIP(ctx, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? ISO_ERR_OUT_OF_MEMORY : ISO_ERR_DEVICE, \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+
+hipError_t ensure(DevBuf* b, size_t bytes) {
+  if (b->bytes >= bytes && b->ptr) return hipSuccess;
+  if (b->ptr) {
+    hipError_t e = hipFree(b->ptr);
+    if (e != hipSuccess) return e;
+    b->ptr = nullptr;
+    b->bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  hipError_t e = hipMalloc(&b->ptr, want);
+  if (e != hipSuccess) {
+    b->ptr = nullptr;
+    return e;
+  }
+  b->bytes = want;
+  return hipSuccess;
+}
+
+void release(DevBuf* b) {
+  if (b->ptr) (void)hipFree(b->ptr);
+  b->ptr = nullptr;
+  b->bytes = 0;
+}
+
+hipEvent_t take_event(iso_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one kernel launch with events when timing is on.
+template <typename F>
+hipError_t timed(iso_ctx* c, const char* name, F&& launch) {
+  if (!c->timing) return launch();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return launch();
+  (void)hipEventRecord(a, c->stream);
+  hipError_t e = launch();
+  (void)hipEventRecord(b, c->stream);
+  c->pending.push_back({name, a, b});
+  return e;
+}
+
+// After the stream has been synchronised: fold pending event pairs into stats.
+void collect_timing(iso_ctx* c) {
+  for (const PendingTiming& p : c->pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      KernelStat& s = c->stats[p.name];
+      s.launches++;
+      s.total_ms += ms;
+    }
+    c->event_pool.push_back(p.start);
+    c->event_pool.push_back(p.stop);
+  }
+  c->pending.clear();
+}
+
+int check_lattice(iso_ctx* c, const iso_grid_desc* d, iso::Lattice* L) {
+  if (!d) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null grid descriptor");
+  if (d->nx < 2 || d->ny < 2 || d->nz < 2)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice needs at least 2 samples per axis");
+  const double n = static_cast<double>(d->nx) * static_cast<double>(d->ny) * static_cast<double>(d->nz);
+  if (n >= 4294967296.0)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^32 or more samples");
+  if (d->source != ISO_SOURCE_PROGRAM && d->source != ISO_SOURCE_GRID)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown source kind");
+  if (d->flags != 0) return fail(c, ISO_ERR_INVALID_ARGUMENT, "flags must be 0");
+  L->nx = d->nx;
+  L->ny = d->ny;
+  L->nz = d->nz;
+  for (int i = 0; i < 3; ++i) {
+    L->origin[i] = d->origin[i];
+    L->spacing[i] = d->spacing[i];
+  }
+  return ISO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iso_abi_version(void) { return ISO_ABI_VERSION; }
+
+const char* iso_status_string(int status) {
+  switch (status) {
+    case ISO_OK: return "ok";
+    case ISO_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case ISO_ERR_CAPACITY: return "output capacity too small";
+    case ISO_ERR_DEVICE: return "device error";
+    case ISO_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case ISO_ERR_PROGRAM: return "malformed program";
+    case ISO_ERR_NO_DEVICE: return "no such device";
+    case ISO_ERR_NO_SOURCE: return "no source bound";
+    case ISO_ERR_OVERFLOW: return "mesh exceeds u32 index range";
+  }
+  return "unknown status";
+}
+
+const char* iso_ctx_last_error(const iso_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int iso_ctx_create(int device, void* stream, iso_ctx** out) {
+  if (!out) return ISO_ERR_INVALID_ARGUMENT;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return ISO_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return ISO_ERR_DEVICE;
+  iso_ctx* c = new iso_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return ISO_ERR_DEVICE;
+    }
+    c->own_stream = true;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_totals), 4 * sizeof(unsigned long long),
+                    hipHostMallocDefault) != hipSuccess) {
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ISO_ERR_OUT_OF_MEMORY;
+  }
+  *out = c;
+  return ISO_OK;
+}
+
+int iso_ctx_destroy(iso_ctx* c) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  DevBuf* bufs[] = {&c->prog, &c->grid_owned, &c->signs, &c->seg, &c->blocks, &c->totals,
+                    &c->vout, &c->iout, &c->pts, &c->vals};
+  for (DevBuf* b : bufs) release(b);
+  if (c->h_totals) (void)hipHostFree(c->h_totals);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return ISO_OK;
+}
+
+int iso_ctx_synchronize(iso_ctx* c) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
+}
+
+int iso_program_validate(const iso_node* nodes, uint32_t n_nodes, char* message,
+                         size_t message_len) {
+  std::vector<iso::DevOp> ops;
+  std::string err;
+  const int st = iso::lower_program(nodes, n_nodes, &ops, &err);
+  if (message && message_len) std::snprintf(message, message_len, "%s", err.c_str());
+  return st;
+}
+
+int iso_program_upload(iso_ctx* c, const iso_node* nodes, uint32_t n_nodes) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  std::vector<iso::DevOp> ops;
+  std::string err;
+  int st = iso::lower_program(nodes, n_nodes, &ops, &err);
+  if (st != ISO_OK) return fail(c, st, err);
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, ensure(&c->prog, ops.size() * sizeof(iso::DevOp)));
+  ISO_HIP(c, hipMemcpyAsync(c->prog.ptr, ops.data(), ops.size() * sizeof(iso::DevOp),
+                            hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  c->n_prog = static_cast<int>(ops.size());
+  return ISO_OK;
+}
+
+int iso_program_sample(iso_ctx* c, const float* points, uint64_t n, float* values) {
+  if (!c || (n && (!points || !values))) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null buffer");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (n == 0) return ISO_OK;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, ensure(&c->pts, n * 3 * sizeof(float)));
+  ISO_HIP(c, ensure(&c->vals, n * sizeof(float)));
+  ISO_HIP(c, hipMemcpyAsync(c->pts.ptr, points, n * 3 * sizeof(float), hipMemcpyHostToDevice,
+                            c->stream));
+  ISO_HIP(c, timed(c, "sample_points", [&] {
+            return iso::launch_sample_points(static_cast<const iso::DevOp*>(c->prog.ptr),
+                                             c->n_prog, static_cast<const float*>(c->pts.ptr), n,
+                                             static_cast<float*>(c->vals.ptr), c->stream);
+          }));
+  ISO_HIP(c, hipMemcpyAsync(values, c->vals.ptr, n * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
+}
+
+int iso_program_sample_lattice(iso_ctx* c, const iso_grid_desc* desc, float* out,
+                               int32_t memory) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (memory != ISO_MEMORY_HOST && memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  ISO_HIP(c, hipSetDevice(c->device));
+  const size_t bytes = static_cast<size_t>(L.nx * L.ny * L.nz) * sizeof(float);
+  float* dst = out;
+  if (memory == ISO_MEMORY_HOST) {
+    ISO_HIP(c, ensure(&c->vals, bytes));
+    dst = static_cast<float*>(c->vals.ptr);
+  }
+  ISO_HIP(c, timed(c, "sample_lattice", [&] {
+            return iso::launch_sample_lattice(static_cast<const iso::DevOp*>(c->prog.ptr),
+                                              c->n_prog, L, dst, c->stream);
+          }));
+  if (memory == ISO_MEMORY_HOST)
+    ISO_HIP(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
+}
+
+int iso_grid_upload(iso_ctx* c, const float* values, int64_t nx, int64_t ny, int64_t nz) {
+  if (!c || !values || nx < 1 || ny < 1 || nz < 1)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "bad grid upload arguments");
+  const size_t bytes = static_cast<size_t>(nx) * ny * nz * sizeof(float);
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, ensure(&c->grid_owned, bytes));
+  ISO_HIP(c, hipMemcpyAsync(c->grid_owned.ptr, values, bytes, hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  c->grid = static_cast<const float*>(c->grid_owned.ptr);
+  c->gnx = nx;
+  c->gny = ny;
+  c->gnz = nz;
+  return ISO_OK;
+}
+
+int iso_grid_bind_device(iso_ctx* c, const float* dev, int64_t nx, int64_t ny, int64_t nz) {
+  if (!c || !dev || nx < 1 || ny < 1 || nz < 1)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "bad grid bind arguments");
+  c->grid = dev;
+  c->gnx = nx;
+  c->gny = ny;
+  c->gnz = nz;
+  return ISO_OK;
+}
+
+int iso_grid_desc_reference(int64_t size, int32_t source, iso_grid_desc* out) {
+  if (!out || size < 2) return ISO_ERR_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  out->nx = size;
+  out->ny = size;
+  out->nz = size + 1;
+  // one_over_size = 1.0 / (size_minus_one as f32)  (primal_grid.rs:44-45)
+  const float ovs = 1.0f / static_cast<float>(size - 1);
+  for (int i = 0; i < 3; ++i) {
+    out->origin[i] = 0.0f;
+    out->spacing[i] = ovs;
+  }
+  out->source = source;
+  return ISO_OK;
+}
+
+int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  const bool from_grid = desc->source == ISO_SOURCE_GRID;
+  if (from_grid) {
+    if (!c->grid) return fail(c, ISO_ERR_NO_SOURCE, "no grid bound");
+    if (c->gnx != L.nx || c->gny != L.ny || c->gnz != L.nz)
+      return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice dims differ from the bound grid");
+  } else if (c->n_prog == 0) {
+    return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  }
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+
+  ISO_HIP(c, hipSetDevice(c->device));
+  const iso::McDims D = iso::mc_dims(L);
+  ISO_HIP(c, ensure(&c->signs, (D.n_sign_words + 3) * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->seg, D.n_segs * sizeof(uint2)));
+  ISO_HIP(c, ensure(&c->blocks, 2 * D.n_scan_blocks * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->totals, 4 * sizeof(unsigned long long)));
+
+  iso::McBuffers B{};
+  uint64_t* sbase = static_cast<uint64_t*>(c->signs.ptr);
+  B.signs = sbase + 1;
+  B.seg = static_cast<uint2*>(c->seg.ptr);
+  B.blocks = static_cast<uint64_t*>(c->blocks.ptr);
+  B.totals = static_cast<unsigned long long*>(c->totals.ptr);
+
+  // Zero the guard words around the sign bits and the totals / error flags.
+  ISO_HIP(c, hipMemsetAsync(sbase, 0, sizeof(uint64_t), c->stream));
+  ISO_HIP(c, hipMemsetAsync(sbase + 1 + D.n_sign_words, 0, 2 * sizeof(uint64_t), c->stream));
+  ISO_HIP(c, hipMemsetAsync(B.totals, 0, 4 * sizeof(unsigned long long), c->stream));
+
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  if (from_grid) {
+    ISO_HIP(c, timed(c, "mc_signs_grid",
+                     [&] { return iso::launch_signs_grid(c->grid, L, D, B.signs, c->stream); }));
+  } else {
+    ISO_HIP(c, timed(c, "mc_signs_program", [&] {
+              return iso::launch_signs_program(ops, c->n_prog, L, D, B.signs, c->stream);
+            }));
+  }
+  ISO_HIP(c, timed(c, "mc_count", [&] { return iso::launch_count(L, D, B, c->stream); }));
+  ISO_HIP(c, timed(c, "mc_scan_blocks", [&] { return iso::launch_scan_blocks(D, B, c->stream); }));
+  ISO_HIP(c, timed(c, "mc_scan_top", [&] { return iso::launch_scan_top(D, B, c->stream); }));
+
+  const float* grid = from_grid ? c->grid : nullptr;
+  uint64_t nv = 0, ni = 0;
+  if (out->memory == ISO_MEMORY_DEVICE) {
+    B.vertices = out->vertices;
+    B.indices = out->indices;
+    B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
+    B.index_capacity = out->indices ? out->index_capacity : 0;
+    ISO_HIP(c, timed(c, "mc_emit", [&] {
+              return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
+            }));
+    ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.totals, 3 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    nv = c->h_totals[0];
+    ni = 3 * c->h_totals[1];
+    out->vertex_count = nv;
+    out->index_count = ni;
+    if (c->h_totals[2] || nv > 0xffffffffull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+    if (nv > B.vertex_capacity || ni > B.index_capacity)
+      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    return ISO_OK;
+  }
+
+  // Host memory: size the context's device buffers from the scan totals.
+  ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.totals, 2 * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  nv = c->h_totals[0];
+  ni = 3 * c->h_totals[1];
+  if (nv > 0xffffffffull) {
+    collect_timing(c);
+    out->vertex_count = nv;
+    out->index_count = ni;
+    return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+  }
+  ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
+  ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
+  B.vertices = static_cast<float*>(c->vout.ptr);
+  B.indices = static_cast<uint32_t*>(c->iout.ptr);
+  B.vertex_capacity = nv;
+  B.index_capacity = ni;
+  ISO_HIP(c, timed(c, "mc_emit", [&] {
+            return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
+          }));
+  out->vertex_count = nv;
+  out->index_count = ni;
+  if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+      (ni && (!out->indices || out->index_capacity < ni))) {
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    c->kept = true;
+    c->kept_vertices = nv;
+    c->kept_indices = ni;
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small; mesh kept for iso_mesh_fetch");
+  }
+  if (nv)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, B.vertices, nv * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+  if (ni)
+    ISO_HIP(c, hipMemcpyAsync(out->indices, B.indices, ni * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
+}
+
+int iso_mesh_fetch(iso_ctx* c, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  if (!c->kept) return fail(c, ISO_ERR_NO_SOURCE, "no kept mesh");
+  const uint64_t nv = c->kept_vertices, ni = c->kept_indices;
+  out->vertex_count = nv;
+  out->index_count = ni;
+  if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+      (ni && (!out->indices || out->index_capacity < ni)))
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+  const hipMemcpyKind kind =
+      out->memory == ISO_MEMORY_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  ISO_HIP(c, hipSetDevice(c->device));
+  if (nv)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, c->vout.ptr, nv * 3 * sizeof(float), kind, c->stream));
+  if (ni)
+    ISO_HIP(c, hipMemcpyAsync(out->indices, c->iout.ptr, ni * sizeof(uint32_t), kind, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  c->kept = false;
+  return ISO_OK;
+}
+
+int iso_device_alloc(iso_ctx* c, uint64_t bytes, void** out) {
+  if (!c || !out) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, hipMalloc(out, bytes ? bytes : 1));
+  return ISO_OK;
+}
+
+int iso_device_free(iso_ctx* c, void* p) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  if (p) ISO_HIP(c, hipFree(p));
+  return ISO_OK;
+}
+
+int iso_memcpy_to_device(iso_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
+}
+
+int iso_memcpy_to_host(iso_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
+}
+
+int iso_ctx_set_timing(iso_ctx* c, int enable) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  c->timing = enable != 0;
+  return ISO_OK;
+}
+
+int iso_ctx_kernel_stats(iso_ctx* c, iso_kernel_stat* out, int max, int* n_out) {
+  if (!c || !n_out || (max > 0 && !out)) return ISO_ERR_INVALID_ARGUMENT;
+  *n_out = static_cast<int>(c->stats.size());
+  int i = 0;
+  for (const auto& kv : c->stats) {
+    if (i >= max) break;
+    std::memset(&out[i], 0, sizeof(out[i]));
+    std::snprintf(out[i].name, sizeof(out[i].name), "%s", kv.first.c_str());
+    out[i].launches = kv.second.launches;
+    out[i].total_ms = kv.second.total_ms;
+    ++i;
+  }
+  return ISO_OK;
+}
+
+int iso_ctx_reset_kernel_stats(iso_ctx* c) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  c->stats.clear();
+  return ISO_OK;
+}
+
+}  // extern "C"

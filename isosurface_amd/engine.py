@@ -1,0 +1,248 @@
+"""GPU contexts and the extractor front doors (Python mirror of the crate API).
+
+  MarchingCubes(size).extract(source, extractor)   marching_cubes.rs:46-82
+
+The extraction runs on the GPU through the C ABI; the result is then
+replayed into the extractor with the reference's sink protocol: every
+extract_vertex first (in first-encounter order, which the GPU reproduces),
+then every extract_index (mesh.rs:91-100, extractor.rs:17-20).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _abi as A
+from .sources import Source, to_nodes
+
+
+class Extractor:
+    """Sink of mesh data (extractor.rs:17-20)."""
+
+    def extract_vertex(self, vertex) -> None:  # pragma: no cover - interface
+        raise NotImplementedError
+
+    def extract_index(self, index: int) -> None:  # pragma: no cover - interface
+        raise NotImplementedError
+
+    # Bulk forms; sinks may override them to avoid per-element calls.
+    def extract_vertices(self, vertices: np.ndarray) -> None:
+        for v in vertices:
+            self.extract_vertex(v)
+
+    def extract_indices(self, indices: np.ndarray) -> None:
+        for i in indices:
+            self.extract_index(int(i))
+
+
+class IndexedVertices(Extractor):
+    """xyz f32 per vertex and u32 indices (extractor.rs:72-94)."""
+
+    def __init__(self, vertices: Optional[list] = None, indices: Optional[list] = None):
+        self.vertices = vertices if vertices is not None else []
+        self.indices = indices if indices is not None else []
+
+    def extract_vertex(self, v):
+        self.vertices.extend(float(c) for c in v)
+
+    def extract_index(self, index):
+        self.indices.append(int(index) & 0xFFFFFFFF)
+
+    def extract_vertices(self, vertices):
+        self.vertices.extend(np.asarray(vertices, dtype=np.float32).reshape(-1).tolist())
+
+    def extract_indices(self, indices):
+        self.indices.extend(np.asarray(indices, dtype=np.uint32).tolist())
+
+
+class OnlyVertices(Extractor):
+    """Vertices only; indices are discarded (extractor.rs:24-42)."""
+
+    def __init__(self, vertices: Optional[list] = None):
+        self.vertices = vertices if vertices is not None else []
+
+    def extract_vertex(self, v):
+        self.vertices.extend(float(c) for c in v)
+
+    def extract_index(self, index):
+        pass
+
+    def extract_vertices(self, vertices):
+        self.vertices.extend(np.asarray(vertices, dtype=np.float32).reshape(-1).tolist())
+
+    def extract_indices(self, indices):
+        pass
+
+
+@dataclass
+class Mesh:
+    vertices: np.ndarray  # (V, 3) float32
+    indices: np.ndarray   # (3T,) uint32
+
+
+class Context:
+    """One device context (iso_ctx): owns the GPU scratch buffers."""
+
+    def __init__(self, device: int = 0):
+        self.lib = A.load()
+        h = C.c_void_p()
+        st = self.lib.iso_ctx_create(device, None, C.byref(h))
+        if st != A.ISO_OK:
+            raise A.IsoError(st, "iso_ctx_create: " + self.lib.iso_status_string(st).decode())
+        self.handle = h
+        self._program_key = None
+
+    def close(self):
+        if self.handle:
+            self.lib.iso_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- plumbing
+    def check(self, st: int, what: str) -> int:
+        if st != A.ISO_OK:
+            msg = self.lib.iso_ctx_last_error(self.handle).decode()
+            raise A.IsoError(st, f"{what}: {msg}")
+        return st
+
+    def upload_program(self, program) -> None:
+        key = tuple((op, tuple(p)) for op, p in program)
+        if key == self._program_key:
+            return
+        nodes = to_nodes(program)
+        self.check(self.lib.iso_program_upload(self.handle, nodes, len(program)),
+                   "iso_program_upload")
+        self._program_key = key
+
+    def upload_grid(self, values: np.ndarray) -> None:
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        nz, ny, nx = values.shape
+        self.check(self.lib.iso_grid_upload(self.handle, values.ctypes.data, nx, ny, nz),
+                   "iso_grid_upload")
+        self._program_key = self._program_key  # grids and programs are independent
+
+    def sample(self, source: Source, points: np.ndarray) -> np.ndarray:
+        """ScalarSource::sample_scalar at each point, on the GPU."""
+        self.upload_program(source.program())
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty(len(pts), dtype=np.float32)
+        self.check(self.lib.iso_program_sample(self.handle, pts.ctypes.data, len(pts),
+                                               out.ctypes.data), "iso_program_sample")
+        return out
+
+    def reference_desc(self, size: int, source_kind: int) -> A.IsoGridDesc:
+        d = A.IsoGridDesc()
+        self.check(self.lib.iso_grid_desc_reference(size, source_kind, C.byref(d)),
+                   "iso_grid_desc_reference")
+        return d
+
+    def extract_mc(self, desc: A.IsoGridDesc, vertex_capacity: int = 0,
+                   index_capacity: int = 0) -> Mesh:
+        """iso_extract_mc into host arrays (grown on ISO_ERR_CAPACITY)."""
+        vcap = max(int(vertex_capacity), 1)
+        icap = max(int(index_capacity), 3)
+        verts = np.empty((vcap, 3), dtype=np.float32)
+        idx = np.empty(icap, dtype=np.uint32)
+        out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
+                           A.ISO_MEMORY_HOST, 0)
+        st = self.lib.iso_extract_mc(self.handle, C.byref(desc), C.byref(out))
+        if st == A.ISO_ERR_CAPACITY:
+            nv, ni = int(out.vertex_count), int(out.index_count)
+            verts = np.empty((max(nv, 1), 3), dtype=np.float32)
+            idx = np.empty(max(ni, 3), dtype=np.uint32)
+            out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, max(nv, 1), max(ni, 3), 0, 0,
+                               A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.handle, C.byref(out))
+        self.check(st, "iso_extract_mc")
+        nv, ni = int(out.vertex_count), int(out.index_count)
+        return Mesh(verts[:nv].copy(), idx[:ni].copy())
+
+    # ---- timing
+    def set_timing(self, on: bool) -> None:
+        self.check(self.lib.iso_ctx_set_timing(self.handle, 1 if on else 0), "set_timing")
+
+    def kernel_stats(self) -> dict:
+        arr = (A.IsoKernelStat * 64)()
+        n = C.c_int()
+        self.check(self.lib.iso_ctx_kernel_stats(self.handle, arr, 64, C.byref(n)),
+                   "kernel_stats")
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms))
+                for i in range(min(n.value, 64))}
+
+    def reset_kernel_stats(self) -> None:
+        self.check(self.lib.iso_ctx_reset_kernel_stats(self.handle), "reset_kernel_stats")
+
+    def synchronize(self) -> None:
+        self.check(self.lib.iso_ctx_synchronize(self.handle), "synchronize")
+
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _replay(mesh: Mesh, extractor: Extractor) -> None:
+    extractor.extract_vertices(mesh.vertices)
+    extractor.extract_indices(mesh.indices)
+
+
+def sample_lattice_host(source, desc: A.IsoGridDesc) -> np.ndarray:
+    """Samples an arbitrary host source over the lattice (x fastest).
+
+    The coordinates are the reference's: i as f32 * one_over_size
+    (primal_grid.rs:50, 63-67)."""
+    def axis(n, o, s):
+        return (np.float32(o) + np.arange(n, dtype=np.float32) * np.float32(s)).astype(np.float32)
+
+    xs = axis(desc.nx, desc.origin[0], desc.spacing[0])
+    ys = axis(desc.ny, desc.origin[1], desc.spacing[1])
+    zs = axis(desc.nz, desc.origin[2], desc.spacing[2])
+    z, y, x = np.meshgrid(zs, ys, xs, indexing="ij")
+    v = source.sample_scalar(x, y, z)
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float32).reshape(desc.nz, desc.ny,
+                                                                         desc.nx))
+
+
+class MarchingCubes:
+    """MarchingCubes::<Signed>::new(size) (marching_cubes.rs:46-50) on the GPU."""
+
+    def __init__(self, size: int, context: Optional[Context] = None):
+        if size < 2:
+            raise ValueError("size must be at least 2")
+        self.size = int(size)
+        self.ctx = context or default_context()
+        self._last_counts = (0, 0)
+
+    def extract_mesh(self, source) -> Mesh:
+        if isinstance(source, Source):
+            self.ctx.upload_program(source.program())
+            desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_PROGRAM)
+        else:
+            desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_GRID)
+            self.ctx.upload_grid(sample_lattice_host(source, desc))
+        mesh = self.ctx.extract_mc(desc, *self._last_counts)
+        self._last_counts = (len(mesh.vertices), len(mesh.indices))
+        return mesh
+
+    def extract(self, source, extractor: Extractor) -> None:
+        """MarchingCubes::extract (marching_cubes.rs:59-82)."""
+        _replay(self.extract_mesh(source), extractor)

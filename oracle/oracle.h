@@ -1,0 +1,66 @@
+/*
+ * CPU oracle for isosurface_amd — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain C++ restatement of the reference crate's CPU extraction path
+ * (swiftcoder/isosurface), used by the test suite, the smoke check and the
+ * benchmark's CPU-baseline leg as the checker.  The product never links,
+ * loads or calls it.  Built with -O2 -ffp-contract=off so every f32 operation
+ * is rounded exactly once, as rustc does.
+ *
+ * Each function cites the reference file:line it restates.
+ */
+#ifndef ISO_ORACLE_H
+#define ISO_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/isosurface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_mesh {
+  float* vertices;     /* 3 per vertex, emission order */
+  uint32_t* indices;   /* 3 per triangle */
+  int64_t* edge_keys;  /* 4 per vertex: min lattice point x,y,z and axis 0/1/2 */
+  uint64_t n_vertices;
+  uint64_t n_indices;
+  /* per-cell classification (cube index), z-major, if requested */
+  uint8_t* cases;
+  uint64_t n_cases;
+} or_mesh;
+
+/* Scalar SDF of a program at one point (ScalarSource::sample_scalar). */
+float or_sample(const iso_node* prog, uint32_t n, float x, float y, float z);
+/* The same SDF evaluated in f64 (the FP64 twin). */
+double or_sample_f64(const iso_node* prog, uint32_t n, double x, double y, double z);
+/* Validates a program; returns 0 when well formed. */
+int or_program_check(const iso_node* prog, uint32_t n);
+
+/* Dense samples of a program over a lattice (x fastest). */
+int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* desc, float* out);
+
+/* MarchingCubes::extract with IndexedVertices over a lattice whose samples
+ * come from `prog` (if grid == NULL) or from the dense `grid`.
+ * want_cases != 0 also records every cell's cube index. */
+int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid,
+                  const iso_grid_desc* desc, int want_cases, or_mesh* out);
+
+/* f64 crossing point of lattice edges (FP64 twin): for each key
+ * (x, y, z, axis) the zero of the linear interpolant between the f64 SDF
+ * values at the edge's endpoints. */
+int or_edge_points_f64(const iso_node* prog, uint32_t n, const iso_grid_desc* desc,
+                       const int64_t* keys, uint64_t n_keys, double* out_xyz);
+
+void or_mesh_free(or_mesh* m);
+
+/* Decoded tables, for checksum tests. */
+void or_tri_table(int8_t out[256 * 16]);
+void or_edge_mask_table(uint16_t out[256]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,189 @@
+"""ctypes wrapper of the CPU oracle — TEST INFRASTRUCTURE ONLY.
+
+Only the test suite, the smoke check and the benchmark's CPU-baseline leg
+import this module, and only as the checker.  The product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+class IsoNode(C.Structure):
+    _fields_ = [("op", C.c_uint32), ("reserved", C.c_uint32), ("p", C.c_float * 6)]
+
+
+class IsoGridDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("origin", C.c_float * 3), ("spacing", C.c_float * 3),
+        ("source", C.c_int32), ("flags", C.c_int32),
+    ]
+
+
+class OrMesh(C.Structure):
+    _fields_ = [
+        ("vertices", C.POINTER(C.c_float)),
+        ("indices", C.POINTER(C.c_uint32)),
+        ("edge_keys", C.POINTER(C.c_int64)),
+        ("n_vertices", C.c_uint64),
+        ("n_indices", C.c_uint64),
+        ("cases", C.POINTER(C.c_uint8)),
+        ("n_cases", C.c_uint64),
+    ]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = C.CDLL(str(LIB_PATH))
+        lib.or_sample.restype = C.c_float
+        lib.or_sample.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_float, C.c_float, C.c_float]
+        lib.or_sample_f64.restype = C.c_double
+        lib.or_sample_f64.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_double, C.c_double,
+                                      C.c_double]
+        lib.or_program_check.restype = C.c_int
+        lib.or_program_check.argtypes = [C.POINTER(IsoNode), C.c_uint32]
+        lib.or_sample_lattice.restype = C.c_int
+        lib.or_sample_lattice.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.POINTER(IsoGridDesc),
+                                          C.c_void_p]
+        lib.or_extract_mc.restype = C.c_int
+        lib.or_extract_mc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
+                                      C.POINTER(IsoGridDesc), C.c_int, C.POINTER(OrMesh)]
+        lib.or_edge_points_f64.restype = C.c_int
+        lib.or_edge_points_f64.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.POINTER(IsoGridDesc),
+                                           C.c_void_p, C.c_uint64, C.c_void_p]
+        lib.or_mesh_free.restype = None
+        lib.or_mesh_free.argtypes = [C.POINTER(OrMesh)]
+        lib.or_tri_table.restype = None
+        lib.or_tri_table.argtypes = [C.c_void_p]
+        lib.or_edge_mask_table.restype = None
+        lib.or_edge_mask_table.argtypes = [C.c_void_p]
+        _lib = lib
+    return _lib
+
+
+def nodes(program):
+    arr = (IsoNode * len(program))()
+    for i, (op, params) in enumerate(program):
+        arr[i].op = op
+        for j, v in enumerate(params):
+            arr[i].p[j] = v
+    return arr
+
+
+def reference_desc(size: int, source: int = 0) -> IsoGridDesc:
+    """The lattice of MarchingCubes::new(size) (primal_grid.rs:44-69)."""
+    d = IsoGridDesc()
+    d.nx, d.ny, d.nz = size, size, size + 1
+    ovs = np.float32(1.0) / np.float32(size - 1)
+    for i in range(3):
+        d.origin[i] = 0.0
+        d.spacing[i] = float(ovs)
+    d.source = source
+    return d
+
+
+def lattice_desc(nx, ny, nz, origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0), source=1):
+    d = IsoGridDesc()
+    d.nx, d.ny, d.nz = nx, ny, nz
+    for i in range(3):
+        d.origin[i] = origin[i]
+        d.spacing[i] = spacing[i]
+    d.source = source
+    return d
+
+
+def sample(program, x, y, z) -> float:
+    lib = load()
+    arr = nodes(program)
+    return float(np.float32(lib.or_sample(arr, len(program), x, y, z)))
+
+
+def sample_f64(program, x, y, z) -> float:
+    lib = load()
+    arr = nodes(program)
+    return float(lib.or_sample_f64(arr, len(program), x, y, z))
+
+
+def sample_lattice(program, desc: IsoGridDesc) -> np.ndarray:
+    lib = load()
+    arr = nodes(program)
+    out = np.empty((desc.nz, desc.ny, desc.nx), dtype=np.float32)
+    st = lib.or_sample_lattice(arr, len(program), C.byref(desc), out.ctypes.data)
+    assert st == 0, st
+    return out
+
+
+class OracleMesh:
+    def __init__(self, vertices, indices, keys, cases):
+        self.vertices = vertices  # (V,3) f32
+        self.indices = indices    # (3T,) u32
+        self.keys = keys          # (V,4) i64: lattice point + axis
+        self.cases = cases        # per-cell cube index (or None)
+
+
+def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=False) -> OracleMesh:
+    """MarchingCubes::extract + IndexedVertices, restated on the CPU."""
+    lib = load()
+    arr = nodes(program) if program is not None else None
+    n = len(program) if program is not None else 0
+    gptr = None
+    if grid is not None:
+        grid = np.ascontiguousarray(grid, dtype=np.float32)
+        gptr = grid.ctypes.data
+    m = OrMesh()
+    st = lib.or_extract_mc(arr, n, gptr, C.byref(desc), 1 if want_cases else 0, C.byref(m))
+    assert st == 0, st
+    try:
+        nv, ni = int(m.n_vertices), int(m.n_indices)
+        v = np.ctypeslib.as_array(m.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else \
+            np.zeros((0, 3), np.float32)
+        i = np.ctypeslib.as_array(m.indices, shape=(ni,)).copy() if ni else np.zeros(0, np.uint32)
+        k = np.ctypeslib.as_array(m.edge_keys, shape=(nv * 4,)).reshape(nv, 4).copy() if nv else \
+            np.zeros((0, 4), np.int64)
+        c = None
+        if want_cases and m.n_cases:
+            c = np.ctypeslib.as_array(m.cases, shape=(int(m.n_cases),)).copy()
+    finally:
+        lib.or_mesh_free(C.byref(m))
+    return OracleMesh(v, i, k, c)
+
+
+def edge_points_f64(program, desc: IsoGridDesc, keys: np.ndarray) -> np.ndarray:
+    lib = load()
+    arr = nodes(program)
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    out = np.empty((len(keys), 3), dtype=np.float64)
+    st = lib.or_edge_points_f64(arr, len(program), C.byref(desc), keys.ctypes.data, len(keys),
+                                out.ctypes.data)
+    assert st == 0
+    return out
+
+
+def tri_table() -> np.ndarray:
+    out = np.empty(256 * 16, dtype=np.int8)
+    load().or_tri_table(out.ctypes.data)
+    return out.reshape(256, 16)
+
+
+def edge_mask_table() -> np.ndarray:
+    out = np.empty(256, dtype=np.uint16)
+    load().or_edge_mask_table(out.ctypes.data)
+    return out

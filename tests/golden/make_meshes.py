@@ -1,0 +1,49 @@
+"""Writes mesh_digests.json: counts and SHA-256 digests of the oracle's
+marching-cubes meshes for the benchmark workloads.
+
+The oracle (oracle/, a restatement of the reference's CPU extraction) is run
+here once; the GPU tests compare their output digests with these values, so
+full-size parity needs no CPU re-run on the GPU host.
+
+usage: python tests/golden/make_meshes.py
+"""
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+from isosurface_amd import workloads as W  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def digest(a) -> str:
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def main() -> None:
+    cases = {
+        "sphere_r0.35_size65": (W.sphere_source(), 65),
+        "spheres64_minus_torus_size65": (W.spheres_minus_torus()[0], 65),
+        "spheres64_minus_torus_size129": (W.spheres_minus_torus()[0], 129),
+        "spheres64_minus_torus_size257": (W.spheres_minus_torus()[0], 257),
+    }
+    out = {}
+    for name, (src, size) in cases.items():
+        m = O.extract_mc(src.program(), desc=O.reference_desc(size))
+        out[name] = {
+            "size": size,
+            "vertices": int(len(m.vertices)),
+            "triangles": int(len(m.indices) // 3),
+            "vertices_sha256": digest(m.vertices.astype("<f4")),
+            "indices_sha256": digest(m.indices.astype("<u4")),
+        }
+        print(name, out[name]["vertices"], out[name]["triangles"], flush=True)
+    path = Path(__file__).with_name("mesh_digests.json")
+    path.write_text(json.dumps(out, indent=1) + "\n")
+
+
+if __name__ == "__main__":
+    main()

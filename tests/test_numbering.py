@@ -1,0 +1,132 @@
+"""The GPU's closed-form vertex numbering, checked against the oracle.
+
+The emit kernel never hashes: it derives every vertex id from the lattice
+position of its edge.  A lattice edge's vertex is owned by the lowest cell
+(z-major) containing the edge; owners own e5, e6, e10 always and the
+low-face edges only on the lattice boundary; a vertex's id is the owner's
+running count plus the rank of the edge among the owner's owned edges in
+first-appearance order of its triangle list.  This file restates that rule
+in plain Python and checks it reproduces the oracle's first-encounter
+numbering (mesh.rs:240-251) exactly, including lattice boundaries and zero
+samples."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+CORNER = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6),
+         (3, 7)]
+TRI = O.tri_table()
+MASK = [int(v) for v in O.edge_mask_table()]
+
+
+def _before():
+    out = [[0] * 12 for _ in range(256)]
+    for c in range(256):
+        seen = []
+        for e in map(int, TRI[c]):
+            if e < 0:
+                break
+            if e not in seen:
+                seen.append(e)
+        acc = 0
+        for e in seen:
+            out[c][e] = acc
+            acc |= 1 << e
+    return out
+
+
+BEFORE = _before()
+GEOM = []
+OF_AXIS_MIN = {}
+for _e, (_u, _v) in enumerate(EDGES):
+    _a, _b = CORNER[_u], CORNER[_v]
+    _axis = [d for d in range(3) if _a[d] != _b[d]][0]
+    _mn = tuple(min(_a[d], _b[d]) for d in range(3))
+    GEOM.append((_axis, _mn))
+    OF_AXIS_MIN[(_axis, _mn)] = _e
+
+
+def owned(x, y, z):
+    m = (1 << 5) | (1 << 6) | (1 << 10)
+    if y == 0:
+        m |= (1 << 4) | (1 << 9)
+    if z == 0:
+        m |= (1 << 2) | (1 << 1)
+    if x == 0:
+        m |= (1 << 7) | (1 << 11)
+    if y == 0 and z == 0:
+        m |= 1
+    if x == 0 and z == 0:
+        m |= 1 << 3
+    if x == 0 and y == 0:
+        m |= 1 << 8
+    return m
+
+
+def closed_form_indices(g):
+    nz, ny, nx = g.shape
+    s = ~(g > 0)
+
+    def case(x, y, z):
+        return sum(1 << i for i, (dx, dy, dz) in enumerate(CORNER) if s[z + dz, y + dy, x + dx])
+
+    cells = [(x, y, z) for z in range(nz - 1) for y in range(ny - 1) for x in range(nx - 1)]
+    base, acc = {}, 0
+    for c in cells:
+        base[c] = acc
+        acc += bin(owned(*c) & MASK[case(*c)]).count("1")
+    idx = []
+    for (x, y, z) in cells:
+        c = case(x, y, z)
+        for e in map(int, TRI[c]):
+            if e < 0:
+                break
+            axis, mn = GEOM[e]
+            cc = (x, y, z)
+            r, rel = [0, 0, 0], [0, 0, 0]
+            for d in range(3):
+                if d != axis:
+                    if mn[d] == 0 and cc[d] >= 1:
+                        r[d], rel[d] = -1, 1
+                    else:
+                        rel[d] = mn[d]
+            o = (x + r[0], y + r[1], z + r[2])
+            eo = OF_AXIS_MIN[(axis, tuple(rel))]
+            oc = case(*o)
+            oown = owned(*o) & MASK[oc]
+            idx.append(base[o] + bin(oown & BEFORE[oc][eo]).count("1"))
+    return acc, np.array(idx, np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(120))
+def test_closed_form_numbering_matches_first_encounter(seed):
+    rng = np.random.RandomState(1000 + seed)
+    nx, ny, nz = rng.randint(2, 8, size=3)
+    g = rng.randn(nz, ny, nx).astype(np.float32)
+    if seed % 3 == 0:
+        g[rng.rand(nz, ny, nx) < 0.3] = 0.0
+    m = O.extract_mc(grid=g, desc=O.lattice_desc(nx, ny, nz))
+    nv, idx = closed_form_indices(g)
+    assert nv == len(m.vertices)
+    assert np.array_equal(idx, m.indices)
+
+
+def test_owned_edges_partition_lattice_edges():
+    # Every crossed lattice edge is owned by exactly one cell.
+    rng = np.random.RandomState(7)
+    g = rng.randn(5, 6, 7).astype(np.float32)
+    nz, ny, nx = g.shape
+    seen = {}
+    for z in range(nz - 1):
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                for e in range(12):
+                    if owned(x, y, z) >> e & 1:
+                        axis, mn = GEOM[e]
+                        key = (x + mn[0], y + mn[1], z + mn[2], axis)
+                        assert key not in seen
+                        seen[key] = (x, y, z)
+    n_edges = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
+    assert len(seen) == n_edges

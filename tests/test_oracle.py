@@ -1,0 +1,219 @@
+"""Pins the CPU oracle: the reference's own known-answer tests, closed forms,
+an independent tiny restatement of the traversal, and mesh invariants that a
+plausible mistake (dropped term, swapped operand, wrong index) would break."""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from isosurface_amd import workloads as W
+from isosurface_amd.sources import (Cylinder, DemoSource, Difference, Intersection,
+                                    RectangularPrism, SmoothUnion, Sphere, Torus, Translate, Union)
+from oracle import oracle as O
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+KATS = json.loads((GOLDEN / "reference_kats.json").read_text())
+
+CORNER = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+EDGES = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6),
+         (3, 7)]
+F32 = np.float32
+
+
+@pytest.mark.parametrize("kat", KATS, ids=[k["where"] for k in KATS])
+def test_reference_known_answers(kat):
+    prog = [(op, tuple(p)) for op, p in kat["program"]]
+    assert O.sample(prog, *kat["point"]) == kat["expected"]
+
+
+def test_fold_len_sq():
+    # vector.rs:353-356 (test_fold_works): len_sq(1,2,3) == 14 -> sphere(0) at (1,2,3) == sqrt(14)
+    assert O.sample(Sphere(0.0).program(), 1.0, 2.0, 3.0) == float(np.sqrt(F32(14.0)))
+
+
+def test_difference_is_b_minus_a():
+    # csg.rs:96-100: Difference(a, b) = b.max(-a).  Small sphere a removed from big sphere b:
+    # at the centre the value is -(-1) = 1 (outside); swapping the operands gives 2.
+    prog = Difference(Sphere(1.0), Sphere(2.0)).program()
+    assert O.sample(prog, 0.0, 0.0, 0.0) == 1.0
+    assert O.sample(prog, 1.5, 0.0, 0.0) == -0.5
+
+
+def test_translate_and_demo_source_shift():
+    # DemoSource samples at p - 0.5 (examples/common/sources.rs:38-43)
+    prog = DemoSource(Sphere(0.25)).program()
+    assert O.sample(prog, 0.75, 0.5, 0.5) == 0.0
+    assert O.sample(prog, 0.5, 0.5, 0.5) == -0.25
+    prog = Translate((0.25, 0.0, -0.5), Sphere(0.5)).program()
+    assert O.sample(prog, 0.25, 0.0, 0.0) == 0.0
+
+
+def test_smooth_union_closed_forms():
+    k = 0.5
+    prog = SmoothUnion(Sphere(1.0), Sphere(1.0), k).program()
+    # a == b: h = 1 -> a - k/4
+    assert O.sample(prog, 0.0, 0.0, 0.0) == float(F32(-1.0) - F32(k) * F32(0.25))
+    far = SmoothUnion(Sphere(1.0), Translate((10.0, 0.0, 0.0), Sphere(1.0)), k).program()
+    # |a - b| >= k: plain minimum
+    assert O.sample(far, 0.0, 0.0, 0.0) == -1.0
+
+
+def test_f64_twin_agrees():
+    src, _ = W.spheres_minus_torus()
+    prog = src.program()
+    rng = np.random.RandomState(0)
+    for p in rng.rand(200, 3):
+        a = O.sample(prog, *map(float, p.astype(F32)))
+        b = O.sample_f64(prog, *map(float, p.astype(F32)))
+        assert abs(a - b) <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Marching cubes
+# ---------------------------------------------------------------------------
+
+def tiny_reference_mc(g: np.ndarray, spacing: float):
+    """A second, independent restatement of MarchingCubes::extract for tiny
+    lattices: explicit layer loop (primal_grid.rs:39-89), dict IndexCache
+    keyed by sorted corner tuples (index_cache.rs:48-61), f32 crossing points
+    (distance.rs:64-70), first-encounter numbering (mesh.rs:240-251)."""
+    tri = O.tri_table()
+    mask = O.edge_mask_table()
+    nz, ny, nx = g.shape
+    sp = F32(spacing)
+    verts, faces, cache = [], [], {}
+    for z in range(nz - 1):
+        for y in range(ny - 1):
+            for x in range(nx - 1):
+                keys = [(x + c[0], y + c[1], z + c[2]) for c in CORNER]
+                vals = [g[k[2], k[1], k[0]] for k in keys]
+                pts = [tuple(F32(F32(i) * sp) for i in k) for k in keys]
+                cube = sum(1 << i for i in range(8) if not vals[i] > 0)
+                pos = {}
+                for e, (u, v) in enumerate(EDGES):
+                    if mask[cube] >> e & 1:
+                        a, b = vals[u], vals[v]
+                        d = F32(b - a)
+                        t = F32(0.5) if d == 0 else F32(-a / d)
+                        s = F32(F32(1.0) - t)
+                        pos[e] = tuple(F32(F32(pa * s) + F32(pb * t))
+                                       for pa, pb in zip(pts[u], pts[v]))
+                for i in range(5):
+                    if tri[cube][3 * i] < 0:
+                        break
+                    face = []
+                    for e in tri[cube][3 * i:3 * i + 3]:
+                        u, v = EDGES[e]
+                        key = tuple(sorted((keys[u], keys[v])))
+                        if key not in cache:
+                            cache[key] = len(verts)
+                            verts.append(pos[e])
+                        face.append(cache[key])
+                    faces.extend(face)
+    return np.array(verts, dtype=F32).reshape(-1, 3), np.array(faces, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_oracle_matches_tiny_restatement(seed):
+    rng = np.random.RandomState(seed)
+    nx, ny, nz = rng.randint(2, 6, size=3)
+    g = rng.randn(nz, ny, nx).astype(F32)
+    if seed % 4 == 0:
+        g[rng.rand(nz, ny, nx) < 0.25] = 0.0
+    spacing = 0.25
+    m = O.extract_mc(grid=g, desc=O.lattice_desc(nx, ny, nz, spacing=(spacing,) * 3))
+    v, i = tiny_reference_mc(g, spacing)
+    assert np.array_equal(m.indices, i)
+    assert m.vertices.tobytes() == v.tobytes()
+
+
+def test_classification_all_256_cases_and_special_values():
+    # classify_corners: bit i set iff !(v_i > 0); 0.0, -0.0 and NaN are inside.
+    for case in range(256):
+        for inside_value in (-1.0, 0.0, -0.0, float("nan")):
+            g = np.empty((2, 2, 2), F32)
+            for i, (x, y, z) in enumerate(CORNER):
+                g[z, y, x] = inside_value if case >> i & 1 else 1.0
+            m = O.extract_mc(grid=g, desc=O.lattice_desc(2, 2, 2), want_cases=True)
+            assert int(m.cases[0]) == case
+            ntri = int(np.sum(O.tri_table()[case] >= 0) // 3)
+            assert len(m.indices) == 3 * ntri
+
+
+def _edge_count(g):
+    s = ~(g > 0)
+    return int((s[:, :, 1:] != s[:, :, :-1]).sum() + (s[:, 1:, :] != s[:, :-1, :]).sum() +
+               (s[1:, :, :] != s[:-1, :, :]).sum())
+
+
+def _tri_count(g):
+    s = (~(g > 0)).astype(np.int64)
+    case = np.zeros((g.shape[0] - 1, g.shape[1] - 1, g.shape[2] - 1), np.int64)
+    for i, (x, y, z) in enumerate(CORNER):
+        case |= s[z:z + case.shape[0], y:y + case.shape[1], x:x + case.shape[2]] << i
+    ntri = (O.tri_table() >= 0).sum(axis=1) // 3
+    return int(ntri[case].sum())
+
+
+def test_sphere_size65_counts_and_closed_surface():
+    desc = O.reference_desc(65)
+    prog = W.sphere_source().program()
+    m = O.extract_mc(prog, desc=desc)
+    g = O.sample_lattice(prog, desc)
+    assert len(m.vertices) == 9486 == _edge_count(g)
+    assert len(m.indices) // 3 == 18968 == _tri_count(g)
+    tri = m.indices.reshape(-1, 3).astype(np.int64)
+    e = np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1)
+    uniq, counts = np.unique(e[:, 0] * (1 << 32) + e[:, 1], return_counts=True)
+    assert np.all(counts == 2)  # closed 2-manifold
+    assert len(m.vertices) - len(uniq) + len(tri) == 2  # Euler characteristic of a sphere
+
+
+def test_vertices_lie_on_their_edges_at_the_zero_crossing():
+    desc = O.reference_desc(33)
+    src, _ = W.spheres_minus_torus()
+    prog = src.program()
+    m = O.extract_mc(prog, desc=desc)
+    g = O.sample_lattice(prog, desc).astype(np.float64)
+    h = float(F32(1.0) / F32(32))
+    for (x, y, z, ax), p in zip(m.keys, m.vertices.astype(np.float64)):
+        a = np.array([x, y, z], np.float64) * h
+        step = np.zeros(3)
+        step[ax] = h
+        off = np.delete(p - a, ax)
+        assert np.all(np.abs(off) <= 1e-6)  # on the edge
+        t = (p[ax] - a[ax]) / h
+        assert -1e-6 <= t <= 1 + 1e-6
+        va = g[z, y, x]
+        vb = g[z + (ax == 2), y + (ax == 1), x + (ax == 0)]
+        assert (va > 0) != (vb > 0)
+        assert abs(va + t * (vb - va)) <= 1e-5 * max(1.0, abs(va) + abs(vb))
+
+
+def test_counts_match_independent_counts_on_csg():
+    desc = O.reference_desc(49)
+    src, _ = W.spheres_minus_torus()
+    prog = src.program()
+    m = O.extract_mc(prog, desc=desc)
+    g = O.sample_lattice(prog, desc)
+    assert len(m.vertices) == _edge_count(g)
+    assert len(m.indices) // 3 == _tri_count(g)
+
+
+def test_program_and_grid_sources_agree():
+    desc = O.reference_desc(33)
+    prog = W.sphere_source().program()
+    a = O.extract_mc(prog, desc=desc)
+    g = O.sample_lattice(prog, desc)
+    gd = O.reference_desc(33, source=1)
+    b = O.extract_mc(grid=g, desc=gd)
+    assert a.vertices.tobytes() == b.vertices.tobytes()
+    assert np.array_equal(a.indices, b.indices)
+
+
+def test_oracle_rejects_malformed_programs():
+    assert O.load().or_program_check(O.nodes([(16, ())]), 1) != 0
+    assert O.load().or_program_check(O.nodes([(1, (1.0,)), (1, (1.0,))]), 2) != 0
+    assert O.load().or_program_check(O.nodes([(32, (0.0, 0.0, 0.0)), (1, (1.0,))]), 2) != 0
