@@ -58,7 +58,21 @@ def build(verbose: bool = False) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=str(ROOT))
     os.replace(tmp, LIB)
+    build_example(verbose)
     return LIB
+
+
+EXAMPLE = PKG / "host" / "example_mc"
+
+
+def build_example(verbose: bool = False) -> Path:
+    """Compiles the C++ mirror's example program against the library."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", str(PKG / "host" / "example_mc.cpp"),
+           "-o", str(EXAMPLE), f"-L{PKG}", "-lisosurface_amd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=str(ROOT))
+    return EXAMPLE
 
 
 if __name__ == "__main__":

@@ -47,7 +47,8 @@ struct iso_ctx {
   int64_t gnx = 0, gny = 0, gnz = 0;
 
   // scratch
-  DevBuf signs, seg, blocks, totals, vout, iout, pts, vals;
+  DevBuf signs, seg, tplanes, flags, agg, incl, active, totals, vout, iout, pts, vals;
+  uint64_t active_capacity = 0;
   unsigned long long* h_totals = nullptr;  // pinned
 
   // mesh retained after a host-memory extraction whose buffers were too small
@@ -199,7 +200,7 @@ int iso_ctx_create(int device, void* stream, iso_ctx** out) {
     }
     c->own_stream = true;
   }
-  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_totals), 4 * sizeof(unsigned long long),
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_totals), iso::kTotSlots * sizeof(unsigned long long),
                     hipHostMallocDefault) != hipSuccess) {
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -215,8 +216,9 @@ int iso_ctx_destroy(iso_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
-  DevBuf* bufs[] = {&c->prog, &c->grid_owned, &c->signs, &c->seg, &c->blocks, &c->totals,
-                    &c->vout, &c->iout, &c->pts, &c->vals};
+  DevBuf* bufs[] = {&c->prog, &c->grid_owned, &c->signs,  &c->seg,    &c->tplanes,
+                    &c->flags, &c->agg,        &c->incl,   &c->active, &c->totals,
+                    &c->vout, &c->iout,        &c->pts,    &c->vals};
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -345,6 +347,67 @@ int iso_grid_desc_reference(int64_t size, int32_t source, iso_grid_desc* out) {
   return ISO_OK;
 }
 
+namespace {
+
+// Sizes the context's scratch for a lattice; the active-cell list starts at
+// 1/16 of the cells (at least 2^20) and grows when a mesh needs more.
+int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuffers* B) {
+  ISO_HIP(c, ensure(&c->signs, (D.n_sign_words + 3) * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->seg, D.n_segs * sizeof(uint2)));
+  ISO_HIP(c, ensure(&c->tplanes, 3 * D.n_segs * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->flags, D.n_blocks * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->agg, D.n_blocks * sizeof(iso::ScanTriple)));
+  ISO_HIP(c, ensure(&c->incl, D.n_blocks * sizeof(iso::ScanTriple)));
+  ISO_HIP(c, ensure(&c->totals, iso::kTotSlots * sizeof(unsigned long long)));
+  uint64_t cap = static_cast<uint64_t>(D.n_cells / 16);
+  if (cap < (1u << 20)) cap = 1u << 20;
+  if (cap < min_active) cap = min_active;
+  if (cap > static_cast<uint64_t>(D.n_cells)) cap = static_cast<uint64_t>(D.n_cells);
+  if (cap < 1) cap = 1;
+  if (c->active_capacity < cap || !c->active.ptr) {
+    ISO_HIP(c, ensure(&c->active, cap * sizeof(uint32_t)));
+    c->active_capacity = c->active.bytes / sizeof(uint32_t);
+  }
+  B->signs = static_cast<uint64_t*>(c->signs.ptr) + 1;
+  B->seg = static_cast<uint2*>(c->seg.ptr);
+  B->tplanes = static_cast<uint64_t*>(c->tplanes.ptr);
+  B->flags = static_cast<uint32_t*>(c->flags.ptr);
+  B->agg = static_cast<iso::ScanTriple*>(c->agg.ptr);
+  B->incl = static_cast<iso::ScanTriple*>(c->incl.ptr);
+  B->active = static_cast<uint32_t*>(c->active.ptr);
+  B->active_capacity = c->active_capacity;
+  B->totals = static_cast<unsigned long long*>(c->totals.ptr);
+  return ISO_OK;
+}
+
+// signs (with state reset) + count/scan.
+int classify_and_count(iso_ctx* c, bool from_grid, const iso::Lattice& L, const iso::McDims& D,
+                       const iso::McBuffers& B) {
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  if (from_grid) {
+    ISO_HIP(c, timed(c, "mc_signs_grid",
+                     [&] { return iso::launch_signs_grid(c->grid, L, D, B, c->stream); }));
+  } else {
+    ISO_HIP(c, timed(c, "mc_signs_program", [&] {
+              return iso::launch_signs_program(ops, c->n_prog, L, D, B, c->stream);
+            }));
+  }
+  ISO_HIP(c, timed(c, "mc_count", [&] { return iso::launch_count(L, D, B, c->stream); }));
+  return ISO_OK;
+}
+
+int read_totals(iso_ctx* c, const iso::McBuffers& B) {
+  ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.totals, iso::kTotSlots * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (c->h_totals[iso::kTotErrors] & 2ull)
+    return fail(c, ISO_ERR_DEVICE, "count scan look-back did not complete");
+  return ISO_OK;
+}
+
+}  // namespace
+
 int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
   if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
   iso::Lattice L;
@@ -364,72 +427,54 @@ int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
 
   ISO_HIP(c, hipSetDevice(c->device));
   const iso::McDims D = iso::mc_dims(L);
-  ISO_HIP(c, ensure(&c->signs, (D.n_sign_words + 3) * sizeof(uint64_t)));
-  ISO_HIP(c, ensure(&c->seg, D.n_segs * sizeof(uint2)));
-  ISO_HIP(c, ensure(&c->blocks, 2 * D.n_scan_blocks * sizeof(uint64_t)));
-  ISO_HIP(c, ensure(&c->totals, 4 * sizeof(unsigned long long)));
-
+  if (D.n_segs >= (int64_t(1) << 26))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");
   iso::McBuffers B{};
-  uint64_t* sbase = static_cast<uint64_t*>(c->signs.ptr);
-  B.signs = sbase + 1;
-  B.seg = static_cast<uint2*>(c->seg.ptr);
-  B.blocks = static_cast<uint64_t*>(c->blocks.ptr);
-  B.totals = static_cast<unsigned long long*>(c->totals.ptr);
-
-  // Zero the guard words around the sign bits and the totals / error flags.
-  ISO_HIP(c, hipMemsetAsync(sbase, 0, sizeof(uint64_t), c->stream));
-  ISO_HIP(c, hipMemsetAsync(sbase + 1 + D.n_sign_words, 0, 2 * sizeof(uint64_t), c->stream));
-  ISO_HIP(c, hipMemsetAsync(B.totals, 0, 4 * sizeof(unsigned long long), c->stream));
-
-  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
-  if (from_grid) {
-    ISO_HIP(c, timed(c, "mc_signs_grid",
-                     [&] { return iso::launch_signs_grid(c->grid, L, D, B.signs, c->stream); }));
-  } else {
-    ISO_HIP(c, timed(c, "mc_signs_program", [&] {
-              return iso::launch_signs_program(ops, c->n_prog, L, D, B.signs, c->stream);
-            }));
-  }
-  ISO_HIP(c, timed(c, "mc_count", [&] { return iso::launch_count(L, D, B, c->stream); }));
-  ISO_HIP(c, timed(c, "mc_scan_blocks", [&] { return iso::launch_scan_blocks(D, B, c->stream); }));
-  ISO_HIP(c, timed(c, "mc_scan_top", [&] { return iso::launch_scan_top(D, B, c->stream); }));
-
+  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
   const float* grid = from_grid ? c->grid : nullptr;
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   uint64_t nv = 0, ni = 0;
+
   if (out->memory == ISO_MEMORY_DEVICE) {
     B.vertices = out->vertices;
     B.indices = out->indices;
     B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
     B.index_capacity = out->indices ? out->index_capacity : 0;
-    ISO_HIP(c, timed(c, "mc_emit", [&] {
-              return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
-            }));
-    ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.totals, 3 * sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, c->stream));
-    ISO_HIP(c, hipStreamSynchronize(c->stream));
-    collect_timing(c);
-    nv = c->h_totals[0];
-    ni = 3 * c->h_totals[1];
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if ((st = classify_and_count(c, from_grid, L, D, B)) != ISO_OK) return st;
+      ISO_HIP(c, timed(c, "mc_emit", [&] {
+                return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
+              }));
+      if ((st = read_totals(c, B)) != ISO_OK) return st;
+      const uint64_t n_act = c->h_totals[iso::kTotActive];
+      if (n_act <= B.active_capacity) break;
+      // The active-cell list was too short: grow it and extract again.
+      if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+    }
+    nv = c->h_totals[iso::kTotVertices];
+    ni = 3 * c->h_totals[iso::kTotTriangles];
     out->vertex_count = nv;
     out->index_count = ni;
-    if (c->h_totals[2] || nv > 0xffffffffull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+    if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
     if (nv > B.vertex_capacity || ni > B.index_capacity)
       return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
     return ISO_OK;
   }
 
   // Host memory: size the context's device buffers from the scan totals.
-  ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.totals, 2 * sizeof(unsigned long long),
-                            hipMemcpyDeviceToHost, c->stream));
-  ISO_HIP(c, hipStreamSynchronize(c->stream));
-  nv = c->h_totals[0];
-  ni = 3 * c->h_totals[1];
-  if (nv > 0xffffffffull) {
-    collect_timing(c);
-    out->vertex_count = nv;
-    out->index_count = ni;
-    return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+  if ((st = classify_and_count(c, from_grid, L, D, B)) != ISO_OK) return st;
+  if ((st = read_totals(c, B)) != ISO_OK) return st;
+  const uint64_t n_act = c->h_totals[iso::kTotActive];
+  if (n_act > B.active_capacity) {
+    if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+    if ((st = classify_and_count(c, from_grid, L, D, B)) != ISO_OK) return st;
+    if ((st = read_totals(c, B)) != ISO_OK) return st;
   }
+  nv = c->h_totals[iso::kTotVertices];
+  ni = 3 * c->h_totals[iso::kTotTriangles];
+  out->vertex_count = nv;
+  out->index_count = ni;
+  if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
   ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
   ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
   B.vertices = static_cast<float*>(c->vout.ptr);
@@ -439,8 +484,6 @@ int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
   ISO_HIP(c, timed(c, "mc_emit", [&] {
             return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
           }));
-  out->vertex_count = nv;
-  out->index_count = ni;
   if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
       (ni && (!out->indices || out->index_capacity < ni))) {
     ISO_HIP(c, hipStreamSynchronize(c->stream));

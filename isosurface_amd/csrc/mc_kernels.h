@@ -16,40 +16,58 @@ struct Lattice {
 
 // Sizes derived from a lattice.
 struct McDims {
-  int64_t n_points;     // nx*ny*nz
-  int64_t n_sign_words; // ceil(n_points/64)
-  int64_t n_cell_rows;  // (ny-1)*(nz-1)
-  int64_t segs_per_row; // ceil((nx-1)/64)
-  int64_t n_segs;       // n_cell_rows * segs_per_row
-  int64_t n_scan_blocks;
+  int64_t n_points;      // nx*ny*nz
+  int64_t n_sign_words;  // ceil(n_points/64)
+  int64_t n_cell_rows;   // (ny-1)*(nz-1)
+  int64_t segs_per_row;  // ceil((nx-1)/64)
+  int64_t n_segs;        // n_cell_rows * segs_per_row
+  int64_t n_blocks;      // count workgroups (kCountBlock segments each)
+  int64_t n_cells;       // (nx-1)*(ny-1)*(nz-1)
 };
 
-constexpr int kScanItems = 8;
-constexpr int kScanBlock = 256 * kScanItems;  // segments per scan block
+constexpr int kCountBlock = 256;  // segments per count workgroup
+
+// Running totals of the decoupled look-back scan.
+struct ScanTriple {
+  unsigned long long v, t, a;  // vertices, triangles, active cells
+};
+
+// Slots of McBuffers::totals.
+enum : int {
+  kTotVertices = 0,
+  kTotTriangles = 1,
+  kTotActive = 2,
+  kTotErrors = 3,  // bit 0: u32 id overflow, bit 1: look-back gave up
+  kTotTicket = 4,  // workgroup ticket of the count scan
+  kTotSlots = 8,
+};
 
 McDims mc_dims(const Lattice& L);
 
-// Buffers of one extraction.  `signs` has one zero word before element 0 and
-// two after the last.
+// Buffers of one extraction.
 struct McBuffers {
-  uint64_t* signs;          // n_sign_words (+ padding)
-  uint2* seg;               // n_segs: counts, then in-block exclusive prefix
-  uint64_t* blocks;         // 2 * n_scan_blocks: exclusive block prefix (verts, tris)
-  unsigned long long* totals;  // [0] verts, [1] tris, [2] error flags
+  uint64_t* signs;          // n_sign_words; signs[-1], signs[n], signs[n+1] are guard words
+  uint2* seg;               // n_segs: global first (vertex, triangle) id of each segment
+  uint64_t* tplanes;        // 3 * n_segs: per-cell triangle counts, bit-sliced
+  uint32_t* flags;          // n_blocks: look-back status (0, 1 aggregate, 2 inclusive)
+  ScanTriple* agg;          // n_blocks
+  ScanTriple* incl;         // n_blocks
+  uint32_t* active;         // active-cell list: (segment << 6) | bit, cell order
+  uint64_t active_capacity;
+  unsigned long long* totals;  // kTotSlots
   float* vertices;
   uint32_t* indices;
   uint64_t vertex_capacity;
   uint64_t index_capacity;
 };
 
-// Each returns hipSuccess or the launch error.
+// Each returns hipSuccess or the launch error.  The signs launchers also
+// reset the per-extraction state (guard words, totals, look-back flags).
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,
-                             uint64_t* signs, hipStream_t s);
+                             const McBuffers& B, hipStream_t s);
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
-                                uint64_t* signs, hipStream_t s);
+                                const McBuffers& B, hipStream_t s);
 hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
-hipError_t launch_scan_blocks(const McDims& D, const McBuffers& B, hipStream_t s);
-hipError_t launch_scan_top(const McDims& D, const McBuffers& B, hipStream_t s);
 hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lattice& L,
                        const McDims& D, const McBuffers& B, hipStream_t s);
 hipError_t launch_sample_lattice(const DevOp* ops, int n_ops, const Lattice& L, float* out,

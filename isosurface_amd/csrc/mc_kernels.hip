@@ -1,28 +1,31 @@
 // Marching-cubes kernels for gfx950 (MI355X).
 //
-// Pipeline (one extraction):
-//   1. signs   one bit per lattice sample, "inside" = !(v > 0) exactly as
-//              classify_corners (marching_cubes_impl.rs:26-37, distance.rs:52-54).
-//              From a dense FP32 grid this is a pure HBM stream; from a
-//              program it is the SDF evaluation.  Wave64 ballots pack 64
-//              consecutive samples (flat x-fastest order) into one word.
-//   2. count   one wavefront per 64-cell segment of a cell row: cube indices
-//              come from the sign words alone (no FP32 re-read); per segment
-//              it counts the vertices the segment's cells own and the
-//              triangles they emit.
-//   3. scan    exclusive scan of the segment counts in cell order.
-//   4. emit    recomputes the cube indices of active segments, writes vertex
-//              positions (FP32 crossing points, from the grid or the program)
-//              and the u32 index triples.
+// One extraction is three kernels:
+//   1. signs  one bit per lattice sample, "inside" = !(v > 0) exactly as
+//             classify_corners (marching_cubes_impl.rs:26-37, distance.rs:52-54).
+//             From a dense FP32 grid this is a pure HBM stream (16-byte
+//             loads, wave64 ballots); from a program it is the SDF
+//             evaluation.  The launch also resets the extraction's state.
+//   2. count  one thread per 64-cell segment of a cell row: the segment's
+//             eight corner bit-planes come from four row words, vertex
+//             ownership counts are popcounts of XORed planes, triangles are
+//             counted over the mixed cells only.  A single-pass decoupled
+//             look-back scan turns the counts into global first ids and
+//             writes the list of active (mixed) cells in cell order.
+//   3. emit   one thread per active cell: its 3x3x3 sign neighbourhood holds
+//             every cell that can own one of its edges; each needed owner's
+//             first vertex id is resolved once, the cell's own vertices are
+//             written (FP32 crossing points from the grid or the program)
+//             and its index triples are written.
 //
 // Vertex numbering is the reference's first-encounter order
 // (MeshTopologyBuilder::add_vertex, mesh.rs:240-251, driven by
-// PrimalGrid::traverse's z-major cell order, primal_grid.rs:59-85): a lattice
-// edge's vertex belongs to the lowest-numbered cell that contains the edge,
-// its position is computed with that cell's local edge direction, and
+// PrimalGrid::traverse's z-major cell order, primal_grid.rs:59-85): a
+// lattice edge's vertex belongs to the lowest-numbered cell containing the
+// edge, its position is computed along that cell's local edge direction, and
 // vertices are numbered by (owner cell, first appearance in the owner's
 // triangle list).  The owner of every edge follows from the edge's position
-// alone, so each wavefront computes vertex ids without any hash table.
+// alone, so no hash table is needed.
 #include <hip/hip_runtime.h>
 
 #include "mc_kernels.h"
@@ -34,7 +37,7 @@ namespace iso {
 namespace {
 
 // ---------------------------------------------------------------------------
-// Edge geometry derived from the corner / edge tables.
+// Tables
 // ---------------------------------------------------------------------------
 struct EdgeGeom {
   int8_t axis[12];
@@ -62,10 +65,9 @@ constexpr EdgeGeom make_geom() {
 }
 
 __constant__ tables::McTables c_tables = tables::kTables;
-__constant__ EdgeGeom c_geom = make_geom();
 
-// Lower endpoint offsets and axis of each edge, packed for register use:
-// bits 0-2 = min point (x|y<<1|z<<2), bits 3-4 = axis.
+// Per edge, packed for register use: bits 0-2 = lower endpoint (x|y<<1|z<<2),
+// bits 3-4 = axis.
 constexpr uint64_t pack_edge_geom() {
   EdgeGeom g = make_geom();
   uint64_t r = 0;
@@ -78,6 +80,28 @@ constexpr uint64_t pack_edge_geom() {
 }
 constexpr uint64_t kEdgeGeomPacked = pack_edge_geom();
 
+// Edge of a given axis whose lower endpoint sits at corner offset r (3 bits),
+// four bits per entry [axis * 8 + r] over two words.
+constexpr uint64_t pack_of_axis_min(int half) {
+  EdgeGeom g = make_geom();
+  uint64_t r = 0;
+  for (int i = 0; i < 16; ++i) {
+    const int k = 16 * half + i;
+    if (k >= 24) break;
+    const int e = g.of_axis_min[k / 8][k % 8];
+    r |= static_cast<uint64_t>(e < 0 ? 15 : e) << (4 * i);
+  }
+  return r;
+}
+constexpr uint64_t kOfAxisMin0 = pack_of_axis_min(0);
+constexpr uint64_t kOfAxisMin1 = pack_of_axis_min(1);
+
+__device__ __forceinline__ int edge_of_axis_min(int axis, int rel) {
+  const int k = 8 * axis + rel;
+  const uint64_t w = k < 16 ? kOfAxisMin0 : kOfAxisMin1;
+  return static_cast<int>((w >> (4 * (k & 15))) & 15u);
+}
+
 __device__ __forceinline__ void load_tables(tables::McTables* dst) {
   const uint32_t* src = reinterpret_cast<const uint32_t*>(&c_tables);
   uint32_t* d = reinterpret_cast<uint32_t*>(dst);
@@ -87,69 +111,43 @@ __device__ __forceinline__ void load_tables(tables::McTables* dst) {
 }
 
 // ---------------------------------------------------------------------------
-// Wavefront helpers.
+// Small helpers
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
-  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+__device__ __forceinline__ uint32_t popc64(uint64_t v) {
+  return static_cast<uint32_t>(__popcll(v));
 }
 
-// Exclusive wave prefix and wave total of v < 2^BITS (bit-sliced ballots).
-template <int BITS>
-__device__ __forceinline__ uint32_t wave_excl_small(uint32_t v, uint32_t* total) {
-  uint32_t pre = 0, tot = 0;
+__device__ __forceinline__ uint64_t low_mask(int b) { return (1ull << b) - 1ull; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = __lane_id();
 #pragma unroll
-  for (int b = 0; b < BITS; ++b) {
-    const uint64_t m = __ballot((v >> b) & 1u);
-    pre += mbcnt64(m) << b;
-    tot += static_cast<uint32_t>(__popcll(m)) << b;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
   }
-  *total = tot;
-  return pre;
+  return v;
 }
 
-__device__ __forceinline__ int64_t uniform64(int64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
-  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
-  return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+__device__ __forceinline__ float lattice_coord(const Lattice& L, int axis, int64_t i) {
+  return L.origin[axis] + static_cast<float>(i) * L.spacing[axis];
 }
 
-// 64 sign bits of a point row starting at flat sample index f:
-// cur bit l = sample f+l, hi bit l = sample f+l+1, prev bit l = sample f+l-1.
-struct Win {
-  uint64_t cur, hi, prev;
-};
-
-__device__ __forceinline__ Win load_win(const uint64_t* __restrict__ S, int64_t f) {
-  const int64_t w0 = f >> 6;
-  const int s = static_cast<int>(f & 63);
-  const uint64_t a = S[w0 - 1], b = S[w0], c = S[w0 + 1];
-  const uint64_t cur = s ? ((b >> s) | (c << (64 - s))) : b;
-  const uint64_t nextbit = (c >> s) & 1ull;
-  const uint64_t prevbit = s ? ((b >> (s - 1)) & 1ull) : (a >> 63);
-  Win r;
-  r.cur = cur;
-  r.hi = (cur >> 1) | (nextbit << 63);
-  r.prev = (cur << 1) | prevbit;
-  return r;
-}
-
-__device__ __forceinline__ uint32_t bit(uint64_t w, int l) {
-  return static_cast<uint32_t>((w >> l) & 1ull);
-}
-
-// Cube index from the x/x+1 sign bits of the four point rows around a cell
-// row: 00=(y,z) 10=(y+1,z) 01=(y,z+1) 11=(y+1,z+1).  Corner order is CORNERS
-// (marching_cubes_tables.rs:16-25).
-__device__ __forceinline__ uint32_t cube_index(uint64_t lo00, uint64_t hi00, uint64_t lo10,
-                                               uint64_t hi10, uint64_t lo01, uint64_t hi01,
-                                               uint64_t lo11, uint64_t hi11, int l) {
-  return bit(lo00, l) | (bit(hi00, l) << 1) | (bit(hi10, l) << 2) | (bit(lo10, l) << 3) |
-         (bit(lo01, l) << 4) | (bit(hi01, l) << 5) | (bit(hi11, l) << 6) | (bit(lo11, l) << 7);
+// Signed::find_crossing_point (distance.rs:64-70): t = d == 0 ? 0.5 : -a/d,
+// p = pa*(1-t) + pb*t per component, every operation rounded once.
+__device__ __forceinline__ void crossing(float a, float b, float pax, float pay, float paz,
+                                         float pbx, float pby, float pbz, float* o) {
+  const float delta = b - a;
+  const float t = (delta == 0.0f) ? 0.5f : -a / delta;
+  const float s = 1.0f - t;
+  o[0] = pax * s + pbx * t;
+  o[1] = pay * s + pby * t;
+  o[2] = paz * s + pbz * t;
 }
 
 // Local edges whose vertex the cell (x, y, z) owns, i.e. for which it is the
-// first cell in z-major order that contains the edge.
+// first cell in z-major order that contains the edge: e5, e6, e10 always,
+// and the low-face edges on the lattice's low faces.
 __device__ __forceinline__ uint32_t owned_edges(int64_t x, int64_t y, int64_t z) {
   uint32_t m = (1u << 5) | (1u << 6) | (1u << 10);
   if (y == 0) m |= (1u << 4) | (1u << 9);
@@ -161,29 +159,173 @@ __device__ __forceinline__ uint32_t owned_edges(int64_t x, int64_t y, int64_t z)
   return m;
 }
 
-__device__ __forceinline__ float lattice_coord(const Lattice& L, int axis, int64_t i) {
-  return L.origin[axis] + static_cast<float>(i) * L.spacing[axis];
-}
-
-// Signed::find_crossing_point (distance.rs:64-70).
-__device__ __forceinline__ void crossing(float a, float b, float pax, float pay, float paz,
-                                         float pbx, float pby, float pbz, float* o) {
-  const float delta = b - a;
-  const float t = (delta == 0.0f) ? 0.5f : -a / delta;
-  const float s = 1.0f - t;
-  o[0] = pax * s + pbx * t;
-  o[1] = pay * s + pby * t;
-  o[2] = paz * s + pbz * t;
+// Row-aligned 64 sign bits starting at flat sample f, and the low bit of the
+// next word (sample f + 64) in *next.
+__device__ __forceinline__ uint64_t row_word(const uint64_t* __restrict__ S, int64_t f,
+                                             uint64_t* next) {
+  const int64_t w0 = f >> 6;
+  const int s = static_cast<int>(f & 63);
+  const uint64_t a = S[w0], b = S[w0 + 1];
+  *next = (b >> s) & 1ull;
+  return s ? ((a >> s) | (b << (64 - s))) : a;
 }
 
 // ---------------------------------------------------------------------------
-// 1. signs
+// Segment words: the eight corner bit-planes of 64 consecutive cells of a
+// cell row (bit b of c[i] = "inside" for corner i of cell x0 + b), so every
+// per-cell quantity below is computed for 64 cells per 64-bit operation.
 // ---------------------------------------------------------------------------
-constexpr int kSignUnroll = 8;  // words per wavefront iteration
+struct SegWords {
+  uint64_t c[8];
+};
+
+// From the row words of the four point rows around a cell row
+// (00=(y,z) 10=(y+1,z) 01=(y,z+1) 11=(y+1,z+1)) and their next bits.
+__device__ __forceinline__ SegWords seg_words(uint64_t w00, uint64_t n00, uint64_t w10,
+                                              uint64_t n10, uint64_t w01, uint64_t n01,
+                                              uint64_t w11, uint64_t n11) {
+  SegWords s;
+  s.c[0] = w00;
+  s.c[1] = (w00 >> 1) | (n00 << 63);
+  s.c[2] = (w10 >> 1) | (n10 << 63);
+  s.c[3] = w10;
+  s.c[4] = w01;
+  s.c[5] = (w01 >> 1) | (n01 << 63);
+  s.c[6] = (w11 >> 1) | (n11 << 63);
+  s.c[7] = w11;
+  return s;
+}
+
+__device__ __forceinline__ uint32_t case_of(const SegWords& s, int b) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) c |= static_cast<uint32_t>((s.c[i] >> b) & 1ull) << i;
+  return c;
+}
+
+// Cells with a surface crossing (cube index not 0 or 255).
+__device__ __forceinline__ uint64_t mixed_cells(const SegWords& s) {
+  uint64_t any = 0, all = ~0ull;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    any |= s.c[i];
+    all &= s.c[i];
+  }
+  return any & ~all;
+}
+
+// Valid cells (x < nx - 1) of segment w.
+__device__ __forceinline__ uint64_t seg_valid(int64_t nx, int64_t w) {
+  const int64_t n = nx - 1 - 64 * w;
+  return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+}
+
+// Vertices owned by the cells selected by `m` (a subset of one segment): a
+// crossed edge is the XOR of its endpoints' planes.  `first` marks the
+// segment holding cell x = 0.
+__device__ __forceinline__ uint32_t owned_count(const SegWords& s, uint64_t m, bool y0, bool z0,
+                                                bool first) {
+  const uint64_t* c = s.c;
+  uint32_t n = popc64(m & (c[6] ^ c[7])) + popc64(m & (c[5] ^ c[6])) + popc64(m & (c[2] ^ c[6]));
+  if (y0) n += popc64(m & (c[4] ^ c[5])) + popc64(m & (c[1] ^ c[5]));
+  if (z0) n += popc64(m & (c[2] ^ c[3])) + popc64(m & (c[1] ^ c[2]));
+  if (y0 && z0) n += popc64(m & (c[0] ^ c[1]));
+  if (first && (m & 1ull)) {
+    n += static_cast<uint32_t>(((c[4] ^ c[7]) & 1ull) + ((c[3] ^ c[7]) & 1ull));
+    if (z0) n += static_cast<uint32_t>((c[0] ^ c[3]) & 1ull);
+    if (y0) n += static_cast<uint32_t>((c[0] ^ c[4]) & 1ull);
+  }
+  return n;
+}
+
+// ---------------------------------------------------------------------------
+// 1. signs (+ reset of the extraction's scratch state)
+// ---------------------------------------------------------------------------
+struct ResetArgs {
+  uint64_t* signs;  // guard words at [-1], [n_words], [n_words + 1]
+  int64_t n_words;
+  uint32_t* flags;
+  int64_t n_blocks;
+  unsigned long long* totals;
+};
+
+__device__ __forceinline__ void reset_state(const ResetArgs& r) {
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = i0; i < r.n_blocks; i += stride) r.flags[i] = 0u;
+  if (i0 < kTotSlots) r.totals[i0] = 0ull;
+  if (i0 == 0) {
+    r.signs[-1] = 0ull;
+    r.signs[r.n_words] = 0ull;
+    r.signs[r.n_words + 1] = 0ull;
+  }
+}
+
+constexpr int kSignVec = 4;  // float4 loads per lane per iteration (16 words)
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Moves bit i of a 16-bit value to bit 4i.
+__device__ __forceinline__ uint64_t spread4(uint64_t x) {
+  x &= 0xFFFFull;
+  x = (x | (x << 24)) & 0x000000FF000000FFull;
+  x = (x | (x << 12)) & 0x000F000F000F000Full;
+  x = (x | (x << 6)) & 0x0303030303030303ull;
+  x = (x | (x << 3)) & 0x1111111111111111ull;
+  return x;
+}
+
+// 16-byte loads: lane l of a wavefront reads samples 4l..4l+3 of a
+// 256-sample chunk, so ballot j holds component j of every lane and word k
+// of the chunk interleaves the 16-lane slices [16k, 16k+16) of the ballots.
+__global__ __launch_bounds__(256) void k_signs_grid4(const f32x4* __restrict__ g4,
+                                                     int64_t n_points, int64_t n_words,
+                                                     uint64_t* __restrict__ S, ResetArgs R) {
+  reset_state(R);
+  const int lane = __lane_id();
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
+  const float* g = reinterpret_cast<const float*>(g4);
+  constexpr int64_t kSpan = 256 * kSignVec;  // samples per wavefront iteration
+  for (int64_t base = wave * kSpan; base < n_points; base += n_waves * kSpan) {
+    f32x4 v[kSignVec];
+#pragma unroll
+    for (int j = 0; j < kSignVec; ++j) {
+      const int64_t f = base + 256 * j + 4 * lane;
+      if (f + 3 < n_points) {
+        v[j] = __builtin_nontemporal_load(g4 + (f >> 2));
+      } else {
+        v[j].x = f + 0 < n_points ? g[f + 0] : 1.0f;
+        v[j].y = f + 1 < n_points ? g[f + 1] : 1.0f;
+        v[j].z = f + 2 < n_points ? g[f + 2] : 1.0f;
+        v[j].w = 1.0f;
+      }
+    }
+    uint64_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kSignVec; ++j) {
+      const uint64_t b0 = __ballot(!(v[j].x > 0.0f));
+      const uint64_t b1 = __ballot(!(v[j].y > 0.0f));
+      const uint64_t b2 = __ballot(!(v[j].z > 0.0f));
+      const uint64_t b3 = __ballot(!(v[j].w > 0.0f));
+      if ((lane >> 2) == j) {
+        const int sh = 16 * (lane & 3);
+        mine = spread4(b0 >> sh) | (spread4(b1 >> sh) << 1) | (spread4(b2 >> sh) << 2) |
+               (spread4(b3 >> sh) << 3);
+      }
+    }
+    const int64_t w = (base >> 6) + lane;
+    if (lane < 4 * kSignVec && w < n_words) S[w] = mine;
+  }
+}
+
+// Grids that are not 16-byte aligned: one float per lane per ballot.
+constexpr int kSignUnroll = 8;
 
 __global__ __launch_bounds__(256) void k_signs_grid(const float* __restrict__ g,
                                                     int64_t n_points, int64_t n_words,
-                                                    uint64_t* __restrict__ S) {
+                                                    uint64_t* __restrict__ S, ResetArgs R) {
+  reset_state(R);
   const int lane = __lane_id();
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
@@ -192,7 +334,7 @@ __global__ __launch_bounds__(256) void k_signs_grid(const float* __restrict__ g,
 #pragma unroll
     for (int j = 0; j < kSignUnroll; ++j) {
       const int64_t f = (w0 + j) * 64 + lane;
-      v[j] = f < n_points ? __builtin_nontemporal_load(g + f) : 1.0f;
+      v[j] = f < n_points ? g[f] : 1.0f;
     }
     uint64_t mine = 0;
 #pragma unroll
@@ -207,7 +349,8 @@ __global__ __launch_bounds__(256) void k_signs_grid(const float* __restrict__ g,
 __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__ ops, int n_ops,
                                                        Lattice L, int64_t n_points,
                                                        int64_t n_words,
-                                                       uint64_t* __restrict__ S) {
+                                                       uint64_t* __restrict__ S, ResetArgs R) {
+  reset_state(R);
   const int lane = __lane_id();
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
@@ -232,170 +375,158 @@ __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// 2. count
+// 2. count + single-pass scan (decoupled look-back)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_count(const uint64_t* __restrict__ S, Lattice L,
-                                               McDims D, uint2* __restrict__ seg) {
-  __shared__ uint16_t s_mask[256];
+// A predecessor publishes its aggregate right after its own counting, so the
+// wait is short; the bound only guarantees that a broken invariant ends the
+// kernel (with an error flag) instead of spinning forever.
+constexpr uint32_t kLookBackSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kCountBlock) void k_count(const uint64_t* __restrict__ S, Lattice L,
+                                                       McDims D, McBuffers B) {
   __shared__ uint8_t s_ntri[256];
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) {
-    s_mask[i] = c_tables.edge_mask[i];
-    s_ntri[i] = c_tables.tri_count[i];
-  }
+  __shared__ uint32_t s_wv[4], s_wt[4], s_wa[4];
+  __shared__ uint32_t s_block;
+  __shared__ ScanTriple s_prefix;
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_ntri[i] = c_tables.tri_count[i];
+  if (threadIdx.x == 0)
+    s_block = atomicAdd(reinterpret_cast<unsigned int*>(&B.totals[kTotTicket]), 1u);
   __syncthreads();
-  const int lane = __lane_id();
-  const int64_t nx = L.nx, ny = L.ny;
-  const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
-  const uint32_t cyn = static_cast<uint32_t>(ny - 1);
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
-  for (int64_t g = wave; g < D.n_segs; g += n_waves) {
-    const uint32_t gg = static_cast<uint32_t>(uniform64(g));
+  // Workgroups scan in ticket order, so every predecessor is already running.
+  const uint32_t blk = s_block;
+  const int64_t g = static_cast<int64_t>(blk) * kCountBlock + threadIdx.x;
+
+  uint32_t nv = 0, nt = 0, na = 0;
+  uint64_t act = 0;
+  if (g < D.n_segs) {
+    const int64_t nx = L.nx, ny = L.ny;
+    const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
+    const uint32_t cyn = static_cast<uint32_t>(ny - 1);
+    const uint32_t gg = static_cast<uint32_t>(g);
     const uint32_t row = gg / spr;
     const uint32_t w = gg - row * spr;
     const uint32_t cz = row / cyn;
     const uint32_t cy = row - cz * cyn;
-    const int64_t x0 = static_cast<int64_t>(w) * 64;
-    const int64_t f00 = (static_cast<int64_t>(cz) * ny + cy) * nx + x0;
-    const Win r00 = load_win(S, f00);
-    const Win r10 = load_win(S, f00 + nx);
-    const Win r01 = load_win(S, f00 + nx * ny);
-    const Win r11 = load_win(S, f00 + nx * ny + nx);
-    const int64_t cx = x0 + lane;
-    const bool valid = cx < nx - 1;
-    const uint32_t c = cube_index(r00.cur, r00.hi, r10.cur, r10.hi, r01.cur, r01.hi, r11.cur,
-                                  r11.hi, lane);
-    const uint32_t own = owned_edges(cx, cy, cz) & s_mask[c];
-    const uint32_t nv = valid ? static_cast<uint32_t>(__popc(own)) : 0u;
-    const uint32_t nt = valid ? s_ntri[c] : 0u;
-    uint32_t tv, tt;
-    wave_excl_small<4>(nv, &tv);
-    wave_excl_small<3>(nt, &tt);
-    if (lane == 0) seg[gg] = make_uint2(tv, tt);
+    const int64_t f00 = (static_cast<int64_t>(cz) * ny + cy) * nx + 64 * static_cast<int64_t>(w);
+    uint64_t n00, n10, n01, n11;
+    const uint64_t w00 = row_word(S, f00, &n00);
+    const uint64_t w10 = row_word(S, f00 + nx, &n10);
+    const uint64_t w01 = row_word(S, f00 + nx * ny, &n01);
+    const uint64_t w11 = row_word(S, f00 + nx * ny + nx, &n11);
+    const SegWords sw = seg_words(w00, n00, w10, n10, w01, n01, w11, n11);
+    const uint64_t valid = seg_valid(nx, w);
+    nv = owned_count(sw, valid, cy == 0, cz == 0, w == 0);
+    act = mixed_cells(sw) & valid;
+    na = popc64(act);
+    uint64_t p0 = 0, p1 = 0, p2 = 0;  // per-cell triangle counts, bit-sliced
+    for (uint64_t m = act; m; m &= m - 1) {
+      const int b = __builtin_ctzll(m);
+      const uint32_t c = s_ntri[case_of(sw, b)];
+      nt += c;
+      p0 |= static_cast<uint64_t>(c & 1u) << b;
+      p1 |= static_cast<uint64_t>((c >> 1) & 1u) << b;
+      p2 |= static_cast<uint64_t>((c >> 2) & 1u) << b;
+    }
+    B.tplanes[3 * g + 0] = p0;
+    B.tplanes[3 * g + 1] = p1;
+    B.tplanes[3 * g + 2] = p2;
   }
-}
 
-// ---------------------------------------------------------------------------
-// 3. scan (two levels: blocks of kScanBlock segments, then block totals)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = __lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
-__device__ __forceinline__ uint64_t wave_incl_scan64(uint64_t v) {
-  const int lane = __lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint64_t o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
-__global__ __launch_bounds__(256) void k_scan_blocks(uint2* __restrict__ seg, int64_t n_segs,
-                                                     uint64_t* __restrict__ blocks,
-                                                     int64_t n_blocks) {
-  __shared__ uint32_t s_v[4], s_t[4];
-  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock + threadIdx.x * kScanItems;
-  uint2 it[kScanItems];
-  uint32_t sv = 0, st = 0;
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const int64_t k = base + i;
-    it[i] = k < n_segs ? seg[k] : make_uint2(0u, 0u);
-    sv += it[i].x;
-    st += it[i].y;
-  }
-  const uint32_t iv = wave_incl_scan(sv), ivt = wave_incl_scan(st);
-  const int wv = threadIdx.x >> 6, lane = __lane_id();
-  if (lane == 63) {
-    s_v[wv] = iv;
-    s_t[wv] = ivt;
+  // Workgroup scan of (vertices, triangles, active cells).
+  const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt), ia = wave_incl_scan(na);
+  const int wv = threadIdx.x >> 6;
+  if (__lane_id() == 63) {
+    s_wv[wv] = iv;
+    s_wt[wv] = it;
+    s_wa[wv] = ia;
   }
   __syncthreads();
-  uint32_t off_v = 0, off_t = 0;
+  uint32_t ov = iv - nv, ot = it - nt, oa = ia - na;
   for (int i = 0; i < wv; ++i) {
-    off_v += s_v[i];
-    off_t += s_t[i];
+    ov += s_wv[i];
+    ot += s_wt[i];
+    oa += s_wa[i];
   }
-  uint32_t pv = off_v + iv - sv, pt = off_t + ivt - st;
-#pragma unroll
-  for (int i = 0; i < kScanItems; ++i) {
-    const int64_t k = base + i;
-    if (k < n_segs) seg[k] = make_uint2(pv, pt);
-    pv += it[i].x;
-    pt += it[i].y;
-  }
-  if (threadIdx.x == 255) {
-    blocks[2 * blockIdx.x] = pv;
-    blocks[2 * blockIdx.x + 1] = pt;
-  }
-}
 
-// One workgroup: exclusive scan of the block totals, grand totals.
-__global__ __launch_bounds__(1024) void k_scan_top(uint64_t* __restrict__ blocks,
-                                                   int64_t n_blocks,
-                                                   unsigned long long* __restrict__ totals) {
-  __shared__ uint64_t s_v[16], s_t[16];
-  const int64_t per = (n_blocks + 1023) / 1024;
-  const int64_t b0 = threadIdx.x * per;
-  uint64_t sv = 0, st = 0;
-  for (int64_t i = 0; i < per; ++i) {
-    const int64_t k = b0 + i;
-    if (k < n_blocks) {
-      sv += blocks[2 * k];
-      st += blocks[2 * k + 1];
+  if (threadIdx.x == 0) {
+    ScanTriple total;
+    total.v = static_cast<unsigned long long>(s_wv[0]) + s_wv[1] + s_wv[2] + s_wv[3];
+    total.t = static_cast<unsigned long long>(s_wt[0]) + s_wt[1] + s_wt[2] + s_wt[3];
+    total.a = static_cast<unsigned long long>(s_wa[0]) + s_wa[1] + s_wa[2] + s_wa[3];
+    ScanTriple prefix{0ull, 0ull, 0ull};
+    if (blk == 0) {
+      B.incl[0] = total;
+      store_release(&B.flags[0], 2u);
+    } else {
+      B.agg[blk] = total;
+      store_release(&B.flags[blk], 1u);
+      int64_t j = static_cast<int64_t>(blk) - 1;
+      uint32_t spins = 0;
+      while (j >= 0) {
+        const uint32_t f = load_acquire(&B.flags[j]);
+        if (f == 0u) {
+          if (++spins > kLookBackSpinLimit) {
+            atomicOr(&B.totals[kTotErrors], 2ull);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          continue;
+        }
+        const ScanTriple x = (f == 2u) ? B.incl[j] : B.agg[j];
+        prefix.v += x.v;
+        prefix.t += x.t;
+        prefix.a += x.a;
+        if (f == 2u) break;
+        --j;
+      }
+      B.incl[blk] = ScanTriple{prefix.v + total.v, prefix.t + total.t, prefix.a + total.a};
+      store_release(&B.flags[blk], 2u);
     }
-  }
-  const uint64_t iv = wave_incl_scan64(sv), it = wave_incl_scan64(st);
-  const int wv = threadIdx.x >> 6, lane = __lane_id();
-  if (lane == 63) {
-    s_v[wv] = iv;
-    s_t[wv] = it;
+    s_prefix = prefix;
+    if (static_cast<int64_t>(blk) == D.n_blocks - 1) {
+      B.totals[kTotVertices] = prefix.v + total.v;
+      B.totals[kTotTriangles] = prefix.t + total.t;
+      B.totals[kTotActive] = prefix.a + total.a;
+      if (prefix.v + total.v > 0xffffffffull || 3 * (prefix.t + total.t) > 0xffffffffull)
+        atomicOr(&B.totals[kTotErrors], 1ull);
+    }
   }
   __syncthreads();
-  uint64_t off_v = 0, off_t = 0;
-  for (int i = 0; i < wv; ++i) {
-    off_v += s_v[i];
-    off_t += s_t[i];
-  }
-  uint64_t pv = off_v + iv - sv, pt = off_t + it - st;
-  for (int64_t i = 0; i < per; ++i) {
-    const int64_t k = b0 + i;
-    if (k < n_blocks) {
-      const uint64_t v = blocks[2 * k], t = blocks[2 * k + 1];
-      blocks[2 * k] = pv;
-      blocks[2 * k + 1] = pt;
-      pv += v;
-      pt += t;
-    }
-  }
-  if (threadIdx.x == 1023) {
-    totals[0] = pv;
-    totals[1] = pt;
+
+  if (g < D.n_segs) {
+    const ScanTriple p = s_prefix;
+    B.seg[g] = make_uint2(static_cast<uint32_t>(p.v + ov), static_cast<uint32_t>(p.t + ot));
+    uint64_t k = p.a + oa;
+    for (uint64_t m = act; m; m &= m - 1, ++k)
+      if (k < B.active_capacity)
+        B.active[k] = (static_cast<uint32_t>(g) << 6) | static_cast<uint32_t>(__builtin_ctzll(m));
   }
 }
 
 // ---------------------------------------------------------------------------
-// 4. emit
+// 3. emit — one thread per active cell
 // ---------------------------------------------------------------------------
 struct EmitArgs {
   const uint64_t* S;
   const uint2* seg;
-  const uint64_t* blocks;
+  const uint64_t* tplanes;
+  const uint32_t* active;
+  uint64_t active_capacity;
   const float* grid;
   const DevOp* ops;
   int n_ops;
   float* vertices;
   uint32_t* indices;
   uint64_t vcap, icap;
-  unsigned long long* totals;
+  const unsigned long long* totals;
 };
 
 template <bool kFromGrid>
@@ -409,111 +540,133 @@ __device__ __forceinline__ float lattice_value(const EmitArgs& A, const Lattice&
   }
 }
 
-// Per-lane view of one owner cell row: cube index, owned-and-crossed edges
-// and first vertex id of the lane's cell (x) and its left neighbour (x-1).
-struct RowOwn {
-  uint32_t case_x, case_xm;
-  uint32_t own_x, own_xm;
-  uint64_t base_x, base_xm;
-};
-
-// Branch-free choice among the eight (z, y, x) owner candidates, indexed
-// [z][y][x] with 0 = one cell back, 1 = this cell's coordinate.
-template <typename T>
-__device__ __forceinline__ T pick(bool x, bool y, bool z, T zyx000, T zyx001, T zyx010, T zyx011,
-                                  T zyx100, T zyx101, T zyx110, T zyx111) {
-  const T a = y ? (x ? zyx011 : zyx010) : (x ? zyx001 : zyx000);
-  const T b = y ? (x ? zyx111 : zyx110) : (x ? zyx101 : zyx100);
-  return z ? b : a;
+// Cube index of the cell whose low corner sits at (dx, dy, dz) of the 3x3x3
+// sign neighbourhood `cube` (bit dx + 3 dy + 9 dz).
+__device__ __forceinline__ uint32_t case_in_cube(uint32_t cube, int dx, int dy, int dz) {
+  const uint32_t s = cube >> (dx + 3 * dy + 9 * dz);
+  return (s & 1u) | ((s >> 1) & 1u) << 1 | ((s >> 4) & 1u) << 2 | ((s >> 3) & 1u) << 3 |
+         ((s >> 9) & 1u) << 4 | ((s >> 10) & 1u) << 5 | ((s >> 13) & 1u) << 6 |
+         ((s >> 12) & 1u) << 7;
 }
 
+constexpr int kEmitThreads = 256;
+
 template <bool kFromGrid>
-__global__ __launch_bounds__(256) void k_emit(EmitArgs A, Lattice L, McDims D) {
+__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, McDims D) {
   __shared__ tables::McTables s_t;
+  // Per thread: first vertex id and (cube index | owned crossed edges << 8)
+  // of each of the eight cells that can own one of its edges, indexed
+  // rx | ry<<1 | rz<<2 for the cell at (x-rx, y-ry, z-rz).
+  __shared__ uint32_t s_obase[8][kEmitThreads];
+  __shared__ uint32_t s_oinfo[8][kEmitThreads];
   load_tables(&s_t);
   __syncthreads();
 
-  const int lane = __lane_id();
+  const int tid = threadIdx.x;
   const int64_t nx = L.nx, ny = L.ny;
+  const int64_t nxy = nx * ny;
   const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
   const uint32_t cyn = static_cast<uint32_t>(ny - 1);
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
+  uint64_t n_act = A.totals[kTotActive];
+  if (n_act > A.active_capacity) n_act = A.active_capacity;
 
-  for (int64_t g = wave; g < D.n_segs; g += n_waves) {
-    const uint32_t gg = static_cast<uint32_t>(uniform64(g));
-    const uint32_t row = gg / spr;
-    const uint32_t w = gg - row * spr;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kEmitThreads + tid; i < n_act;
+       i += static_cast<uint64_t>(gridDim.x) * kEmitThreads) {
+    const uint32_t rec = A.active[i];
+    const uint32_t g = rec >> 6;
+    const int b = static_cast<int>(rec & 63u);
+    const uint32_t row = g / spr;
+    const uint32_t w = g - row * spr;
     const int64_t cz = row / cyn;
     const int64_t cy = row - cz * cyn;
-    const int64_t x0 = static_cast<int64_t>(w) * 64;
-    const int64_t cx = x0 + lane;
-    const bool valid = cx < nx - 1;
+    const int64_t cx = 64 * static_cast<int64_t>(w) + b;
+    const bool prev_seg = (b == 0) && (w > 0);  // owners at x-1 live in segment w-1
 
-    // Point rows y in {cy-1, cy, cy+1} x z in {cz-1, cz, cz+1}; index [dz+1][dy+1].
-    Win P[3][3];
+    // Row words of point rows y-1..y+1, z-1..z+1 at segment w (and w-1 when
+    // the cell is its segment's first), indexed [dz][dy].
+    uint64_t rw[3][3], rn[3][3], pw[3][3], pn[3][3];
 #pragma unroll
-    for (int dz = -1; dz <= 1; ++dz) {
+    for (int dz = 0; dz < 3; ++dz) {
 #pragma unroll
-      for (int dy = -1; dy <= 1; ++dy) {
-        Win r{0ull, 0ull, 0ull};
-        if (cy + dy >= 0 && cz + dz >= 0) r = load_win(A.S, ((cz + dz) * ny + cy + dy) * nx + x0);
-        P[dz + 1][dy + 1] = r;
-      }
-    }
-
-    const uint32_t c0 = cube_index(P[1][1].cur, P[1][1].hi, P[1][2].cur, P[1][2].hi, P[2][1].cur,
-                                   P[2][1].hi, P[2][2].cur, P[2][2].hi, lane);
-    const uint32_t ntri = valid ? s_t.tri_count[c0] : 0u;
-    if (__ballot(ntri > 0) == 0ull) continue;
-
-    // Owner rows (oy, oz) = (cy + ry, cz + rz), ry, rz in {-1, 0}; index [rz+1][ry+1].
-    RowOwn R[2][2];
-#pragma unroll
-    for (int rz = -1; rz <= 0; ++rz) {
-#pragma unroll
-      for (int ry = -1; ry <= 0; ++ry) {
-        RowOwn o{};
-        const int64_t oy = cy + ry, oz = cz + rz;
-        if (oy >= 0 && oz >= 0) {
-          const Win& q00 = P[rz + 1][ry + 1];
-          const Win& q10 = P[rz + 1][ry + 2];
-          const Win& q01 = P[rz + 2][ry + 1];
-          const Win& q11 = P[rz + 2][ry + 2];
-          o.case_x = cube_index(q00.cur, q00.hi, q10.cur, q10.hi, q01.cur, q01.hi, q11.cur,
-                                q11.hi, lane);
-          o.case_xm = cube_index(q00.prev, q00.cur, q10.prev, q10.cur, q01.prev, q01.cur,
-                                 q11.prev, q11.cur, lane);
-          o.own_x = valid ? (owned_edges(cx, oy, oz) & s_t.edge_mask[o.case_x]) : 0u;
-          o.own_xm = (cx >= 1) ? (owned_edges(cx - 1, oy, oz) & s_t.edge_mask[o.case_xm]) : 0u;
-          uint32_t tot;
-          const uint32_t pre = wave_excl_small<4>(static_cast<uint32_t>(__popc(o.own_x)), &tot);
-          const uint64_t sidx = (static_cast<uint64_t>(oz) * cyn + oy) * spr + w;
-          const uint64_t seg_base =
-              A.seg[sidx].x + A.blocks[2 * (sidx / kScanBlock)];
-          o.base_x = seg_base + pre;
-          o.base_xm = o.base_x - static_cast<uint32_t>(__popc(o.own_xm));
+      for (int dy = 0; dy < 3; ++dy) {
+        rw[dz][dy] = rn[dz][dy] = pw[dz][dy] = pn[dz][dy] = 0ull;
+        const int64_t y = cy - 1 + dy, z = cz - 1 + dz;
+        if (y >= 0 && z >= 0) {
+          const int64_t f = z * nxy + y * nx + 64 * static_cast<int64_t>(w);
+          rw[dz][dy] = row_word(A.S, f, &rn[dz][dy]);
+          if (prev_seg) pw[dz][dy] = row_word(A.S, f - 64, &pn[dz][dy]);
         }
-        R[rz + 1][ry + 1] = o;
       }
     }
+    // 3x3x3 sign neighbourhood: every cell that can own one of this cell's
+    // edges lies inside it.
+    uint32_t cube = 0;
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz) {
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        const uint64_t v = rw[dz][dy];
+        const uint32_t lo = b ? static_cast<uint32_t>((v >> (b - 1)) & 1ull)
+                              : static_cast<uint32_t>(pw[dz][dy] >> 63);
+        const uint32_t mid = static_cast<uint32_t>((v >> b) & 1ull);
+        const uint32_t hi = b < 63 ? static_cast<uint32_t>((v >> (b + 1)) & 1ull)
+                                   : static_cast<uint32_t>(rn[dz][dy]);
+        cube |= (lo | (mid << 1) | (hi << 2)) << (3 * dy + 9 * dz);
+      }
+    }
+    const uint32_t c0 = case_in_cube(cube, 1, 1, 1);
+    const uint32_t emask = s_t.edge_mask[c0];
 
-    // Triangle numbering: cell order, then table order (mesh.rs:91-100).
-    uint32_t tri_tot;
-    const uint32_t tri_pre = wave_excl_small<3>(ntri, &tri_tot);
-    const uint64_t sidx0 = static_cast<uint64_t>(row) * spr + w;
-    const uint64_t tri_base = A.seg[sidx0].y + A.blocks[2 * (sidx0 / kScanBlock) + 1] + tri_pre;
+    // Which owner cells the cell's crossed edges need.
+    uint32_t needed = 0;
+    for (uint32_t m = emask; m; m &= m - 1) {
+      const int e = __builtin_ctz(m);
+      const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
+      const int axis = static_cast<int>(gm >> 3);
+      const int rx = (axis != 0 && (gm & 1u) == 0 && cx >= 1) ? 1 : 0;
+      const int ry = (axis != 1 && (gm & 2u) == 0 && cy >= 1) ? 1 : 0;
+      const int rz = (axis != 2 && (gm & 4u) == 0 && cz >= 1) ? 1 : 0;
+      needed |= 1u << (rx | (ry << 1) | (rz << 2));
+    }
 
-    if (ntri == 0) continue;  // inactive lane (its wave still helped with the prefixes)
+    // First vertex id, cube index and owned crossed edges of each needed owner.
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      if (!(needed & (1u << o))) continue;
+      const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
+      const int64_t ox = cx - rx, oy = cy - ry, oz = cz - rz;
+      const uint32_t oc = case_in_cube(cube, 1 - rx, 1 - ry, 1 - rz);
+      const uint32_t oown = owned_edges(ox, oy, oz) & s_t.edge_mask[oc];
+      // Owner's segment words: point rows oy..oy+1 (dy = 1-ry, 2-ry) and
+      // oz..oz+1 (dz = 1-rz, 2-rz), segment w, or w-1 for x-1 at bit 0.
+      const bool pseg = rx && b == 0;
+      const int y0i = 1 - ry, z0i = 1 - rz;
+      SegWords os;
+      if (pseg) {
+        os = seg_words(pw[z0i][y0i], pn[z0i][y0i], pw[z0i][y0i + 1], pn[z0i][y0i + 1],
+                       pw[z0i + 1][y0i], pn[z0i + 1][y0i], pw[z0i + 1][y0i + 1],
+                       pn[z0i + 1][y0i + 1]);
+      } else {
+        os = seg_words(rw[z0i][y0i], rn[z0i][y0i], rw[z0i][y0i + 1], rn[z0i][y0i + 1],
+                       rw[z0i + 1][y0i], rn[z0i + 1][y0i], rw[z0i + 1][y0i + 1],
+                       rn[z0i + 1][y0i + 1]);
+      }
+      const int64_t ow = pseg ? static_cast<int64_t>(w) - 1 : static_cast<int64_t>(w);
+      const int ob = pseg ? 63 : b - rx;
+      const uint64_t og = (static_cast<uint64_t>(oz) * cyn + oy) * spr + ow;
+      const uint32_t pre =
+          owned_count(os, seg_valid(nx, ow) & low_mask(ob), oy == 0, oz == 0, ow == 0);
+      s_obase[o][tid] = A.seg[og].x + pre;
+      s_oinfo[o][tid] = oc | (oown << 8);
+    }
 
-    // --- vertices owned by this cell (mesh.rs:240-251: extract_vertex on first use)
-    const RowOwn& me = R[1][1];
-    {
-      uint32_t todo = me.own_x;
-      while (todo) {
+    // Vertices this cell owns: positions along its own edge directions.
+    const uint32_t mine = owned_edges(cx, cy, cz) & emask;
+    if (mine) {
+      const uint32_t base = s_obase[0][tid];
+      for (uint32_t todo = mine; todo; todo &= todo - 1) {
         const int e = __builtin_ctz(todo);
-        todo &= todo - 1;
-        const uint64_t id = me.base_x + static_cast<uint32_t>(__popc(me.own_x & s_t.before[c0][e]));
+        const uint32_t id = base + static_cast<uint32_t>(__popc(mine & s_t.before[c0][e]));
         const int u = tables::kEdgeCorners[e][0], v = tables::kEdgeCorners[e][1];
         const int64_t ux = cx + tables::kCorner[u][0], uy = cy + tables::kCorner[u][1],
                       uz = cz + tables::kCorner[u][2];
@@ -525,50 +678,44 @@ __global__ __launch_bounds__(256) void k_emit(EmitArgs A, Lattice L, McDims D) {
         crossing(va, vb, lattice_coord(L, 0, ux), lattice_coord(L, 1, uy), lattice_coord(L, 2, uz),
                  lattice_coord(L, 0, vx), lattice_coord(L, 1, vy), lattice_coord(L, 2, vz), p);
         if (id < A.vcap) {
-          A.vertices[3 * id + 0] = p[0];
-          A.vertices[3 * id + 1] = p[1];
-          A.vertices[3 * id + 2] = p[2];
+          A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
+          A.vertices[3 * static_cast<uint64_t>(id) + 1] = p[1];
+          A.vertices[3 * static_cast<uint64_t>(id) + 2] = p[2];
         }
-        if (id > 0xffffffffull) atomicOr(&A.totals[2], 1ull);
       }
     }
 
-    // --- triangles (march_cube, marching_cubes_impl.rs:102-117)
-    for (uint32_t j = 0; j < ntri; ++j) {
-      const uint64_t tix = tri_base + j;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int e = s_t.tri[c0][3 * j + k];
-        const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
-        const int axis = static_cast<int>(gm >> 3);
-        const int px = gm & 1, py = (gm >> 1) & 1, pz = (gm >> 2) & 1;
-        // Owner of the lattice edge: step back by one cell along each
-        // non-axis direction where the edge lies on this cell's low face
-        // and a lower cell exists.
-        const int rx = (axis != 0 && px == 0 && cx >= 1) ? -1 : 0;
-        const int ry = (axis != 1 && py == 0 && cy >= 1) ? -1 : 0;
-        const int rz = (axis != 2 && pz == 0 && cz >= 1) ? -1 : 0;
-        const int relx = px - rx, rely = py - ry, relz = pz - rz;
-        const int eo = c_geom.of_axis_min[axis][relx | (rely << 1) | (relz << 2)];
-        // Select the owner row / x without dynamic register indexing.
-        const bool y0 = ry == 0, z0 = rz == 0, x0s = rx == 0;
-        const uint32_t oc = pick(x0s, y0, z0, R[0][0].case_xm, R[0][0].case_x, R[0][1].case_xm,
-                                 R[0][1].case_x, R[1][0].case_xm, R[1][0].case_x,
-                                 R[1][1].case_xm, R[1][1].case_x);
-        const uint32_t oown = pick(x0s, y0, z0, R[0][0].own_xm, R[0][0].own_x, R[0][1].own_xm,
-                                   R[0][1].own_x, R[1][0].own_xm, R[1][0].own_x,
-                                   R[1][1].own_xm, R[1][1].own_x);
-        const uint64_t obase = pick(x0s, y0, z0, R[0][0].base_xm, R[0][0].base_x,
-                                    R[0][1].base_xm, R[0][1].base_x, R[1][0].base_xm,
-                                    R[1][0].base_x, R[1][1].base_xm, R[1][1].base_x);
-        const uint64_t id = obase + static_cast<uint32_t>(__popc(oown & s_t.before[oc][eo]));
-        if (3 * tix + k < A.icap) A.indices[3 * tix + k] = static_cast<uint32_t>(id);
-      }
+    // Triangles: cell order, then table order (mesh.rs:91-100).
+    const uint64_t lm = low_mask(b);
+    const uint64_t* tp = A.tplanes + 3 * static_cast<uint64_t>(g);
+    const uint64_t tbase = A.seg[g].y + popc64(tp[0] & lm) + 2 * popc64(tp[1] & lm) +
+                           4 * popc64(tp[2] & lm);
+    const uint32_t nidx = 3u * s_t.tri_count[c0];
+    for (uint32_t j = 0; j < nidx; ++j) {
+      const int e = s_t.tri[c0][j];
+      const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
+      const int axis = static_cast<int>(gm >> 3);
+      const int px = gm & 1, py = (gm >> 1) & 1, pz = (gm >> 2) & 1;
+      // Owner: one cell back along each non-axis direction where the edge
+      // lies on this cell's low face and a lower cell exists.
+      const int rx = (axis != 0 && px == 0 && cx >= 1) ? 1 : 0;
+      const int ry = (axis != 1 && py == 0 && cy >= 1) ? 1 : 0;
+      const int rz = (axis != 2 && pz == 0 && cz >= 1) ? 1 : 0;
+      const int o = rx | (ry << 1) | (rz << 2);
+      const int eo = edge_of_axis_min(axis, (px | rx) | ((py | ry) << 1) | ((pz | rz) << 2));
+      const uint32_t info = s_oinfo[o][tid];
+      const uint32_t oc = info & 0xffu, oown = info >> 8;
+      const uint32_t id =
+          s_obase[o][tid] + static_cast<uint32_t>(__popc(oown & s_t.before[oc][eo]));
+      const uint64_t ix = 3 * tbase + j;
+      if (ix < A.icap) A.indices[ix] = id;
     }
   }
 }
 
-// Dense program samples over a lattice (iso_program_sample_lattice).
+// ---------------------------------------------------------------------------
+// Program sampling (iso_program_sample_lattice / iso_program_sample)
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_sample_lattice(const DevOp* __restrict__ ops, int n_ops,
                                                         Lattice L, int64_t n_points,
                                                         float* __restrict__ out) {
@@ -587,7 +734,6 @@ __global__ __launch_bounds__(256) void k_sample_lattice(const DevOp* __restrict_
   }
 }
 
-// Program samples at arbitrary points (iso_program_sample).
 __global__ __launch_bounds__(256) void k_sample_points(const DevOp* __restrict__ ops, int n_ops,
                                                        const float* __restrict__ pts, uint64_t n,
                                                        float* __restrict__ out) {
@@ -602,6 +748,23 @@ int grid_for(int64_t waves_needed) {
   return static_cast<int>(blocks);
 }
 
+int device_cus() {
+  int device = 0, cus = 256;
+  if (hipGetDevice(&device) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  return cus > 0 ? cus : 256;
+}
+
+ResetArgs reset_args(const McDims& D, const McBuffers& B) {
+  ResetArgs r;
+  r.signs = B.signs;
+  r.n_words = D.n_sign_words;
+  r.flags = B.flags;
+  r.n_blocks = D.n_blocks;
+  r.totals = B.totals;
+  return r;
+}
+
 }  // namespace
 
 McDims mc_dims(const Lattice& L) {
@@ -611,39 +774,46 @@ McDims mc_dims(const Lattice& L) {
   D.n_cell_rows = (L.ny - 1) * (L.nz - 1);
   D.segs_per_row = (L.nx - 1 + 63) / 64;
   D.n_segs = D.n_cell_rows * D.segs_per_row;
-  D.n_scan_blocks = (D.n_segs + kScanBlock - 1) / kScanBlock;
+  D.n_blocks = (D.n_segs + kCountBlock - 1) / kCountBlock;
+  D.n_cells = (L.nx - 1) * (L.ny - 1) * (L.nz - 1);
   return D;
 }
 
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,
-                             uint64_t* signs, hipStream_t s) {
-  const int blocks = grid_for((D.n_sign_words + kSignUnroll - 1) / kSignUnroll);
-  hipLaunchKernelGGL(k_signs_grid, dim3(blocks), dim3(256), 0, s, grid, D.n_points,
-                     D.n_sign_words, signs);
+                             const McBuffers& B, hipStream_t s) {
+  (void)L;
+  const ResetArgs R = reset_args(D, B);
+  // At least enough threads to clear the look-back flags in one sweep.
+  const int64_t min_blocks = (D.n_blocks + 255) / 256;
+  if ((reinterpret_cast<uintptr_t>(grid) & 15u) == 0) {
+    int64_t blocks = grid_for((D.n_points + 256 * kSignVec - 1) / (256 * kSignVec));
+    if (blocks < min_blocks) blocks = min_blocks;
+    hipLaunchKernelGGL(k_signs_grid4, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                       reinterpret_cast<const f32x4*>(grid), D.n_points, D.n_sign_words, B.signs,
+                       R);
+  } else {
+    int64_t blocks = grid_for((D.n_sign_words + kSignUnroll - 1) / kSignUnroll);
+    if (blocks < min_blocks) blocks = min_blocks;
+    hipLaunchKernelGGL(k_signs_grid, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, grid,
+                       D.n_points, D.n_sign_words, B.signs, R);
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
-                                uint64_t* signs, hipStream_t s) {
-  const int blocks = grid_for(D.n_sign_words);
-  hipLaunchKernelGGL(k_signs_program, dim3(blocks), dim3(256), 0, s, ops, n_ops, L, D.n_points,
-                     D.n_sign_words, signs);
+                                const McBuffers& B, hipStream_t s) {
+  int64_t blocks = grid_for(D.n_sign_words);
+  const int64_t min_blocks = (D.n_blocks + 255) / 256;
+  if (blocks < min_blocks) blocks = min_blocks;
+  hipLaunchKernelGGL(k_signs_program, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
+                     n_ops, L, D.n_points, D.n_sign_words, B.signs, reset_args(D, B));
   return hipGetLastError();
 }
 
 hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_count, dim3(grid_for(D.n_segs)), dim3(256), 0, s, B.signs, L, D, B.seg);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_blocks(const McDims& D, const McBuffers& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan_blocks, dim3(static_cast<unsigned>(D.n_scan_blocks)), dim3(256), 0, s,
-                     B.seg, D.n_segs, B.blocks, D.n_scan_blocks);
-  return hipGetLastError();
-}
-
-hipError_t launch_scan_top(const McDims& D, const McBuffers& B, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, B.blocks, D.n_scan_blocks, B.totals);
+  if (D.n_blocks > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_count, dim3(static_cast<unsigned>(D.n_blocks)), dim3(kCountBlock), 0, s,
+                     B.signs, L, D, B);
   return hipGetLastError();
 }
 
@@ -652,7 +822,9 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lat
   EmitArgs A;
   A.S = B.signs;
   A.seg = B.seg;
-  A.blocks = B.blocks;
+  A.tplanes = B.tplanes;
+  A.active = B.active;
+  A.active_capacity = B.active_capacity;
   A.grid = grid;
   A.ops = ops;
   A.n_ops = n_ops;
@@ -661,11 +833,17 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lat
   A.vcap = B.vertex_capacity;
   A.icap = B.index_capacity;
   A.totals = B.totals;
-  const int blocks = grid_for(D.n_segs);
+  // Grid-stride over the active list (its length is only known on the device).
+  int64_t blocks = (static_cast<int64_t>(B.active_capacity) + kEmitThreads - 1) / kEmitThreads;
+  const int64_t cap = 8LL * device_cus();
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
   if (grid) {
-    hipLaunchKernelGGL(k_emit<true>, dim3(blocks), dim3(256), 0, s, A, L, D);
+    hipLaunchKernelGGL(k_emit<true>, dim3(static_cast<unsigned>(blocks)), dim3(kEmitThreads), 0,
+                       s, A, L, D);
   } else {
-    hipLaunchKernelGGL(k_emit<false>, dim3(blocks), dim3(256), 0, s, A, L, D);
+    hipLaunchKernelGGL(k_emit<false>, dim3(static_cast<unsigned>(blocks)), dim3(kEmitThreads), 0,
+                       s, A, L, D);
   }
   return hipGetLastError();
 }

@@ -1,0 +1,68 @@
+// Drives the C++ mirror the way the crate's callers drive the Rust API:
+//   MarchingCubes::new(65).extract(&DemoSource(Sphere(0.35)), &mut IndexedVertices)
+// and, for a source the GPU cannot evaluate, the same sphere written as a
+// host lambda (sampled on the host, uploaded, extracted on the GPU).
+// Writes the meshes as raw little-endian arrays for the test suite:
+//   <out>.program.f32 / .program.u32 and <out>.host.f32 / .host.u32
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "isosurface.hpp"
+
+namespace {
+
+struct HostSphere {  // a user source: Sphere(0.35) behind DemoSource, on the host
+  isosurface::Signed sample_scalar(isosurface::Vec3 p) const {
+    const float x = p.x - 0.5f, y = p.y - 0.5f, z = p.z - 0.5f;
+    return {std::sqrt((x * x + y * y) + z * z) - 0.35f};
+  }
+};
+
+template <class T>
+void dump(const std::string& path, const std::vector<T>& v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(v.size() * sizeof(T)));
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const std::string out = argc > 1 ? argv[1] : "example_mc";
+  try {
+    isosurface::Context ctx(0);
+    isosurface::MarchingCubes mc(65, ctx);
+
+    std::vector<float> v;
+    std::vector<uint32_t> i;
+    isosurface::IndexedVertices sink(v, i);
+    mc.extract(isosurface::DemoSource(isosurface::Sphere{0.35f}), sink);
+    dump(out + ".program.f32", v);
+    dump(out + ".program.u32", i);
+    std::printf("program source: %zu vertices, %zu triangles\n", v.size() / 3, i.size() / 3);
+
+    std::vector<float> hv;
+    std::vector<uint32_t> hi;
+    isosurface::IndexedVertices hsink(hv, hi);
+    mc.extract(HostSphere{}, hsink);
+    dump(out + ".host.f32", hv);
+    dump(out + ".host.u32", hi);
+    std::printf("host source: %zu vertices, %zu triangles\n", hv.size() / 3, hi.size() / 3);
+
+    // Error channel: a malformed program surfaces as isosurface::Error.
+    try {
+      isosurface::Program bad{isosurface::detail::node(ISO_OP_UNION)};
+      ctx.check(iso_program_upload(ctx.get(), bad.data(), 1), "iso_program_upload");
+      std::printf("error path: missing\n");
+      return 1;
+    } catch (const isosurface::Error& e) {
+      std::printf("error path: status %d\n", e.status());
+    }
+  } catch (const isosurface::Error& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 2;
+  }
+  return 0;
+}

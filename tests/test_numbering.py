@@ -130,3 +130,66 @@ def test_owned_edges_partition_lattice_edges():
                         seen[key] = (x, y, z)
     n_edges = (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
     assert len(seen) == n_edges
+
+
+M64 = (1 << 64) - 1
+
+
+def _popc(v):
+    return bin(v).count("1")
+
+
+def owned_count_planes(c, m, y0, z0, first):
+    """The kernels' bit-sliced owned-vertex count over the cells in mask m of
+    one 64-cell segment (c = the eight corner bit-planes)."""
+    n = _popc(m & (c[6] ^ c[7])) + _popc(m & (c[5] ^ c[6])) + _popc(m & (c[2] ^ c[6]))
+    if y0:
+        n += _popc(m & (c[4] ^ c[5])) + _popc(m & (c[1] ^ c[5]))
+    if z0:
+        n += _popc(m & (c[2] ^ c[3])) + _popc(m & (c[1] ^ c[2]))
+    if y0 and z0:
+        n += _popc(m & (c[0] ^ c[1]))
+    if first and (m & 1):
+        n += ((c[4] ^ c[7]) & 1) + ((c[3] ^ c[7]) & 1)
+        if z0:
+            n += (c[0] ^ c[3]) & 1
+        if y0:
+            n += (c[0] ^ c[4]) & 1
+    return n
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_bit_sliced_owned_counts_match_per_cell_rule(seed):
+    rng = np.random.RandomState(500 + seed)
+    nx, ny, nz = int(rng.randint(2, 200)), int(rng.randint(2, 4)), int(rng.randint(2, 4))
+    s = ~(rng.randn(nz, ny, nx).astype(np.float32) > 0)
+
+    def rowbits(y, z, x0):
+        v = 0
+        for b in range(65):
+            if x0 + b < nx and s[z, y, x0 + b]:
+                v |= 1 << b
+        return v & M64, (v >> 64) & 1
+
+    def hi(a, n):
+        return ((a >> 1) | (n << 63)) & M64
+
+    for cz in range(nz - 1):
+        for cy in range(ny - 1):
+            for w in range((nx - 1 + 63) // 64):
+                x0 = 64 * w
+                w00, n00 = rowbits(cy, cz, x0)
+                w10, n10 = rowbits(cy + 1, cz, x0)
+                w01, n01 = rowbits(cy, cz + 1, x0)
+                w11, n11 = rowbits(cy + 1, cz + 1, x0)
+                c = [w00, hi(w00, n00), hi(w10, n10), w10, w01, hi(w01, n01), hi(w11, n11), w11]
+                left = nx - 1 - x0
+                valid = M64 if left >= 64 else (1 << left) - 1
+                for b in (0, 1, 13, 63, 64):
+                    lm = M64 if b == 64 else (1 << b) - 1
+                    want = 0
+                    for x in range(x0, min(x0 + b, nx - 1)):
+                        case = sum(1 << i for i, (dx, dy, dz) in enumerate(CORNER)
+                                   if s[cz + dz, cy + dy, x + dx])
+                        want += _popc(owned(x, cy, cz) & MASK[case])
+                    assert owned_count_planes(c, valid & lm, cy == 0, cz == 0, w == 0) == want
