@@ -5,341 +5,46 @@
 // bit-identical.  Compiled with -O2 -ffp-contract=off: no FMA contraction,
 // SSE scalar f32 arithmetic, correctly rounded sqrt and division — the same
 // IEEE-754 semantics rustc gives f32 on x86-64.
-#include "oracle.h"
-
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
-#include <unordered_map>
-#include <unordered_set>
 #include <vector>
 
-#include "../isosurface_amd/csrc/mc_tables.h"
+#include "oracle.h"
+#include "oracle_internal.h"
 
-namespace {
-
+using orc::F3;
 using iso::tables::kCorner;
-using iso::tables::kEdgeCorners;
 using iso::tables::kTables;
-
-// ---------------------------------------------------------------------------
-// Vec3 (reference src/math/vector.rs)
-// ---------------------------------------------------------------------------
-template <typename T>
-struct V3 {
-  T x, y, z;
-};
-
-template <typename T>
-V3<T> sub(V3<T> a, V3<T> b) {
-  return {a.x - b.x, a.y - b.y, a.z - b.z};
-}
-// len_sq is fold!(+, x*x, y*y, z*z) = (x*x + y*y) + z*z  (vector.rs:45-52,125-127)
-template <typename T>
-T len_sq(V3<T> v) {
-  return (v.x * v.x + v.y * v.y) + v.z * v.z;
-}
-template <typename T>
-T len(V3<T> v) {
-  return std::sqrt(len_sq(v));  // vector.rs:130-132
-}
-// f32::min / f32::max: NaN-ignoring IEEE minNum/maxNum, as fmin/fmax.
-template <typename T>
-T fmn(T a, T b) {
-  return std::fmin(a, b);
-}
-template <typename T>
-T fmx(T a, T b) {
-  return std::fmax(a, b);
-}
-
-// ---------------------------------------------------------------------------
-// Sources: the program is a post-order list; rebuild the tree of the crate's
-// source types and evaluate it recursively, as the monomorphised
-// sample_scalar calls do.
-// ---------------------------------------------------------------------------
-struct TreeNode {
-  uint32_t op;
-  float p[6];
-  int a = -1;  // first operand / translated child
-  int b = -1;  // second operand
-};
-
-struct Tree {
-  std::vector<TreeNode> nodes;
-  int root = -1;
-};
-
-bool build_tree(const iso_node* prog, uint32_t n, Tree* t) {
-  std::vector<int> st;
-  std::vector<std::pair<size_t, int>> scopes;  // (value stack size, translate node)
-  t->nodes.clear();
-  for (uint32_t i = 0; i < n; ++i) {
-    const iso_node& nd = prog[i];
-    if (nd.reserved != 0) return false;
-    TreeNode tn;
-    tn.op = nd.op;
-    std::memcpy(tn.p, nd.p, sizeof(tn.p));
-    switch (nd.op) {
-      case ISO_OP_SPHERE:
-      case ISO_OP_TORUS:
-      case ISO_OP_BOX:
-      case ISO_OP_CYLINDER:
-        t->nodes.push_back(tn);
-        st.push_back(static_cast<int>(t->nodes.size()) - 1);
-        break;
-      case ISO_OP_UNION:
-      case ISO_OP_INTERSECTION:
-      case ISO_OP_DIFFERENCE:
-      case ISO_OP_SMOOTH_UNION: {
-        size_t floor_sz = scopes.empty() ? 0 : scopes.back().first;
-        if (st.size() < floor_sz + 2) return false;
-        tn.b = st.back();
-        st.pop_back();
-        tn.a = st.back();
-        st.pop_back();
-        t->nodes.push_back(tn);
-        st.push_back(static_cast<int>(t->nodes.size()) - 1);
-        break;
-      }
-      case ISO_OP_TRANSLATE:
-        t->nodes.push_back(tn);
-        scopes.push_back({st.size(), static_cast<int>(t->nodes.size()) - 1});
-        break;
-      case ISO_OP_END_TRANSLATE: {
-        if (scopes.empty()) return false;
-        auto sc = scopes.back();
-        scopes.pop_back();
-        if (st.size() != sc.first + 1) return false;
-        t->nodes[sc.second].a = st.back();
-        st.pop_back();
-        st.push_back(sc.second);
-        break;
-      }
-      default:
-        return false;
-    }
-  }
-  if (!scopes.empty() || st.size() != 1) return false;
-  t->root = st.back();
-  return true;
-}
-
-template <typename T>
-T eval(const Tree& t, int id, V3<T> p) {
-  const TreeNode& n = t.nodes[id];
-  const T p0 = static_cast<T>(n.p[0]), p1 = static_cast<T>(n.p[1]), p2 = static_cast<T>(n.p[2]);
-  switch (n.op) {
-    case ISO_OP_SPHERE:
-      // sphere.rs:35-39: p.len() - radius
-      return len(p) - p0;
-    case ISO_OP_TORUS: {
-      // torus.rs:40-46
-      T q_x = std::fabs(std::sqrt(p.x * p.x + p.y * p.y)) - p0;
-      T l = std::sqrt(q_x * q_x + p.z * p.z);
-      return l - p1;
-    }
-    case ISO_OP_BOX: {
-      // rectangular_prism.rs:36-41: q = |p| - h;
-      // q.max(0).len() + q.max_component().min(0)
-      V3<T> q{std::fabs(p.x) - p0, std::fabs(p.y) - p1, std::fabs(p.z) - p2};
-      V3<T> qm{fmx(q.x, T(0)), fmx(q.y, T(0)), fmx(q.z, T(0))};
-      T maxc = fmx(q.x, fmx(q.y, q.z));  // vector.rs max_component
-      return len(qm) + fmn(maxc, T(0));
-    }
-    case ISO_OP_CYLINDER: {
-      // cylinder.rs:41-48
-      T q_x = std::fabs(std::sqrt(p.x * p.x + p.y * p.y)) - p0;  // p.xy().len().abs() - r
-      T q_z = std::fabs(p.z) - p1;
-      V3<T> d{fmx(q_x, T(0)), fmx(q_z, T(0)), T(0)};
-      return fmn(fmx(q_x, q_z), T(0)) + len(d);
-    }
-    case ISO_OP_UNION:
-      // csg.rs:35-39: a.min(b)
-      return fmn(eval(t, n.a, p), eval(t, n.b, p));
-    case ISO_OP_INTERSECTION:
-      // csg.rs:68-72: a.max(b)
-      return fmx(eval(t, n.a, p), eval(t, n.b, p));
-    case ISO_OP_DIFFERENCE: {
-      // csg.rs:96-100: b.max(-a)
-      T b = eval(t, n.b, p);
-      T a = eval(t, n.a, p);
-      return fmx(b, -a);
-    }
-    case ISO_OP_SMOOTH_UNION: {
-      // Polynomial smooth minimum (no reference counterpart):
-      //   h = max(k - |a - b|, 0) / k;  min(a, b) - h*h*k*0.25
-      T a = eval(t, n.a, p);
-      T b = eval(t, n.b, p);
-      T h = fmx(p0 - std::fabs(a - b), T(0)) / p0;
-      return fmn(a, b) - ((h * h) * p0) * T(0.25);
-    }
-    case ISO_OP_TRANSLATE: {
-      // DemoSource: q = p - Vec3::from_scalar(0.5) (examples/common/sources.rs:38-43)
-      V3<T> q{p.x - p0, p.y - p1, p.z - p2};
-      return eval(t, n.a, q);
-    }
-  }
-  return T(0);
-}
-
-// ---------------------------------------------------------------------------
-// Lattice coordinates (primal_grid.rs:45-67: x as f32 * one_over_size)
-// ---------------------------------------------------------------------------
-inline float coord(const iso_grid_desc& d, int axis, int64_t i) {
-  return d.origin[axis] + static_cast<float>(i) * d.spacing[axis];
-}
-
-// ---------------------------------------------------------------------------
-// MC cell kernels (marching_cubes_impl.rs) and Signed (distance.rs)
-// ---------------------------------------------------------------------------
-inline int classify_corners(const float v[8]) {
-  // marching_cubes_impl.rs:26-37 with Signed::is_positive = v > 0.0 (distance.rs:52-54)
-  int cube = 0;
-  for (int i = 0; i < 8; ++i)
-    if (!(v[i] > 0.0f)) cube |= 1 << i;
-  return cube;
-}
-
-inline V3<float> find_crossing_point(float a, float b, V3<float> pa, V3<float> pb) {
-  // distance.rs:64-70: t = delta == 0 ? 0.5 : -a/delta; pa*(1-t) + pb*t
-  float delta = b - a;
-  float t = (delta == 0.0f) ? 0.5f : -a / delta;
-  float s = 1.0f - t;
-  return {pa.x * s + pb.x * t, pa.y * s + pb.y * t, pa.z * s + pb.z * t};
-}
-
-// ---------------------------------------------------------------------------
-// IndexCache + MeshTopology (index_cache.rs, mesh.rs)
-// ---------------------------------------------------------------------------
-struct Pt {
-  int64_t x, y, z;
-  bool operator<(const Pt& o) const {
-    if (x != o.x) return x < o.x;
-    if (y != o.y) return y < o.y;
-    return z < o.z;
-  }
-  bool operator==(const Pt& o) const { return x == o.x && y == o.y && z == o.z; }
-};
-
-struct GridKey {  // index_cache.rs:18, 48-61
-  Pt a, b;
-  bool operator==(const GridKey& o) const { return a == o.a && b == o.b; }
-};
-
-struct GridKeyHash {
-  size_t operator()(const GridKey& k) const {
-    uint64_t h = 1469598103934665603ull;
-    const int64_t v[6] = {k.a.x, k.a.y, k.a.z, k.b.x, k.b.y, k.b.z};
-    for (int i = 0; i < 6; ++i) {
-      h ^= static_cast<uint64_t>(v[i]) + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2);
-    }
-    return static_cast<size_t>(h);
-  }
-};
-
-struct EdgeHandle {  // mesh.rs Edge::new orders the two handles
-  uint32_t a, b;
-  bool operator==(const EdgeHandle& o) const { return a == o.a && b == o.b; }
-};
-struct EdgeHash {
-  size_t operator()(const EdgeHandle& e) const {
-    uint64_t k = (static_cast<uint64_t>(e.a) << 32) | e.b;
-    k ^= k >> 33;
-    k *= 0xff51afd7ed558ccdull;
-    k ^= k >> 33;
-    return static_cast<size_t>(k);
-  }
-};
-
-struct MeshBuilder {
-  // MeshTopologyBuilder (mesh.rs:226-260) + MeshTopology (mesh.rs:44-100)
-  std::unordered_map<GridKey, uint32_t, GridKeyHash> cache;
-  uint32_t next_vertex = 0;
-  std::vector<uint32_t> faces;
-  std::unordered_set<EdgeHandle, EdgeHash> edges;
-  std::unordered_map<EdgeHandle, std::vector<uint32_t>, EdgeHash> edge_to_face;
-  // IndexedVertices sink (extractor.rs:72-94)
-  std::vector<float> vertices;
-  std::vector<int64_t> keys;
-
-  uint32_t add_vertex(const GridKey& key, V3<float> v) {
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    uint32_t idx = next_vertex++;
-    cache.emplace(key, idx);
-    vertices.push_back(v.x);
-    vertices.push_back(v.y);
-    vertices.push_back(v.z);
-    int axis = key.a.x != key.b.x ? 0 : (key.a.y != key.b.y ? 1 : 2);
-    keys.push_back(key.a.x);
-    keys.push_back(key.a.y);
-    keys.push_back(key.a.z);
-    keys.push_back(axis);
-    return idx;
-  }
-
-  void add_edge(uint32_t u, uint32_t v, uint32_t face) {
-    EdgeHandle e = u > v ? EdgeHandle{v, u} : EdgeHandle{u, v};
-    edges.insert(e);
-    edge_to_face[e].push_back(face);
-  }
-
-  void add_face(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t face = static_cast<uint32_t>(faces.size() / 3);
-    faces.push_back(a);
-    faces.push_back(b);
-    faces.push_back(c);
-    add_edge(a, b, face);
-    add_edge(b, c, face);
-    add_edge(c, a, face);
-  }
-};
-
-GridKey grid_key(const Pt keys[8], int edge) {
-  const Pt& a = keys[kEdgeCorners[edge][0]];
-  const Pt& b = keys[kEdgeCorners[edge][1]];
-  if (b < a) return {b, a};
-  return {a, b};
-}
-
-template <typename T>
-T* dup(const std::vector<T>& v) {
-  T* p = static_cast<T*>(std::malloc(v.size() * sizeof(T) + 1));
-  if (!v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
-  return p;
-}
-
-}  // namespace
 
 extern "C" {
 
 int or_program_check(const iso_node* prog, uint32_t n) {
-  Tree t;
-  return build_tree(prog, n, &t) ? 0 : 1;
+  orc::Program t;
+  return orc::load_program(prog, n, &t) ? 0 : 1;
 }
 
 float or_sample(const iso_node* prog, uint32_t n, float x, float y, float z) {
-  Tree t;
-  if (!build_tree(prog, n, &t)) return NAN;
-  return eval<float>(t, t.root, V3<float>{x, y, z});
+  orc::Program t;
+  if (!orc::load_program(prog, n, &t)) return NAN;
+  return orc::sample(t, F3{x, y, z});
 }
 
 double or_sample_f64(const iso_node* prog, uint32_t n, double x, double y, double z) {
-  Tree t;
-  if (!build_tree(prog, n, &t)) return NAN;
-  return eval<double>(t, t.root, V3<double>{x, y, z});
+  orc::Program t;
+  if (!orc::load_program(prog, n, &t)) return NAN;
+  return orc::eval<double>(t, t.root, orc::V3<double>{x, y, z});
 }
 
 int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* desc, float* out) {
-  Tree t;
-  if (!build_tree(prog, n, &t)) return ISO_ERR_PROGRAM;
+  orc::Program t;
+  if (!orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
   const iso_grid_desc& d = *desc;
   for (int64_t z = 0; z < d.nz; ++z)
     for (int64_t y = 0; y < d.ny; ++y)
       for (int64_t x = 0; x < d.nx; ++x) {
-        V3<float> p{coord(d, 0, x), coord(d, 1, y), coord(d, 2, z)};
-        out[(z * d.ny + y) * d.nx + x] = eval<float>(t, t.root, p);
+        const F3 p{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
+        out[(z * d.ny + y) * d.nx + x] = orc::sample(t, p);
       }
   return ISO_OK;
 }
@@ -351,31 +56,31 @@ int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso
   std::memset(out, 0, sizeof(*out));
   const iso_grid_desc& d = *desc;
   if (d.nx < 2 || d.ny < 2 || d.nz < 2) return ISO_ERR_INVALID_ARGUMENT;
-  Tree t;
-  if (!grid && !build_tree(prog, n, &t)) return ISO_ERR_PROGRAM;
+  orc::Program t;
+  if (!grid && !orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
 
   const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
   struct Sample {
-    V3<float> p;
+    F3 p;
     float v;
   };
   std::vector<Sample> layers[2] = {std::vector<Sample>(nx * ny), std::vector<Sample>(nx * ny)};
   auto sample = [&](int64_t x, int64_t y, int64_t z) -> Sample {
-    V3<float> p{coord(d, 0, x), coord(d, 1, y), coord(d, 2, z)};
-    float v = grid ? grid[(z * ny + y) * nx + x] : eval<float>(t, t.root, p);
+    const F3 p{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
+    const float v = grid ? grid[(z * ny + y) * nx + x] : orc::sample(t, p);
     return {p, v};
   };
 
-  MeshBuilder mb;
-  std::vector<uint8_t> cases;
-  if (want_cases) cases.reserve((nx - 1) * (ny - 1) * (nz - 1));
+  orc::MeshBuilder mb;
+  mb.want_cases = want_cases != 0;
+  if (mb.want_cases) mb.cases.reserve((nx - 1) * (ny - 1) * (nz - 1));
 
   // Cache layer zero (primal_grid.rs:47-53)
   for (int64_t y = 0; y < ny; ++y)
     for (int64_t x = 0; x < nx; ++x) layers[0][y * nx + x] = sample(x, y, 0);
 
-  Pt keys[8];
-  V3<float> corners[8];
+  orc::Pt keys[8];
+  F3 corners[8];
   float values[8];
   for (int64_t z = 0; z + 1 < nz; ++z) {
     // Cache layer z+1 (primal_grid.rs:60-70)
@@ -390,57 +95,29 @@ int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso
           corners[i] = s.p;
           values[i] = s.v;
         }
-        // callback: marching_cubes.rs:66-79
-        int cube = classify_corners(values);
-        if (want_cases) cases.push_back(static_cast<uint8_t>(cube));
-        V3<float> verts[12];
-        uint16_t mask = kTables.edge_mask[cube];
-        for (int e = 0; e < 12; ++e) {
-          if (mask & (1u << e)) {
-            int u = kEdgeCorners[e][0], v = kEdgeCorners[e][1];
-            verts[e] = find_crossing_point(values[u], values[v], corners[u], corners[v]);
-          }
-        }
-        // march_cube (marching_cubes_impl.rs:102-117)
-        for (int i = 0; i < 5; ++i) {
-          const int8_t* tri = &kTables.tri[cube][3 * i];
-          if (tri[0] < 0) break;
-          uint32_t a = mb.add_vertex(grid_key(keys, tri[0]), verts[tri[0]]);
-          uint32_t b = mb.add_vertex(grid_key(keys, tri[1]), verts[tri[1]]);
-          uint32_t c = mb.add_vertex(grid_key(keys, tri[2]), verts[tri[2]]);
-          mb.add_face(a, b, c);
-        }
+        orc::march(mb, keys, corners, values);  // marching_cubes.rs:66-79
       }
     }
     std::swap(layers[0], layers[1]);
   }
-
-  // MeshTopology::extract_indices (mesh.rs:91-100)
-  out->vertices = dup(mb.vertices);
-  out->indices = dup(mb.faces);
-  out->edge_keys = dup(mb.keys);
-  out->n_vertices = mb.vertices.size() / 3;
-  out->n_indices = mb.faces.size();
-  if (want_cases) {
-    out->cases = dup(cases);
-    out->n_cases = cases.size();
-  }
+  orc::finish(mb, out);
   return ISO_OK;
 }
 
 int or_edge_points_f64(const iso_node* prog, uint32_t n, const iso_grid_desc* desc,
                        const int64_t* keys, uint64_t n_keys, double* out_xyz) {
-  Tree t;
-  if (!build_tree(prog, n, &t)) return ISO_ERR_PROGRAM;
+  orc::Program t;
+  if (!orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
   const iso_grid_desc& d = *desc;
   for (uint64_t i = 0; i < n_keys; ++i) {
-    int64_t ix = keys[4 * i], iy = keys[4 * i + 1], iz = keys[4 * i + 2], ax = keys[4 * i + 3];
-    V3<double> a{coord(d, 0, ix), coord(d, 1, iy), coord(d, 2, iz)};
-    V3<double> b{coord(d, 0, ix + (ax == 0)), coord(d, 1, iy + (ax == 1)),
-                 coord(d, 2, iz + (ax == 2))};
-    double va = eval<double>(t, t.root, a), vb = eval<double>(t, t.root, b);
-    double delta = vb - va;
-    double s = delta == 0.0 ? 0.5 : -va / delta;
+    const int64_t ix = keys[4 * i], iy = keys[4 * i + 1], iz = keys[4 * i + 2],
+                  ax = keys[4 * i + 3];
+    const orc::V3<double> a{orc::coord(d, 0, ix), orc::coord(d, 1, iy), orc::coord(d, 2, iz)};
+    const orc::V3<double> b{orc::coord(d, 0, ix + (ax == 0)), orc::coord(d, 1, iy + (ax == 1)),
+                            orc::coord(d, 2, iz + (ax == 2))};
+    const double va = orc::eval<double>(t, t.root, a), vb = orc::eval<double>(t, t.root, b);
+    const double delta = vb - va;
+    const double s = delta == 0.0 ? 0.5 : -va / delta;
     out_xyz[3 * i + 0] = a.x + (b.x - a.x) * s;
     out_xyz[3 * i + 1] = a.y + (b.y - a.y) * s;
     out_xyz[3 * i + 2] = a.z + (b.z - a.z) * s;

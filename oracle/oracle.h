@@ -53,6 +53,16 @@ int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid,
 int or_edge_points_f64(const iso_node* prog, uint32_t n, const iso_grid_desc* desc,
                        const int64_t* keys, uint64_t n_keys, double* out_xyz);
 
+/* DualContouring::extract with MinimiseQEF and CentralDifference normals
+ * (epsilon as in CentralDifference::new_with_epsilon) over a program lattice. */
+int or_extract_dc(const iso_node* prog, uint32_t n, const iso_grid_desc* desc, float epsilon,
+                  or_mesh* out);
+/* The reference SVD solve of an n x 3 system (row-major a, n <= 12). */
+int or_svd_solve(const double* a, const double* b, int rows, int drop_min, double* x_out,
+                 double* d_out);
+/* The dual vertex of a mixed cell from its corners and corner normals. */
+void or_place_feature(const float* corners, const float* normals, float* out);
+
 void or_mesh_free(or_mesh* m);
 
 /* Decoded tables, for checksum tests. */

@@ -187,3 +187,61 @@ def edge_mask_table() -> np.ndarray:
     out = np.empty(256, dtype=np.uint16)
     load().or_edge_mask_table(out.ctypes.data)
     return out
+
+
+def _dc_bind(lib):
+    if getattr(lib, "_dc_bound", False):
+        return
+    lib.or_extract_dc.restype = C.c_int
+    lib.or_extract_dc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.POINTER(IsoGridDesc),
+                                  C.c_float, C.POINTER(OrMesh)]
+    lib.or_svd_solve.restype = C.c_int
+    lib.or_svd_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p]
+    lib.or_place_feature.restype = None
+    lib.or_place_feature.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib._dc_bound = True
+
+
+def extract_dc(program, desc: IsoGridDesc, epsilon: float = 1e-6) -> OracleMesh:
+    """DualContouring::extract (MinimiseQEF, CentralDifference), restated."""
+    lib = load()
+    _dc_bind(lib)
+    arr = nodes(program)
+    m = OrMesh()
+    st = lib.or_extract_dc(arr, len(program), C.byref(desc), epsilon, C.byref(m))
+    assert st == 0, st
+    try:
+        nv, ni = int(m.n_vertices), int(m.n_indices)
+        v = np.ctypeslib.as_array(m.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else \
+            np.zeros((0, 3), np.float32)
+        i = np.ctypeslib.as_array(m.indices, shape=(ni,)).copy() if ni else np.zeros(0, np.uint32)
+        k = np.ctypeslib.as_array(m.edge_keys, shape=(nv * 4,)).reshape(nv, 4).copy() if nv else \
+            np.zeros((0, 4), np.int64)
+    finally:
+        lib.or_mesh_free(C.byref(m))
+    return OracleMesh(v, i, k, None)
+
+
+def svd_solve(a: np.ndarray, b: np.ndarray, drop_min: bool = False):
+    """SVD::new + (optional Edge rank drop) + SVD::solve on an n x 3 system."""
+    lib = load()
+    _dc_bind(lib)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.empty(3, np.float64)
+    d = np.empty(3, np.float64)
+    st = lib.or_svd_solve(a.ctypes.data, b.ctypes.data, len(a), 1 if drop_min else 0,
+                          x.ctypes.data, d.ctypes.data)
+    assert st == 0
+    return x, d
+
+
+def place_feature(corners: np.ndarray, normals: np.ndarray) -> np.ndarray:
+    lib = load()
+    _dc_bind(lib)
+    c = np.ascontiguousarray(corners, dtype=np.float32)
+    n = np.ascontiguousarray(normals, dtype=np.float32)
+    out = np.empty(3, np.float32)
+    lib.or_place_feature(c.ctypes.data, n.ctypes.data, out.ctypes.data)
+    return out

@@ -1,0 +1,195 @@
+"""Pins the dual-contouring oracle: the restated SVD (math/svd.rs) against
+numpy's SVD and least-squares solutions, the feature placement's closed
+forms, and DC mesh invariants."""
+import numpy as np
+import pytest
+
+from isosurface_amd import workloads as W
+from isosurface_amd.sources import DemoSource, RectangularPrism, Sphere, Translate
+from oracle import oracle as O
+
+
+def check_solve(a, b, x, d, drop_min):
+    """The reference's solve: w = U^T b; w_i /= s_i only where s_i was kept
+    (s_i >= 0.1 and not the dropped minimum); x = V w (svd.rs:55-84, 928-972).
+    A zeroed s_i leaves its component unscaled rather than removing it.  The
+    kept components are unique; a zeroed one only when its singular value is
+    isolated (otherwise its left singular vector is not determined)."""
+    u, s, vt = np.linalg.svd(a, full_matrices=False)
+    # chop() stops the QR sweep once off-diagonals fall below 1e-5 of their
+    # neighbours (svd.rs:481-493): singular values agree to ~1e-10 relative,
+    # singular vectors only to ~1e-5 * s / gap.
+    assert np.all(d[:-1] >= d[1:]) and np.all(d >= 0)
+    assert np.allclose(d, s, rtol=1e-8, atol=1e-10)
+    keep = s >= 0.1
+    if drop_min:
+        keep[np.argmin(s)] = False
+    scale = 1.0 + np.abs(b).sum() / max(s.min(initial=1.0), 0.1)
+    for i in range(3):
+        proj = vt[i] @ x
+        gap = min(abs(s[i] - s[j]) for j in range(3) if j != i)
+        if keep[i]:
+            assert abs(proj - (u[:, i] @ b) / s[i]) <= 1e-4 * scale
+        elif s[i] > 1e-6 and gap > 1e-6:
+            assert abs(proj - u[:, i] @ b) <= 1e-4 * scale
+        else:
+            assert abs(proj) <= np.linalg.norm(b) * (1 + 1e-9)
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_svd_matches_numpy(seed):
+    rng = np.random.RandomState(seed)
+    rows = 8 if seed < 40 else rng.randint(3, 13)
+    a = rng.randn(rows, 3)
+    if seed % 5 == 1:  # rank 2: third column a combination of the first two
+        a[:, 2] = 0.3 * a[:, 0] - 1.7 * a[:, 1]
+    if seed % 5 == 2:  # rank 1: unit normals all alike (a planar cell)
+        n = rng.randn(3)
+        a = np.tile(n / np.linalg.norm(n), (rows, 1)) + 1e-9 * rng.randn(rows, 3)
+    if seed % 5 == 3:  # one small singular value between 0 and the 0.1 threshold
+        a[:, 2] = 0.3 * a[:, 0] - 1.7 * a[:, 1] + 0.01 * rng.randn(rows)
+    b = rng.randn(rows)
+    for drop in (False, True):
+        x, d = O.svd_solve(a, b, drop)
+        check_solve(a, b, x, d, drop)
+
+
+def test_svd_zero_matrix_and_identity():
+    # A = 0: the Householder steps are skipped, U = [I; 0], V = I, all s = 0,
+    # and the unscaled components return b's first three entries.
+    x, d = O.svd_solve(np.zeros((8, 3)), np.arange(1.0, 9.0))
+    assert np.all(d == 0) and np.array_equal(x, [1.0, 2.0, 3.0])
+    a = np.vstack([np.eye(3), np.eye(3)[:1]])
+    x, d = O.svd_solve(a, np.array([1.0, 2.0, 3.0, 1.0]))
+    assert np.allclose(x, [1.0, 2.0, 3.0])
+
+
+CUBE = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1],
+                 [0, 1, 1]], np.float32) * np.float32(0.125)
+
+
+def test_planar_cell_gets_centre_of_mass():
+    normals = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (8, 1))
+    p = O.place_feature(CUBE, normals)
+    assert np.array_equal(p, np.float32([0.0625, 0.0625, 0.0625]))
+
+
+def test_corner_feature_is_solved_exactly():
+    # Three axis planes through (0.05, 0.08, 0.03) seen at the eight corners:
+    # the QEF minimiser is the corner point.
+    c = np.float32([0.05, 0.08, 0.03])
+    normals = np.zeros((8, 3), np.float32)
+    for i in range(8):
+        normals[i, i % 3] = 1.0
+    p = O.place_feature(CUBE, normals)
+    # planes n.x = n.c: the solution is c; offsets are relative to the centre of mass
+    d = np.array([(CUBE[i] - CUBE.mean(0)) @ normals[i] for i in range(8)])
+    x, _ = O.svd_solve(normals.astype(np.float64), d)
+    assert np.allclose(p, CUBE.mean(0) + x, atol=1e-6)
+    assert np.all(np.isfinite(p))
+
+
+F = np.float32
+FEATURE_ANGLE = F(0.8660254037844387)
+
+
+def model_place_feature(corners, normals):
+    """TangentPlanes::new (feature/mod.rs:91-148) in f32 and the reference
+    solve (qef.rs:36-68) with numpy's SVD; None when the zeroed singular
+    value is not isolated (its vector is then not determined)."""
+    dot = lambda a, b: F(F(F(a[0] * b[0]) + F(a[1] * b[1])) + F(a[2] * b[2]))
+    min_angle, axis = F(3.4028235e38), np.zeros(3, F)
+    for i in range(8):
+        for j in range(8):
+            ang = dot(normals[i], normals[j])
+            if ang < min_angle:
+                a, b = normals[i], normals[j]
+                axis = np.array([F(a[1] * b[2]) - F(a[2] * b[1]), F(a[2] * b[0]) - F(a[0] * b[2]),
+                                 F(a[0] * b[1]) - F(a[1] * b[0])], F)
+                min_angle = ang
+    com = np.zeros(3, F)
+    for i in range(8):
+        com = (com + corners[i]).astype(F)
+    com = (com / F(8)).astype(F)
+    if min_angle > FEATURE_ANGLE:
+        return com, "planar"
+    l = np.sqrt(dot(axis, axis))
+    axis = (axis / l).astype(F) if l >= F(1.1920929e-07) else np.zeros(3, F)
+    cs = [dot(axis, n) for n in normals]
+    c = max(abs(min(cs + [F(1)])), abs(max(cs + [F(-1)])))
+    c = np.sqrt(F(1) - F(c * c))
+    edge = c > FEATURE_ANGLE
+    a = normals.astype(np.float64)
+    b = np.array([dot((corners[i] - com).astype(F), normals[i]) for i in range(8)], np.float64)
+    u, s, vt = np.linalg.svd(a, full_matrices=False)
+    keep = s >= 0.1
+    if edge:
+        keep[np.argmin(s)] = False
+    for i in range(3):
+        gap = min(abs(s[i] - s[j]) for j in range(3) if j != i)
+        if not keep[i] and (s[i] < 1e-4 or gap < 1e-4):
+            return None, "ambiguous"
+    w = u.T @ b
+    w = np.where(keep, w / np.where(keep, s, 1.0), w)
+    return (com + (vt.T @ w).astype(F)).astype(F), "edge" if edge else "corner"
+
+
+@pytest.mark.parametrize("seed", range(60))
+def test_place_feature_matches_model(seed):
+    rng = np.random.RandomState(seed)
+    h = F(rng.uniform(0.01, 0.2))
+    corners = (CUBE / F(0.125) * h + rng.uniform(0, 1, 3).astype(F)).astype(F)
+    kind = seed % 3
+    fam = [np.array([1, 0, 0.0]), np.array([0, 1, 0.0]), np.array([0, 0, 1.0])]
+    normals = np.zeros((8, 3), F)
+    for i in range(8):
+        n = fam[i % (kind + 1)] + rng.normal(0, 0.05, 3)
+        normals[i] = (n / np.linalg.norm(n)).astype(F)
+    want, how = model_place_feature(corners, normals)
+    got = O.place_feature(corners, normals)
+    if want is None:
+        assert np.all(np.isfinite(got))
+        return
+    assert np.allclose(got, want, atol=2e-5 * (1 + float(h))), how
+
+
+def _dual_counts(prog, size):
+    desc = O.reference_desc(size)
+    g = O.sample_lattice(prog, desc).astype(np.float32)
+    dual = (np.float32(0.5) * g[:-1, :-1, :-1] + np.float32(0.5) * g[1:, 1:, 1:]).astype(np.float32)
+    s = ~(dual > 0)
+    edges = int((s[:, :, 1:] != s[:, :, :-1]).sum() + (s[:, 1:, :] != s[:, :-1, :]).sum() +
+                (s[1:, :, :] != s[:-1, :, :]).sum())
+    corner = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    cs = s.astype(np.int64)
+    case = np.zeros(tuple(d - 1 for d in s.shape), np.int64)
+    for i, (x, y, z) in enumerate(corner):
+        case |= cs[z:z + case.shape[0], y:y + case.shape[1], x:x + case.shape[2]] << i
+    ntri = (O.tri_table() >= 0).sum(axis=1) // 3
+    return edges, int(ntri[case].sum())
+
+
+@pytest.mark.parametrize("src,size", [(DemoSource(Sphere(0.3)), 17),
+                                      (DemoSource(RectangularPrism((0.21, 0.17, 0.26))), 21),
+                                      (W.boxes_union(8, seed=3), 25)])
+def test_dc_topology_is_mc_on_the_dual_lattice(src, size):
+    prog = src.program()
+    m = O.extract_dc(prog, O.reference_desc(size))
+    v, t = _dual_counts(prog, size)
+    assert len(m.vertices) == v and len(m.indices) // 3 == t
+    tri = m.indices.reshape(-1, 3).astype(np.int64)
+    e = np.sort(np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]]), axis=1)
+    _, counts = np.unique(e[:, 0] * (1 << 32) + e[:, 1], return_counts=True)
+    assert np.all(counts == 2)  # closed surface
+
+
+def test_dc_vertices_hug_the_box_surface():
+    # Every DC vertex is interpolated between dual vertices of adjacent cells,
+    # so it lies within one cell diagonal of the surface.
+    h = np.float32([0.21, 0.17, 0.26])
+    size = 21
+    m = O.extract_dc(DemoSource(RectangularPrism(tuple(h))).program(), O.reference_desc(size))
+    q = np.abs(m.vertices.astype(np.float64) - 0.5) - h
+    sdf = np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(q.max(axis=1), 0)
+    cell = 1.0 / (size - 1)
+    assert np.all(np.abs(sdf) <= cell * np.sqrt(3) + 1e-6)
