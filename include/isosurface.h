@@ -168,6 +168,27 @@ typedef struct iso_mesh_out {
  * Vertex order, positions and index lists are the reference's. */
 int iso_extract_mc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
 
+/* Dual contouring with MinimiseQEF feature placement and CentralDifference
+ * normals of the current program (step `epsilon`, the reference's default is
+ * 1e-6): replaces DualContouring::new(size, MinimiseQEF {}).extract over a
+ * CentralDifference::new_with_epsilon source with an IndexedVertices
+ * extractor (dual_contouring.rs:65-105, source.rs:52-94).  `desc` is the
+ * primal lattice and must name ISO_SOURCE_PROGRAM; the mesh is the
+ * reference's: vertex order, positions (feature points from the reference's
+ * f64 SVD) and index lists.  ISO_ERR_DEVICE if a cell's SVD fails to
+ * converge within a fixed bound of QR sweeps (the reference loops on). */
+int iso_extract_dc(iso_ctx* ctx, const iso_grid_desc* desc, float epsilon, iso_mesh_out* out);
+
+/* Marching cubes over a linear-hashed octree of the current program:
+ * replaces LinearHashedMarchingCubes::new(max_depth).extract with an
+ * IndexedVertices extractor (linear_hashed_marching_cubes.rs:51-84).  The
+ * domain is the unit cube, as the reference's Morton centres.  The mesh is
+ * the reference's for the primal-vertex visiting order "ascending Morton
+ * key" (the reference visits them in std HashMap order, which differs from
+ * run to run); Morton::level()'s truncated-constant quirk is reproduced.
+ * max_depth in [0, 9]. */
+int iso_extract_lhmc(iso_ctx* ctx, int32_t max_depth, iso_mesh_out* out);
+
 /* After an ISO_MEMORY_HOST extraction returned ISO_ERR_CAPACITY, the mesh is
  * kept in the context; this copies it into larger buffers without
  * re-extracting (the reference's Vec-growing sink needs no second pass). */

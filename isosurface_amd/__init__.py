@@ -4,16 +4,20 @@ The hot path of the swiftcoder/isosurface crate (SDF sampling, marching-cubes
 classification, edge interpolation, vertex dedup and triangle emission) runs
 in hand-written HIP kernels behind the C ABI of include/isosurface.h.  This
 package is the Python view of that ABI, mirroring the crate's
-Source / Extractor / MarchingCubes API; the C++ mirror for compiled hosts is
+Source / Extractor / MarchingCubes / DualContouring /
+LinearHashedMarchingCubes API; the C++ mirror for compiled hosts is
 isosurface_amd/host/isosurface.hpp.
 """
-from .engine import (Context, Extractor, IndexedVertices, MarchingCubes, Mesh, OnlyVertices,
+from .engine import (Context, DualContouring, Extractor, IndexedVertices,
+                     LinearHashedMarchingCubes, MarchingCubes, Mesh, MinimiseQEF, OnlyVertices,
                      default_context, sample_lattice_host)
-from .sources import (Cylinder, DemoSource, Difference, Intersection, RectangularPrism,
+from .sources import (CentralDifference, Cylinder, DemoSource, Difference, Intersection,
+                      RectangularPrism,
                       SmoothUnion, Source, Sphere, Torus, Translate, Union, smooth_union_all,
                       to_nodes, union_all)
 
 __all__ = [
+    "CentralDifference", "DualContouring", "LinearHashedMarchingCubes", "MinimiseQEF",
     "Context", "Extractor", "IndexedVertices", "MarchingCubes", "Mesh", "OnlyVertices",
     "default_context", "sample_lattice_host", "Cylinder", "DemoSource", "Difference",
     "Intersection", "RectangularPrism", "SmoothUnion", "Source", "Sphere", "Torus",

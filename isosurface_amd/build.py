@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["api.hip", "mc_kernels.hip", "program.cpp"]
+SOURCES = ["api.hip", "mc_kernels.hip", "dc_kernels.hip", "lhmc_kernels.hip", "program.cpp"]
 
 
 def hipcc() -> str:

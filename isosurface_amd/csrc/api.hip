@@ -8,6 +8,8 @@
 #include <vector>
 
 #include "../../include/isosurface.h"
+#include "dc_kernels.h"
+#include "lhmc_kernels.h"
 #include "mc_kernels.h"
 #include "program.h"
 
@@ -48,6 +50,13 @@ struct iso_ctx {
 
   // scratch
   DevBuf signs, seg, tplanes, flags, agg, incl, active, totals, vout, iout, pts, vals;
+  // dual contouring: primal samples, dual values and points, mixed list, counters
+  DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_counters;
+  unsigned long long* h_dc = nullptr;  // pinned
+  // octree marching cubes: nodes, dense primal lattice, cubes, edge table, scan state
+  DevBuf oc_keys, oc_values, oc_leaf, oc_dense, oc_primal, oc_plevel, oc_corner, oc_case,
+      oc_base, oc_tkeys, oc_towner, oc_tid, sc_flags, sc_agg, sc_incl, sc_ctl;
+  unsigned long long* h_stage = nullptr;  // pinned, 8 words
   uint64_t active_capacity = 0;
   unsigned long long* h_totals = nullptr;  // pinned
 
@@ -201,7 +210,13 @@ int iso_ctx_create(int device, void* stream, iso_ctx** out) {
     c->own_stream = true;
   }
   if (hipHostMalloc(reinterpret_cast<void**>(&c->h_totals), iso::kTotSlots * sizeof(unsigned long long),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->h_dc), iso::kDcSlots * sizeof(unsigned long long),
+                    hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc(reinterpret_cast<void**>(&c->h_stage), 8 * sizeof(unsigned long long),
                     hipHostMallocDefault) != hipSuccess) {
+    if (c->h_totals) (void)hipHostFree(c->h_totals);
+    if (c->h_dc) (void)hipHostFree(c->h_dc);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ISO_ERR_OUT_OF_MEMORY;
@@ -216,11 +231,18 @@ int iso_ctx_destroy(iso_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
-  DevBuf* bufs[] = {&c->prog, &c->grid_owned, &c->signs,  &c->seg,    &c->tplanes,
-                    &c->flags, &c->agg,        &c->incl,   &c->active, &c->totals,
-                    &c->vout, &c->iout,        &c->pts,    &c->vals};
+  DevBuf* bufs[] = {&c->prog,       &c->grid_owned, &c->signs,     &c->seg,      &c->tplanes,
+                    &c->flags,      &c->agg,        &c->incl,      &c->active,   &c->totals,
+                    &c->vout,       &c->iout,       &c->pts,       &c->vals,     &c->dc_samples,
+                    &c->dc_values,  &c->dc_points,  &c->dc_mixed,  &c->dc_counters,
+                    &c->oc_keys,    &c->oc_values,  &c->oc_leaf,   &c->oc_dense, &c->oc_primal,
+                    &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
+                    &c->oc_towner,  &c->oc_tid,     &c->sc_flags,  &c->sc_agg,   &c->sc_incl,
+                    &c->sc_ctl};
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
+  if (c->h_dc) (void)hipHostFree(c->h_dc);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return ISO_OK;
@@ -380,13 +402,14 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
   return ISO_OK;
 }
 
-// signs (with state reset) + count/scan.
-int classify_and_count(iso_ctx* c, bool from_grid, const iso::Lattice& L, const iso::McDims& D,
+// signs (with state reset) + count/scan; samples from `grid` (device) or,
+// when it is null, from the context's program.
+int classify_and_count(iso_ctx* c, const float* grid, const iso::Lattice& L, const iso::McDims& D,
                        const iso::McBuffers& B) {
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
-  if (from_grid) {
+  if (grid) {
     ISO_HIP(c, timed(c, "mc_signs_grid",
-                     [&] { return iso::launch_signs_grid(c->grid, L, D, B, c->stream); }));
+                     [&] { return iso::launch_signs_grid(grid, L, D, B, c->stream); }));
   } else {
     ISO_HIP(c, timed(c, "mc_signs_program", [&] {
               return iso::launch_signs_program(ops, c->n_prog, L, D, B, c->stream);
@@ -403,6 +426,89 @@ int read_totals(iso_ctx* c, const iso::McBuffers& B) {
   collect_timing(c);
   if (c->h_totals[iso::kTotErrors] & 2ull)
     return fail(c, ISO_ERR_DEVICE, "count scan look-back did not complete");
+  return ISO_OK;
+}
+
+// Marching cubes over lattice L whose samples come from `grid` (device) or,
+// when grid is null, from the context's program; with `positions` the
+// vertices interpolate between per-point positions.  Writes the mesh to out.
+int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const float* positions,
+                    iso_mesh_out* out) {
+  int st = ISO_OK;
+  const iso::McDims D = iso::mc_dims(L);
+  if (D.n_segs >= (int64_t(1) << 26))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");
+  iso::McBuffers B{};
+  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  uint64_t nv = 0, ni = 0;
+
+  if (out->memory == ISO_MEMORY_DEVICE) {
+    B.vertices = out->vertices;
+    B.indices = out->indices;
+    B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
+    B.index_capacity = out->indices ? out->index_capacity : 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+      ISO_HIP(c, timed(c, "mc_emit", [&] {
+                return iso::launch_emit(grid, ops, c->n_prog, positions, L, D, B, c->stream);
+              }));
+      if ((st = read_totals(c, B)) != ISO_OK) return st;
+      const uint64_t n_act = c->h_totals[iso::kTotActive];
+      if (n_act <= B.active_capacity) break;
+      // The active-cell list was too short: grow it and extract again.
+      if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+    }
+    nv = c->h_totals[iso::kTotVertices];
+    ni = 3 * c->h_totals[iso::kTotTriangles];
+    out->vertex_count = nv;
+    out->index_count = ni;
+    if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+    if (nv > B.vertex_capacity || ni > B.index_capacity)
+      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    return ISO_OK;
+  }
+
+  // Host memory: size the context's device buffers from the scan totals.
+  if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+  if ((st = read_totals(c, B)) != ISO_OK) return st;
+  const uint64_t n_act = c->h_totals[iso::kTotActive];
+  if (n_act > B.active_capacity) {
+    if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+    if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+    if ((st = read_totals(c, B)) != ISO_OK) return st;
+  }
+  nv = c->h_totals[iso::kTotVertices];
+  ni = 3 * c->h_totals[iso::kTotTriangles];
+  out->vertex_count = nv;
+  out->index_count = ni;
+  if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+  ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
+  ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
+  B.vertices = static_cast<float*>(c->vout.ptr);
+  B.indices = static_cast<uint32_t*>(c->iout.ptr);
+  B.vertex_capacity = nv;
+  B.index_capacity = ni;
+  ISO_HIP(c, timed(c, "mc_emit", [&] {
+            return iso::launch_emit(grid, ops, c->n_prog, positions, L, D, B, c->stream);
+          }));
+  if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+      (ni && (!out->indices || out->index_capacity < ni))) {
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    c->kept = true;
+    c->kept_vertices = nv;
+    c->kept_indices = ni;
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small; mesh kept for iso_mesh_fetch");
+  }
+  if (nv)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, B.vertices, nv * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+  if (ni)
+    ISO_HIP(c, hipMemcpyAsync(out->indices, B.indices, ni * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
   return ISO_OK;
 }
 
@@ -424,83 +530,280 @@ int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
   if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
   c->kept = false;
-
   ISO_HIP(c, hipSetDevice(c->device));
-  const iso::McDims D = iso::mc_dims(L);
-  if (D.n_segs >= (int64_t(1) << 26))
-    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");
-  iso::McBuffers B{};
-  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
-  const float* grid = from_grid ? c->grid : nullptr;
-  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
-  uint64_t nv = 0, ni = 0;
+  return extract_lattice(c, L, from_grid ? c->grid : nullptr, nullptr, out);
+}
 
-  if (out->memory == ISO_MEMORY_DEVICE) {
-    B.vertices = out->vertices;
-    B.indices = out->indices;
-    B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
-    B.index_capacity = out->indices ? out->index_capacity : 0;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      if ((st = classify_and_count(c, from_grid, L, D, B)) != ISO_OK) return st;
-      ISO_HIP(c, timed(c, "mc_emit", [&] {
-                return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
-              }));
-      if ((st = read_totals(c, B)) != ISO_OK) return st;
-      const uint64_t n_act = c->h_totals[iso::kTotActive];
-      if (n_act <= B.active_capacity) break;
-      // The active-cell list was too short: grow it and extract again.
-      if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
-    }
-    nv = c->h_totals[iso::kTotVertices];
-    ni = 3 * c->h_totals[iso::kTotTriangles];
-    out->vertex_count = nv;
-    out->index_count = ni;
-    if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
-    if (nv > B.vertex_capacity || ni > B.index_capacity)
-      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  if (desc->source != ISO_SOURCE_PROGRAM)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "dual contouring needs a program source (normals)");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (!(epsilon > 0.0f) || !(epsilon < 3.0e38f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "epsilon must be positive and finite");
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+
+  // The dual lattice has one point per primal cell; with fewer than two
+  // dual points along an axis there is no dual cube and the mesh is empty.
+  iso::Lattice Ld = L;
+  Ld.nx = L.nx - 1;
+  Ld.ny = L.ny - 1;
+  Ld.nz = L.nz - 1;
+  if (Ld.nx < 2 || Ld.ny < 2 || Ld.nz < 2) {
+    out->vertex_count = 0;
+    out->index_count = 0;
     return ISO_OK;
   }
+  const uint64_t n_points = static_cast<uint64_t>(L.nx * L.ny * L.nz);
+  const uint64_t n_cells = static_cast<uint64_t>(Ld.nx * Ld.ny * Ld.nz);
+  ISO_HIP(c, ensure(&c->dc_samples, n_points * sizeof(float)));
+  ISO_HIP(c, ensure(&c->dc_values, n_cells * sizeof(float)));
+  ISO_HIP(c, ensure(&c->dc_points, 3 * n_cells * sizeof(float)));
+  ISO_HIP(c, ensure(&c->dc_mixed, n_cells * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->dc_counters, iso::kDcSlots * sizeof(unsigned long long)));
+  iso::DcBuffers B;
+  B.samples = static_cast<float*>(c->dc_samples.ptr);
+  B.dual_values = static_cast<float*>(c->dc_values.ptr);
+  B.dual_points = static_cast<float*>(c->dc_points.ptr);
+  B.mixed = static_cast<uint32_t*>(c->dc_mixed.ptr);
+  B.counters = static_cast<unsigned long long*>(c->dc_counters.ptr);
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  ISO_HIP(c, timed(c, "dc_sample",
+                   [&] { return iso::launch_dc_sample(ops, c->n_prog, L, B, c->stream); }));
+  ISO_HIP(c, timed(c, "dc_dual", [&] { return iso::launch_dc_dual(L, B, c->stream); }));
+  ISO_HIP(c, timed(c, "dc_feature", [&] {
+            return iso::launch_dc_feature(ops, c->n_prog, L, epsilon, B, c->stream);
+          }));
+  st = extract_lattice(c, Ld, B.dual_values, B.dual_points, out);
+  // The counters are final once the stream has passed the feature kernel,
+  // which every extract_lattice path has synchronised past.
+  ISO_HIP(c, hipMemcpyAsync(c->h_dc, B.counters, iso::kDcSlots * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->h_dc[iso::kDcStalled])
+    return fail(c, ISO_ERR_DEVICE, "a feature point's SVD did not converge");
+  return st;
+}
 
-  // Host memory: size the context's device buffers from the scan totals.
-  if ((st = classify_and_count(c, from_grid, L, D, B)) != ISO_OK) return st;
-  if ((st = read_totals(c, B)) != ISO_OK) return st;
-  const uint64_t n_act = c->h_totals[iso::kTotActive];
-  if (n_act > B.active_capacity) {
-    if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
-    if ((st = classify_and_count(c, from_grid, L, D, B)) != ISO_OK) return st;
-    if ((st = read_totals(c, B)) != ISO_OK) return st;
+}  // extern "C"
+
+namespace {
+
+// Grows a device buffer to `bytes`, keeping its first `keep` bytes.
+int grow_keep(iso_ctx* c, DevBuf* b, size_t bytes, size_t keep) {
+  if (b->bytes >= bytes && b->ptr) return ISO_OK;
+  size_t want = b->bytes ? b->bytes : 256;
+  while (want < bytes) want *= 2;
+  void* p = nullptr;
+  ISO_HIP(c, hipMalloc(&p, want));
+  if (keep && b->ptr) {
+    hipError_t e = hipMemcpyAsync(p, b->ptr, keep, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return fail(c, ISO_ERR_DEVICE, std::string("grow copy: ") + hipGetErrorString(e));
+    }
   }
-  nv = c->h_totals[iso::kTotVertices];
-  ni = 3 * c->h_totals[iso::kTotTriangles];
+  release(b);
+  b->ptr = p;
+  b->bytes = want;
+  return ISO_OK;
+}
+
+// Scan state for scans of up to n items; err points at the status word.
+int prepare_scan(iso_ctx* c, uint64_t n, iso::scan::State* st) {
+  const size_t nb = static_cast<size_t>(iso::scan::blocks_for(n));
+  ISO_HIP(c, ensure(&c->sc_flags, nb * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->sc_agg, nb * sizeof(unsigned long long)));
+  ISO_HIP(c, ensure(&c->sc_incl, nb * sizeof(unsigned long long)));
+  ISO_HIP(c, ensure(&c->sc_ctl, 4 * sizeof(unsigned long long)));
+  st->flags = static_cast<uint32_t*>(c->sc_flags.ptr);
+  st->agg = static_cast<unsigned long long*>(c->sc_agg.ptr);
+  st->incl = static_cast<unsigned long long*>(c->sc_incl.ptr);
+  st->ctl = static_cast<unsigned long long*>(c->sc_ctl.ptr);
+  st->err = st->ctl + 2;
+  return ISO_OK;
+}
+
+// Reads ctl[0..4) of the scan state (total in [1], status bits in [2]).
+int read_ctl(iso_ctx* c, const iso::scan::State& st) {
+  ISO_HIP(c, hipMemcpyAsync(c->h_stage, st.ctl, 4 * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (c->h_stage[2]) {
+    const unsigned long long e = c->h_stage[2];
+    return fail(c, ISO_ERR_DEVICE,
+                std::string("octree extraction failed:") + ((e & iso::kLhmcErrScan) ? " scan" : "") +
+                    ((e & iso::kLhmcErrWalk) ? " dual-corner walk" : "") +
+                    ((e & iso::kLhmcErrTable) ? " edge table" : ""));
+  }
+  return ISO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iso_extract_lhmc(iso_ctx* c, int32_t max_depth, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (max_depth < 0 || max_depth > iso::kLhmcMaxDepth)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "max_depth must be in [0, 9]");
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+  int st = ISO_OK;
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  const int d1 = (max_depth > 2 ? max_depth : 2) + 1;  // deepest possible node
+  const uint64_t n_dense = 1ull << (3 * d1);
+  iso::scan::State S{};
+  if ((st = prepare_scan(c, n_dense, &S)) != ISO_OK) return st;
+  // status word: cleared once, accumulated by every kernel below
+  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(S.err), 2, 0u, c->stream));
+
+  // 1. octree, breadth first
+  iso::LhmcNodes N{};
+  N.n_levels = d1 + 1;
+  int64_t off[iso::kLhmcMaxLevels + 2] = {0};
+  off[0] = 0;
+  off[1] = 1;
+  {
+    const size_t cap = 4096;
+    if ((st = grow_keep(c, &c->oc_keys, cap * 8, 0)) != ISO_OK) return st;
+    c->h_stage[4] = 1ull;  // the root key
+    ISO_HIP(c, hipMemcpyAsync(c->oc_keys.ptr, &c->h_stage[4], sizeof(uint64_t),
+                              hipMemcpyHostToDevice, c->stream));
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  for (int d = 0; d <= d1; ++d) {
+    const int64_t n = off[d + 1] - off[d];
+    const int64_t next_cap = d < d1 ? 8 * n : 0;
+    const size_t need = static_cast<size_t>(off[d + 1] + next_cap);
+    if ((st = grow_keep(c, &c->oc_keys, need * 8, static_cast<size_t>(off[d + 1]) * 8)) != ISO_OK ||
+        (st = grow_keep(c, &c->oc_values, need * 4, static_cast<size_t>(off[d]) * 4)) != ISO_OK ||
+        (st = grow_keep(c, &c->oc_leaf, need, static_cast<size_t>(off[d]))) != ISO_OK)
+      return st;
+    ISO_HIP(c, timed(c, "lhmc_level", [&] {
+              return iso::launch_lhmc_level(
+                  ops, c->n_prog, max_depth, static_cast<uint64_t*>(c->oc_keys.ptr),
+                  static_cast<float*>(c->oc_values.ptr), static_cast<uint8_t*>(c->oc_leaf.ptr),
+                  off[d], n, next_cap, S, c->stream);
+            }));
+    if ((st = read_ctl(c, S)) != ISO_OK) return st;
+    const int64_t children = 8 * static_cast<int64_t>(c->h_stage[1]);
+    if (children > next_cap)
+      return fail(c, ISO_ERR_DEVICE, "octree refined past its deepest possible depth");
+    off[d + 2] = off[d + 1] + children;
+  }
+  N.keys = static_cast<uint64_t*>(c->oc_keys.ptr);
+  N.values = static_cast<float*>(c->oc_values.ptr);
+  N.leaf = static_cast<uint8_t*>(c->oc_leaf.ptr);
+  for (int d = 0; d <= N.n_levels; ++d) N.level_off[d] = off[d];
+  const int64_t n_nodes = off[N.n_levels];
+  if (n_nodes >= 0xffffffffll)
+    return fail(c, ISO_ERR_OVERFLOW, "octree has 2^32 or more nodes");
+
+  // 2. primal vertices in ascending Morton order
+  iso::LhmcCubes C{};
+  uint64_t pv_cap = 8 * static_cast<uint64_t>(n_nodes);
+  if (pv_cap > n_dense) pv_cap = n_dense;
+  ISO_HIP(c, ensure(&c->oc_dense, n_dense * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->oc_primal, pv_cap * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->oc_plevel, pv_cap));
+  C.primal = static_cast<uint64_t*>(c->oc_primal.ptr);
+  C.primal_level = static_cast<uint8_t*>(c->oc_plevel.ptr);
+  ISO_HIP(c, timed(c, "lhmc_primal", [&] {
+            return iso::launch_lhmc_primal(N, d1, static_cast<uint32_t*>(c->oc_dense.ptr),
+                                           n_dense, C, pv_cap, S, c->stream);
+          }));
+  if ((st = read_ctl(c, S)) != ISO_OK) return st;
+  const uint64_t n_pv = c->h_stage[1];
+  if (n_pv > pv_cap) return fail(c, ISO_ERR_DEVICE, "primal vertex list overflow");
+  if (n_pv > 0x0fffffffull) return fail(c, ISO_ERR_OVERFLOW, "too many dual cubes");
+  C.n = static_cast<uint32_t>(n_pv);
+
+  // 3. dual cubes
+  ISO_HIP(c, ensure(&c->oc_corner, (n_pv ? n_pv : 1) * 8 * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->oc_case, n_pv ? n_pv : 1));
+  ISO_HIP(c, ensure(&c->oc_base, (n_pv ? n_pv : 1) * sizeof(unsigned long long)));
+  C.corner = static_cast<uint32_t*>(c->oc_corner.ptr);
+  C.cube_case = static_cast<uint8_t*>(c->oc_case.ptr);
+  C.base = static_cast<unsigned long long*>(c->oc_base.ptr);
+  ISO_HIP(c, timed(c, "lhmc_cubes", [&] { return iso::launch_lhmc_cubes(N, d1, C, S, c->stream); }));
+  if ((st = read_ctl(c, S)) != ISO_OK) return st;
+  const uint64_t n_tri = c->h_stage[1];
+
+  // 4. edge ownership and first ids
+  iso::LhmcEdgeTable T{};
+  uint64_t tcap = 1024;
+  while (tcap < 6 * n_tri) tcap *= 2;
+  ISO_HIP(c, ensure(&c->oc_tkeys, tcap * sizeof(unsigned long long)));
+  ISO_HIP(c, ensure(&c->oc_towner, tcap * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->oc_tid, tcap * sizeof(uint32_t)));
+  T.keys = static_cast<unsigned long long*>(c->oc_tkeys.ptr);
+  T.owner = static_cast<uint32_t*>(c->oc_towner.ptr);
+  T.id = static_cast<uint32_t*>(c->oc_tid.ptr);
+  T.mask = tcap - 1;
+  ISO_HIP(c, timed(c, "lhmc_own", [&] {
+            hipError_t e = iso::launch_lhmc_table_clear(T, c->stream);
+            if (e != hipSuccess) return e;
+            return iso::launch_lhmc_own(N, C, T, S.err, S, c->stream);
+          }));
+  if ((st = read_ctl(c, S)) != ISO_OK) return st;
+  const uint64_t nv = c->h_stage[1] & 0xffffffffull;
+  const uint64_t ni = 3 * (c->h_stage[1] >> 32);
+  if ((c->h_stage[1] >> 32) != n_tri) return fail(c, ISO_ERR_DEVICE, "triangle count mismatch");
+  if (ni > 0xffffffffull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
   out->vertex_count = nv;
   out->index_count = ni;
-  if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
-  ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
-  ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
-  B.vertices = static_cast<float*>(c->vout.ptr);
-  B.indices = static_cast<uint32_t*>(c->iout.ptr);
-  B.vertex_capacity = nv;
-  B.index_capacity = ni;
-  ISO_HIP(c, timed(c, "mc_emit", [&] {
-            return iso::launch_emit(grid, ops, c->n_prog, L, D, B, c->stream);
+
+  // 5. emit
+  float* vdst = nullptr;
+  uint32_t* idst = nullptr;
+  uint64_t vcap = 0, icap = 0;
+  if (out->memory == ISO_MEMORY_DEVICE) {
+    vdst = out->vertices;
+    idst = out->indices;
+    vcap = out->vertices ? out->vertex_capacity : 0;
+    icap = out->indices ? out->index_capacity : 0;
+  } else {
+    ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
+    ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
+    vdst = static_cast<float*>(c->vout.ptr);
+    idst = static_cast<uint32_t*>(c->iout.ptr);
+    vcap = nv;
+    icap = ni;
+  }
+  ISO_HIP(c, timed(c, "lhmc_emit", [&] {
+            return iso::launch_lhmc_emit(N, C, T, vdst, vcap, idst, icap, S.err, c->stream);
           }));
+  if ((st = read_ctl(c, S)) != ISO_OK) return st;
+  if (out->memory == ISO_MEMORY_DEVICE) {
+    if (nv > vcap || ni > icap) return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    return ISO_OK;
+  }
   if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
       (ni && (!out->indices || out->index_capacity < ni))) {
-    ISO_HIP(c, hipStreamSynchronize(c->stream));
-    collect_timing(c);
     c->kept = true;
     c->kept_vertices = nv;
     c->kept_indices = ni;
     return fail(c, ISO_ERR_CAPACITY, "output buffers too small; mesh kept for iso_mesh_fetch");
   }
   if (nv)
-    ISO_HIP(c, hipMemcpyAsync(out->vertices, B.vertices, nv * 3 * sizeof(float),
-                              hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, vdst, nv * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
   if (ni)
-    ISO_HIP(c, hipMemcpyAsync(out->indices, B.indices, ni * sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipMemcpyAsync(out->indices, idst, ni * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
-  collect_timing(c);
   return ISO_OK;
 }
 

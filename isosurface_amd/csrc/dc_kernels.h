@@ -1,0 +1,41 @@
+// Launch interface of the dual-contouring kernels (host side).
+//
+// Dual contouring is marching cubes on the dual lattice: one dual point per
+// primal cell, carrying the cell's dual value (the lerp of its corners 0 and
+// 6) and its dual vertex (the cell centre, or for a mixed cell the feature
+// point placed by the QEF).  These kernels build that lattice; the
+// marching-cubes kernels then extract it with per-point positions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mc_kernels.h"
+#include "sdf_program.h"
+
+namespace iso {
+
+// Slots of DcBuffers::counters.
+enum : int {
+  kDcMixed = 0,    // number of mixed primal cells listed
+  kDcStalled = 1,  // nonzero when a QEF's SVD hit the sweep bound
+  kDcSlots = 2,
+};
+
+struct DcBuffers {
+  float* samples;       // primal lattice samples, nx*ny*nz
+  float* dual_values;   // one per primal cell, z-major
+  float* dual_points;   // 3 per primal cell
+  uint32_t* mixed;      // mixed primal cells, capacity = number of cells
+  unsigned long long* counters;  // kDcSlots
+};
+
+// Samples the program over the primal lattice and clears the counters.
+hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const DcBuffers& B,
+                            hipStream_t s);
+// Dual values, centre dual vertices and the mixed-cell list.
+hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, hipStream_t s);
+// Feature points of the listed mixed cells (normals by central differences).
+hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
+                             const DcBuffers& B, hipStream_t s);
+
+}  // namespace iso

@@ -1,0 +1,203 @@
+// Dual-contouring kernels for gfx950 (MI355X).
+//
+// DualContouring::extract (dual_contouring.rs:65-105) over
+// DualGrid::traverse (traversal/dual_grid.rs:52-95) in three kernels before
+// the marching-cubes pass over the dual lattice:
+//   1. sample   the program at every primal lattice point;
+//   2. dual     one thread per primal cell: dual value
+//               lerp(v0, v6, 0.5) = 0.5*v0 + 0.5*v6, and either the centre
+//               lerp(c0, c6, 0.5) as dual vertex (cube index 0 or 255) or an
+//               entry in the mixed-cell list;
+//   3. feature  one thread per listed mixed cell: corner normals by central
+//               differences of the program (source.rs:82-94), normalised or
+//               zero (marching_cubes_impl.rs:58-69), then TangentPlanes and
+//               the QEF solve (feature/mod.rs:91-148, feature/qef.rs:36-68)
+//               with the reference's f64 SVD.
+// The list order is whatever the atomics give, but every result is written
+// to its cell's own slot, so the output is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "dc_kernels.h"
+#include "mc_tables.h"
+#include "qef_device.h"
+#include "sdf_eval.h"
+
+namespace iso {
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_dc_sample(const DevOp* __restrict__ ops, int n_ops,
+                                                   Lattice L, int64_t n_points,
+                                                   float* __restrict__ out,
+                                                   unsigned long long* __restrict__ counters) {
+  if (blockIdx.x == 0 && threadIdx.x < kDcSlots) counters[threadIdx.x] = 0ull;
+  const uint32_t nx = static_cast<uint32_t>(L.nx);
+  const uint32_t nxy = static_cast<uint32_t>(L.nx * L.ny);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < n_points;
+       f += stride) {
+    const uint32_t ff = static_cast<uint32_t>(f);
+    const uint32_t z = ff / nxy;
+    const uint32_t r = ff - z * nxy;
+    const uint32_t y = r / nx;
+    const uint32_t x = r - y * nx;
+    out[f] = sdf_eval(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
+                      lattice_coord(L, 2, z));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dc_dual(Lattice L, int64_t n_cells,
+                                                 const float* __restrict__ samples,
+                                                 float* __restrict__ dual_values,
+                                                 float* __restrict__ dual_points,
+                                                 uint32_t* __restrict__ mixed,
+                                                 unsigned long long* __restrict__ counters) {
+  const int64_t nx = L.nx, ny = L.ny;
+  const uint32_t cnx = static_cast<uint32_t>(nx - 1);
+  const uint32_t cnxy = static_cast<uint32_t>((nx - 1) * (ny - 1));
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  const int lane = __lane_id();
+  // Every lane of a wave runs the same number of iterations, so the ballot
+  // below sees the whole wave.
+  const int64_t start = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t wave_start = start - lane;
+  for (int64_t base = wave_start; base < n_cells; base += stride) {
+    const int64_t c = base + lane;
+    bool is_mixed = false;
+    if (c < n_cells) {
+      const uint32_t cc = static_cast<uint32_t>(c);
+      const uint32_t z = cc / cnxy;
+      const uint32_t r = cc - z * cnxy;
+      const uint32_t y = r / cnx;
+      const uint32_t x = r - y * cnx;
+      float v[8];
+      uint32_t cube = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int64_t px = x + tables::kCorner[i][0], py = y + tables::kCorner[i][1],
+                      pz = z + tables::kCorner[i][2];
+        v[i] = samples[(pz * ny + py) * nx + px];
+        if (!(v[i] > 0.0f)) cube |= 1u << i;  // classify_corners, distance.rs:52-54
+      }
+      // Signed::lerp (distance.rs:56-58) with factor 0.5: (1 - 0.5) * a + 0.5 * b
+      dual_values[c] = 0.5f * v[0] + 0.5f * v[6];
+      is_mixed = cube != 0u && cube != 255u;
+      if (!is_mixed) {
+        // Vec3::lerp (vector.rs:325-333) of corners 0 and 6
+        const float c0x = lattice_coord(L, 0, x), c0y = lattice_coord(L, 1, y),
+                    c0z = lattice_coord(L, 2, z);
+        const float c6x = lattice_coord(L, 0, x + 1), c6y = lattice_coord(L, 1, y + 1),
+                    c6z = lattice_coord(L, 2, z + 1);
+        dual_points[3 * c + 0] = 0.5f * c0x + 0.5f * c6x;
+        dual_points[3 * c + 1] = 0.5f * c0y + 0.5f * c6y;
+        dual_points[3 * c + 2] = 0.5f * c0z + 0.5f * c6z;
+      }
+    }
+    // One atomic per wave reserves the wave's slots in the mixed-cell list.
+    const uint64_t ballot = __ballot(is_mixed);
+    if (ballot) {
+      unsigned long long first = 0;
+      const int leader = __builtin_ctzll(ballot);
+      if (lane == leader) first = atomicAdd(&counters[kDcMixed], static_cast<unsigned long long>(__popcll(ballot)));
+      first = __shfl(first, leader, 64);
+      if (is_mixed) {
+        const uint64_t below = ballot & ((1ull << lane) - 1ull);
+        mixed[first + __popcll(below)] = static_cast<uint32_t>(c);
+      }
+    }
+  }
+}
+
+// CentralDifference::sample_normal (source.rs:82-94) at p, then
+// normalised().unwrap_or_default() (marching_cubes_impl.rs:63-67).
+__device__ __forceinline__ qef::Vec3f central_normal(const DevOp* ops, int n_ops, float px,
+                                                     float py, float pz, float eps) {
+  const float vx = sdf_eval(ops, n_ops, px + eps, py + 0.0f, pz + 0.0f) -
+                   sdf_eval(ops, n_ops, px - eps, py - 0.0f, pz - 0.0f);
+  const float vy = sdf_eval(ops, n_ops, px + 0.0f, py + eps, pz + 0.0f) -
+                   sdf_eval(ops, n_ops, px - 0.0f, py - eps, pz - 0.0f);
+  const float vz = sdf_eval(ops, n_ops, px + 0.0f, py + 0.0f, pz + eps) -
+                   sdf_eval(ops, n_ops, px - 0.0f, py - 0.0f, pz - eps);
+  const float e2 = 2.0f * eps;
+  return qef::normalised_or_zero(qef::Vec3f{vx / e2, vy / e2, vz / e2});
+}
+
+constexpr int kFeatureThreads = 128;
+
+__global__ __launch_bounds__(kFeatureThreads) void k_dc_feature(
+    const DevOp* __restrict__ ops, int n_ops, Lattice L, float eps,
+    const uint32_t* __restrict__ mixed, float* __restrict__ dual_points,
+    unsigned long long* __restrict__ counters) {
+  const uint64_t n_mixed = counters[kDcMixed];
+  const uint32_t cnx = static_cast<uint32_t>(L.nx - 1);
+  const uint32_t cnxy = static_cast<uint32_t>((L.nx - 1) * (L.ny - 1));
+  bool stalled = false;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kFeatureThreads + threadIdx.x; i < n_mixed;
+       i += static_cast<uint64_t>(gridDim.x) * kFeatureThreads) {
+    const uint32_t c = mixed[i];
+    const uint32_t z = c / cnxy;
+    const uint32_t r = c - z * cnxy;
+    const uint32_t y = r / cnx;
+    const uint32_t x = r - y * cnx;
+    qef::Vec3f corners[8], normals[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      corners[k] = {lattice_coord(L, 0, x + tables::kCorner[k][0]),
+                    lattice_coord(L, 1, y + tables::kCorner[k][1]),
+                    lattice_coord(L, 2, z + tables::kCorner[k][2])};
+      normals[k] = central_normal(ops, n_ops, corners[k].x, corners[k].y, corners[k].z, eps);
+    }
+    const qef::Vec3f p = qef::place_feature(corners, normals, &stalled);
+    dual_points[3 * static_cast<uint64_t>(c) + 0] = p.x;
+    dual_points[3 * static_cast<uint64_t>(c) + 1] = p.y;
+    dual_points[3 * static_cast<uint64_t>(c) + 2] = p.z;
+  }
+  if (stalled) atomicOr(&counters[kDcStalled], 1ull);
+}
+
+int device_cus_dc() {
+  int device = 0, cus = 256;
+  if (hipGetDevice(&device) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  return cus > 0 ? cus : 256;
+}
+
+}  // namespace
+
+hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const DcBuffers& B,
+                            hipStream_t s) {
+  if (!ops || !B.samples || !B.counters) return hipErrorInvalidValue;
+  const int64_t n = L.nx * L.ny * L.nz;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_dc_sample, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops, n_ops,
+                     L, n, B.samples, B.counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, hipStream_t s) {
+  const int64_t n_cells = (L.nx - 1) * (L.ny - 1) * (L.nz - 1);
+  int64_t blocks = (n_cells + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_dc_dual, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L, n_cells,
+                     B.samples, B.dual_values, B.dual_points, B.mixed, B.counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
+                             const DcBuffers& B, hipStream_t s) {
+  if (!ops) return hipErrorInvalidValue;
+  // Grid-stride over the mixed list, whose length is only known on the device.
+  const int64_t n_cells = (L.nx - 1) * (L.ny - 1) * (L.nz - 1);
+  int64_t blocks = (n_cells + kFeatureThreads - 1) / kFeatureThreads;
+  const int64_t cap = 16LL * device_cus_dc();
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_dc_feature, dim3(static_cast<unsigned>(blocks)), dim3(kFeatureThreads), 0,
+                     s, ops, n_ops, L, epsilon, B.mixed, B.dual_points, B.counters);
+  return hipGetLastError();
+}
+
+}  // namespace iso

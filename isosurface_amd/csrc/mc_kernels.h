@@ -44,6 +44,13 @@ enum : int {
 
 McDims mc_dims(const Lattice& L);
 
+// Coordinate of lattice index i along an axis: origin + i * spacing, the
+// reference's `x as f32 * one_over_size` when origin is 0
+// (primal_grid.rs:50-51, 62-63).
+__device__ __forceinline__ float lattice_coord(const Lattice& L, int axis, int64_t i) {
+  return L.origin[axis] + static_cast<float>(i) * L.spacing[axis];
+}
+
 // Buffers of one extraction.
 struct McBuffers {
   uint64_t* signs;          // n_sign_words; signs[-1], signs[n], signs[n+1] are guard words
@@ -68,8 +75,10 @@ hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& 
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
                                 const McBuffers& B, hipStream_t s);
 hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
-hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lattice& L,
-                       const McDims& D, const McBuffers& B, hipStream_t s);
+// With `positions` (3 floats per lattice point, requires `grid`) vertices
+// interpolate between those positions instead of lattice coordinates.
+hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const float* positions,
+                       const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
 hipError_t launch_sample_lattice(const DevOp* ops, int n_ops, const Lattice& L, float* out,
                                  hipStream_t s);
 hipError_t launch_sample_points(const DevOp* ops, int n_ops, const float* pts, uint64_t n,

@@ -129,10 +129,6 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ float lattice_coord(const Lattice& L, int axis, int64_t i) {
-  return L.origin[axis] + static_cast<float>(i) * L.spacing[axis];
-}
-
 // Signed::find_crossing_point (distance.rs:64-70): t = d == 0 ? 0.5 : -a/d,
 // p = pa*(1-t) + pb*t per component, every operation rounded once.
 __device__ __forceinline__ void crossing(float a, float b, float pax, float pay, float paz,
@@ -523,16 +519,24 @@ struct EmitArgs {
   const float* grid;
   const DevOp* ops;
   int n_ops;
+  const float* positions;  // per lattice point (3 floats), kEmitPositions only
   float* vertices;
   uint32_t* indices;
   uint64_t vcap, icap;
   const unsigned long long* totals;
 };
 
-template <bool kFromGrid>
+// Where an extraction's sample values and vertex positions come from.
+enum EmitSource : int {
+  kEmitProgram = 0,    // values from the SDF program, positions from the lattice
+  kEmitGrid = 1,       // values from a dense grid, positions from the lattice
+  kEmitPositions = 2,  // values from a dense grid, positions per lattice point
+};
+
+template <int kSrc>
 __device__ __forceinline__ float lattice_value(const EmitArgs& A, const Lattice& L, int64_t x,
                                                int64_t y, int64_t z) {
-  if (kFromGrid) {
+  if (kSrc != kEmitProgram) {
     return A.grid[(z * L.ny + y) * L.nx + x];
   } else {
     return sdf_eval(A.ops, A.n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
@@ -551,7 +555,22 @@ __device__ __forceinline__ uint32_t case_in_cube(uint32_t cube, int dx, int dy, 
 
 constexpr int kEmitThreads = 256;
 
-template <bool kFromGrid>
+template <int kSrc>
+__device__ __forceinline__ void lattice_position(const EmitArgs& A, const Lattice& L, int64_t x,
+                                                 int64_t y, int64_t z, float* p) {
+  if (kSrc == kEmitPositions) {
+    const float* q = A.positions + 3 * ((z * L.ny + y) * L.nx + x);
+    p[0] = q[0];
+    p[1] = q[1];
+    p[2] = q[2];
+  } else {
+    p[0] = lattice_coord(L, 0, x);
+    p[1] = lattice_coord(L, 1, y);
+    p[2] = lattice_coord(L, 2, z);
+  }
+}
+
+template <int kSrc>
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, McDims D) {
   __shared__ tables::McTables s_t;
   // Per thread: first vertex id and (cube index | owned crossed edges << 8)
@@ -672,11 +691,12 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, Mc
                       uz = cz + tables::kCorner[u][2];
         const int64_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
                       vz = cz + tables::kCorner[v][2];
-        const float va = lattice_value<kFromGrid>(A, L, ux, uy, uz);
-        const float vb = lattice_value<kFromGrid>(A, L, vx, vy, vz);
-        float p[3];
-        crossing(va, vb, lattice_coord(L, 0, ux), lattice_coord(L, 1, uy), lattice_coord(L, 2, uz),
-                 lattice_coord(L, 0, vx), lattice_coord(L, 1, vy), lattice_coord(L, 2, vz), p);
+        const float va = lattice_value<kSrc>(A, L, ux, uy, uz);
+        const float vb = lattice_value<kSrc>(A, L, vx, vy, vz);
+        float pa[3], pb[3], p[3];
+        lattice_position<kSrc>(A, L, ux, uy, uz, pa);
+        lattice_position<kSrc>(A, L, vx, vy, vz, pb);
+        crossing(va, vb, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], p);
         if (id < A.vcap) {
           A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
           A.vertices[3 * static_cast<uint64_t>(id) + 1] = p[1];
@@ -782,6 +802,7 @@ McDims mc_dims(const Lattice& L) {
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,
                              const McBuffers& B, hipStream_t s) {
   (void)L;
+  if (!grid) return hipErrorInvalidValue;
   const ResetArgs R = reset_args(D, B);
   // At least enough threads to clear the look-back flags in one sweep.
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
@@ -802,6 +823,7 @@ hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& 
 
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
                                 const McBuffers& B, hipStream_t s) {
+  if (!ops) return hipErrorInvalidValue;
   int64_t blocks = grid_for(D.n_sign_words);
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
   if (blocks < min_blocks) blocks = min_blocks;
@@ -817,8 +839,9 @@ hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, h
   return hipGetLastError();
 }
 
-hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lattice& L,
-                       const McDims& D, const McBuffers& B, hipStream_t s) {
+hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const float* positions,
+                       const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s) {
+  if ((positions && !grid) || (!grid && !ops)) return hipErrorInvalidValue;
   EmitArgs A;
   A.S = B.signs;
   A.seg = B.seg;
@@ -828,6 +851,7 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lat
   A.grid = grid;
   A.ops = ops;
   A.n_ops = n_ops;
+  A.positions = positions;
   A.vertices = B.vertices;
   A.indices = B.indices;
   A.vcap = B.vertex_capacity;
@@ -838,12 +862,15 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const Lat
   const int64_t cap = 8LL * device_cus();
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  if (grid) {
-    hipLaunchKernelGGL(k_emit<true>, dim3(static_cast<unsigned>(blocks)), dim3(kEmitThreads), 0,
-                       s, A, L, D);
+  if (grid && positions) {
+    hipLaunchKernelGGL(k_emit<kEmitPositions>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(kEmitThreads), 0, s, A, L, D);
+  } else if (grid) {
+    hipLaunchKernelGGL(k_emit<kEmitGrid>, dim3(static_cast<unsigned>(blocks)), dim3(kEmitThreads),
+                       0, s, A, L, D);
   } else {
-    hipLaunchKernelGGL(k_emit<false>, dim3(static_cast<unsigned>(blocks)), dim3(kEmitThreads), 0,
-                       s, A, L, D);
+    hipLaunchKernelGGL(k_emit<kEmitProgram>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(kEmitThreads), 0, s, A, L, D);
   }
   return hipGetLastError();
 }

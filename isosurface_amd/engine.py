@@ -1,6 +1,10 @@
 """GPU contexts and the extractor front doors (Python mirror of the crate API).
 
   MarchingCubes(size).extract(source, extractor)   marching_cubes.rs:46-82
+  DualContouring(size, MinimiseQEF()).extract(CentralDifference(source), extractor)
+                                                   dual_contouring.rs:48-105
+  LinearHashedMarchingCubes(max_depth).extract(source, extractor)
+                                                   linear_hashed_marching_cubes.rs:51-84
 
 The extraction runs on the GPU through the C ABI; the result is then
 replayed into the extractor with the reference's sink protocol: every
@@ -16,7 +20,7 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from . import _abi as A
-from .sources import Source, to_nodes
+from .sources import CentralDifference, Source, to_nodes
 
 
 class Extractor:
@@ -171,6 +175,49 @@ class Context:
         nv, ni = int(out.vertex_count), int(out.index_count)
         return Mesh(verts[:nv].copy(), idx[:ni].copy())
 
+    def extract_dc(self, desc: A.IsoGridDesc, epsilon: float, vertex_capacity: int = 0,
+                   index_capacity: int = 0) -> Mesh:
+        """iso_extract_dc into host arrays (grown on ISO_ERR_CAPACITY)."""
+        vcap = max(int(vertex_capacity), 1)
+        icap = max(int(index_capacity), 3)
+        verts = np.empty((vcap, 3), dtype=np.float32)
+        idx = np.empty(icap, dtype=np.uint32)
+        out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
+                           A.ISO_MEMORY_HOST, 0)
+        st = self.lib.iso_extract_dc(self.handle, C.byref(desc), C.c_float(epsilon),
+                                     C.byref(out))
+        if st == A.ISO_ERR_CAPACITY:
+            nv, ni = int(out.vertex_count), int(out.index_count)
+            verts = np.empty((max(nv, 1), 3), dtype=np.float32)
+            idx = np.empty(max(ni, 3), dtype=np.uint32)
+            out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, max(nv, 1), max(ni, 3), 0, 0,
+                               A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.handle, C.byref(out))
+        self.check(st, "iso_extract_dc")
+        nv, ni = int(out.vertex_count), int(out.index_count)
+        return Mesh(verts[:nv].copy(), idx[:ni].copy())
+
+    def extract_lhmc(self, max_depth: int, vertex_capacity: int = 0,
+                     index_capacity: int = 0) -> Mesh:
+        """iso_extract_lhmc into host arrays (grown on ISO_ERR_CAPACITY)."""
+        vcap = max(int(vertex_capacity), 1)
+        icap = max(int(index_capacity), 3)
+        verts = np.empty((vcap, 3), dtype=np.float32)
+        idx = np.empty(icap, dtype=np.uint32)
+        out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
+                           A.ISO_MEMORY_HOST, 0)
+        st = self.lib.iso_extract_lhmc(self.handle, int(max_depth), C.byref(out))
+        if st == A.ISO_ERR_CAPACITY:
+            nv, ni = int(out.vertex_count), int(out.index_count)
+            verts = np.empty((max(nv, 1), 3), dtype=np.float32)
+            idx = np.empty(max(ni, 3), dtype=np.uint32)
+            out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, max(nv, 1), max(ni, 3), 0, 0,
+                               A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.handle, C.byref(out))
+        self.check(st, "iso_extract_lhmc")
+        nv, ni = int(out.vertex_count), int(out.index_count)
+        return Mesh(verts[:nv].copy(), idx[:ni].copy())
+
     # ---- timing
     def set_timing(self, on: bool) -> None:
         self.check(self.lib.iso_ctx_set_timing(self.handle, 1 if on else 0), "set_timing")
@@ -245,4 +292,66 @@ class MarchingCubes:
 
     def extract(self, source, extractor: Extractor) -> None:
         """MarchingCubes::extract (marching_cubes.rs:59-82)."""
+        _replay(self.extract_mesh(source), extractor)
+
+
+class MinimiseQEF:
+    """MinimiseQEF (feature/qef.rs:24-68): the feature point minimising the
+    quadratic error of the cell corners' tangent planes, by the reference's
+    f64 SVD with singular values below 0.1 dropped (and the smallest one on
+    an edge feature).  Evaluated on the GPU."""
+
+
+class DualContouring:
+    """DualContouring::new(size, MinimiseQEF {}) (dual_contouring.rs:48-54)
+    on the GPU."""
+
+    def __init__(self, size: int, place_feature: Optional[MinimiseQEF] = None,
+                 context: Optional[Context] = None):
+        if size < 2:
+            raise ValueError("size must be at least 2")
+        if place_feature is not None and not isinstance(place_feature, MinimiseQEF):
+            raise TypeError("only MinimiseQEF feature placement runs on the GPU")
+        self.size = int(size)
+        self.place_feature = place_feature or MinimiseQEF()
+        self.ctx = context or default_context()
+        self._last_counts = (0, 0)
+
+    def extract_mesh(self, source: CentralDifference) -> Mesh:
+        if not isinstance(source, CentralDifference):
+            raise TypeError("DualContouring needs a HermiteSource: wrap the program source "
+                            "in CentralDifference")
+        self.ctx.upload_program(source.program())
+        desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_PROGRAM)
+        mesh = self.ctx.extract_dc(desc, source.epsilon, *self._last_counts)
+        self._last_counts = (len(mesh.vertices), len(mesh.indices))
+        return mesh
+
+    def extract(self, source: CentralDifference, extractor: Extractor) -> None:
+        """DualContouring::extract (dual_contouring.rs:65-105)."""
+        _replay(self.extract_mesh(source), extractor)
+
+
+class LinearHashedMarchingCubes:
+    """LinearHashedMarchingCubes::new(max_depth)
+    (linear_hashed_marching_cubes.rs:51-56) on the GPU.  The source is
+    sampled over the unit cube at octree node centres; max_depth <= 9."""
+
+    def __init__(self, max_depth: int, context: Optional[Context] = None):
+        if not 0 <= int(max_depth) <= 9:
+            raise ValueError("max_depth must be in [0, 9]")
+        self.max_depth = int(max_depth)
+        self.ctx = context or default_context()
+        self._last_counts = (0, 0)
+
+    def extract_mesh(self, source: Source) -> Mesh:
+        if not isinstance(source, Source):
+            raise TypeError("LinearHashedMarchingCubes evaluates program sources on the GPU")
+        self.ctx.upload_program(source.program())
+        mesh = self.ctx.extract_lhmc(self.max_depth, *self._last_counts)
+        self._last_counts = (len(mesh.vertices), len(mesh.indices))
+        return mesh
+
+    def extract(self, source: Source, extractor: Extractor) -> None:
+        """LinearHashedMarchingCubes::extract (linear_hashed_marching_cubes.rs:62-84)."""
         _replay(self.extract_mesh(source), extractor)

@@ -124,6 +124,21 @@ class Translate(Source):
         out.append((A.ISO_OP_END_TRANSLATE, ()))
 
 
+class CentralDifference:
+    """CentralDifference::new / new_with_epsilon (source.rs:52-94): a
+    HermiteSource whose normals are central differences of `source`
+    (step epsilon, default 1e-6), as dual contouring needs."""
+
+    def __init__(self, source: Source, epsilon: float = 0.000001):
+        if not isinstance(source, Source):
+            raise TypeError("CentralDifference wraps a program source")
+        self.source = source
+        self.epsilon = float(epsilon)
+
+    def program(self) -> List[Node]:
+        return self.source.program()
+
+
 def DemoSource(source: Source) -> Source:
     """The examples' recentring wrapper: q = p - Vec3::from_scalar(0.5)."""
     return Translate((0.5, 0.5, 0.5), source)

@@ -63,6 +63,23 @@ int or_svd_solve(const double* a, const double* b, int rows, int drop_min, doubl
 /* The dual vertex of a mixed cell from its corners and corner normals. */
 void or_place_feature(const float* corners, const float* normals, float* out);
 
+/* LinearHashedMarchingCubes::new(max_depth).extract with IndexedVertices.
+ * Primal vertices are visited in ascending Morton order (order_seed == 0) or
+ * in a seeded shuffle of it; edge_keys holds (a, b, 0, 0) MortonKeys. */
+int or_extract_lhmc(const iso_node* prog, uint32_t n, int max_depth, uint64_t order_seed,
+                    or_mesh* out);
+/* The octree alone: node keys in BFS order, values, leaf flags. */
+int or_lhmc_octree(const iso_node* prog, uint32_t n, int max_depth, uint64_t capacity,
+                   uint64_t* keys, float* values, uint8_t* is_leaf, uint64_t* n_nodes);
+/* Morton helpers (morton.rs), for direct checks. */
+int or_morton_level(uint64_t k);
+void or_morton_center(uint64_t k, float* out);
+float or_morton_size(uint64_t k);
+uint64_t or_morton_add(uint64_t a, uint64_t b);
+uint64_t or_morton_sub(uint64_t a, uint64_t b);
+uint64_t or_morton_primal_vertex(uint64_t k, int level, unsigned which);
+uint64_t or_morton_dual_vertex(uint64_t k, int level, unsigned which);
+
 void or_mesh_free(or_mesh* m);
 
 /* Decoded tables, for checksum tests. */

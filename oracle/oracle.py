@@ -245,3 +245,82 @@ def place_feature(corners: np.ndarray, normals: np.ndarray) -> np.ndarray:
     out = np.empty(3, np.float32)
     lib.or_place_feature(c.ctypes.data, n.ctypes.data, out.ctypes.data)
     return out
+
+
+def _lhmc_bind(lib):
+    if getattr(lib, "_lhmc_bound", False):
+        return
+    lib.or_extract_lhmc.restype = C.c_int
+    lib.or_extract_lhmc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_int, C.c_uint64,
+                                    C.POINTER(OrMesh)]
+    lib.or_lhmc_octree.restype = C.c_int
+    lib.or_lhmc_octree.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_int, C.c_uint64,
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]
+    lib.or_morton_level.restype = C.c_int
+    lib.or_morton_level.argtypes = [C.c_uint64]
+    lib.or_morton_center.restype = None
+    lib.or_morton_center.argtypes = [C.c_uint64, C.c_void_p]
+    lib.or_morton_size.restype = C.c_float
+    lib.or_morton_size.argtypes = [C.c_uint64]
+    for name in ("or_morton_add", "or_morton_sub"):
+        getattr(lib, name).restype = C.c_uint64
+        getattr(lib, name).argtypes = [C.c_uint64, C.c_uint64]
+    for name in ("or_morton_primal_vertex", "or_morton_dual_vertex"):
+        getattr(lib, name).restype = C.c_uint64
+        getattr(lib, name).argtypes = [C.c_uint64, C.c_int, C.c_uint]
+    lib._lhmc_bound = True
+
+
+def morton():
+    """The oracle's Morton helpers (morton.rs)."""
+    lib = load()
+    _lhmc_bind(lib)
+    return lib
+
+
+def morton_center(k: int) -> np.ndarray:
+    lib = morton()
+    out = np.empty(3, np.float32)
+    lib.or_morton_center(k, out.ctypes.data)
+    return out
+
+
+def extract_lhmc(program, max_depth: int, order_seed: int = 0) -> OracleMesh:
+    """LinearHashedMarchingCubes::new(max_depth).extract, restated; primal
+    vertices visited in ascending Morton order (seed 0) or a seeded shuffle."""
+    lib = load()
+    _lhmc_bind(lib)
+    arr = nodes(program)
+    m = OrMesh()
+    st = lib.or_extract_lhmc(arr, len(program), max_depth, order_seed, C.byref(m))
+    assert st == 0, st
+    try:
+        nv, ni = int(m.n_vertices), int(m.n_indices)
+        v = np.ctypeslib.as_array(m.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else \
+            np.zeros((0, 3), np.float32)
+        i = np.ctypeslib.as_array(m.indices, shape=(ni,)).copy() if ni else np.zeros(0, np.uint32)
+        k = np.ctypeslib.as_array(m.edge_keys, shape=(nv * 4,)).reshape(nv, 4).copy() if nv else \
+            np.zeros((0, 4), np.int64)
+    finally:
+        lib.or_mesh_free(C.byref(m))
+    return OracleMesh(v, i, k[:, :2].astype(np.uint64), None)
+
+
+def lhmc_octree(program, max_depth: int):
+    """(keys in BFS order, values, leaf flags) of the implicit octree."""
+    lib = load()
+    _lhmc_bind(lib)
+    arr = nodes(program)
+    n = C.c_uint64(0)
+    cap = 1
+    while True:
+        keys = np.empty(cap, np.uint64)
+        vals = np.empty(cap, np.float32)
+        leaf = np.empty(cap, np.uint8)
+        st = lib.or_lhmc_octree(arr, len(program), max_depth, cap, keys.ctypes.data,
+                                vals.ctypes.data, leaf.ctypes.data, C.byref(n))
+        if st == 0:
+            k = int(n.value)
+            return keys[:k], vals[:k], leaf[:k].astype(bool)
+        assert st == 2, st  # ISO_ERR_CAPACITY
+        cap = int(n.value)

@@ -1,0 +1,303 @@
+// CPU oracle, linear-hashed octree marching cubes — TEST INFRASTRUCTURE ONLY
+// (see oracle.h).
+//
+// Restates LinearHashedMarchingCubes::extract
+// (linear_hashed_marching_cubes.rs:62-84) over ImplicitOctree::traverse
+// (traversal/implicit_octree.rs:39-99), LinearHashedOctree
+// (linear_hashed_octree.rs:30-65), Morton (morton.rs:17-180) and MortonKey
+// (index_cache.rs:63-73), including the level() quirk of the truncated
+// 1/(3 ln 2) constant: level() is computed with the same libm log.
+//
+// The reference iterates its primal vertices out of a std HashMap, whose
+// order is random per process.  Here that order is a parameter: ascending
+// Morton key (order_seed == 0) or a seeded shuffle of it.  Every order is
+// one the reference can produce.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <random>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../isosurface_amd/csrc/mc_tables.h"
+#include "oracle.h"
+#include "oracle_internal.h"
+
+namespace {
+
+using orc::F3;
+
+// morton.rs:17-32
+const uint64_t kDilateMask2 = 0x7FC0000FF80001FFull;
+const uint64_t kDilateMask1 = 0x01C0E070381C0E07ull;
+const uint64_t kDilateMask0 = 0x9249249249249249ull;
+const uint64_t kTz = 0x4924924924924924ull;
+const uint64_t kTy = 0x2492492492492492ull;
+const uint64_t kTx = 0x9249249249249249ull;
+const uint64_t kT1 = 0xB6DB6DB6DB6DB6DBull;
+const uint64_t kT2 = 0xDB6DB6DB6DB6DB6Dull;
+const uint64_t kT3 = 0x6DB6DB6DB6DB6DB6ull;
+const double kLg2_3 = 0.48089834696;
+const int kMaxLevel = (8 * 8 - 1) / 3;
+const float kSqrtOf3 = 1.73205080757f;  // distance.rs:18
+const int kRemapCube[8] = {2, 3, 1, 0, 6, 7, 5, 4};  // marching_cubes_tables.rs:29
+
+int level(uint64_t k) {  // morton.rs:53-59
+  if (k == 0) return 0;
+  return static_cast<int>(std::floor(std::log(static_cast<double>(k)) * kLg2_3));
+}
+uint64_t parent(uint64_t k) { return std::max<uint64_t>(k >> 3, 1); }          // :62-64
+uint64_t child(uint64_t k, unsigned w) { return (k << 3) | w; }                  // :67-69
+float size(uint64_t k) { return 1.0f / static_cast<float>(2ull << level(k)); }  // :73-75
+
+F3 center(uint64_t k) {  // morton.rs:78-116
+  uint64_t bz = (k >> 2) & kDilateMask0;
+  uint64_t by = (k >> 1) & kDilateMask0;
+  uint64_t bx = k & kDilateMask0;
+  const int lv = level(k);
+  if (lv > 1) {
+    bz = (bz | (bz >> 2) | (bz >> 4)) & kDilateMask1;
+    by = (by | (by >> 2) | (by >> 4)) & kDilateMask1;
+    bx = (bx | (bx >> 2) | (bx >> 4)) & kDilateMask1;
+    if (lv > 3) {
+      bz = (bz | (bz >> 6) | (bz >> 12)) & kDilateMask2;
+      by = (by | (by >> 6) | (by >> 12)) & kDilateMask2;
+      bx = (bx | (bx >> 6) | (bx >> 12)) & kDilateMask2;
+      if (lv > 9) {
+        bz = bz | (bz >> 18) | (bz >> 36);
+        by = by | (by >> 18) | (by >> 36);
+        bx = bx | (bx >> 18) | (bx >> 36);
+      }
+    }
+  }
+  const uint64_t length_mask = (1ull << lv) - 1;
+  bz &= length_mask;
+  by &= length_mask;
+  bx &= length_mask;
+  const float s = size(k);
+  const float s2 = s * 2.0f;
+  return {static_cast<float>(bx) * s2 + s, static_cast<float>(by) * s2 + s,
+          static_cast<float>(bz) * s2 + s};
+}
+
+uint64_t add(uint64_t a, uint64_t b) {  // morton.rs:153-163
+  return (((a | kT1) + (b & kTz)) & kTz) | (((a | kT2) + (b & kTy)) & kTy) |
+         (((a | kT3) + (b & kTx)) & kTx);
+}
+
+uint64_t sub(uint64_t a, uint64_t b) {  // morton.rs:165-175
+  return (((a & kTz) - (b & kTz)) & kTz) | (((a & kTy) - (b & kTy)) & kTy) |
+         (((a & kTx) - (b & kTx)) & kTx);
+}
+
+uint64_t primal_vertex(uint64_t key, int lv, unsigned which) {  // morton.rs:120-138
+  const uint64_t k = 1ull << (3 * lv);
+  const uint64_t k_plus_one = k << 1;
+  const uint64_t vk = add(key, which);
+  const uint64_t dk = sub(vk, k);
+  if (vk >= k_plus_one || (dk & kTx) == 0 || (dk & kTy) == 0 || (dk & kTz) == 0) return 0;
+  return vk << (3 * (kMaxLevel - lv));
+}
+
+uint64_t dual_vertex(uint64_t key, int lv, unsigned which) {  // morton.rs:142-146
+  return sub(key >> (3 * (kMaxLevel - lv)), which);
+}
+
+struct Octree {
+  std::vector<uint64_t> order;  // BFS order of construction
+  std::unordered_map<uint64_t, float> nodes;
+  std::vector<uint64_t> leaves;
+};
+
+// LinearHashedOctree::build (linear_hashed_octree.rs:30-51) with the
+// ImplicitOctree refinement rule (implicit_octree.rs:47-59).
+void build(const orc::Program& p, int max_depth, Octree* t) {
+  std::deque<uint64_t> queue;
+  queue.push_back(1);
+  while (!queue.empty()) {
+    const uint64_t key = queue.front();
+    queue.pop_front();
+    const float node = orc::sample(p, center(key));
+    const int lv = level(key);
+    const float sz = size(key);
+    const bool refine = lv < 2 || (lv < max_depth && std::fabs(node) < sz * kSqrtOf3);
+    if (refine) {
+      for (unsigned i = 0; i < 8; ++i) queue.push_back(child(key, i));
+    } else {
+      t->leaves.push_back(key);
+    }
+    t->nodes[key] = node;
+    t->order.push_back(key);
+  }
+}
+
+struct EdgeKey {
+  uint64_t a, b;
+  bool operator==(const EdgeKey& o) const { return a == o.a && b == o.b; }
+};
+struct EdgeKeyHash {
+  size_t operator()(const EdgeKey& k) const {
+    uint64_t h = k.a * 0x9e3779b97f4a7c15ull ^ (k.b + 0x632be59bd9b4e019ull + (k.a << 6));
+    h ^= h >> 31;
+    return static_cast<size_t>(h * 0xbf58476d1ce4e5b9ull);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int or_morton_level(uint64_t k) { return level(k); }
+void or_morton_center(uint64_t k, float* out) {
+  const F3 c = center(k);
+  out[0] = c.x;
+  out[1] = c.y;
+  out[2] = c.z;
+}
+float or_morton_size(uint64_t k) { return size(k); }
+uint64_t or_morton_add(uint64_t a, uint64_t b) { return add(a, b); }
+uint64_t or_morton_sub(uint64_t a, uint64_t b) { return sub(a, b); }
+uint64_t or_morton_primal_vertex(uint64_t k, int lv, unsigned which) {
+  return primal_vertex(k, lv, which);
+}
+uint64_t or_morton_dual_vertex(uint64_t k, int lv, unsigned which) {
+  return dual_vertex(k, lv, which);
+}
+
+// The octree alone: keys in BFS order with values and leaf flags.
+int or_lhmc_octree(const iso_node* prog, uint32_t n, int max_depth, uint64_t capacity,
+                   uint64_t* keys, float* values, uint8_t* is_leaf, uint64_t* n_nodes) {
+  orc::Program p;
+  if (!orc::load_program(prog, n, &p)) return ISO_ERR_PROGRAM;
+  if (max_depth < 0 || max_depth > 10) return ISO_ERR_INVALID_ARGUMENT;
+  Octree t;
+  build(p, max_depth, &t);
+  *n_nodes = t.order.size();
+  if (t.order.size() > capacity) return ISO_ERR_CAPACITY;
+  std::unordered_map<uint64_t, bool> leaf;
+  for (uint64_t k : t.leaves) leaf[k] = true;
+  for (size_t i = 0; i < t.order.size(); ++i) {
+    keys[i] = t.order[i];
+    values[i] = t.nodes[t.order[i]];
+    is_leaf[i] = leaf.count(t.order[i]) ? 1 : 0;
+  }
+  return ISO_OK;
+}
+
+// LinearHashedMarchingCubes::new(max_depth).extract + IndexedVertices.
+// edge_keys receives (a, b, 0, 0) per vertex: the MortonKey of its edge.
+int or_extract_lhmc(const iso_node* prog, uint32_t n, int max_depth, uint64_t order_seed,
+                    or_mesh* out) {
+  std::memset(out, 0, sizeof(*out));
+  orc::Program p;
+  if (!orc::load_program(prog, n, &p)) return ISO_ERR_PROGRAM;
+  if (max_depth < 0 || max_depth > 10) return ISO_ERR_INVALID_ARGUMENT;
+  Octree t;
+  build(p, max_depth, &t);
+
+  // walk_leaves: primal vertices with the deepest touching leaf level
+  // (implicit_octree.rs:61-79).
+  std::map<uint64_t, int> primal;
+  for (uint64_t key : t.leaves) {
+    const int lv = level(key);
+    for (unsigned i = 0; i < 8; ++i) {
+      const uint64_t v = primal_vertex(key, lv, i);
+      if (v != 0) {
+        auto it = primal.find(v);
+        if (it == primal.end()) {
+          primal.emplace(v, lv);
+        } else if (lv > it->second) {
+          it->second = lv;
+        }
+      }
+    }
+  }
+  std::vector<std::pair<uint64_t, int>> seq(primal.begin(), primal.end());
+  if (order_seed != 0) {
+    std::mt19937_64 rng(order_seed);
+    std::shuffle(seq.begin(), seq.end(), rng);
+  }
+
+  // Dual cubes (implicit_octree.rs:81-98); keys/corners/values carry over
+  // between cubes exactly as the reference's reused arrays do.
+  uint64_t keys[8];
+  F3 corners[8];
+  float values[8];
+  for (int i = 0; i < 8; ++i) {
+    keys[i] = 1;
+    corners[i] = {0.0f, 0.0f, 0.0f};
+    values[i] = 0.0f;
+  }
+  std::unordered_map<EdgeKey, uint32_t, EdgeKeyHash> cache;
+  std::vector<float> verts;
+  std::vector<int64_t> ekeys;
+  std::vector<uint32_t> faces;
+  using iso::tables::kEdgeCorners;
+  using iso::tables::kTables;
+  for (const auto& pv : seq) {
+    for (int i = 0; i < 8; ++i) {
+      uint64_t m = dual_vertex(pv.first, pv.second, kRemapCube[i]);
+      while (m > 1) {
+        auto it = t.nodes.find(m);
+        if (it != t.nodes.end()) {
+          keys[i] = m;
+          corners[i] = center(m);
+          values[i] = it->second;
+          break;
+        }
+        m = parent(m);
+      }
+    }
+    // linear_hashed_marching_cubes.rs:71-81
+    const int cube = orc::classify(values);
+    F3 vx[12];
+    const uint16_t mask = kTables.edge_mask[cube];
+    for (int e = 0; e < 12; ++e) {
+      if (mask & (1u << e)) {
+        const int u = kEdgeCorners[e][0], v = kEdgeCorners[e][1];
+        vx[e] = orc::find_crossing_point(values[u], values[v], corners[u], corners[v]);
+      }
+    }
+    for (int tri = 0; tri < 5; ++tri) {
+      const int8_t* te = &kTables.tri[cube][3 * tri];
+      if (te[0] < 0) break;
+      uint32_t ids[3];
+      for (int j = 0; j < 3; ++j) {
+        const int e = te[j];
+        uint64_t a = keys[kEdgeCorners[e][0]], b = keys[kEdgeCorners[e][1]];
+        if (a > b) std::swap(a, b);
+        const EdgeKey ek{a, b};
+        auto it = cache.find(ek);
+        if (it != cache.end()) {
+          ids[j] = it->second;
+        } else {
+          const uint32_t id = static_cast<uint32_t>(verts.size() / 3);
+          cache.emplace(ek, id);
+          verts.push_back(vx[e].x);
+          verts.push_back(vx[e].y);
+          verts.push_back(vx[e].z);
+          ekeys.push_back(static_cast<int64_t>(a));
+          ekeys.push_back(static_cast<int64_t>(b));
+          ekeys.push_back(0);
+          ekeys.push_back(0);
+          ids[j] = id;
+        }
+      }
+      faces.push_back(ids[0]);
+      faces.push_back(ids[1]);
+      faces.push_back(ids[2]);
+    }
+  }
+  out->vertices = orc::dup(verts);
+  out->indices = orc::dup(faces);
+  out->edge_keys = orc::dup(ekeys);
+  out->n_vertices = verts.size() / 3;
+  out->n_indices = faces.size();
+  return ISO_OK;
+}
+
+}  // extern "C"

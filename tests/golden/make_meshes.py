@@ -1,5 +1,5 @@
 """Writes mesh_digests.json: counts and SHA-256 digests of the oracle's
-marching-cubes meshes for the benchmark workloads.
+marching-cubes and dual-contouring meshes for the benchmark workloads.
 
 The oracle (oracle/, a restatement of the reference's CPU extraction) is run
 here once; the GPU tests compare their output digests with these values, so
@@ -30,9 +30,24 @@ def main() -> None:
         "spheres64_minus_torus_size129": (W.spheres_minus_torus()[0], 129),
         "spheres64_minus_torus_size257": (W.spheres_minus_torus()[0], 257),
     }
+    dc_cases = {
+        "dc_sphere_r0.35_size33": (W.sphere_source(), 33),
+        "dc_boxes32_size65": (W.boxes_union(32, seed=4), 65),
+        "dc_boxes32_size128": (W.boxes_union(32, seed=4), 128),
+        "dc_spheres64_minus_torus_size65": (W.spheres_minus_torus()[0], 65),
+    }
     out = {}
-    for name, (src, size) in cases.items():
-        m = O.extract_mc(src.program(), desc=O.reference_desc(size))
+    meshes = [(name, size, O.extract_mc(src.program(), desc=O.reference_desc(size)))
+              for name, (src, size) in cases.items()]
+    meshes += [(name, size, O.extract_dc(src.program(), O.reference_desc(size), 1e-6))
+               for name, (src, size) in dc_cases.items()]
+    lhmc_cases = {
+        "lhmc_boxes32_depth7": (W.boxes_union(32, seed=4), 7),
+        "lhmc_sphere_r0.35_depth8": (W.sphere_source(), 8),
+    }
+    meshes += [(name, depth, O.extract_lhmc(src.program(), depth, 0))
+               for name, (src, depth) in lhmc_cases.items()]
+    for name, size, m in meshes:
         out[name] = {
             "size": size,
             "vertices": int(len(m.vertices)),
