@@ -19,6 +19,15 @@ Printed JSON line:
                 (4 B per sample + 12 B per vertex + 12 B per triangle)
   program_path  the same extraction sampling the CSG program on the GPU
                 instead of reading the grid (VALU-bound, reported aside)
+  batched_chunks
+                4096 chunks of 16^3..64^3 cells, each its own seeded 1-8
+                primitive CSG program, extracted by one captured HIP graph
+                per batch after the autotuner picked the launch shapes
+  dual_contouring, octree_mc
+                the sharp-feature workload (union of 32 seeded boxes):
+                DualContouring at 128^3 with central-difference normals and
+                the f64 QEF, and LinearHashedMarchingCubes at max_depth 7;
+                each checked against the oracle's golden digests
   cpu_baseline  the oracle (a single-threaded restatement of the reference
                 CPU path) on the same lattice, median of a few runs
 """
@@ -136,6 +145,123 @@ def cpu_baseline(grid: np.ndarray, size: int, budget_s: float) -> dict:
     }
 
 
+def side_workloads(ctx: Context, steps: int, with_cpu: bool) -> dict:
+    """Dual contouring at 128^3 and octree MC at depth 7 on the 32-box union."""
+    golden = json.loads((ROOT / "tests" / "golden" / "mesh_digests.json").read_text())
+    src = W.boxes_union(32, seed=4)
+    ctx.upload_program(src.program())
+    res = {}
+
+    def run(name, call, key, cells, oracle_call):
+        probe = DeviceMesh(ctx, 1, 3)
+        out = probe.out()
+        st = call(out)
+        if st not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+            ctx.check(st, name)
+        nv, ni = int(out.vertex_count), int(out.index_count)
+        probe.free()
+        mesh = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
+        out = mesh.out()
+        ctx.check(call(out), name)
+        dv, di = digest(ctx, mesh, out)
+        parity = dv == golden[key]["vertices_sha256"] and di == golden[key]["indices_sha256"]
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.check(call(mesh.out()), name)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        ctx.reset_kernel_stats()
+        ctx.set_timing(True)
+        ctx.check(call(mesh.out()), name)
+        ctx.synchronize()
+        ctx.set_timing(False)
+        ks = {k: round(1e3 * ms / max(n, 1), 2) for k, (n, ms) in ctx.kernel_stats().items()}
+        ctx.reset_kernel_stats()
+        mesh.free()
+        entry = {"ms_per_extraction": 1e3 * dt, "cells_per_s": cells / dt, "vertices": nv,
+                 "triangles": ni // 3, "parity_vs_oracle": parity, "kernels_us": ks}
+        if with_cpu:
+            t0 = time.perf_counter()
+            oracle_call()
+            entry["cpu_oracle_s"] = time.perf_counter() - t0
+        log(f"{name}: {1e3 * dt:.3f} ms, parity {parity}")
+        return entry
+
+    from_oracle = None
+    if with_cpu:
+        from oracle import oracle as O
+        from_oracle = O
+    dc_desc = ctx.reference_desc(128, A.ISO_SOURCE_PROGRAM)
+    res["dual_contouring"] = run(
+        "iso_extract_dc",
+        lambda out: ctx.lib.iso_extract_dc(ctx.handle, C.byref(dc_desc), C.c_float(1e-6),
+                                           C.byref(out)),
+        "dc_boxes32_size128", 127 * 127 * 128,
+        lambda: from_oracle.extract_dc(src.program(), from_oracle.reference_desc(128), 1e-6))
+    res["dual_contouring"]["workload"] = (
+        "DualContouring::new(128, MinimiseQEF) on CentralDifference(union of 32 seeded boxes), "
+        "128x128x129 primal samples, f64 QEF per mixed cell")
+    res["octree_mc"] = run(
+        "iso_extract_lhmc",
+        lambda out: ctx.lib.iso_extract_lhmc(ctx.handle, 7, C.byref(out)),
+        "lhmc_boxes32_depth7", 128 ** 3,
+        lambda: from_oracle.extract_lhmc(src.program(), 7, 0))
+    res["octree_mc"]["workload"] = ("LinearHashedMarchingCubes::new(7) on the union of 32 "
+                                    "seeded boxes (cells/s counted against the 128^3 grid)")
+    return res
+
+
+def batched_chunks(ctx: Context, steps: int) -> dict:
+    """The batched-chunk workload: 4096 chunks, one graph replay per batch."""
+    from isosurface_amd.engine import chunk_desc
+    golden = json.loads((ROOT / "tests" / "golden" / "mesh_digests.json").read_text())
+    g = golden["batch4096_seed6"]
+    work = W.batch_chunks(4096, seed=6)
+    cells = sum(size ** 3 for _, size in work)
+    ctx.upload_batch([(src, chunk_desc(size)) for src, size in work])
+    n = len(work)
+    nv, ni = g["vertices"], 3 * g["triangles"]
+    v, i = C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 12 * nv, C.byref(v)), "alloc")
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 4 * ni, C.byref(i)), "alloc")
+    voff = np.zeros(n + 1, np.uint64)
+    ioff = np.zeros(n + 1, np.uint64)
+    flags = A.ISO_BATCH_GRAPH | A.ISO_BATCH_AUTOTUNE
+
+    def call():
+        out = A.IsoBatchOut(v, i, nv, ni, voff.ctypes.data, ioff.ctypes.data, 0, 0,
+                            A.ISO_MEMORY_DEVICE, flags)
+        ctx.check(ctx.lib.iso_extract_batch(ctx.handle, C.byref(out)), "iso_extract_batch")
+        return out
+
+    t0 = time.perf_counter()
+    out = call()  # eager pass, autotuning, graph capture
+    first = time.perf_counter() - t0
+    call()  # first replay
+    vv = np.empty((int(out.vertex_count), 3), np.float32)
+    ii = np.empty(int(out.index_count), np.uint32)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, vv.ctypes.data, v, vv.nbytes)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, ii.ctypes.data, i, ii.nbytes)
+    parity = (hashlib.sha256(vv.astype("<f4").tobytes()).hexdigest() == g["vertices_sha256"] and
+              hashlib.sha256(ii.astype("<u4").tobytes()).hexdigest() == g["indices_sha256"])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        call()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.lib.iso_device_free(ctx.handle, v)
+    ctx.lib.iso_device_free(ctx.handle, i)
+    log(f"batched chunks: {1e3 * dt:.3f} ms per batch, parity {parity}")
+    return {"workload": ("4096 chunks, S^3 cells with S uniform in {16,32,48,64}, each a seeded "
+                         "1-8 sphere/box/torus CSG program (seed 6); one HIP graph replay per "
+                         "batch, launch shapes autotuned on the first extraction"),
+            "cells": cells, "ms_per_batch": 1e3 * dt, "cells_per_s": cells / dt,
+            "first_call_ms": 1e3 * first, "vertices": int(out.vertex_count),
+            "triangles": int(out.index_count) // 3, "parity_vs_oracle": parity}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +271,8 @@ def main() -> int:
     ap.add_argument("--program-steps", type=int, default=5)
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-side", action="store_true",
+                    help="skip the dual-contouring and octree workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the extraction path has no multi-GPU mode "
@@ -260,6 +388,10 @@ def main() -> int:
         },
         "program_path": program_path,
     }
+
+    if not args.no_side:
+        result["batched_chunks"] = batched_chunks(ctx, max(2, args.program_steps))
+        result.update(side_workloads(ctx, max(1, args.program_steps), not args.no_cpu_baseline))
 
     if not args.no_cpu_baseline:
         host_grid = np.empty((desc.nz, desc.ny, desc.nx), np.float32)

@@ -189,6 +189,47 @@ int iso_extract_dc(iso_ctx* ctx, const iso_grid_desc* desc, float epsilon, iso_m
  * max_depth in [0, 9]. */
 int iso_extract_lhmc(iso_ctx* ctx, int32_t max_depth, iso_mesh_out* out);
 
+/* ---- batched chunks ---------------------------------------------------------
+ * Many independent chunks (each its own lattice and program, e.g. the chunks
+ * of a paged or LOD terrain) extracted in one pass: one sign pass, one count
+ * scan and one emit pass over all chunks.  No reference counterpart as a
+ * single call: each chunk's mesh is exactly what iso_extract_mc gives for
+ * that chunk's lattice and program (MarchingCubes::extract with an
+ * IndexedVertices extractor per chunk); the batch only shares the launches. */
+typedef struct iso_chunk_desc {
+  iso_grid_desc grid;   /* the chunk's lattice; grid.source must be ISO_SOURCE_PROGRAM */
+  uint32_t node_offset; /* the chunk's program: nodes[node_offset, node_offset + node_count) */
+  uint32_t node_count;
+} iso_chunk_desc;
+
+/* Validates and lowers every chunk's program and lays the batch out on the
+ * device; replaces any previous batch of the context. */
+int iso_batch_upload(iso_ctx* ctx, const iso_node* nodes, uint64_t n_nodes,
+                     const iso_chunk_desc* chunks, uint32_t n_chunks);
+
+#define ISO_BATCH_GRAPH 1u    /* capture the extraction in a HIP graph, replay it on later calls */
+#define ISO_BATCH_AUTOTUNE 2u /* on the first extraction of a batch, time the launch shapes
+                                 (count tile, sign words per wavefront, emit residency) and
+                                 keep the fastest */
+
+typedef struct iso_batch_out {
+  float* vertices;           /* all chunks' vertices, chunk after chunk */
+  uint32_t* indices;         /* all chunks' index lists, chunk after chunk, chunk-local ids */
+  uint64_t vertex_capacity;  /* in vertices */
+  uint64_t index_capacity;   /* in indices */
+  uint64_t* vertex_offsets;  /* host, n_chunks + 1 (out): chunk c owns [off[c], off[c+1]) */
+  uint64_t* index_offsets;   /* host, n_chunks + 1 (out) */
+  uint64_t vertex_count;     /* out */
+  uint64_t index_count;      /* out */
+  int32_t memory;            /* ISO_MEMORY_* of vertices / indices */
+  uint32_t flags;            /* ISO_BATCH_* */
+} iso_batch_out;
+
+/* Marching cubes over every chunk of the uploaded batch.  ISO_ERR_CAPACITY
+ * when the outputs are too small (counts and offsets are still written;
+ * host-memory results are kept for iso_mesh_fetch). */
+int iso_extract_batch(iso_ctx* ctx, iso_batch_out* out);
+
 /* After an ISO_MEMORY_HOST extraction returned ISO_ERR_CAPACITY, the mesh is
  * kept in the context; this copies it into larger buffers without
  * re-extracting (the reference's Vec-growing sink needs no second pass). */

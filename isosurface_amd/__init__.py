@@ -8,16 +8,16 @@ Source / Extractor / MarchingCubes / DualContouring /
 LinearHashedMarchingCubes API; the C++ mirror for compiled hosts is
 isosurface_amd/host/isosurface.hpp.
 """
-from .engine import (Context, DualContouring, Extractor, IndexedVertices,
+from .engine import (BatchedMarchingCubes, Context, DualContouring, Extractor, IndexedVertices,
                      LinearHashedMarchingCubes, MarchingCubes, Mesh, MinimiseQEF, OnlyVertices,
-                     default_context, sample_lattice_host)
+                     chunk_desc, default_context, sample_lattice_host)
 from .sources import (CentralDifference, Cylinder, DemoSource, Difference, Intersection,
                       RectangularPrism,
                       SmoothUnion, Source, Sphere, Torus, Translate, Union, smooth_union_all,
                       to_nodes, union_all)
 
 __all__ = [
-    "CentralDifference", "DualContouring", "LinearHashedMarchingCubes", "MinimiseQEF",
+    "BatchedMarchingCubes", "chunk_desc", "CentralDifference", "DualContouring", "LinearHashedMarchingCubes", "MinimiseQEF",
     "Context", "Extractor", "IndexedVertices", "MarchingCubes", "Mesh", "OnlyVertices",
     "default_context", "sample_lattice_host", "Cylinder", "DemoSource", "Difference",
     "Intersection", "RectangularPrism", "SmoothUnion", "Source", "Sphere", "Torus",

@@ -32,6 +32,8 @@ ISO_SOURCE_PROGRAM = 0
 ISO_SOURCE_GRID = 1
 ISO_MEMORY_HOST = 0
 ISO_MEMORY_DEVICE = 1
+ISO_BATCH_GRAPH = 1
+ISO_BATCH_AUTOTUNE = 2
 
 
 class IsoNode(C.Structure):
@@ -63,6 +65,25 @@ class IsoMeshOut(C.Structure):
     ]
 
 
+class IsoChunkDesc(C.Structure):
+    _fields_ = [("grid", IsoGridDesc), ("node_offset", C.c_uint32), ("node_count", C.c_uint32)]
+
+
+class IsoBatchOut(C.Structure):
+    _fields_ = [
+        ("vertices", C.c_void_p),
+        ("indices", C.c_void_p),
+        ("vertex_capacity", C.c_uint64),
+        ("index_capacity", C.c_uint64),
+        ("vertex_offsets", C.c_void_p),
+        ("index_offsets", C.c_void_p),
+        ("vertex_count", C.c_uint64),
+        ("index_count", C.c_uint64),
+        ("memory", C.c_int32),
+        ("flags", C.c_uint32),
+    ]
+
+
 class IsoKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -87,6 +108,9 @@ SIGNATURES = [
     ("iso_extract_dc", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float,
                                  C.POINTER(IsoMeshOut)]),
     ("iso_extract_lhmc", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(IsoMeshOut)]),
+    ("iso_batch_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(IsoChunkDesc),
+                                   C.c_uint32]),
+    ("iso_extract_batch", C.c_int, [C.c_void_p, C.POINTER(IsoBatchOut)]),
     ("iso_mesh_fetch", C.c_int, [C.c_void_p, C.POINTER(IsoMeshOut)]),
     ("iso_device_alloc", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     ("iso_device_free", C.c_int, [C.c_void_p, C.c_void_p]),

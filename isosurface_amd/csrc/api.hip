@@ -49,14 +49,26 @@ struct iso_ctx {
   int64_t gnx = 0, gny = 0, gnz = 0;
 
   // scratch
-  DevBuf signs, seg, tplanes, flags, agg, incl, active, totals, vout, iout, pts, vals;
+  DevBuf signs, seg, tplanes, agg, incl, active, totals, vout, iout, pts, vals;
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
   // octree marching cubes: nodes, dense primal lattice, cubes, edge table, scan state
   DevBuf oc_keys, oc_values, oc_leaf, oc_dense, oc_primal, oc_plevel, oc_corner, oc_case,
-      oc_base, oc_tkeys, oc_towner, oc_tid, sc_flags, sc_agg, sc_incl, sc_ctl;
+      oc_base, oc_tkeys, oc_towner, oc_tid, sc_slot, sc_ctl;
   unsigned long long* h_stage = nullptr;  // pinned, 8 words
+
+  // batched chunks: lowered programs, chunk table, sign words, offsets
+  DevBuf b_ops, b_chunks, b_segstart, b_wordstart, b_signs, b_voff, b_ioff;
+  iso::BatchPlan plan{};
+  uint64_t b_cells = 0;
+  bool b_ready = false;
+  // launch shapes (chosen by the autotuner when asked)
+  int b_rounds = 1, b_words_per_wave = 1, b_emit_per_cu = 0;
+  bool b_tuned = false;
+  // captured extraction and the buffers it was captured with
+  hipGraphExec_t b_graph = nullptr;
+  iso::McBuffers g_buffers{};
   uint64_t active_capacity = 0;
   unsigned long long* h_totals = nullptr;  // pinned
 
@@ -232,13 +244,15 @@ int iso_ctx_destroy(iso_ctx* c) {
   collect_timing(c);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&c->prog,       &c->grid_owned, &c->signs,     &c->seg,      &c->tplanes,
-                    &c->flags,      &c->agg,        &c->incl,      &c->active,   &c->totals,
+                    &c->agg,        &c->incl,       &c->active,    &c->totals,
                     &c->vout,       &c->iout,       &c->pts,       &c->vals,     &c->dc_samples,
                     &c->dc_values,  &c->dc_points,  &c->dc_mixed,  &c->dc_counters,
                     &c->oc_keys,    &c->oc_values,  &c->oc_leaf,   &c->oc_dense, &c->oc_primal,
                     &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
-                    &c->oc_towner,  &c->oc_tid,     &c->sc_flags,  &c->sc_agg,   &c->sc_incl,
-                    &c->sc_ctl};
+                    &c->oc_towner,  &c->oc_tid,     &c->sc_slot,   &c->sc_ctl,
+                    &c->b_ops,      &c->b_chunks,   &c->b_segstart, &c->b_wordstart,
+                    &c->b_signs,    &c->b_voff,     &c->b_ioff};
+  if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
   if (c->h_dc) (void)hipHostFree(c->h_dc);
@@ -377,7 +391,6 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
   ISO_HIP(c, ensure(&c->signs, (D.n_sign_words + 3) * sizeof(uint64_t)));
   ISO_HIP(c, ensure(&c->seg, D.n_segs * sizeof(uint2)));
   ISO_HIP(c, ensure(&c->tplanes, 3 * D.n_segs * sizeof(uint64_t)));
-  ISO_HIP(c, ensure(&c->flags, D.n_blocks * sizeof(uint32_t)));
   ISO_HIP(c, ensure(&c->agg, D.n_blocks * sizeof(iso::ScanTriple)));
   ISO_HIP(c, ensure(&c->incl, D.n_blocks * sizeof(iso::ScanTriple)));
   ISO_HIP(c, ensure(&c->totals, iso::kTotSlots * sizeof(unsigned long long)));
@@ -393,7 +406,6 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
   B->signs = static_cast<uint64_t*>(c->signs.ptr) + 1;
   B->seg = static_cast<uint2*>(c->seg.ptr);
   B->tplanes = static_cast<uint64_t*>(c->tplanes.ptr);
-  B->flags = static_cast<uint32_t*>(c->flags.ptr);
   B->agg = static_cast<iso::ScanTriple*>(c->agg.ptr);
   B->incl = static_cast<iso::ScanTriple*>(c->incl.ptr);
   B->active = static_cast<uint32_t*>(c->active.ptr);
@@ -616,16 +628,13 @@ int grow_keep(iso_ctx* c, DevBuf* b, size_t bytes, size_t keep) {
   return ISO_OK;
 }
 
-// Scan state for scans of up to n items; err points at the status word.
+// Scan state for scans of up to n items at one item per thread; err points
+// at the status word.
 int prepare_scan(iso_ctx* c, uint64_t n, iso::scan::State* st) {
-  const size_t nb = static_cast<size_t>(iso::scan::blocks_for(n));
-  ISO_HIP(c, ensure(&c->sc_flags, nb * sizeof(uint32_t)));
-  ISO_HIP(c, ensure(&c->sc_agg, nb * sizeof(unsigned long long)));
-  ISO_HIP(c, ensure(&c->sc_incl, nb * sizeof(unsigned long long)));
+  const size_t nb = static_cast<size_t>(iso::scan::blocks_for(n, 1));
+  ISO_HIP(c, ensure(&c->sc_slot, nb * sizeof(unsigned long long)));
   ISO_HIP(c, ensure(&c->sc_ctl, 4 * sizeof(unsigned long long)));
-  st->flags = static_cast<uint32_t*>(c->sc_flags.ptr);
-  st->agg = static_cast<unsigned long long*>(c->sc_agg.ptr);
-  st->incl = static_cast<unsigned long long*>(c->sc_incl.ptr);
+  st->slot = static_cast<unsigned long long*>(c->sc_slot.ptr);
   st->ctl = static_cast<unsigned long long*>(c->sc_ctl.ptr);
   st->err = st->ctl + 2;
   return ISO_OK;
@@ -804,6 +813,317 @@ int iso_extract_lhmc(iso_ctx* c, int32_t max_depth, iso_mesh_out* out) {
     ISO_HIP(c, hipMemcpyAsync(out->indices, idst, ni * sizeof(uint32_t), hipMemcpyDeviceToHost,
                               c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
+}
+
+int iso_batch_upload(iso_ctx* c, const iso_node* nodes, uint64_t n_nodes,
+                     const iso_chunk_desc* chunks, uint32_t n_chunks) {
+  if (!c || !nodes || !chunks || n_chunks == 0)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "empty or null batch");
+  ISO_HIP(c, hipSetDevice(c->device));
+  c->b_ready = false;
+  c->b_tuned = false;
+  if (c->b_graph) {
+    (void)hipGraphExecDestroy(c->b_graph);
+    c->b_graph = nullptr;
+  }
+  std::vector<iso::DevOp> ops;
+  std::vector<uint32_t> op_off(n_chunks), seg_start(n_chunks + 1);
+  std::vector<uint64_t> word_start(n_chunks + 1), sign_off(n_chunks);
+  uint64_t segs = 0, words = 0, sign_words = 0, cells = 0;
+  for (uint32_t i = 0; i < n_chunks; ++i) {
+    const iso_chunk_desc& ch = chunks[i];
+    iso::Lattice L;
+    int st = check_lattice(c, &ch.grid, &L);
+    if (st != ISO_OK) return st;
+    if (ch.grid.source != ISO_SOURCE_PROGRAM)
+      return fail(c, ISO_ERR_INVALID_ARGUMENT, "batch chunks must name ISO_SOURCE_PROGRAM");
+    if (static_cast<uint64_t>(ch.node_offset) + ch.node_count > n_nodes || ch.node_count == 0)
+      return fail(c, ISO_ERR_INVALID_ARGUMENT,
+                  "chunk " + std::to_string(i) + ": program outside the node array");
+    std::vector<iso::DevOp> part;
+    std::string err;
+    st = iso::lower_program(nodes + ch.node_offset, ch.node_count, &part, &err);
+    if (st != ISO_OK) return fail(c, st, "chunk " + std::to_string(i) + ": " + err);
+    op_off[i] = static_cast<uint32_t>(ops.size());
+    ops.insert(ops.end(), part.begin(), part.end());
+    const iso::McDims D = iso::mc_dims(L);
+    seg_start[i] = static_cast<uint32_t>(segs);
+    word_start[i] = words;
+    sign_off[i] = sign_words + 1;  // one guard word before, two after
+    segs += static_cast<uint64_t>(D.n_segs);
+    words += static_cast<uint64_t>(D.n_sign_words);
+    sign_words += static_cast<uint64_t>(D.n_sign_words) + 3;
+    cells += static_cast<uint64_t>(D.n_cells);
+    if (segs >= (1ull << 26))
+      return fail(c, ISO_ERR_INVALID_ARGUMENT, "batch has 2^26 or more 64-cell segments");
+  }
+  seg_start[n_chunks] = static_cast<uint32_t>(segs);
+  word_start[n_chunks] = words;
+  ISO_HIP(c, ensure(&c->b_ops, ops.size() * sizeof(iso::DevOp)));
+  ISO_HIP(c, ensure(&c->b_signs, sign_words * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->b_chunks, n_chunks * sizeof(iso::CellSpace)));
+  ISO_HIP(c, ensure(&c->b_segstart, (n_chunks + 1) * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->b_wordstart, (n_chunks + 1) * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->b_voff, (n_chunks + 1) * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->b_ioff, (n_chunks + 1) * sizeof(uint64_t)));
+  std::vector<iso::CellSpace> table(n_chunks);
+  const iso::DevOp* dops = static_cast<const iso::DevOp*>(c->b_ops.ptr);
+  const uint64_t* dsigns = static_cast<const uint64_t*>(c->b_signs.ptr);
+  for (uint32_t i = 0; i < n_chunks; ++i) {
+    const iso_grid_desc& g = chunks[i].grid;
+    iso::CellSpace& C = table[i];
+    std::memset(&C, 0, sizeof(C));
+    C.nx = g.nx;
+    C.ny = g.ny;
+    C.nz = g.nz;
+    for (int k = 0; k < 3; ++k) {
+      C.origin[k] = g.origin[k];
+      C.spacing[k] = g.spacing[k];
+    }
+    C.signs = dsigns + sign_off[i];
+    C.segs_per_row = static_cast<uint32_t>((g.nx - 1 + 63) / 64);
+    C.seg_off = seg_start[i];
+    C.ops = dops + op_off[i];
+    C.n_ops = static_cast<int32_t>((i + 1 < n_chunks ? op_off[i + 1] : ops.size()) - op_off[i]);
+  }
+  ISO_HIP(c, hipMemcpyAsync(c->b_ops.ptr, ops.data(), ops.size() * sizeof(iso::DevOp),
+                            hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipMemcpyAsync(c->b_chunks.ptr, table.data(), n_chunks * sizeof(iso::CellSpace),
+                            hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipMemcpyAsync(c->b_segstart.ptr, seg_start.data(), (n_chunks + 1) * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipMemcpyAsync(c->b_wordstart.ptr, word_start.data(),
+                            (n_chunks + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  c->plan.chunks = static_cast<const iso::CellSpace*>(c->b_chunks.ptr);
+  c->plan.seg_start = static_cast<const uint32_t*>(c->b_segstart.ptr);
+  c->plan.word_start = static_cast<const uint64_t*>(c->b_wordstart.ptr);
+  c->plan.n_chunks = n_chunks;
+  c->plan.n_words = static_cast<int64_t>(words);
+  c->plan.n_segs = static_cast<int64_t>(segs);
+  c->b_cells = cells;
+  c->b_ready = true;
+  return ISO_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The batch's launches: signs (with state reset), count, emit (unless only
+// counting), offsets.
+hipError_t batch_launches(iso_ctx* c, const iso::McBuffers& B, bool emit = true) {
+  const iso::BatchPlan& P = c->plan;
+  hipError_t e = timed(c, "batch_signs", [&] {
+    return iso::launch_batch_signs(P, B, c->b_words_per_wave, c->stream);
+  });
+  if (e == hipSuccess)
+    e = timed(c, "batch_count", [&] { return iso::launch_batch_count(P, B, c->b_rounds, c->stream); });
+  if (e == hipSuccess && emit)
+    e = timed(c, "batch_emit", [&] {
+      return iso::launch_batch_emit(P, B, c->b_emit_per_cu, c->stream);
+    });
+  if (e == hipSuccess)
+    e = iso::launch_batch_offsets(P, B, static_cast<uint64_t*>(c->b_voff.ptr),
+                                  static_cast<uint64_t*>(c->b_ioff.ptr), c->stream);
+  return e;
+}
+
+// Times every launch shape on the batch (outputs already sized) and keeps
+// the fastest: count tile (segments per thread), sign words per wavefront,
+// emit workgroups per CU.
+int autotune_batch(iso_ctx* c, const iso::McBuffers& B) {
+  const bool timing = c->timing;
+  c->timing = false;
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return fail(c, ISO_ERR_DEVICE, "no events for the autotuner");
+  const int rounds[] = {1, 2};
+  const int wpw[] = {1, 2, 4};
+  const int per_cu[] = {0, 2, 4, 8};
+  float best = 1e30f;
+  int br = c->b_rounds, bw = c->b_words_per_wave, be = c->b_emit_per_cu;
+  auto measure = [&](float* ms) -> int {
+    float t = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      ISO_HIP(c, hipEventRecord(a, c->stream));
+      ISO_HIP(c, batch_launches(c, B));
+      ISO_HIP(c, hipEventRecord(b, c->stream));
+      ISO_HIP(c, hipEventSynchronize(b));
+      float x = 0.0f;
+      ISO_HIP(c, hipEventElapsedTime(&x, a, b));
+      if (x < t) t = x;
+    }
+    *ms = t;
+    return ISO_OK;
+  };
+  // Coordinate search: each shape in turn with the others at their best.
+  int st = ISO_OK;
+  for (int r : rounds) {
+    c->b_rounds = r;
+    float ms;
+    if ((st = measure(&ms)) != ISO_OK) break;
+    if (ms < best) best = ms, br = r;
+  }
+  c->b_rounds = br;
+  for (int w : wpw) {
+    if (st != ISO_OK) break;
+    c->b_words_per_wave = w;
+    float ms;
+    if ((st = measure(&ms)) != ISO_OK) break;
+    if (ms < best) best = ms, bw = w;
+  }
+  c->b_words_per_wave = bw;
+  for (int e : per_cu) {
+    if (st != ISO_OK) break;
+    c->b_emit_per_cu = e;
+    float ms;
+    if ((st = measure(&ms)) != ISO_OK) break;
+    if (ms < best) best = ms, be = e;
+  }
+  c->b_emit_per_cu = be;
+  c->event_pool.push_back(a);
+  c->event_pool.push_back(b);
+  c->timing = timing;
+  c->b_tuned = st == ISO_OK;
+  return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iso_extract_batch(iso_ctx* c, iso_batch_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  if (!c->b_ready) return fail(c, ISO_ERR_NO_SOURCE, "no batch uploaded");
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  if (!out->vertex_offsets || !out->index_offsets)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "offset arrays are required");
+  c->kept = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+  const iso::BatchPlan& P = c->plan;
+  iso::McDims D{};
+  D.n_segs = P.n_segs;
+  D.n_blocks = iso::batch_count_blocks(P, 1);  // look-back words for the finest tile
+  D.n_cells = static_cast<int64_t>(c->b_cells);
+  int st = ISO_OK;
+  iso::McBuffers B{};
+  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
+  const bool device = out->memory == ISO_MEMORY_DEVICE;
+  const bool graph = device && (out->flags & ISO_BATCH_GRAPH);
+
+  auto finish = [&](uint64_t nv, uint64_t ni) -> int {
+    ISO_HIP(c, hipMemcpyAsync(out->vertex_offsets, c->b_voff.ptr, (P.n_chunks + 1) * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipMemcpyAsync(out->index_offsets, c->b_ioff.ptr, (P.n_chunks + 1) * sizeof(uint64_t),
+                              hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    out->vertex_count = nv;
+    out->index_count = ni;
+    return ISO_OK;
+  };
+
+  if (device) {
+    B.vertices = out->vertices;
+    B.indices = out->indices;
+    B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
+    B.index_capacity = out->indices ? out->index_capacity : 0;
+    // A graph replays only with exactly the buffers it was captured with.
+    const iso::McBuffers& G = c->g_buffers;
+    const bool replay = graph && c->b_graph && G.seg == B.seg && G.tplanes == B.tplanes &&
+                        G.agg == B.agg && G.incl == B.incl && G.active == B.active &&
+                        G.active_capacity == B.active_capacity && G.totals == B.totals &&
+                        G.vertices == B.vertices && G.indices == B.indices &&
+                        G.vertex_capacity == B.vertex_capacity &&
+                        G.index_capacity == B.index_capacity;
+    if (replay) {
+      ISO_HIP(c, hipGraphLaunch(c->b_graph, c->stream));
+      if ((st = read_totals(c, B)) != ISO_OK) return st;
+    } else {
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        ISO_HIP(c, batch_launches(c, B));
+        if ((st = read_totals(c, B)) != ISO_OK) return st;
+        const uint64_t n_act = c->h_totals[iso::kTotActive];
+        if (n_act <= B.active_capacity) break;
+        if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+      }
+    }
+    const uint64_t nv = c->h_totals[iso::kTotVertices];
+    const uint64_t ni = 3 * c->h_totals[iso::kTotTriangles];
+    if ((st = finish(nv, ni)) != ISO_OK) return st;
+    if (c->h_totals[iso::kTotErrors] & 1ull)
+      return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+    if (nv > B.vertex_capacity || ni > B.index_capacity)
+      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    if ((out->flags & ISO_BATCH_AUTOTUNE) && !c->b_tuned) {
+      if ((st = autotune_batch(c, B)) != ISO_OK) return st;
+      if (c->b_graph) {
+        (void)hipGraphExecDestroy(c->b_graph);
+        c->b_graph = nullptr;
+      }
+    }
+    if (graph && !replay) {
+      // Capture the launches for this output binding; later calls replay it.
+      if (c->b_graph) {
+        (void)hipGraphExecDestroy(c->b_graph);
+        c->b_graph = nullptr;
+      }
+      const bool timing = c->timing;
+      c->timing = false;
+      hipGraph_t g = nullptr;
+      ISO_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t le = batch_launches(c, B);
+      const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      c->timing = timing;
+      ISO_HIP(c, le);
+      ISO_HIP(c, ce);
+      const hipError_t ie = hipGraphInstantiate(&c->b_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      ISO_HIP(c, ie);
+      c->g_buffers = B;
+    }
+    return ISO_OK;
+  }
+
+  // Host memory: count, size the context's buffers, emit, copy.
+  B.vertices = nullptr;
+  B.indices = nullptr;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    ISO_HIP(c, batch_launches(c, B, false));
+    if ((st = read_totals(c, B)) != ISO_OK) return st;
+    const uint64_t n_act = c->h_totals[iso::kTotActive];
+    if (n_act <= B.active_capacity) break;
+    if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+  }
+  const uint64_t nv = c->h_totals[iso::kTotVertices];
+  const uint64_t ni = 3 * c->h_totals[iso::kTotTriangles];
+  if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+  ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
+  ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
+  B.vertices = static_cast<float*>(c->vout.ptr);
+  B.indices = static_cast<uint32_t*>(c->iout.ptr);
+  B.vertex_capacity = nv;
+  B.index_capacity = ni;
+  ISO_HIP(c, timed(c, "batch_emit", [&] {
+            return iso::launch_batch_emit(P, B, c->b_emit_per_cu, c->stream);
+          }));
+  if ((st = finish(nv, ni)) != ISO_OK) return st;
+  if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+      (ni && (!out->indices || out->index_capacity < ni))) {
+    c->kept = true;
+    c->kept_vertices = nv;
+    c->kept_indices = ni;
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small; mesh kept for iso_mesh_fetch");
+  }
+  if (nv)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, B.vertices, nv * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+  if (ni)
+    ISO_HIP(c, hipMemcpyAsync(out->indices, B.indices, ni * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
   return ISO_OK;
 }
 

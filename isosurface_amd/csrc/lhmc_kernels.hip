@@ -416,7 +416,7 @@ hipError_t launch_lhmc_level(const DevOp* ops, int n_ops, int max_depth, uint64_
                              int64_t next_capacity, const scan::State& st, hipStream_t s) {
   LevelLoad load{ops, n_ops, max_depth, keys, values, leaf, off};
   LevelSink sink{keys, off, n, next_capacity};
-  return scan::launch(static_cast<uint64_t>(n), load, sink, st, s);
+  return scan::launch<1>(static_cast<uint64_t>(n), load, sink, st, s);
 }
 
 hipError_t launch_lhmc_primal(const LhmcNodes& N, int dense_depth, uint32_t* dense,
@@ -430,13 +430,13 @@ hipError_t launch_lhmc_primal(const LhmcNodes& N, int dense_depth, uint32_t* den
   if (e != hipSuccess) return e;
   PrimalLoad load{dense};
   PrimalSink sink{dense, C.primal, C.primal_level, capacity};
-  return scan::launch(n_dense, load, sink, st, s);
+  return scan::launch<scan::kItems>(n_dense, load, sink, st, s);
 }
 
 hipError_t launch_lhmc_cubes(const LhmcNodes& N, int dense_depth, const LhmcCubes& C,
                              const scan::State& st, hipStream_t s) {
   CubeLoad load{N, C, dense_depth, st.err};
-  return scan::launch(C.n, load, NoSink{}, st, s);
+  return scan::launch<1>(C.n, load, NoSink{}, st, s);
 }
 
 hipError_t launch_lhmc_own(const LhmcNodes& N, const LhmcCubes& C, const LhmcEdgeTable& T,
@@ -447,7 +447,7 @@ hipError_t launch_lhmc_own(const LhmcNodes& N, const LhmcCubes& C, const LhmcEdg
   if (e != hipSuccess) return e;
   OwnLoad load{C, T};
   OwnSink sink{C.base};
-  return scan::launch(C.n, load, sink, st, s);
+  return scan::launch<1>(C.n, load, sink, st, s);
 }
 
 hipError_t launch_lhmc_emit(const LhmcNodes& N, const LhmcCubes& C, const LhmcEdgeTable& T,

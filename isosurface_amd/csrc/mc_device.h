@@ -1,0 +1,241 @@
+// Device helpers shared by the marching-cubes kernels (single lattice and
+// batched chunks): edge geometry tables, the bit-sliced segment words of 64
+// cells, vertex-ownership counts, sign neighbourhoods, crossing points and
+// the look-back primitives.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mc_tables.h"
+
+namespace iso {
+namespace mcd {
+
+// ---------------------------------------------------------------------------
+// Tables
+// ---------------------------------------------------------------------------
+struct EdgeGeom {
+  int8_t axis[12];
+  int8_t minp[12][3];        // lattice offset of the edge's lower endpoint
+  int8_t of_axis_min[3][8];  // [axis][x | y<<1 | z<<2 of lower endpoint] -> edge
+};
+
+constexpr EdgeGeom make_geom() {
+  EdgeGeom g{};
+  for (int a = 0; a < 3; ++a)
+    for (int r = 0; r < 8; ++r) g.of_axis_min[a][r] = -1;
+  for (int e = 0; e < 12; ++e) {
+    const int* u = tables::kCorner[tables::kEdgeCorners[e][0]];
+    const int* v = tables::kCorner[tables::kEdgeCorners[e][1]];
+    int axis = 0;
+    for (int d = 0; d < 3; ++d) {
+      if (u[d] != v[d]) axis = d;
+      g.minp[e][d] = static_cast<int8_t>(u[d] < v[d] ? u[d] : v[d]);
+    }
+    g.axis[e] = static_cast<int8_t>(axis);
+    g.of_axis_min[axis][g.minp[e][0] | (g.minp[e][1] << 1) | (g.minp[e][2] << 2)] =
+        static_cast<int8_t>(e);
+  }
+  return g;
+}
+
+
+// Per edge, packed for register use: bits 0-2 = lower endpoint (x|y<<1|z<<2),
+// bits 3-4 = axis.
+constexpr uint64_t pack_edge_geom() {
+  EdgeGeom g = make_geom();
+  uint64_t r = 0;
+  for (int e = 0; e < 12; ++e) {
+    uint64_t v = static_cast<uint64_t>(g.minp[e][0] | (g.minp[e][1] << 1) | (g.minp[e][2] << 2) |
+                                       (g.axis[e] << 3));
+    r |= v << (5 * e);
+  }
+  return r;
+}
+constexpr uint64_t kEdgeGeomPacked = pack_edge_geom();
+
+// Edge of a given axis whose lower endpoint sits at corner offset r (3 bits),
+// four bits per entry [axis * 8 + r] over two words.
+constexpr uint64_t pack_of_axis_min(int half) {
+  EdgeGeom g = make_geom();
+  uint64_t r = 0;
+  for (int i = 0; i < 16; ++i) {
+    const int k = 16 * half + i;
+    if (k >= 24) break;
+    const int e = g.of_axis_min[k / 8][k % 8];
+    r |= static_cast<uint64_t>(e < 0 ? 15 : e) << (4 * i);
+  }
+  return r;
+}
+constexpr uint64_t kOfAxisMin0 = pack_of_axis_min(0);
+constexpr uint64_t kOfAxisMin1 = pack_of_axis_min(1);
+
+__device__ __forceinline__ int edge_of_axis_min(int axis, int rel) {
+  const int k = 8 * axis + rel;
+  const uint64_t w = k < 16 ? kOfAxisMin0 : kOfAxisMin1;
+  return static_cast<int>((w >> (4 * (k & 15))) & 15u);
+}
+
+__device__ __forceinline__ void load_tables(tables::McTables* dst,
+                                            const tables::McTables* from) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(from);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  constexpr int kWords = sizeof(tables::McTables) / 4;
+  static_assert(sizeof(tables::McTables) % 4 == 0, "table size");
+  for (int i = threadIdx.x; i < kWords; i += blockDim.x) d[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t popc64(uint64_t v) {
+  return static_cast<uint32_t>(__popcll(v));
+}
+
+__device__ __forceinline__ uint64_t low_mask(int b) { return (1ull << b) - 1ull; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = __lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+// Signed::find_crossing_point (distance.rs:64-70): t = d == 0 ? 0.5 : -a/d,
+// p = pa*(1-t) + pb*t per component, every operation rounded once.
+__device__ __forceinline__ void crossing(float a, float b, float pax, float pay, float paz,
+                                         float pbx, float pby, float pbz, float* o) {
+  const float delta = b - a;
+  const float t = (delta == 0.0f) ? 0.5f : -a / delta;
+  const float s = 1.0f - t;
+  o[0] = pax * s + pbx * t;
+  o[1] = pay * s + pby * t;
+  o[2] = paz * s + pbz * t;
+}
+
+// Local edges whose vertex the cell (x, y, z) owns, i.e. for which it is the
+// first cell in z-major order that contains the edge: e5, e6, e10 always,
+// and the low-face edges on the lattice's low faces.
+__device__ __forceinline__ uint32_t owned_edges(int64_t x, int64_t y, int64_t z) {
+  uint32_t m = (1u << 5) | (1u << 6) | (1u << 10);
+  if (y == 0) m |= (1u << 4) | (1u << 9);
+  if (z == 0) m |= (1u << 2) | (1u << 1);
+  if (x == 0) m |= (1u << 7) | (1u << 11);
+  if (y == 0 && z == 0) m |= 1u << 0;
+  if (x == 0 && z == 0) m |= 1u << 3;
+  if (x == 0 && y == 0) m |= 1u << 8;
+  return m;
+}
+
+// Row-aligned 64 sign bits starting at flat sample f, and the low bit of the
+// next word (sample f + 64) in *next.
+__device__ __forceinline__ uint64_t row_word(const uint64_t* __restrict__ S, int64_t f,
+                                             uint64_t* next) {
+  const int64_t w0 = f >> 6;
+  const int s = static_cast<int>(f & 63);
+  const uint64_t a = S[w0], b = S[w0 + 1];
+  *next = (b >> s) & 1ull;
+  return s ? ((a >> s) | (b << (64 - s))) : a;
+}
+
+// ---------------------------------------------------------------------------
+// Segment words: the eight corner bit-planes of 64 consecutive cells of a
+// cell row (bit b of c[i] = "inside" for corner i of cell x0 + b), so every
+// per-cell quantity below is computed for 64 cells per 64-bit operation.
+// ---------------------------------------------------------------------------
+struct SegWords {
+  uint64_t c[8];
+};
+
+// From the row words of the four point rows around a cell row
+// (00=(y,z) 10=(y+1,z) 01=(y,z+1) 11=(y+1,z+1)) and their next bits.
+__device__ __forceinline__ SegWords seg_words(uint64_t w00, uint64_t n00, uint64_t w10,
+                                              uint64_t n10, uint64_t w01, uint64_t n01,
+                                              uint64_t w11, uint64_t n11) {
+  SegWords s;
+  s.c[0] = w00;
+  s.c[1] = (w00 >> 1) | (n00 << 63);
+  s.c[2] = (w10 >> 1) | (n10 << 63);
+  s.c[3] = w10;
+  s.c[4] = w01;
+  s.c[5] = (w01 >> 1) | (n01 << 63);
+  s.c[6] = (w11 >> 1) | (n11 << 63);
+  s.c[7] = w11;
+  return s;
+}
+
+__device__ __forceinline__ uint32_t case_of(const SegWords& s, int b) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) c |= static_cast<uint32_t>((s.c[i] >> b) & 1ull) << i;
+  return c;
+}
+
+// Cells with a surface crossing (cube index not 0 or 255).
+__device__ __forceinline__ uint64_t mixed_cells(const SegWords& s) {
+  uint64_t any = 0, all = ~0ull;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    any |= s.c[i];
+    all &= s.c[i];
+  }
+  return any & ~all;
+}
+
+// Valid cells (x < nx - 1) of segment w.
+__device__ __forceinline__ uint64_t seg_valid(int64_t nx, int64_t w) {
+  const int64_t n = nx - 1 - 64 * w;
+  return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
+}
+
+// Vertices owned by the cells selected by `m` (a subset of one segment): a
+// crossed edge is the XOR of its endpoints' planes.  `first` marks the
+// segment holding cell x = 0.
+__device__ __forceinline__ uint32_t owned_count(const SegWords& s, uint64_t m, bool y0, bool z0,
+                                                bool first) {
+  const uint64_t* c = s.c;
+  uint32_t n = popc64(m & (c[6] ^ c[7])) + popc64(m & (c[5] ^ c[6])) + popc64(m & (c[2] ^ c[6]));
+  if (y0) n += popc64(m & (c[4] ^ c[5])) + popc64(m & (c[1] ^ c[5]));
+  if (z0) n += popc64(m & (c[2] ^ c[3])) + popc64(m & (c[1] ^ c[2]));
+  if (y0 && z0) n += popc64(m & (c[0] ^ c[1]));
+  if (first && (m & 1ull)) {
+    n += static_cast<uint32_t>(((c[4] ^ c[7]) & 1ull) + ((c[3] ^ c[7]) & 1ull));
+    if (z0) n += static_cast<uint32_t>((c[0] ^ c[3]) & 1ull);
+    if (y0) n += static_cast<uint32_t>((c[0] ^ c[4]) & 1ull);
+  }
+  return n;
+}
+
+// Cube index of the cell whose low corner sits at (dx, dy, dz) of the 3x3x3
+// sign neighbourhood `cube` (bit dx + 3 dy + 9 dz).
+__device__ __forceinline__ uint32_t case_in_cube(uint32_t cube, int dx, int dy, int dz) {
+  const uint32_t s = cube >> (dx + 3 * dy + 9 * dz);
+  return (s & 1u) | ((s >> 1) & 1u) << 1 | ((s >> 4) & 1u) << 2 | ((s >> 3) & 1u) << 3 |
+         ((s >> 9) & 1u) << 4 | ((s >> 10) & 1u) << 5 | ((s >> 13) & 1u) << 6 |
+         ((s >> 12) & 1u) << 7;
+}
+
+// A predecessor publishes its aggregate right after its own counting, so the
+// wait is short; the bound only guarantees that a broken invariant ends the
+// kernel (with an error flag) instead of spinning forever.
+constexpr uint32_t kLookBackSpinLimit = 1u << 22;
+
+// Look-back words: status << 62 | value (1: aggregate, 2: inclusive),
+// relaxed agent-scope atomics; the status travels with the value.
+constexpr unsigned long long kSlotAggregate = 1ull << 62;
+constexpr unsigned long long kSlotInclusive = 2ull << 62;
+constexpr unsigned long long kSlotValue = (1ull << 62) - 1ull;
+
+__device__ __forceinline__ unsigned long long slot_load(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void slot_store(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace mcd
+}  // namespace iso

@@ -25,7 +25,9 @@ struct McDims {
   int64_t n_cells;       // (nx-1)*(ny-1)*(nz-1)
 };
 
-constexpr int kCountBlock = 256;  // segments per count workgroup
+constexpr int kCountBlock = 1024;  // threads per count workgroup (16 waves)
+constexpr int kCountRounds = 1;    // segments per count thread
+constexpr int kCountSegs = kCountBlock * kCountRounds;  // segments per count workgroup
 
 // Running totals of the decoupled look-back scan.
 struct ScanTriple {
@@ -51,14 +53,33 @@ __device__ __forceinline__ float lattice_coord(const Lattice& L, int axis, int64
   return L.origin[axis] + static_cast<float>(i) * L.spacing[axis];
 }
 
+// One lattice as the count and emit kernels see it.  A single extraction
+// passes one as a kernel argument (its fields stay scalar); a batch keeps
+// one per chunk in device memory and finds a segment's chunk by its global
+// index.  The first vertex id of the lattice is the first id of its first
+// segment, which the kernels subtract from the ids they write into index
+// lists (zero for a single lattice), so a chunk's indices are chunk-local.
+struct CellSpace {
+  int64_t nx, ny, nz;
+  float origin[3];
+  float spacing[3];
+  const uint64_t* signs;    // the lattice's flat sign bits
+  uint32_t segs_per_row;
+  uint32_t seg_off;         // global index of the lattice's first segment
+  const DevOp* ops;         // program source (values re-evaluated in emit)
+  int32_t n_ops;
+  int32_t reserved;
+  const float* grid;        // dense samples, or null for a program source
+  const float* positions;   // per-point vertex positions, or null
+};
+
 // Buffers of one extraction.
 struct McBuffers {
   uint64_t* signs;          // n_sign_words; signs[-1], signs[n], signs[n+1] are guard words
   uint2* seg;               // n_segs: global first (vertex, triangle) id of each segment
   uint64_t* tplanes;        // 3 * n_segs: per-cell triangle counts, bit-sliced
-  uint32_t* flags;          // n_blocks: look-back status (0, 1 aggregate, 2 inclusive)
-  ScanTriple* agg;          // n_blocks
-  ScanTriple* incl;         // n_blocks
+  ScanTriple* agg;          // n_blocks: look-back aggregates, each word status << 62 | value
+  ScanTriple* incl;         // n_blocks: look-back inclusive prefixes, same encoding
   uint32_t* active;         // active-cell list: (segment << 6) | bit, cell order
   uint64_t active_capacity;
   unsigned long long* totals;  // kTotSlots
@@ -69,7 +90,7 @@ struct McBuffers {
 };
 
 // Each returns hipSuccess or the launch error.  The signs launchers also
-// reset the per-extraction state (guard words, totals, look-back flags).
+// reset the per-extraction state (guard words, totals, look-back words).
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,
                              const McBuffers& B, hipStream_t s);
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
@@ -81,6 +102,33 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const flo
                        const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
 hipError_t launch_sample_lattice(const DevOp* ops, int n_ops, const Lattice& L, float* out,
                                  hipStream_t s);
+
+// A batch of independent chunk lattices extracted in one pass: the chunks'
+// sign words and segments are concatenated, one count scan runs over all
+// segments, and chunk c's vertex / triangle ids start at the first ids of
+// its first segment.
+struct BatchPlan {
+  const CellSpace* chunks;     // device, n_chunks (signs = the chunk's first data word)
+  const uint32_t* seg_start;   // device, n_chunks + 1
+  const uint64_t* word_start;  // device, n_chunks + 1: first sign word of each chunk
+  uint32_t n_chunks;
+  int64_t n_words;   // sign words of all chunks
+  int64_t n_segs;    // segments of all chunks
+};
+
+// Count workgroups of a batch at `rounds` segments per thread (1 or 2).
+int64_t batch_count_blocks(const BatchPlan& P, int rounds);
+// Program sign bits of every chunk (and the extraction's state reset, sized
+// for the finest count tile); `words_per_wave` sign words per wavefront.
+hipError_t launch_batch_signs(const BatchPlan& P, const McBuffers& B, int words_per_wave,
+                              hipStream_t s);
+hipError_t launch_batch_count(const BatchPlan& P, const McBuffers& B, int rounds,
+                              hipStream_t s);
+hipError_t launch_batch_emit(const BatchPlan& P, const McBuffers& B, int blocks_per_cu,
+                             hipStream_t s);
+// voff / ioff (device, n_chunks + 1): each chunk's first vertex and index.
+hipError_t launch_batch_offsets(const BatchPlan& P, const McBuffers& B, uint64_t* voff,
+                                uint64_t* ioff, hipStream_t s);
 hipError_t launch_sample_points(const DevOp* ops, int n_ops, const float* pts, uint64_t n,
                                 float* out, hipStream_t s);
 

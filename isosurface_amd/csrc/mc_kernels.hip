@@ -28,6 +28,7 @@
 // alone, so no hash table is needed.
 #include <hip/hip_runtime.h>
 
+#include "mc_device.h"
 #include "mc_kernels.h"
 #include "mc_tables.h"
 #include "sdf_eval.h"
@@ -36,203 +37,9 @@ namespace iso {
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// Tables
-// ---------------------------------------------------------------------------
-struct EdgeGeom {
-  int8_t axis[12];
-  int8_t minp[12][3];        // lattice offset of the edge's lower endpoint
-  int8_t of_axis_min[3][8];  // [axis][x | y<<1 | z<<2 of lower endpoint] -> edge
-};
-
-constexpr EdgeGeom make_geom() {
-  EdgeGeom g{};
-  for (int a = 0; a < 3; ++a)
-    for (int r = 0; r < 8; ++r) g.of_axis_min[a][r] = -1;
-  for (int e = 0; e < 12; ++e) {
-    const int* u = tables::kCorner[tables::kEdgeCorners[e][0]];
-    const int* v = tables::kCorner[tables::kEdgeCorners[e][1]];
-    int axis = 0;
-    for (int d = 0; d < 3; ++d) {
-      if (u[d] != v[d]) axis = d;
-      g.minp[e][d] = static_cast<int8_t>(u[d] < v[d] ? u[d] : v[d]);
-    }
-    g.axis[e] = static_cast<int8_t>(axis);
-    g.of_axis_min[axis][g.minp[e][0] | (g.minp[e][1] << 1) | (g.minp[e][2] << 2)] =
-        static_cast<int8_t>(e);
-  }
-  return g;
-}
+using namespace mcd;
 
 __constant__ tables::McTables c_tables = tables::kTables;
-
-// Per edge, packed for register use: bits 0-2 = lower endpoint (x|y<<1|z<<2),
-// bits 3-4 = axis.
-constexpr uint64_t pack_edge_geom() {
-  EdgeGeom g = make_geom();
-  uint64_t r = 0;
-  for (int e = 0; e < 12; ++e) {
-    uint64_t v = static_cast<uint64_t>(g.minp[e][0] | (g.minp[e][1] << 1) | (g.minp[e][2] << 2) |
-                                       (g.axis[e] << 3));
-    r |= v << (5 * e);
-  }
-  return r;
-}
-constexpr uint64_t kEdgeGeomPacked = pack_edge_geom();
-
-// Edge of a given axis whose lower endpoint sits at corner offset r (3 bits),
-// four bits per entry [axis * 8 + r] over two words.
-constexpr uint64_t pack_of_axis_min(int half) {
-  EdgeGeom g = make_geom();
-  uint64_t r = 0;
-  for (int i = 0; i < 16; ++i) {
-    const int k = 16 * half + i;
-    if (k >= 24) break;
-    const int e = g.of_axis_min[k / 8][k % 8];
-    r |= static_cast<uint64_t>(e < 0 ? 15 : e) << (4 * i);
-  }
-  return r;
-}
-constexpr uint64_t kOfAxisMin0 = pack_of_axis_min(0);
-constexpr uint64_t kOfAxisMin1 = pack_of_axis_min(1);
-
-__device__ __forceinline__ int edge_of_axis_min(int axis, int rel) {
-  const int k = 8 * axis + rel;
-  const uint64_t w = k < 16 ? kOfAxisMin0 : kOfAxisMin1;
-  return static_cast<int>((w >> (4 * (k & 15))) & 15u);
-}
-
-__device__ __forceinline__ void load_tables(tables::McTables* dst) {
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(&c_tables);
-  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-  constexpr int kWords = sizeof(tables::McTables) / 4;
-  static_assert(sizeof(tables::McTables) % 4 == 0, "table size");
-  for (int i = threadIdx.x; i < kWords; i += blockDim.x) d[i] = src[i];
-}
-
-// ---------------------------------------------------------------------------
-// Small helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t popc64(uint64_t v) {
-  return static_cast<uint32_t>(__popcll(v));
-}
-
-__device__ __forceinline__ uint64_t low_mask(int b) { return (1ull << b) - 1ull; }
-
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-  const int lane = __lane_id();
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
-
-// Signed::find_crossing_point (distance.rs:64-70): t = d == 0 ? 0.5 : -a/d,
-// p = pa*(1-t) + pb*t per component, every operation rounded once.
-__device__ __forceinline__ void crossing(float a, float b, float pax, float pay, float paz,
-                                         float pbx, float pby, float pbz, float* o) {
-  const float delta = b - a;
-  const float t = (delta == 0.0f) ? 0.5f : -a / delta;
-  const float s = 1.0f - t;
-  o[0] = pax * s + pbx * t;
-  o[1] = pay * s + pby * t;
-  o[2] = paz * s + pbz * t;
-}
-
-// Local edges whose vertex the cell (x, y, z) owns, i.e. for which it is the
-// first cell in z-major order that contains the edge: e5, e6, e10 always,
-// and the low-face edges on the lattice's low faces.
-__device__ __forceinline__ uint32_t owned_edges(int64_t x, int64_t y, int64_t z) {
-  uint32_t m = (1u << 5) | (1u << 6) | (1u << 10);
-  if (y == 0) m |= (1u << 4) | (1u << 9);
-  if (z == 0) m |= (1u << 2) | (1u << 1);
-  if (x == 0) m |= (1u << 7) | (1u << 11);
-  if (y == 0 && z == 0) m |= 1u << 0;
-  if (x == 0 && z == 0) m |= 1u << 3;
-  if (x == 0 && y == 0) m |= 1u << 8;
-  return m;
-}
-
-// Row-aligned 64 sign bits starting at flat sample f, and the low bit of the
-// next word (sample f + 64) in *next.
-__device__ __forceinline__ uint64_t row_word(const uint64_t* __restrict__ S, int64_t f,
-                                             uint64_t* next) {
-  const int64_t w0 = f >> 6;
-  const int s = static_cast<int>(f & 63);
-  const uint64_t a = S[w0], b = S[w0 + 1];
-  *next = (b >> s) & 1ull;
-  return s ? ((a >> s) | (b << (64 - s))) : a;
-}
-
-// ---------------------------------------------------------------------------
-// Segment words: the eight corner bit-planes of 64 consecutive cells of a
-// cell row (bit b of c[i] = "inside" for corner i of cell x0 + b), so every
-// per-cell quantity below is computed for 64 cells per 64-bit operation.
-// ---------------------------------------------------------------------------
-struct SegWords {
-  uint64_t c[8];
-};
-
-// From the row words of the four point rows around a cell row
-// (00=(y,z) 10=(y+1,z) 01=(y,z+1) 11=(y+1,z+1)) and their next bits.
-__device__ __forceinline__ SegWords seg_words(uint64_t w00, uint64_t n00, uint64_t w10,
-                                              uint64_t n10, uint64_t w01, uint64_t n01,
-                                              uint64_t w11, uint64_t n11) {
-  SegWords s;
-  s.c[0] = w00;
-  s.c[1] = (w00 >> 1) | (n00 << 63);
-  s.c[2] = (w10 >> 1) | (n10 << 63);
-  s.c[3] = w10;
-  s.c[4] = w01;
-  s.c[5] = (w01 >> 1) | (n01 << 63);
-  s.c[6] = (w11 >> 1) | (n11 << 63);
-  s.c[7] = w11;
-  return s;
-}
-
-__device__ __forceinline__ uint32_t case_of(const SegWords& s, int b) {
-  uint32_t c = 0;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) c |= static_cast<uint32_t>((s.c[i] >> b) & 1ull) << i;
-  return c;
-}
-
-// Cells with a surface crossing (cube index not 0 or 255).
-__device__ __forceinline__ uint64_t mixed_cells(const SegWords& s) {
-  uint64_t any = 0, all = ~0ull;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    any |= s.c[i];
-    all &= s.c[i];
-  }
-  return any & ~all;
-}
-
-// Valid cells (x < nx - 1) of segment w.
-__device__ __forceinline__ uint64_t seg_valid(int64_t nx, int64_t w) {
-  const int64_t n = nx - 1 - 64 * w;
-  return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1ull));
-}
-
-// Vertices owned by the cells selected by `m` (a subset of one segment): a
-// crossed edge is the XOR of its endpoints' planes.  `first` marks the
-// segment holding cell x = 0.
-__device__ __forceinline__ uint32_t owned_count(const SegWords& s, uint64_t m, bool y0, bool z0,
-                                                bool first) {
-  const uint64_t* c = s.c;
-  uint32_t n = popc64(m & (c[6] ^ c[7])) + popc64(m & (c[5] ^ c[6])) + popc64(m & (c[2] ^ c[6]));
-  if (y0) n += popc64(m & (c[4] ^ c[5])) + popc64(m & (c[1] ^ c[5]));
-  if (z0) n += popc64(m & (c[2] ^ c[3])) + popc64(m & (c[1] ^ c[2]));
-  if (y0 && z0) n += popc64(m & (c[0] ^ c[1]));
-  if (first && (m & 1ull)) {
-    n += static_cast<uint32_t>(((c[4] ^ c[7]) & 1ull) + ((c[3] ^ c[7]) & 1ull));
-    if (z0) n += static_cast<uint32_t>((c[0] ^ c[3]) & 1ull);
-    if (y0) n += static_cast<uint32_t>((c[0] ^ c[4]) & 1ull);
-  }
-  return n;
-}
 
 // ---------------------------------------------------------------------------
 // 1. signs (+ reset of the extraction's scratch state)
@@ -240,7 +47,8 @@ __device__ __forceinline__ uint32_t owned_count(const SegWords& s, uint64_t m, b
 struct ResetArgs {
   uint64_t* signs;  // guard words at [-1], [n_words], [n_words + 1]
   int64_t n_words;
-  uint32_t* flags;
+  ScanTriple* agg;
+  ScanTriple* incl;
   int64_t n_blocks;
   unsigned long long* totals;
 };
@@ -248,9 +56,12 @@ struct ResetArgs {
 __device__ __forceinline__ void reset_state(const ResetArgs& r) {
   const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = i0; i < r.n_blocks; i += stride) r.flags[i] = 0u;
+  for (int64_t i = i0; i < r.n_blocks; i += stride) {
+    r.agg[i] = ScanTriple{0ull, 0ull, 0ull};
+    r.incl[i] = ScanTriple{0ull, 0ull, 0ull};
+  }
   if (i0 < kTotSlots) r.totals[i0] = 0ull;
-  if (i0 == 0) {
+  if (i0 == 0 && r.signs) {
     r.signs[-1] = 0ull;
     r.signs[r.n_words] = 0ull;
     r.signs[r.n_words + 1] = 0ull;
@@ -370,26 +181,94 @@ __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__
   }
 }
 
+// Batched chunks: one wavefront per sign word of the concatenated word
+// numbering; the word's chunk is found by binary search, so a wavefront
+// always evaluates a single chunk's program.  Thread i < n_chunks also
+// clears chunk i's guard words.
+__global__ __launch_bounds__(256) void k_batch_signs(const CellSpace* __restrict__ chunks,
+                                                     const uint64_t* __restrict__ word_start,
+                                                     uint32_t n_chunks, ResetArgs R) {
+  reset_state(R);
+  const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid < n_chunks) {
+    uint64_t* S = const_cast<uint64_t*>(chunks[tid].signs);
+    const int64_t nw = static_cast<int64_t>(word_start[tid + 1] - word_start[tid]);
+    S[-1] = 0ull;
+    S[nw] = 0ull;
+    S[nw + 1] = 0ull;
+  }
+  const int lane = __lane_id();
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
+  const int64_t n_words = static_cast<int64_t>(word_start[n_chunks]);
+  for (int64_t w = wave; w < n_words; w += n_waves) {
+    uint32_t lo = 0, hi = n_chunks;  // last chunk whose first word <= w
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (static_cast<int64_t>(word_start[mid]) <= w) lo = mid;
+      else hi = mid;
+    }
+    const CellSpace& C = chunks[lo];
+    const int64_t lw = w - static_cast<int64_t>(word_start[lo]);
+    const int64_t f = lw * 64 + lane;
+    const int64_t n_points = C.nx * C.ny * C.nz;
+    bool inside = false;
+    if (f < n_points) {
+      const uint32_t nx = static_cast<uint32_t>(C.nx);
+      const uint32_t nxy = static_cast<uint32_t>(C.nx * C.ny);
+      const uint32_t ff = static_cast<uint32_t>(f);
+      const uint32_t z = ff / nxy;
+      const uint32_t r = ff - z * nxy;
+      const uint32_t y = r / nx;
+      const uint32_t x = r - y * nx;
+      const float v = sdf_eval(C.ops, C.n_ops, C.origin[0] + static_cast<float>(x) * C.spacing[0],
+                               C.origin[1] + static_cast<float>(y) * C.spacing[1],
+                               C.origin[2] + static_cast<float>(z) * C.spacing[2]);
+      inside = !(v > 0.0f);
+    }
+    const uint64_t m = __ballot(inside);
+    if (lane == 0) const_cast<uint64_t*>(C.signs)[lw] = m;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 2. count + single-pass scan (decoupled look-back)
 // ---------------------------------------------------------------------------
-// A predecessor publishes its aggregate right after its own counting, so the
-// wait is short; the bound only guarantees that a broken invariant ends the
-// kernel (with an error flag) instead of spinning forever.
-constexpr uint32_t kLookBackSpinLimit = 1u << 22;
+// The lattice of one extraction, passed by value: every field is uniform.
+struct SingleSpace {
+  CellSpace C;
+  __device__ __forceinline__ const CellSpace& at(uint32_t g, uint32_t* local) const {
+    *local = g;
+    return C;
+  }
+};
 
-__device__ __forceinline__ uint32_t load_acquire(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
+// A batch: the chunk owning global segment g is found by binary search over
+// the chunks' first segments.
+struct BatchSpace {
+  const CellSpace* chunks;
+  const uint32_t* seg_start;  // n_chunks + 1 ascending first segments
+  uint32_t n_chunks;
+  __device__ __forceinline__ CellSpace at(uint32_t g, uint32_t* local) const {
+    uint32_t lo = 0, hi = n_chunks;  // last chunk with seg_start <= g
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (seg_start[mid] <= g) lo = mid;
+      else hi = mid;
+    }
+    *local = g - seg_start[lo];
+    return chunks[lo];
+  }
+};
 
-__device__ __forceinline__ void store_release(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ __launch_bounds__(kCountBlock) void k_count(const uint64_t* __restrict__ S, Lattice L,
-                                                       McDims D, McBuffers B) {
+template <class Space, int kCountRounds>
+__global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs, int64_t n_blocks,
+                                                       McBuffers B) {
+  constexpr int kCountSegs = kCountBlock * kCountRounds;  // segments per workgroup
+  constexpr int kWaves = kCountBlock / 64;
   __shared__ uint8_t s_ntri[256];
-  __shared__ uint32_t s_wv[4], s_wt[4], s_wa[4];
+  __shared__ uint32_t s_wv[kCountRounds][kWaves], s_wt[kCountRounds][kWaves],
+      s_wa[kCountRounds][kWaves];
   __shared__ uint32_t s_block;
   __shared__ ScanTriple s_prefix;
   for (int i = threadIdx.x; i < 256; i += blockDim.x) s_ntri[i] = c_tables.tri_count[i];
@@ -398,112 +277,188 @@ __global__ __launch_bounds__(kCountBlock) void k_count(const uint64_t* __restric
   __syncthreads();
   // Workgroups scan in ticket order, so every predecessor is already running.
   const uint32_t blk = s_block;
-  const int64_t g = static_cast<int64_t>(blk) * kCountBlock + threadIdx.x;
-
-  uint32_t nv = 0, nt = 0, na = 0;
-  uint64_t act = 0;
-  if (g < D.n_segs) {
-    const int64_t nx = L.nx, ny = L.ny;
-    const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
-    const uint32_t cyn = static_cast<uint32_t>(ny - 1);
-    const uint32_t gg = static_cast<uint32_t>(g);
-    const uint32_t row = gg / spr;
-    const uint32_t w = gg - row * spr;
-    const uint32_t cz = row / cyn;
-    const uint32_t cy = row - cz * cyn;
-    const int64_t f00 = (static_cast<int64_t>(cz) * ny + cy) * nx + 64 * static_cast<int64_t>(w);
-    uint64_t n00, n10, n01, n11;
-    const uint64_t w00 = row_word(S, f00, &n00);
-    const uint64_t w10 = row_word(S, f00 + nx, &n10);
-    const uint64_t w01 = row_word(S, f00 + nx * ny, &n01);
-    const uint64_t w11 = row_word(S, f00 + nx * ny + nx, &n11);
-    const SegWords sw = seg_words(w00, n00, w10, n10, w01, n01, w11, n11);
-    const uint64_t valid = seg_valid(nx, w);
-    nv = owned_count(sw, valid, cy == 0, cz == 0, w == 0);
-    act = mixed_cells(sw) & valid;
-    na = popc64(act);
-    uint64_t p0 = 0, p1 = 0, p2 = 0;  // per-cell triangle counts, bit-sliced
-    for (uint64_t m = act; m; m &= m - 1) {
-      const int b = __builtin_ctzll(m);
-      const uint32_t c = s_ntri[case_of(sw, b)];
-      nt += c;
-      p0 |= static_cast<uint64_t>(c & 1u) << b;
-      p1 |= static_cast<uint64_t>((c >> 1) & 1u) << b;
-      p2 |= static_cast<uint64_t>((c >> 2) & 1u) << b;
-    }
-    B.tplanes[3 * g + 0] = p0;
-    B.tplanes[3 * g + 1] = p1;
-    B.tplanes[3 * g + 2] = p2;
-  }
-
-  // Workgroup scan of (vertices, triangles, active cells).
-  const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt), ia = wave_incl_scan(na);
   const int wv = threadIdx.x >> 6;
-  if (__lane_id() == 63) {
-    s_wv[wv] = iv;
-    s_wt[wv] = it;
-    s_wa[wv] = ia;
+
+  // Round r handles segments blk * kCountSegs + r * kCountBlock + thread, so
+  // the loads of a round are coalesced and the segments stay in order.
+  uint32_t ov[kCountRounds], ot[kCountRounds], oa[kCountRounds];
+  uint64_t act[kCountRounds];
+#pragma unroll
+  for (int r = 0; r < kCountRounds; ++r) {
+    const int64_t g = static_cast<int64_t>(blk) * kCountSegs + r * kCountBlock + threadIdx.x;
+    uint32_t nv = 0, nt = 0, na = 0;
+    act[r] = 0;
+    if (g < n_segs) {
+      uint32_t gg;
+      const CellSpace& C = sp.at(static_cast<uint32_t>(g), &gg);
+      const uint64_t* __restrict__ S = C.signs;
+      const int64_t nx = C.nx, ny = C.ny;
+      const uint32_t spr = C.segs_per_row;
+      const uint32_t cyn = static_cast<uint32_t>(ny - 1);
+      const uint32_t row = gg / spr;
+      const uint32_t w = gg - row * spr;
+      const uint32_t cz = row / cyn;
+      const uint32_t cy = row - cz * cyn;
+      const int64_t f00 =
+          (static_cast<int64_t>(cz) * ny + cy) * nx + 64 * static_cast<int64_t>(w);
+      uint64_t n00, n10, n01, n11;
+      const uint64_t w00 = row_word(S, f00, &n00);
+      const uint64_t w10 = row_word(S, f00 + nx, &n10);
+      const uint64_t w01 = row_word(S, f00 + nx * ny, &n01);
+      const uint64_t w11 = row_word(S, f00 + nx * ny + nx, &n11);
+      const SegWords sw = seg_words(w00, n00, w10, n10, w01, n01, w11, n11);
+      const uint64_t valid = seg_valid(nx, w);
+      nv = owned_count(sw, valid, cy == 0, cz == 0, w == 0);
+      act[r] = mixed_cells(sw) & valid;
+      na = popc64(act[r]);
+      uint64_t p0 = 0, p1 = 0, p2 = 0;  // per-cell triangle counts, bit-sliced
+      for (uint64_t m = act[r]; m; m &= m - 1) {
+        const int b = __builtin_ctzll(m);
+        const uint32_t c = s_ntri[case_of(sw, b)];
+        nt += c;
+        p0 |= static_cast<uint64_t>(c & 1u) << b;
+        p1 |= static_cast<uint64_t>((c >> 1) & 1u) << b;
+        p2 |= static_cast<uint64_t>((c >> 2) & 1u) << b;
+      }
+      B.tplanes[3 * g + 0] = p0;
+      B.tplanes[3 * g + 1] = p1;
+      B.tplanes[3 * g + 2] = p2;
+    }
+    const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt), ia = wave_incl_scan(na);
+    if (__lane_id() == 63) {
+      s_wv[r][wv] = iv;
+      s_wt[r][wv] = it;
+      s_wa[r][wv] = ia;
+    }
+    ov[r] = iv - nv;
+    ot[r] = it - nt;
+    oa[r] = ia - na;
   }
   __syncthreads();
-  uint32_t ov = iv - nv, ot = it - nt, oa = ia - na;
-  for (int i = 0; i < wv; ++i) {
-    ov += s_wv[i];
-    ot += s_wt[i];
-    oa += s_wa[i];
+  // Workgroup-exclusive offsets: earlier rounds, then earlier waves of this round.
+  {
+    uint32_t bv = 0, bt = 0, ba = 0;
+#pragma unroll
+    for (int r = 0; r < kCountRounds; ++r) {
+      uint32_t xv = bv, xt = bt, xa = ba;
+      for (int i = 0; i < kWaves; ++i) {
+        if (i < wv) {
+          xv += s_wv[r][i];
+          xt += s_wt[r][i];
+          xa += s_wa[r][i];
+        }
+        bv += s_wv[r][i];
+        bt += s_wt[r][i];
+        ba += s_wa[r][i];
+      }
+      ov[r] += xv;
+      ot[r] += xt;
+      oa[r] += xa;
+    }
   }
 
-  if (threadIdx.x == 0) {
-    ScanTriple total;
-    total.v = static_cast<unsigned long long>(s_wv[0]) + s_wv[1] + s_wv[2] + s_wv[3];
-    total.t = static_cast<unsigned long long>(s_wt[0]) + s_wt[1] + s_wt[2] + s_wt[3];
-    total.a = static_cast<unsigned long long>(s_wa[0]) + s_wa[1] + s_wa[2] + s_wa[3];
+  // Look-back by the first wave: its 64 lanes inspect 64 predecessors at a
+  // time, so a chain of published aggregates is summed in one step.
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    ScanTriple total{0ull, 0ull, 0ull};
+    for (int r = 0; r < kCountRounds; ++r)
+      for (int i = 0; i < kWaves; ++i) {
+        total.v += s_wv[r][i];
+        total.t += s_wt[r][i];
+        total.a += s_wa[r][i];
+      }
     ScanTriple prefix{0ull, 0ull, 0ull};
     if (blk == 0) {
-      B.incl[0] = total;
-      store_release(&B.flags[0], 2u);
+      if (lane == 0) {
+        slot_store(&B.incl[0].v, kSlotInclusive | total.v);
+        slot_store(&B.incl[0].t, kSlotInclusive | total.t);
+        slot_store(&B.incl[0].a, kSlotInclusive | total.a);
+      }
     } else {
-      B.agg[blk] = total;
-      store_release(&B.flags[blk], 1u);
-      int64_t j = static_cast<int64_t>(blk) - 1;
+      if (lane == 0) {
+        slot_store(&B.agg[blk].v, kSlotAggregate | total.v);
+        slot_store(&B.agg[blk].t, kSlotAggregate | total.t);
+        slot_store(&B.agg[blk].a, kSlotAggregate | total.a);
+      }
+      int64_t j = static_cast<int64_t>(blk) - 1;  // nearest predecessor of the window
       uint32_t spins = 0;
-      while (j >= 0) {
-        const uint32_t f = load_acquire(&B.flags[j]);
-        if (f == 0u) {
+      while (true) {
+        const int64_t idx = j - lane;
+        // A predecessor is inclusive when all three inclusive words carry
+        // the inclusive status, aggregate when all three aggregate words do.
+        // Blocks before the first one act as an inclusive zero.
+        unsigned long long xv = 0ull, xt = 0ull, xa = 0ull;
+        int state = 2;
+        if (idx >= 0) {
+          xv = slot_load(&B.incl[idx].v);
+          xt = slot_load(&B.incl[idx].t);
+          xa = slot_load(&B.incl[idx].a);
+          if (((xv & xt & xa) >> 62) != 2ull) {
+            xv = slot_load(&B.agg[idx].v);
+            xt = slot_load(&B.agg[idx].t);
+            xa = slot_load(&B.agg[idx].a);
+            state = ((xv >> 62) == 1ull && (xt >> 62) == 1ull && (xa >> 62) == 1ull) ? 1 : 0;
+          }
+        }
+        const uint64_t incl = __ballot(state == 2);
+        const uint64_t zero = __ballot(state == 0);
+        const int stop = incl ? __builtin_ctzll(incl) : 64;
+        const uint64_t upto = stop < 64 ? (2ull << stop) - 1ull : ~0ull;
+        if (zero & upto) {
           if (++spins > kLookBackSpinLimit) {
-            atomicOr(&B.totals[kTotErrors], 2ull);
+            if (lane == 0) atomicOr(&B.totals[kTotErrors], 2ull);
             break;
           }
           __builtin_amdgcn_s_sleep(1);
           continue;
         }
-        const ScanTriple x = (f == 2u) ? B.incl[j] : B.agg[j];
+        ScanTriple x{0ull, 0ull, 0ull};
+        if (lane <= stop) x = ScanTriple{xv & kSlotValue, xt & kSlotValue, xa & kSlotValue};
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+          x.v += __shfl_xor(x.v, d, 64);
+          x.t += __shfl_xor(x.t, d, 64);
+          x.a += __shfl_xor(x.a, d, 64);
+        }
         prefix.v += x.v;
         prefix.t += x.t;
         prefix.a += x.a;
-        if (f == 2u) break;
-        --j;
+        if (stop < 64) break;
+        j -= 64;
       }
-      B.incl[blk] = ScanTriple{prefix.v + total.v, prefix.t + total.t, prefix.a + total.a};
-      store_release(&B.flags[blk], 2u);
+      if (lane == 0) {
+        slot_store(&B.incl[blk].v, kSlotInclusive | (prefix.v + total.v));
+        slot_store(&B.incl[blk].t, kSlotInclusive | (prefix.t + total.t));
+        slot_store(&B.incl[blk].a, kSlotInclusive | (prefix.a + total.a));
+      }
     }
-    s_prefix = prefix;
-    if (static_cast<int64_t>(blk) == D.n_blocks - 1) {
-      B.totals[kTotVertices] = prefix.v + total.v;
-      B.totals[kTotTriangles] = prefix.t + total.t;
-      B.totals[kTotActive] = prefix.a + total.a;
-      if (prefix.v + total.v > 0xffffffffull || 3 * (prefix.t + total.t) > 0xffffffffull)
-        atomicOr(&B.totals[kTotErrors], 1ull);
+    if (lane == 0) {
+      s_prefix = prefix;
+      if (static_cast<int64_t>(blk) == n_blocks - 1) {
+        B.totals[kTotVertices] = prefix.v + total.v;
+        B.totals[kTotTriangles] = prefix.t + total.t;
+        B.totals[kTotActive] = prefix.a + total.a;
+        if (prefix.v + total.v > 0xffffffffull || 3 * (prefix.t + total.t) > 0xffffffffull)
+          atomicOr(&B.totals[kTotErrors], 1ull);
+      }
     }
   }
   __syncthreads();
 
-  if (g < D.n_segs) {
-    const ScanTriple p = s_prefix;
-    B.seg[g] = make_uint2(static_cast<uint32_t>(p.v + ov), static_cast<uint32_t>(p.t + ot));
-    uint64_t k = p.a + oa;
-    for (uint64_t m = act; m; m &= m - 1, ++k)
-      if (k < B.active_capacity)
-        B.active[k] = (static_cast<uint32_t>(g) << 6) | static_cast<uint32_t>(__builtin_ctzll(m));
+  const ScanTriple p = s_prefix;
+#pragma unroll
+  for (int r = 0; r < kCountRounds; ++r) {
+    const int64_t g = static_cast<int64_t>(blk) * kCountSegs + r * kCountBlock + threadIdx.x;
+    if (g < n_segs) {
+      B.seg[g] =
+          make_uint2(static_cast<uint32_t>(p.v + ov[r]), static_cast<uint32_t>(p.t + ot[r]));
+      uint64_t k = p.a + oa[r];
+      for (uint64_t m = act[r]; m; m &= m - 1, ++k)
+        if (k < B.active_capacity)
+          B.active[k] =
+              (static_cast<uint32_t>(g) << 6) | static_cast<uint32_t>(__builtin_ctzll(m));
+    }
   }
 }
 
@@ -511,15 +466,10 @@ __global__ __launch_bounds__(kCountBlock) void k_count(const uint64_t* __restric
 // 3. emit — one thread per active cell
 // ---------------------------------------------------------------------------
 struct EmitArgs {
-  const uint64_t* S;
   const uint2* seg;
   const uint64_t* tplanes;
   const uint32_t* active;
   uint64_t active_capacity;
-  const float* grid;
-  const DevOp* ops;
-  int n_ops;
-  const float* positions;  // per lattice point (3 floats), kEmitPositions only
   float* vertices;
   uint32_t* indices;
   uint64_t vcap, icap;
@@ -533,104 +483,119 @@ enum EmitSource : int {
   kEmitPositions = 2,  // values from a dense grid, positions per lattice point
 };
 
-template <int kSrc>
-__device__ __forceinline__ float lattice_value(const EmitArgs& A, const Lattice& L, int64_t x,
-                                               int64_t y, int64_t z) {
-  if (kSrc != kEmitProgram) {
-    return A.grid[(z * L.ny + y) * L.nx + x];
-  } else {
-    return sdf_eval(A.ops, A.n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
-                    lattice_coord(L, 2, z));
-  }
+__device__ __forceinline__ float space_coord(const CellSpace& C, int axis, int64_t i) {
+  return C.origin[axis] + static_cast<float>(i) * C.spacing[axis];
 }
 
-// Cube index of the cell whose low corner sits at (dx, dy, dz) of the 3x3x3
-// sign neighbourhood `cube` (bit dx + 3 dy + 9 dz).
-__device__ __forceinline__ uint32_t case_in_cube(uint32_t cube, int dx, int dy, int dz) {
-  const uint32_t s = cube >> (dx + 3 * dy + 9 * dz);
-  return (s & 1u) | ((s >> 1) & 1u) << 1 | ((s >> 4) & 1u) << 2 | ((s >> 3) & 1u) << 3 |
-         ((s >> 9) & 1u) << 4 | ((s >> 10) & 1u) << 5 | ((s >> 13) & 1u) << 6 |
-         ((s >> 12) & 1u) << 7;
+template <int kSrc>
+__device__ __forceinline__ float space_value(const CellSpace& C, int64_t x, int64_t y,
+                                             int64_t z) {
+  if (kSrc != kEmitProgram) {
+    return C.grid[(z * C.ny + y) * C.nx + x];
+  } else {
+    return sdf_eval(C.ops, C.n_ops, space_coord(C, 0, x), space_coord(C, 1, y),
+                    space_coord(C, 2, z));
+  }
 }
 
 constexpr int kEmitThreads = 256;
 
 template <int kSrc>
-__device__ __forceinline__ void lattice_position(const EmitArgs& A, const Lattice& L, int64_t x,
-                                                 int64_t y, int64_t z, float* p) {
+__device__ __forceinline__ void space_position(const CellSpace& C, int64_t x, int64_t y,
+                                               int64_t z, float* p) {
   if (kSrc == kEmitPositions) {
-    const float* q = A.positions + 3 * ((z * L.ny + y) * L.nx + x);
+    const float* q = C.positions + 3 * ((z * C.ny + y) * C.nx + x);
     p[0] = q[0];
     p[1] = q[1];
     p[2] = q[2];
   } else {
-    p[0] = lattice_coord(L, 0, x);
-    p[1] = lattice_coord(L, 1, y);
-    p[2] = lattice_coord(L, 2, z);
+    p[0] = space_coord(C, 0, x);
+    p[1] = space_coord(C, 1, y);
+    p[2] = space_coord(C, 2, z);
   }
 }
 
-template <int kSrc>
-__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, McDims D) {
+template <int kSrc, class Space>
+__global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
   __shared__ tables::McTables s_t;
   // Per thread: first vertex id and (cube index | owned crossed edges << 8)
   // of each of the eight cells that can own one of its edges, indexed
   // rx | ry<<1 | rz<<2 for the cell at (x-rx, y-ry, z-rz).
   __shared__ uint32_t s_obase[8][kEmitThreads];
   __shared__ uint32_t s_oinfo[8][kEmitThreads];
-  load_tables(&s_t);
+  load_tables(&s_t, &c_tables);
   __syncthreads();
 
   const int tid = threadIdx.x;
-  const int64_t nx = L.nx, ny = L.ny;
-  const int64_t nxy = nx * ny;
-  const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
-  const uint32_t cyn = static_cast<uint32_t>(ny - 1);
   uint64_t n_act = A.totals[kTotActive];
   if (n_act > A.active_capacity) n_act = A.active_capacity;
 
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kEmitThreads + tid; i < n_act;
        i += static_cast<uint64_t>(gridDim.x) * kEmitThreads) {
     const uint32_t rec = A.active[i];
-    const uint32_t g = rec >> 6;
+    const uint32_t g = rec >> 6;  // global segment
     const int b = static_cast<int>(rec & 63u);
-    const uint32_t row = g / spr;
-    const uint32_t w = g - row * spr;
+    uint32_t lg;                  // segment within its lattice
+    const CellSpace& C = sp.at(g, &lg);
+    const uint64_t* __restrict__ S = C.signs;
+    const int64_t nx = C.nx, ny = C.ny;
+    const int64_t nxy = nx * ny;
+    const uint32_t spr = C.segs_per_row;
+    const uint32_t cyn = static_cast<uint32_t>(ny - 1);
+    // ids written into index lists are relative to the lattice's first vertex
+    const uint32_t vbase = A.seg[C.seg_off].x;
+    const uint32_t row = lg / spr;
+    const uint32_t w = lg - row * spr;
     const int64_t cz = row / cyn;
     const int64_t cy = row - cz * cyn;
     const int64_t cx = 64 * static_cast<int64_t>(w) + b;
-    const bool prev_seg = (b == 0) && (w > 0);  // owners at x-1 live in segment w-1
 
-    // Row words of point rows y-1..y+1, z-1..z+1 at segment w (and w-1 when
-    // the cell is its segment's first), indexed [dz][dy].
-    uint64_t rw[3][3], rn[3][3], pw[3][3], pn[3][3];
+    // Loads that depend only on the cell's position go out together with the
+    // sign words below, so the cell costs one dependent memory round trip
+    // before the owners' segment words: the first vertex id of every cell
+    // that can own one of its edges (segment of (x-rx, y-ry, z-rz)), the
+    // cell's triangle planes and first triangle id, and for a grid source
+    // its eight corner values.
+    uint32_t obase_pre[8];
 #pragma unroll
-    for (int dz = 0; dz < 3; ++dz) {
-#pragma unroll
-      for (int dy = 0; dy < 3; ++dy) {
-        rw[dz][dy] = rn[dz][dy] = pw[dz][dy] = pn[dz][dy] = 0ull;
-        const int64_t y = cy - 1 + dy, z = cz - 1 + dz;
-        if (y >= 0 && z >= 0) {
-          const int64_t f = z * nxy + y * nx + 64 * static_cast<int64_t>(w);
-          rw[dz][dy] = row_word(A.S, f, &rn[dz][dy]);
-          if (prev_seg) pw[dz][dy] = row_word(A.S, f - 64, &pn[dz][dy]);
-        }
-      }
+    for (int o = 0; o < 8; ++o) {
+      const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
+      obase_pre[o] = 0;
+      if ((rx && cx < 1) || (ry && cy < 1) || (rz && cz < 1)) continue;
+      const int64_t ow = (rx && b == 0) ? static_cast<int64_t>(w) - 1 : static_cast<int64_t>(w);
+      const uint64_t og = C.seg_off + (static_cast<uint64_t>(cz - rz) * cyn + (cy - ry)) * spr + ow;
+      obase_pre[o] = A.seg[og].x;
     }
+    const uint64_t* tp = A.tplanes + 3 * static_cast<uint64_t>(g);
+    const uint64_t tp0 = tp[0], tp1 = tp[1], tp2 = tp[2];
+    const uint32_t tseg = A.seg[g].y;
+    float cv[8];
+    if (kSrc != kEmitProgram) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        cv[k] = C.grid[((cz + tables::kCorner[k][2]) * ny + cy + tables::kCorner[k][1]) * nx + cx +
+                       tables::kCorner[k][0]];
+    }
+
     // 3x3x3 sign neighbourhood: every cell that can own one of this cell's
-    // edges lies inside it.
+    // edges lies inside it.  Row (dy, dz) contributes the bits of samples
+    // x-1, x, x+1, read straight from the flat sign array (the guard words
+    // cover x-1 at the array's start; bits of rows or columns outside the
+    // lattice are never used).
     uint32_t cube = 0;
 #pragma unroll
     for (int dz = 0; dz < 3; ++dz) {
 #pragma unroll
       for (int dy = 0; dy < 3; ++dy) {
-        const uint64_t v = rw[dz][dy];
-        const uint32_t lo = b ? static_cast<uint32_t>((v >> (b - 1)) & 1ull)
-                              : static_cast<uint32_t>(pw[dz][dy] >> 63);
-        const uint32_t mid = static_cast<uint32_t>((v >> b) & 1ull);
-        const uint32_t hi = b < 63 ? static_cast<uint32_t>((v >> (b + 1)) & 1ull)
-                                   : static_cast<uint32_t>(rn[dz][dy]);
-        cube |= (lo | (mid << 1) | (hi << 2)) << (3 * dy + 9 * dz);
+        const int64_t y = cy - 1 + dy, z = cz - 1 + dz;
+        if (y >= 0 && z >= 0) {
+          const int64_t q = z * nxy + y * nx + cx - 1;  // sample x-1 (>= -1)
+          const int64_t wq = q >> 6;
+          const int sq = static_cast<int>(q & 63);
+          const uint64_t lo = S[wq], hi = S[wq + 1];
+          const uint64_t v = sq ? ((lo >> sq) | (hi << (64 - sq))) : lo;
+          cube |= static_cast<uint32_t>(v & 7ull) << (3 * dy + 9 * dz);
+        }
       }
     }
     const uint32_t c0 = case_in_cube(cube, 1, 1, 1);
@@ -649,33 +614,28 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, Mc
     }
 
     // First vertex id, cube index and owned crossed edges of each needed owner.
-#pragma unroll
+#pragma unroll 1
     for (int o = 0; o < 8; ++o) {
       if (!(needed & (1u << o))) continue;
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
       const int64_t ox = cx - rx, oy = cy - ry, oz = cz - rz;
       const uint32_t oc = case_in_cube(cube, 1 - rx, 1 - ry, 1 - rz);
       const uint32_t oown = owned_edges(ox, oy, oz) & s_t.edge_mask[oc];
-      // Owner's segment words: point rows oy..oy+1 (dy = 1-ry, 2-ry) and
-      // oz..oz+1 (dz = 1-rz, 2-rz), segment w, or w-1 for x-1 at bit 0.
+      // Owner's segment words: point rows oy..oy+1 and oz..oz+1 of its
+      // segment (w, or w-1 for x-1 at bit 0), re-read from the sign array.
       const bool pseg = rx && b == 0;
-      const int y0i = 1 - ry, z0i = 1 - rz;
-      SegWords os;
-      if (pseg) {
-        os = seg_words(pw[z0i][y0i], pn[z0i][y0i], pw[z0i][y0i + 1], pn[z0i][y0i + 1],
-                       pw[z0i + 1][y0i], pn[z0i + 1][y0i], pw[z0i + 1][y0i + 1],
-                       pn[z0i + 1][y0i + 1]);
-      } else {
-        os = seg_words(rw[z0i][y0i], rn[z0i][y0i], rw[z0i][y0i + 1], rn[z0i][y0i + 1],
-                       rw[z0i + 1][y0i], rn[z0i + 1][y0i], rw[z0i + 1][y0i + 1],
-                       rn[z0i + 1][y0i + 1]);
-      }
       const int64_t ow = pseg ? static_cast<int64_t>(w) - 1 : static_cast<int64_t>(w);
       const int ob = pseg ? 63 : b - rx;
-      const uint64_t og = (static_cast<uint64_t>(oz) * cyn + oy) * spr + ow;
+      const int64_t f00 = oz * nxy + oy * nx + 64 * ow;  // re-read: cache hits
+      uint64_t n00, n10, n01, n11;
+      const uint64_t w00 = row_word(S, f00, &n00);
+      const uint64_t w10 = row_word(S, f00 + nx, &n10);
+      const uint64_t w01 = row_word(S, f00 + nxy, &n01);
+      const uint64_t w11 = row_word(S, f00 + nxy + nx, &n11);
+      const SegWords os = seg_words(w00, n00, w10, n10, w01, n01, w11, n11);
       const uint32_t pre =
           owned_count(os, seg_valid(nx, ow) & low_mask(ob), oy == 0, oz == 0, ow == 0);
-      s_obase[o][tid] = A.seg[og].x + pre;
+      s_obase[o][tid] = obase_pre[o] + pre;
       s_oinfo[o][tid] = oc | (oown << 8);
     }
 
@@ -691,11 +651,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, Mc
                       uz = cz + tables::kCorner[u][2];
         const int64_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
                       vz = cz + tables::kCorner[v][2];
-        const float va = lattice_value<kSrc>(A, L, ux, uy, uz);
-        const float vb = lattice_value<kSrc>(A, L, vx, vy, vz);
+        const float va = kSrc != kEmitProgram ? cv[u] : space_value<kSrc>(C, ux, uy, uz);
+        const float vb = kSrc != kEmitProgram ? cv[v] : space_value<kSrc>(C, vx, vy, vz);
         float pa[3], pb[3], p[3];
-        lattice_position<kSrc>(A, L, ux, uy, uz, pa);
-        lattice_position<kSrc>(A, L, vx, vy, vz, pb);
+        space_position<kSrc>(C, ux, uy, uz, pa);
+        space_position<kSrc>(C, vx, vy, vz, pb);
         crossing(va, vb, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], p);
         if (id < A.vcap) {
           A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
@@ -707,9 +667,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, Mc
 
     // Triangles: cell order, then table order (mesh.rs:91-100).
     const uint64_t lm = low_mask(b);
-    const uint64_t* tp = A.tplanes + 3 * static_cast<uint64_t>(g);
-    const uint64_t tbase = A.seg[g].y + popc64(tp[0] & lm) + 2 * popc64(tp[1] & lm) +
-                           4 * popc64(tp[2] & lm);
+    const uint64_t tbase =
+        tseg + popc64(tp0 & lm) + 2 * popc64(tp1 & lm) + 4 * popc64(tp2 & lm);
     const uint32_t nidx = 3u * s_t.tri_count[c0];
     for (uint32_t j = 0; j < nidx; ++j) {
       const int e = s_t.tri[c0][j];
@@ -728,8 +687,27 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Lattice L, Mc
       const uint32_t id =
           s_obase[o][tid] + static_cast<uint32_t>(__popc(oown & s_t.before[oc][eo]));
       const uint64_t ix = 3 * tbase + j;
-      if (ix < A.icap) A.indices[ix] = id;
+      if (ix < A.icap) A.indices[ix] = id - vbase;
     }
+  }
+}
+
+// Per chunk first vertex / index of the batch output (n_chunks + 1 entries).
+__global__ __launch_bounds__(256) void k_batch_offsets(const uint2* __restrict__ seg,
+                                                       const uint32_t* __restrict__ seg_start,
+                                                       uint32_t n_chunks,
+                                                       const unsigned long long* __restrict__ totals,
+                                                       uint64_t* __restrict__ voff,
+                                                       uint64_t* __restrict__ ioff) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n_chunks) return;
+  if (c < n_chunks) {
+    const uint2 f = seg[seg_start[c]];
+    voff[c] = f.x;
+    ioff[c] = 3ull * f.y;
+  } else {
+    voff[c] = totals[kTotVertices];
+    ioff[c] = 3ull * totals[kTotTriangles];
   }
 }
 
@@ -779,7 +757,8 @@ ResetArgs reset_args(const McDims& D, const McBuffers& B) {
   ResetArgs r;
   r.signs = B.signs;
   r.n_words = D.n_sign_words;
-  r.flags = B.flags;
+  r.agg = B.agg;
+  r.incl = B.incl;
   r.n_blocks = D.n_blocks;
   r.totals = B.totals;
   return r;
@@ -794,7 +773,7 @@ McDims mc_dims(const Lattice& L) {
   D.n_cell_rows = (L.ny - 1) * (L.nz - 1);
   D.segs_per_row = (L.nx - 1 + 63) / 64;
   D.n_segs = D.n_cell_rows * D.segs_per_row;
-  D.n_blocks = (D.n_segs + kCountBlock - 1) / kCountBlock;
+  D.n_blocks = (D.n_segs + kCountSegs - 1) / kCountSegs;
   D.n_cells = (L.nx - 1) * (L.ny - 1) * (L.nz - 1);
   return D;
 }
@@ -804,7 +783,7 @@ hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& 
   (void)L;
   if (!grid) return hipErrorInvalidValue;
   const ResetArgs R = reset_args(D, B);
-  // At least enough threads to clear the look-back flags in one sweep.
+  // At least enough threads to clear the look-back words in one sweep.
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
   if ((reinterpret_cast<uintptr_t>(grid) & 15u) == 0) {
     int64_t blocks = grid_for((D.n_points + 256 * kSignVec - 1) / (256 * kSignVec));
@@ -832,46 +811,159 @@ hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, c
   return hipGetLastError();
 }
 
+namespace {
+
+SingleSpace single_space(const Lattice& L, const McDims& D, const uint64_t* signs,
+                         const DevOp* ops, int n_ops, const float* grid,
+                         const float* positions) {
+  SingleSpace sp;
+  sp.C.nx = L.nx;
+  sp.C.ny = L.ny;
+  sp.C.nz = L.nz;
+  for (int i = 0; i < 3; ++i) {
+    sp.C.origin[i] = L.origin[i];
+    sp.C.spacing[i] = L.spacing[i];
+  }
+  sp.C.signs = signs;
+  sp.C.segs_per_row = static_cast<uint32_t>(D.segs_per_row);
+  sp.C.seg_off = 0;
+  sp.C.ops = ops;
+  sp.C.n_ops = n_ops;
+  sp.C.reserved = 0;
+  sp.C.grid = grid;
+  sp.C.positions = positions;
+  return sp;
+}
+
+EmitArgs emit_args(const McBuffers& B) {
+  EmitArgs A;
+  A.seg = B.seg;
+  A.tplanes = B.tplanes;
+  A.active = B.active;
+  A.active_capacity = B.active_capacity;
+  A.vertices = B.vertices;
+  A.indices = B.indices;
+  A.vcap = B.vertex_capacity;
+  A.icap = B.index_capacity;
+  A.totals = B.totals;
+  return A;
+}
+
+// Grid-stride over the active list (its length is only known on the device):
+// at most `per_cu` resident workgroups per CU (0: as many as fit).
+template <class K>
+unsigned emit_blocks(const McBuffers& B, int per_cu, K kernel) {
+  if (per_cu <= 0) {
+    int fit = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, kernel, kEmitThreads, 0) !=
+            hipSuccess ||
+        fit < 1)
+      fit = 4;
+    per_cu = fit;
+  }
+  int64_t blocks = (static_cast<int64_t>(B.active_capacity) + kEmitThreads - 1) / kEmitThreads;
+  const int64_t cap = static_cast<int64_t>(per_cu) * device_cus();
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  return static_cast<unsigned>(blocks);
+}
+
+}  // namespace
+
 hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s) {
   if (D.n_blocks > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_count, dim3(static_cast<unsigned>(D.n_blocks)), dim3(kCountBlock), 0, s,
-                     B.signs, L, D, B);
+  const SingleSpace sp = single_space(L, D, B.signs, nullptr, 0, nullptr, nullptr);
+  hipLaunchKernelGGL((k_count<SingleSpace, kCountRounds>), dim3(static_cast<unsigned>(D.n_blocks)),
+                     dim3(kCountBlock), 0, s, sp, D.n_segs, D.n_blocks, B);
   return hipGetLastError();
 }
 
 hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const float* positions,
                        const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s) {
   if ((positions && !grid) || (!grid && !ops)) return hipErrorInvalidValue;
-  EmitArgs A;
-  A.S = B.signs;
-  A.seg = B.seg;
-  A.tplanes = B.tplanes;
-  A.active = B.active;
-  A.active_capacity = B.active_capacity;
-  A.grid = grid;
-  A.ops = ops;
-  A.n_ops = n_ops;
-  A.positions = positions;
-  A.vertices = B.vertices;
-  A.indices = B.indices;
-  A.vcap = B.vertex_capacity;
-  A.icap = B.index_capacity;
-  A.totals = B.totals;
-  // Grid-stride over the active list (its length is only known on the device).
-  int64_t blocks = (static_cast<int64_t>(B.active_capacity) + kEmitThreads - 1) / kEmitThreads;
-  const int64_t cap = 8LL * device_cus();
-  if (blocks > cap) blocks = cap;
-  if (blocks < 1) blocks = 1;
+  const EmitArgs A = emit_args(B);
+  const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, grid, positions);
   if (grid && positions) {
-    hipLaunchKernelGGL(k_emit<kEmitPositions>, dim3(static_cast<unsigned>(blocks)),
-                       dim3(kEmitThreads), 0, s, A, L, D);
+    auto k = k_emit<kEmitPositions, SingleSpace>;
+    hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
   } else if (grid) {
-    hipLaunchKernelGGL(k_emit<kEmitGrid>, dim3(static_cast<unsigned>(blocks)), dim3(kEmitThreads),
-                       0, s, A, L, D);
+    auto k = k_emit<kEmitGrid, SingleSpace>;
+    hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
   } else {
-    hipLaunchKernelGGL(k_emit<kEmitProgram>, dim3(static_cast<unsigned>(blocks)),
-                       dim3(kEmitThreads), 0, s, A, L, D);
+    auto k = k_emit<kEmitProgram, SingleSpace>;
+    hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_signs(const BatchPlan& P, const McBuffers& B, int words_per_wave,
+                              hipStream_t s) {
+  // The state reset covers the look-back words of the finest count tile.
+  const int64_t n_blocks = batch_count_blocks(P, 1);
+  ResetArgs R;
+  R.signs = nullptr;
+  R.n_words = 0;
+  R.agg = B.agg;
+  R.incl = B.incl;
+  R.n_blocks = n_blocks;
+  R.totals = B.totals;
+  const int64_t wpw = words_per_wave > 0 ? words_per_wave : 1;
+  int64_t blocks = (P.n_words + 4 * wpw - 1) / (4 * wpw);
+  if (blocks < 1) blocks = 1;
+  if (blocks > 65536) blocks = 65536;
+  const int64_t min_blocks = (n_blocks + 255) / 256;
+  const int64_t chunk_blocks = (static_cast<int64_t>(P.n_chunks) + 255) / 256;
+  if (blocks < min_blocks) blocks = min_blocks;
+  if (blocks < chunk_blocks) blocks = chunk_blocks;
+  hipLaunchKernelGGL(k_batch_signs, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, P.chunks,
+                     P.word_start, P.n_chunks, R);
+  return hipGetLastError();
+}
+
+int64_t batch_count_blocks(const BatchPlan& P, int rounds) {
+  const int64_t per = static_cast<int64_t>(kCountBlock) * rounds;
+  return (P.n_segs + per - 1) / per;
+}
+
+hipError_t launch_batch_count(const BatchPlan& P, const McBuffers& B, int rounds,
+                              hipStream_t s) {
+  const int64_t nb = batch_count_blocks(P, rounds);
+  if (nb > 0x7fffffff) return hipErrorInvalidValue;
+  BatchSpace sp;
+  sp.chunks = P.chunks;
+  sp.seg_start = P.seg_start;
+  sp.n_chunks = P.n_chunks;
+  const dim3 grid(static_cast<unsigned>(nb)), block(kCountBlock);
+  switch (rounds) {
+    case 1:
+      hipLaunchKernelGGL((k_count<BatchSpace, 1>), grid, block, 0, s, sp, P.n_segs, nb, B);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_count<BatchSpace, 2>), grid, block, 0, s, sp, P.n_segs, nb, B);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_emit(const BatchPlan& P, const McBuffers& B, int blocks_per_cu,
+                             hipStream_t s) {
+  BatchSpace sp;
+  sp.chunks = P.chunks;
+  sp.seg_start = P.seg_start;
+  sp.n_chunks = P.n_chunks;
+  auto k = k_emit<kEmitProgram, BatchSpace>;
+  hipLaunchKernelGGL(k, dim3(emit_blocks(B, blocks_per_cu, k)), dim3(kEmitThreads), 0, s,
+                     emit_args(B), sp);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_offsets(const BatchPlan& P, const McBuffers& B, uint64_t* voff,
+                                uint64_t* ioff, hipStream_t s) {
+  const unsigned blocks = (P.n_chunks + 1 + 255) / 256;
+  hipLaunchKernelGGL(k_batch_offsets, dim3(blocks), dim3(256), 0, s, B.seg, P.seg_start,
+                     P.n_chunks, B.totals, voff, ioff);
   return hipGetLastError();
 }
 

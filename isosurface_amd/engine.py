@@ -218,6 +218,45 @@ class Context:
         nv, ni = int(out.vertex_count), int(out.index_count)
         return Mesh(verts[:nv].copy(), idx[:ni].copy())
 
+    def upload_batch(self, chunks) -> None:
+        """iso_batch_upload: chunks = [(Source or node list, IsoGridDesc), ...]."""
+        nodes, descs = [], (A.IsoChunkDesc * len(chunks))()
+        for i, (src, desc) in enumerate(chunks):
+            prog = src.program() if isinstance(src, Source) else list(src)
+            descs[i].grid = desc
+            descs[i].grid.source = A.ISO_SOURCE_PROGRAM
+            descs[i].node_offset = len(nodes)
+            descs[i].node_count = len(prog)
+            nodes.extend(prog)
+        arr = to_nodes(nodes)
+        self.check(self.lib.iso_batch_upload(self.handle, C.cast(arr, C.c_void_p), len(nodes),
+                                             descs, len(chunks)), "iso_batch_upload")
+        self._batch_chunks = len(chunks)
+
+    def extract_batch(self, vertex_capacity: int = 0, index_capacity: int = 0,
+                      flags: int = 0) -> List[Mesh]:
+        """iso_extract_batch into host arrays; one Mesh per chunk."""
+        n = self._batch_chunks
+        voff = np.zeros(n + 1, np.uint64)
+        ioff = np.zeros(n + 1, np.uint64)
+        vcap = max(int(vertex_capacity), 1)
+        icap = max(int(index_capacity), 3)
+        verts = np.empty((vcap, 3), dtype=np.float32)
+        idx = np.empty(icap, dtype=np.uint32)
+        out = A.IsoBatchOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, voff.ctypes.data,
+                            ioff.ctypes.data, 0, 0, A.ISO_MEMORY_HOST, flags)
+        st = self.lib.iso_extract_batch(self.handle, C.byref(out))
+        if st == A.ISO_ERR_CAPACITY:
+            nv, ni = int(out.vertex_count), int(out.index_count)
+            verts = np.empty((max(nv, 1), 3), dtype=np.float32)
+            idx = np.empty(max(ni, 3), dtype=np.uint32)
+            mo = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, max(nv, 1), max(ni, 3), 0, 0,
+                              A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.handle, C.byref(mo))
+        self.check(st, "iso_extract_batch")
+        return [Mesh(verts[int(voff[i]):int(voff[i + 1])].copy(),
+                     idx[int(ioff[i]):int(ioff[i + 1])].copy()) for i in range(n)]
+
     # ---- timing
     def set_timing(self, on: bool) -> None:
         self.check(self.lib.iso_ctx_set_timing(self.handle, 1 if on else 0), "set_timing")
@@ -355,3 +394,38 @@ class LinearHashedMarchingCubes:
     def extract(self, source: Source, extractor: Extractor) -> None:
         """LinearHashedMarchingCubes::extract (linear_hashed_marching_cubes.rs:62-84)."""
         _replay(self.extract_mesh(source), extractor)
+
+
+def chunk_desc(size: int) -> A.IsoGridDesc:
+    """A chunk of size^3 cells: (size+1)^3 samples, spacing 1/size, over [0, 1]^3."""
+    d = A.IsoGridDesc()
+    d.nx = d.ny = d.nz = size + 1
+    for i in range(3):
+        d.origin[i] = 0.0
+        d.spacing[i] = float(np.float32(1.0) / np.float32(size))
+    d.source = A.ISO_SOURCE_PROGRAM
+    return d
+
+
+class BatchedMarchingCubes:
+    """Many chunks, each MarchingCubes over its own lattice and program,
+    extracted in one pass (iso_batch_upload + iso_extract_batch).  extract
+    gives one mesh per chunk, each identical to extracting that chunk alone."""
+
+    def __init__(self, context: Optional[Context] = None, graph: bool = False,
+                 autotune: bool = False):
+        self.ctx = context or default_context()
+        self.flags = (A.ISO_BATCH_GRAPH if graph else 0) | (A.ISO_BATCH_AUTOTUNE if autotune else 0)
+        self._last_counts = (0, 0)
+
+    def extract_meshes(self, chunks) -> List[Mesh]:
+        """chunks = [(Source, IsoGridDesc), ...]."""
+        self.ctx.upload_batch(chunks)
+        meshes = self.ctx.extract_batch(*self._last_counts, flags=self.flags)
+        self._last_counts = (sum(len(m.vertices) for m in meshes),
+                             sum(len(m.indices) for m in meshes))
+        return meshes
+
+    def extract(self, chunks, extractors: Sequence[Extractor]) -> None:
+        for mesh, ex in zip(self.extract_meshes(chunks), extractors):
+            _replay(mesh, ex)

@@ -93,3 +93,20 @@ def random_csg(rng: SplitMix64, n_prims: int) -> Source:
         op = rng.next_u64() % 4
         acc = Difference(p, acc) if op == 3 else Union(acc, p)
     return DemoSource(acc)
+
+
+CHUNK_SIZES = (16, 32, 48, 64)
+
+
+def batch_chunks(n_chunks: int = 4096, seed: int = 6) -> List[Tuple[Source, int]]:
+    """The batched-extraction workload: n chunks, each (source, S) with S
+    drawn uniformly from {16, 32, 48, 64} and a seeded CSG of 1-8 spheres,
+    boxes and tori.  Chunk i is S^3 cells: a lattice of (S+1)^3 samples with
+    spacing 1/S over [0, 1]^3 (chunk_desc)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n_chunks):
+        size = CHUNK_SIZES[rng.next_u64() % 4]
+        n_prims = 1 + int(rng.next_u64() % 8)
+        out.append((random_csg(rng, n_prims), size))
+    return out

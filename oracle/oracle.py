@@ -110,6 +110,19 @@ def lattice_desc(nx, ny, nz, origin=(0.0, 0.0, 0.0), spacing=(1.0, 1.0, 1.0), so
     return d
 
 
+def as_desc(d) -> IsoGridDesc:
+    """The oracle's descriptor type from any struct with the same fields."""
+    if isinstance(d, IsoGridDesc):
+        return d
+    o = IsoGridDesc()
+    o.nx, o.ny, o.nz = d.nx, d.ny, d.nz
+    for i in range(3):
+        o.origin[i] = d.origin[i]
+        o.spacing[i] = d.spacing[i]
+    o.source, o.flags = d.source, d.flags
+    return o
+
+
 def sample(program, x, y, z) -> float:
     lib = load()
     arr = nodes(program)
@@ -149,6 +162,7 @@ def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=Fal
         grid = np.ascontiguousarray(grid, dtype=np.float32)
         gptr = grid.ctypes.data
     m = OrMesh()
+    desc = as_desc(desc)
     st = lib.or_extract_mc(arr, n, gptr, C.byref(desc), 1 if want_cases else 0, C.byref(m))
     assert st == 0, st
     try:
@@ -324,3 +338,9 @@ def lhmc_octree(program, max_depth: int):
             return keys[:k], vals[:k], leaf[:k].astype(bool)
         assert st == 2, st  # ISO_ERR_CAPACITY
         cap = int(n.value)
+
+
+def extract_batch(chunks):
+    """Per chunk (program, desc): MarchingCubes over the chunk's lattice —
+    the batched extraction is, by definition, these independent meshes."""
+    return [extract_mc(program, desc=desc) for program, desc in chunks]

@@ -56,6 +56,25 @@ def main() -> None:
             "indices_sha256": digest(m.indices.astype("<u4")),
         }
         print(name, out[name]["vertices"], out[name]["triangles"], flush=True)
+    # The batched-chunk workload: per-chunk oracle meshes concatenated in chunk order.
+    from isosurface_amd.engine import chunk_desc  # noqa: E402
+    vs, ix = [], []
+    for src, size in W.batch_chunks(4096, seed=6):
+        m = O.extract_mc(src.program(), desc=chunk_desc(size))
+        vs.append(m.vertices)
+        ix.append(m.indices)
+    import numpy as np  # noqa: E402
+    v = np.concatenate(vs)
+    i = np.concatenate(ix)
+    out["batch4096_seed6"] = {
+        "size": 0,
+        "vertices": int(len(v)),
+        "triangles": int(len(i) // 3),
+        "vertices_sha256": digest(v.astype("<f4")),
+        "indices_sha256": digest(i.astype("<u4")),
+    }
+    print("batch4096_seed6", out["batch4096_seed6"]["vertices"],
+          out["batch4096_seed6"]["triangles"], flush=True)
     path = Path(__file__).with_name("mesh_digests.json")
     path.write_text(json.dumps(out, indent=1) + "\n")
 
