@@ -23,6 +23,12 @@ Printed JSON line:
                 4096 chunks of 16^3..64^3 cells, each its own seeded 1-8
                 primitive CSG program, extracted by one captured HIP graph
                 per batch after the autotuner picked the launch shapes
+  learned_band  1024^3 cells of the smooth-min (k=0.02) blend of 1024 seeded
+                spheres: the Fourier-feature MLP is trained on the GPU on 2M
+                samples the plain path generates, its margin calibrated on
+                held-out samples; band extraction is timed against the plain
+                dense GPU extraction of the same program, which judges it
+                (identical mesh, missed surface cells)
   dual_contouring, octree_mc
                 the sharp-feature workload (union of 32 seeded boxes):
                 DualContouring at 128^3 with central-difference normals and
@@ -262,6 +268,77 @@ def batched_chunks(ctx: Context, steps: int) -> dict:
             "triangles": int(out.index_count) // 3, "parity_vs_oracle": parity}
 
 
+def learned_band(ctx: Context, size: int, reps: int) -> dict:
+    """The learned narrow band against the dense path at size^3 cells."""
+    from isosurface_amd.band import NarrowBand, training_samples
+    src = W.smooth_spheres(1024, k=0.02, seed=5)
+    t0 = time.perf_counter()
+    pts, sdf = training_samples(ctx, src, 1_000_000, 1_000_000, seed=11, coarse=257)
+    cal_p, cal_s = training_samples(ctx, src, 500_000, 500_000, seed=12, coarse=257)
+    t_samples = time.perf_counter() - t0
+    band = NarrowBand(ctx)
+    band.init(7)
+    steps = 3000
+    t0 = time.perf_counter()
+    loss = band.train(pts, sdf, steps=steps, batch=16384, lr=3e-3, lr_min=5e-5, clamp=0.05,
+                      seed=13)
+    t_train = time.perf_counter() - t0
+    margin = band.calibrate(cal_p, cal_s)
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(size + 1, A.ISO_SOURCE_PROGRAM)
+    cells = size * size * (size + 1)
+    probe = DeviceMesh(ctx, 1, 3)
+    out = probe.out()
+    st = ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out))
+    if st not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+        ctx.check(st, "iso_extract_mc")
+    nv, ni = int(out.vertex_count), int(out.index_count)
+    probe.free()
+    dense = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = dense.out()
+    ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out)), "dense")
+    ctx.synchronize()
+    t_dense = time.perf_counter() - t0
+    dd = digest(ctx, dense, out)
+    dense.free()
+    meshb = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
+    stats = A.IsoBandStats()
+    times = []
+    for _ in range(max(1, reps)):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        outb = meshb.out()
+        ctx.check(ctx.lib.iso_extract_mc_band(ctx.handle, C.byref(desc), margin, C.byref(outb),
+                                              C.byref(stats)), "band")
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    t_band = min(times)
+    same = (int(outb.vertex_count), int(outb.index_count)) == (nv, ni) and \
+        digest(ctx, meshb, outb) == dd
+    meshb.free()
+    missed = C.c_uint64(0)
+    ctx.check(ctx.lib.iso_band_missed_cells(ctx.handle, C.byref(desc), C.byref(missed)), "missed")
+    res = {
+        "workload": (f"MarchingCubes::new({size + 1}) ({cells} cells) on the smooth-min (k=0.02) "
+                     "of 1024 seeded spheres, radii U[0.01,0.05], program source"),
+        "dense_ms": 1e3 * t_dense, "band_ms": 1e3 * t_band, "speedup": t_dense / t_band,
+        "dense_cells_per_s": cells / t_dense, "band_cells_per_s": cells / t_band,
+        "bricks": int(stats.bricks), "band_bricks": int(stats.band_bricks),
+        "band_fraction": int(stats.band_bricks) / max(int(stats.bricks), 1),
+        "exact_samples": int(stats.exact_samples), "margin": margin,
+        "missed_cells": int(missed.value), "identical_to_dense": same,
+        "vertices": nv, "triangles": ni // 3,
+        "training": {"samples": len(sdf), "steps": steps, "batch": 16384, "final_loss": loss,
+                     "train_ms": 1e3 * t_train, "sample_generation_ms": 1e3 * t_samples},
+    }
+    log(f"learned band: dense {1e3 * t_dense:.1f} ms, band {1e3 * t_band:.1f} ms, "
+        f"x{t_dense / t_band:.2f}, band {res['band_fraction']:.3f}, missed {missed.value}, "
+        f"identical {same}")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,7 +349,12 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-side", action="store_true",
-                    help="skip the dual-contouring and octree workloads")
+                    help="skip the batched, dual-contouring and octree workloads")
+    ap.add_argument("--band-size", type=int, default=1024,
+                    help="cells per axis of the learned-band workload (0: skip it)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only a few grid extractions and three 64 MiB device copies (the "
+                         "scale reference), for hardware-counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the extraction path has no multi-GPU mode "
@@ -314,6 +396,22 @@ def main() -> int:
         dv, di = digest(ctx, mesh, out)
         parity = dv == golden[key]["vertices_sha256"] and di == golden[key]["indices_sha256"]
         log(f"parity vs oracle digests: {parity}")
+
+    if args.traffic_probe:
+        # Counter passes read per-dispatch FETCH_SIZE / WRITE_SIZE: the grid
+        # kernels of a few extractions, then copies of exactly 64 MiB.
+        for _ in range(3):
+            extract(ctx, desc, mesh)
+        nb = 64 << 20
+        a, b = C.c_void_p(), C.c_void_p()
+        ctx.check(ctx.lib.iso_device_alloc(ctx.handle, nb, C.byref(a)), "alloc")
+        ctx.check(ctx.lib.iso_device_alloc(ctx.handle, nb, C.byref(b)), "alloc")
+        for _ in range(3):
+            ctx.check(ctx.lib.iso_memcpy_device(ctx.handle, b, a, nb), "copy")
+        ctx.lib.iso_device_free(ctx.handle, a)
+        ctx.lib.iso_device_free(ctx.handle, b)
+        print(json.dumps({"traffic_probe": True, "copy_bytes": nb, "samples": n}), flush=True)
+        return 0
 
     for _ in range(args.warmup):
         extract(ctx, desc, mesh)
@@ -390,6 +488,8 @@ def main() -> int:
     }
 
     if not args.no_side:
+        if args.band_size > 0:
+            result["learned_band"] = learned_band(ctx, args.band_size, 3)
         result["batched_chunks"] = batched_chunks(ctx, max(2, args.program_steps))
         result.update(side_workloads(ctx, max(1, args.program_steps), not args.no_cpu_baseline))
 

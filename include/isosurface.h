@@ -230,6 +230,60 @@ typedef struct iso_batch_out {
  * host-memory results are kept for iso_mesh_fetch). */
 int iso_extract_batch(iso_ctx* ctx, iso_batch_out* out);
 
+/* ---- learned narrow band ----------------------------------------------------
+ * No reference counterpart: this build's own acceleration of dense program
+ * extraction.  A Fourier-feature MLP (99 features: p, sin and cos of
+ * pi 2^(k/2) p for k < 16; three hidden layers of 64 ReLU units; bf16 MFMA
+ * with f32 accumulation) is trained on (point, SDF) samples to predict the
+ * truncated SDF.  iso_extract_mc_band predicts it at the centre of every
+ * 8x8x8-cell brick, evaluates the program exactly only in bricks whose
+ * prediction lies within their half-diagonal plus `margin` of zero, gives
+ * every other sample its brick's predicted sign, and extracts as
+ * iso_extract_mc does.  The result equals iso_extract_mc's whenever no
+ * surface cell lies in a culled brick (iso_band_missed_cells counts them). */
+#define ISO_BAND_PARAMS 15617 /* length of the parameter vector */
+
+typedef struct iso_band_train_params {
+  uint32_t steps;   /* Adam steps */
+  uint32_t batch;   /* samples per step, a multiple of 16 */
+  float lr;         /* learning rate, cosine-decayed to lr_min over the steps */
+  float lr_min;
+  float clamp;      /* targets are clamp(sdf, -clamp, clamp) */
+  uint32_t seed;    /* batch sampling */
+} iso_band_train_params;
+
+typedef struct iso_band_stats {
+  uint64_t bricks;         /* bricks of the lattice */
+  uint64_t band_bricks;    /* bricks evaluated exactly */
+  uint64_t exact_samples;  /* program evaluations of the band (without emit's) */
+  uint64_t reserved;
+} iso_band_stats;
+
+/* Deterministic He-uniform initialisation of the context's network. */
+int iso_band_init(iso_ctx* ctx, uint32_t seed);
+/* Parameter vector (ISO_BAND_PARAMS floats, host memory) in or out:
+ * W1[64][112] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4. */
+int iso_band_set_params(iso_ctx* ctx, const float* theta, uint64_t n);
+int iso_band_get_params(iso_ctx* ctx, float* theta, uint64_t n);
+/* Trains with Adam on n samples (points: 3 floats each; weight may be null)
+ * in `memory`; *last_loss receives the mean weighted squared error / 2 of
+ * the last batch. */
+int iso_band_train(iso_ctx* ctx, const float* points, const float* sdf, const float* weight,
+                   uint64_t n, int32_t memory, const iso_band_train_params* params,
+                   float* last_loss);
+/* Loss and gradient of exactly these n samples (host memory, n a multiple
+ * of 16), without updating the network. */
+int iso_band_gradient(iso_ctx* ctx, const float* points, const float* sdf, const float* weight,
+                      uint64_t n, float clamp, float* grad, float* loss);
+/* Network output at n points (3 floats each) in `memory`. */
+int iso_band_predict(iso_ctx* ctx, const float* points, uint64_t n, float* out, int32_t memory);
+/* Band extraction of the current program over `desc` (ISO_SOURCE_PROGRAM). */
+int iso_extract_mc_band(iso_ctx* ctx, const iso_grid_desc* desc, float margin, iso_mesh_out* out,
+                        iso_band_stats* stats);
+/* Surface cells of the dense extraction of desc that lie in bricks the last
+ * iso_extract_mc_band over the same desc culled. */
+int iso_band_missed_cells(iso_ctx* ctx, const iso_grid_desc* desc, uint64_t* missed);
+
 /* After an ISO_MEMORY_HOST extraction returned ISO_ERR_CAPACITY, the mesh is
  * kept in the context; this copies it into larger buffers without
  * re-extracting (the reference's Vec-growing sink needs no second pass). */
@@ -241,6 +295,7 @@ int iso_device_alloc(iso_ctx* ctx, uint64_t bytes, void** out);
 int iso_device_free(iso_ctx* ctx, void* ptr);
 int iso_memcpy_to_device(iso_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int iso_memcpy_to_host(iso_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int iso_memcpy_device(iso_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ---- per-kernel timing ------------------------------------------------------ */
 

@@ -84,6 +84,19 @@ class IsoBatchOut(C.Structure):
     ]
 
 
+ISO_BAND_PARAMS = 15617
+
+
+class IsoBandTrainParams(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("lr_min", C.c_float), ("clamp", C.c_float), ("seed", C.c_uint32)]
+
+
+class IsoBandStats(C.Structure):
+    _fields_ = [("bricks", C.c_uint64), ("band_bricks", C.c_uint64),
+                ("exact_samples", C.c_uint64), ("reserved", C.c_uint64)]
+
+
 class IsoKernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -111,11 +124,25 @@ SIGNATURES = [
     ("iso_batch_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(IsoChunkDesc),
                                    C.c_uint32]),
     ("iso_extract_batch", C.c_int, [C.c_void_p, C.POINTER(IsoBatchOut)]),
+    ("iso_band_init", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("iso_band_set_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("iso_band_get_params", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("iso_band_train", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                 C.c_int32, C.POINTER(IsoBandTrainParams),
+                                 C.POINTER(C.c_float)]),
+    ("iso_band_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
+                                    C.c_float, C.c_void_p, C.POINTER(C.c_float)]),
+    ("iso_band_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32]),
+    ("iso_extract_mc_band", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float,
+                                      C.POINTER(IsoMeshOut), C.POINTER(IsoBandStats)]),
+    ("iso_band_missed_cells", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc),
+                                        C.POINTER(C.c_uint64)]),
     ("iso_mesh_fetch", C.c_int, [C.c_void_p, C.POINTER(IsoMeshOut)]),
     ("iso_device_alloc", C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     ("iso_device_free", C.c_int, [C.c_void_p, C.c_void_p]),
     ("iso_memcpy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("iso_memcpy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
+    ("iso_memcpy_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     ("iso_ctx_set_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("iso_ctx_kernel_stats", C.c_int, [C.c_void_p, C.POINTER(IsoKernelStat), C.c_int,
                                        C.POINTER(C.c_int)]),

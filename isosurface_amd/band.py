@@ -1,0 +1,137 @@
+"""The learned narrow band (no reference counterpart; see include/isosurface.h).
+
+    band = NarrowBand(ctx)
+    band.init(seed)
+    pts, sdf = training_samples(ctx, source, ...)       # made by the plain path
+    band.train(pts, sdf, steps=..., batch=16384)
+    margin = band.calibrate(held_out_pts, held_out_sdf)  # max |f_hat - clamp(f)|
+    mesh, stats = band.extract_mesh(desc, margin)        # exact SDF only in the band
+    band.missed_cells(desc)                              # judged by the dense path
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _abi as A
+from .engine import Context, MarchingCubes, Mesh, default_context
+from .sources import Source
+
+N_PARAMS = A.ISO_BAND_PARAMS
+
+
+class NarrowBand:
+    def __init__(self, context: Optional[Context] = None):
+        self.ctx = context or default_context()
+        self.clamp = 0.05
+
+    @property
+    def lib(self):
+        return self.ctx.lib
+
+    def init(self, seed: int = 0) -> None:
+        self.ctx.check(self.lib.iso_band_init(self.ctx.handle, seed), "iso_band_init")
+
+    def set_params(self, theta: np.ndarray) -> None:
+        theta = np.ascontiguousarray(theta, np.float32)
+        self.ctx.check(self.lib.iso_band_set_params(self.ctx.handle, theta.ctypes.data,
+                                                    len(theta)), "iso_band_set_params")
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(N_PARAMS, np.float32)
+        self.ctx.check(self.lib.iso_band_get_params(self.ctx.handle, out.ctypes.data, N_PARAMS),
+                       "iso_band_get_params")
+        return out
+
+    def train(self, points: np.ndarray, sdf: np.ndarray, weight: Optional[np.ndarray] = None,
+              steps: int = 2000, batch: int = 16384, lr: float = 3e-3, lr_min: float = 1e-4,
+              clamp: float = 0.05, seed: int = 1) -> float:
+        points = np.ascontiguousarray(points, np.float32)
+        sdf = np.ascontiguousarray(sdf, np.float32)
+        wp = None
+        if weight is not None:
+            weight = np.ascontiguousarray(weight, np.float32)
+            wp = weight.ctypes.data
+        self.clamp = clamp
+        prm = A.IsoBandTrainParams(steps, batch, lr, lr_min, clamp, seed)
+        loss = C.c_float(0.0)
+        self.ctx.check(self.lib.iso_band_train(self.ctx.handle, points.ctypes.data, sdf.ctypes.data,
+                                               wp, len(sdf), A.ISO_MEMORY_HOST, C.byref(prm),
+                                               C.byref(loss)), "iso_band_train")
+        return float(loss.value)
+
+    def gradient(self, points, sdf, weight=None, clamp: float = 0.05) -> Tuple[float, np.ndarray]:
+        points = np.ascontiguousarray(points, np.float32)
+        sdf = np.ascontiguousarray(sdf, np.float32)
+        wp = None
+        if weight is not None:
+            weight = np.ascontiguousarray(weight, np.float32)
+            wp = weight.ctypes.data
+        grad = np.empty(N_PARAMS, np.float32)
+        loss = C.c_float(0.0)
+        self.ctx.check(self.lib.iso_band_gradient(self.ctx.handle, points.ctypes.data,
+                                                  sdf.ctypes.data, wp, len(sdf), clamp,
+                                                  grad.ctypes.data, C.byref(loss)),
+                       "iso_band_gradient")
+        return float(loss.value), grad
+
+    def predict(self, points: np.ndarray) -> np.ndarray:
+        points = np.ascontiguousarray(points, np.float32)
+        out = np.empty(len(points), np.float32)
+        self.ctx.check(self.lib.iso_band_predict(self.ctx.handle, points.ctypes.data, len(points),
+                                                 out.ctypes.data, A.ISO_MEMORY_HOST),
+                       "iso_band_predict")
+        return out
+
+    def calibrate(self, points: np.ndarray, sdf: np.ndarray, safety: float = 1.25) -> float:
+        """Margin: the largest |prediction - truncated target| on held-out
+        samples, times `safety`."""
+        err = np.abs(self.predict(points) - np.clip(sdf, -self.clamp, self.clamp))
+        return float(np.max(err)) * safety
+
+    def extract_mesh(self, desc: A.IsoGridDesc, margin: float,
+                     vertex_capacity: int = 0, index_capacity: int = 0):
+        vcap = max(int(vertex_capacity), 1)
+        icap = max(int(index_capacity), 3)
+        verts = np.empty((vcap, 3), np.float32)
+        idx = np.empty(icap, np.uint32)
+        out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
+                           A.ISO_MEMORY_HOST, 0)
+        stats = A.IsoBandStats()
+        st = self.lib.iso_extract_mc_band(self.ctx.handle, C.byref(desc), margin, C.byref(out),
+                                          C.byref(stats))
+        if st == A.ISO_ERR_CAPACITY:
+            nv, ni = int(out.vertex_count), int(out.index_count)
+            verts = np.empty((max(nv, 1), 3), np.float32)
+            idx = np.empty(max(ni, 3), np.uint32)
+            mo = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, max(nv, 1), max(ni, 3), 0, 0,
+                              A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.ctx.handle, C.byref(mo))
+        self.ctx.check(st, "iso_extract_mc_band")
+        nv, ni = int(out.vertex_count), int(out.index_count)
+        return Mesh(verts[:nv].copy(), idx[:ni].copy()), stats
+
+    def missed_cells(self, desc: A.IsoGridDesc) -> int:
+        n = C.c_uint64(0)
+        self.ctx.check(self.lib.iso_band_missed_cells(self.ctx.handle, C.byref(desc), C.byref(n)),
+                       "iso_band_missed_cells")
+        return int(n.value)
+
+
+def training_samples(ctx: Context, source: Source, n_uniform: int, n_surface: int,
+                     seed: int = 0, jitter: float = 0.01, coarse: int = 129):
+    """(points, sdf) generated by the plain path: uniform points in [0, 1]^3
+    plus vertices of a coarse plain GPU extraction, jittered by
+    N(0, jitter); targets are the program's exact values (GPU)."""
+    rng = np.random.default_rng(seed)
+    pts = [rng.random((n_uniform, 3), dtype=np.float32)]
+    if n_surface:
+        mesh = MarchingCubes(coarse, ctx).extract_mesh(source)
+        pick = rng.integers(0, len(mesh.vertices), n_surface)
+        near = mesh.vertices[pick] + rng.normal(0.0, jitter, (n_surface, 3)).astype(np.float32)
+        pts.append(np.clip(near, 0.0, 1.0).astype(np.float32))
+    points = np.ascontiguousarray(np.concatenate(pts), np.float32)
+    sdf = ctx.sample(source, points)
+    return points, sdf

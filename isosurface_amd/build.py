@@ -32,7 +32,8 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["api.hip", "mc_kernels.hip", "dc_kernels.hip", "lhmc_kernels.hip", "program.cpp"]
+SOURCES = ["api.hip", "mc_kernels.hip", "dc_kernels.hip", "lhmc_kernels.hip", "mlp_kernels.hip",
+           "band_kernels.hip", "program.cpp"]
 
 
 def hipcc() -> str:

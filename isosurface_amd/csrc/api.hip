@@ -2,15 +2,19 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/isosurface.h"
+#include "band_kernels.h"
 #include "dc_kernels.h"
 #include "lhmc_kernels.h"
 #include "mc_kernels.h"
+#include "mlp_kernels.h"
 #include "program.h"
 
 namespace {
@@ -66,6 +70,15 @@ struct iso_ctx {
   // launch shapes (chosen by the autotuner when asked)
   int b_rounds = 1, b_words_per_wave = 1, b_emit_per_cu = 0;
   bool b_tuned = false;
+  // learned narrow band: network, training batch, brick buffers, counters
+  DevBuf nn_theta, nn_m, nn_v, nn_grad, nn_frags, nn_loss, nn_x, nn_h, nn_dz, nn_g, nn_pts, nn_t,
+      nn_w, nn_out, br_centres, br_hd, br_pred, br_flags, br_list, br_count;
+  bool nn_ready = false;
+  uint32_t nn_step = 0;  // Adam steps taken
+  int64_t nn_batch = 0;  // batch the activation buffers are sized for
+  iso_grid_desc band_desc{};  // lattice of the last band extraction
+  bool band_valid = false;
+
   // captured extraction and the buffers it was captured with
   hipGraphExec_t b_graph = nullptr;
   iso::McBuffers g_buffers{};
@@ -251,7 +264,11 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
                     &c->oc_towner,  &c->oc_tid,     &c->sc_slot,   &c->sc_ctl,
                     &c->b_ops,      &c->b_chunks,   &c->b_segstart, &c->b_wordstart,
-                    &c->b_signs,    &c->b_voff,     &c->b_ioff};
+                    &c->b_signs,    &c->b_voff,     &c->b_ioff,    &c->nn_theta, &c->nn_m,
+                    &c->nn_v,       &c->nn_grad,    &c->nn_frags,  &c->nn_loss,  &c->nn_x,
+                    &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
+                    &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
+                    &c->br_flags,   &c->br_list,    &c->br_count};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
@@ -416,10 +433,16 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
 
 // signs (with state reset) + count/scan; samples from `grid` (device) or,
 // when it is null, from the context's program.
+// A replacement for the signs stage (the learned band's).
+using SignsStage = std::function<int(const iso::McBuffers&)>;
+
 int classify_and_count(iso_ctx* c, const float* grid, const iso::Lattice& L, const iso::McDims& D,
-                       const iso::McBuffers& B) {
+                       const iso::McBuffers& B, const SignsStage* signs = nullptr) {
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
-  if (grid) {
+  if (signs) {
+    const int st = (*signs)(B);
+    if (st != ISO_OK) return st;
+  } else if (grid) {
     ISO_HIP(c, timed(c, "mc_signs_grid",
                      [&] { return iso::launch_signs_grid(grid, L, D, B, c->stream); }));
   } else {
@@ -445,7 +468,7 @@ int read_totals(iso_ctx* c, const iso::McBuffers& B) {
 // when grid is null, from the context's program; with `positions` the
 // vertices interpolate between per-point positions.  Writes the mesh to out.
 int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const float* positions,
-                    iso_mesh_out* out) {
+                    iso_mesh_out* out, const SignsStage* signs = nullptr) {
   int st = ISO_OK;
   const iso::McDims D = iso::mc_dims(L);
   if (D.n_segs >= (int64_t(1) << 26))
@@ -461,7 +484,7 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
     B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
     B.index_capacity = out->indices ? out->index_capacity : 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-      if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+      if ((st = classify_and_count(c, grid, L, D, B, signs)) != ISO_OK) return st;
       ISO_HIP(c, timed(c, "mc_emit", [&] {
                 return iso::launch_emit(grid, ops, c->n_prog, positions, L, D, B, c->stream);
               }));
@@ -482,12 +505,12 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
   }
 
   // Host memory: size the context's device buffers from the scan totals.
-  if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+  if ((st = classify_and_count(c, grid, L, D, B, signs)) != ISO_OK) return st;
   if ((st = read_totals(c, B)) != ISO_OK) return st;
   const uint64_t n_act = c->h_totals[iso::kTotActive];
   if (n_act > B.active_capacity) {
     if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
-    if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+    if ((st = classify_and_count(c, grid, L, D, B, signs)) != ISO_OK) return st;
     if ((st = read_totals(c, B)) != ISO_OK) return st;
   }
   nv = c->h_totals[iso::kTotVertices];
@@ -1127,6 +1150,363 @@ int iso_extract_batch(iso_ctx* c, iso_batch_out* out) {
   return ISO_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device model of the context (allocated on first use).
+int nn_model(iso_ctx* c, iso::mlp::Model* M) {
+  const size_t P = iso::mlp::kParams * sizeof(float);
+  ISO_HIP(c, ensure(&c->nn_theta, P));
+  ISO_HIP(c, ensure(&c->nn_m, P));
+  ISO_HIP(c, ensure(&c->nn_v, P));
+  ISO_HIP(c, ensure(&c->nn_grad, P));
+  ISO_HIP(c, ensure(&c->nn_frags, iso::mlp::kFrags * sizeof(uint16_t)));
+  ISO_HIP(c, ensure(&c->nn_loss, 4 * sizeof(float)));
+  M->theta = static_cast<float*>(c->nn_theta.ptr);
+  M->m = static_cast<float*>(c->nn_m.ptr);
+  M->v = static_cast<float*>(c->nn_v.ptr);
+  M->grad = static_cast<float*>(c->nn_grad.ptr);
+  M->frags = static_cast<uint16_t*>(c->nn_frags.ptr);
+  M->loss = static_cast<float*>(c->nn_loss.ptr);
+  return ISO_OK;
+}
+
+// Uploads theta, clears the Adam state and gradient, packs the fragments.
+int nn_load(iso_ctx* c, const float* theta) {
+  iso::mlp::Model M;
+  int st = nn_model(c, &M);
+  if (st != ISO_OK) return st;
+  const size_t P = iso::mlp::kParams * sizeof(float);
+  ISO_HIP(c, hipMemcpyAsync(M.theta, theta, P, hipMemcpyHostToDevice, c->stream));
+  std::vector<float> zeros(iso::mlp::kParams, 0.0f);
+  ISO_HIP(c, hipMemcpyAsync(M.m, zeros.data(), P, hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipMemcpyAsync(M.v, zeros.data(), P, hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, iso::mlp::launch_zero_grad(M, c->stream));
+  ISO_HIP(c, iso::mlp::launch_pack(M, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  c->nn_ready = true;
+  c->nn_step = 0;
+  return ISO_OK;
+}
+
+int nn_batch(iso_ctx* c, int64_t n, iso::mlp::Batch* B) {
+  ISO_HIP(c, ensure(&c->nn_x, static_cast<size_t>(iso::mlp::kFeatPad) * n * sizeof(uint16_t)));
+  ISO_HIP(c, ensure(&c->nn_h, 3 * static_cast<size_t>(iso::mlp::kHidden) * n * sizeof(uint16_t)));
+  ISO_HIP(c, ensure(&c->nn_dz, 3 * static_cast<size_t>(iso::mlp::kHidden) * n * sizeof(uint16_t)));
+  ISO_HIP(c, ensure(&c->nn_g, n * sizeof(float)));
+  uint16_t* h = static_cast<uint16_t*>(c->nn_h.ptr);
+  uint16_t* dz = static_cast<uint16_t*>(c->nn_dz.ptr);
+  const size_t hn = static_cast<size_t>(iso::mlp::kHidden) * n;
+  B->x = static_cast<uint16_t*>(c->nn_x.ptr);
+  B->h1 = h;
+  B->h2 = h + hn;
+  B->h3 = h + 2 * hn;
+  B->dz1 = dz;
+  B->dz2 = dz + hn;
+  B->dz3 = dz + 2 * hn;
+  B->g = static_cast<float*>(c->nn_g.ptr);
+  B->n = n;
+  c->nn_batch = n;
+  return ISO_OK;
+}
+
+// SplitMix64 (Steele, Lea & Flood 2014): the initialiser's generator.
+uint64_t splitmix(uint64_t* s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Copies n elements of `bytes` each from `src` in `memory` into a device
+// buffer (or returns src itself when it already is device memory).
+int stage_in(iso_ctx* c, DevBuf* buf, const void* src, size_t bytes, int32_t memory,
+             const void** dev) {
+  if (memory == ISO_MEMORY_DEVICE) {
+    *dev = src;
+    return ISO_OK;
+  }
+  ISO_HIP(c, ensure(buf, bytes));
+  ISO_HIP(c, hipMemcpyAsync(buf->ptr, src, bytes, hipMemcpyHostToDevice, c->stream));
+  *dev = buf->ptr;
+  return ISO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int iso_band_init(iso_ctx* c, uint32_t seed) {
+  if (!c) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  namespace m = iso::mlp;
+  std::vector<float> theta(m::kParams, 0.0f);
+  uint64_t s = 0x5EED0000ull ^ seed;
+  auto uni = [&](double lim) {
+    const double u = static_cast<double>(splitmix(&s) >> 11) * (1.0 / 9007199254740992.0);
+    return static_cast<float>((2.0 * u - 1.0) * lim);
+  };
+  for (int i = 0; i < m::kHidden; ++i)
+    for (int k = 0; k < m::kFeat; ++k)
+      theta[m::kOffW1 + i * m::kFeatPad + k] = uni(std::sqrt(6.0 / m::kFeat));
+  for (int i = 0; i < m::kHidden * m::kHidden; ++i) {
+    theta[m::kOffW2 + i] = uni(std::sqrt(6.0 / m::kHidden));
+    theta[m::kOffW3 + i] = uni(std::sqrt(6.0 / m::kHidden));
+  }
+  for (int i = 0; i < m::kHidden; ++i) theta[m::kOffW4 + i] = uni(std::sqrt(6.0 / m::kHidden));
+  return nn_load(c, theta.data());
+}
+
+int iso_band_set_params(iso_ctx* c, const float* theta, uint64_t n) {
+  if (!c || !theta || n != static_cast<uint64_t>(iso::mlp::kParams))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "expected ISO_BAND_PARAMS floats");
+  ISO_HIP(c, hipSetDevice(c->device));
+  return nn_load(c, theta);
+}
+
+int iso_band_get_params(iso_ctx* c, float* theta, uint64_t n) {
+  if (!c || !theta || n != static_cast<uint64_t>(iso::mlp::kParams))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "expected ISO_BAND_PARAMS floats");
+  if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, hipMemcpyAsync(theta, c->nn_theta.ptr, n * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
+}
+
+int iso_band_train(iso_ctx* c, const float* points, const float* sdf, const float* weight,
+                   uint64_t n, int32_t memory, const iso_band_train_params* prm,
+                   float* last_loss) {
+  if (!c || !points || !sdf || !prm || n == 0)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument or no samples");
+  if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
+  if (prm->batch == 0 || (prm->batch & 15u) || prm->steps == 0 || !(prm->lr > 0.0f) ||
+      !(prm->clamp > 0.0f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "batch must be a positive multiple of 16");
+  if (memory != ISO_MEMORY_HOST && memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown memory kind");
+  ISO_HIP(c, hipSetDevice(c->device));
+  int st = ISO_OK;
+  iso::mlp::Model M;
+  iso::mlp::Batch B;
+  if ((st = nn_model(c, &M)) != ISO_OK || (st = nn_batch(c, prm->batch, &B)) != ISO_OK) return st;
+  const void *dp = nullptr, *dt = nullptr, *dw = nullptr;
+  if ((st = stage_in(c, &c->nn_pts, points, 3 * n * sizeof(float), memory, &dp)) != ISO_OK ||
+      (st = stage_in(c, &c->nn_t, sdf, n * sizeof(float), memory, &dt)) != ISO_OK)
+    return st;
+  if (weight && (st = stage_in(c, &c->nn_w, weight, n * sizeof(float), memory, &dw)) != ISO_OK)
+    return st;
+  iso::mlp::Samples S;
+  S.points = static_cast<const float*>(dp);
+  S.sdf = static_cast<const float*>(dt);
+  S.weight = static_cast<const float*>(dw);
+  S.count = n;
+  S.clamp = prm->clamp;
+  S.seed = prm->seed;
+  S.gather = 1;
+  ISO_HIP(c, iso::mlp::launch_zero_grad(M, c->stream));
+  for (uint32_t i = 0; i < prm->steps; ++i) {
+    const double frac = prm->steps > 1 ? static_cast<double>(i) / (prm->steps - 1) : 0.0;
+    const float lr = static_cast<float>(prm->lr_min + 0.5 * (prm->lr - prm->lr_min) *
+                                                          (1.0 + std::cos(3.14159265358979 * frac)));
+    ISO_HIP(c, timed(c, "band_train_batch", [&] {
+              return iso::mlp::launch_train_batch(M, B, S, c->nn_step, c->stream);
+            }));
+    if (i + 1 == prm->steps && last_loss) {
+      ISO_HIP(c, hipMemcpyAsync(c->h_stage, M.loss, sizeof(float), hipMemcpyDeviceToHost,
+                                c->stream));
+    }
+    ++c->nn_step;
+    ISO_HIP(c, timed(c, "band_adam", [&] {
+              return iso::mlp::launch_adam(M, c->nn_step, lr, 0.9f, 0.999f, 1e-8f, c->stream);
+            }));
+  }
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (last_loss) {
+    float v;
+    std::memcpy(&v, c->h_stage, sizeof(float));
+    *last_loss = v / (2.0f * static_cast<float>(prm->batch));
+  }
+  return ISO_OK;
+}
+
+int iso_band_gradient(iso_ctx* c, const float* points, const float* sdf, const float* weight,
+                      uint64_t n, float clamp, float* grad, float* loss) {
+  if (!c || !points || !sdf || !grad || n == 0 || (n & 15u) || !(clamp > 0.0f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "n must be a positive multiple of 16");
+  if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
+  ISO_HIP(c, hipSetDevice(c->device));
+  int st = ISO_OK;
+  iso::mlp::Model M;
+  iso::mlp::Batch B;
+  if ((st = nn_model(c, &M)) != ISO_OK || (st = nn_batch(c, static_cast<int64_t>(n), &B)) != ISO_OK)
+    return st;
+  const void *dp = nullptr, *dt = nullptr, *dw = nullptr;
+  if ((st = stage_in(c, &c->nn_pts, points, 3 * n * sizeof(float), ISO_MEMORY_HOST, &dp)) != ISO_OK ||
+      (st = stage_in(c, &c->nn_t, sdf, n * sizeof(float), ISO_MEMORY_HOST, &dt)) != ISO_OK)
+    return st;
+  if (weight && (st = stage_in(c, &c->nn_w, weight, n * sizeof(float), ISO_MEMORY_HOST, &dw)) != ISO_OK)
+    return st;
+  iso::mlp::Samples S;
+  S.points = static_cast<const float*>(dp);
+  S.sdf = static_cast<const float*>(dt);
+  S.weight = static_cast<const float*>(dw);
+  S.count = n;
+  S.clamp = clamp;
+  S.seed = 0;
+  S.gather = 0;
+  ISO_HIP(c, iso::mlp::launch_zero_grad(M, c->stream));
+  ISO_HIP(c, iso::mlp::launch_train_batch(M, B, S, 0, c->stream));
+  ISO_HIP(c, hipMemcpyAsync(grad, M.grad, iso::mlp::kParams * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+  ISO_HIP(c, hipMemcpyAsync(c->h_stage, M.loss, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, iso::mlp::launch_zero_grad(M, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  if (loss) {
+    float v;
+    std::memcpy(&v, c->h_stage, sizeof(float));
+    *loss = v / (2.0f * static_cast<float>(n));
+  }
+  return ISO_OK;
+}
+
+int iso_band_predict(iso_ctx* c, const float* points, uint64_t n, float* out, int32_t memory) {
+  if (!c || (n && (!points || !out))) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
+  if (memory != ISO_MEMORY_HOST && memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown memory kind");
+  if (n == 0) return ISO_OK;
+  ISO_HIP(c, hipSetDevice(c->device));
+  int st = ISO_OK;
+  iso::mlp::Model M;
+  if ((st = nn_model(c, &M)) != ISO_OK) return st;
+  const void* dp = nullptr;
+  if ((st = stage_in(c, &c->nn_pts, points, 3 * n * sizeof(float), memory, &dp)) != ISO_OK) return st;
+  float* dst = out;
+  if (memory == ISO_MEMORY_HOST) {
+    ISO_HIP(c, ensure(&c->nn_out, n * sizeof(float)));
+    dst = static_cast<float*>(c->nn_out.ptr);
+  }
+  ISO_HIP(c, timed(c, "band_predict", [&] {
+            return iso::mlp::launch_forward(M, static_cast<const float*>(dp),
+                                            static_cast<int64_t>(n), dst, c->stream);
+          }));
+  if (memory == ISO_MEMORY_HOST)
+    ISO_HIP(c, hipMemcpyAsync(out, dst, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
+}
+
+int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, iso_mesh_out* out,
+                        iso_band_stats* stats) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  if (desc->source != ISO_SOURCE_PROGRAM)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "band extraction needs a program source");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
+  if (!(margin >= 0.0f) || !(margin < 3.0e38f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "margin must be finite and non-negative");
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+  iso::mlp::Model M;
+  if ((st = nn_model(c, &M)) != ISO_OK) return st;
+  const iso::BrickGrid G = iso::brick_grid(L);
+  const uint64_t nb = static_cast<uint64_t>(G.n);
+  ISO_HIP(c, ensure(&c->br_centres, 3 * nb * sizeof(float)));
+  ISO_HIP(c, ensure(&c->br_hd, nb * sizeof(float)));
+  ISO_HIP(c, ensure(&c->br_pred, nb * sizeof(float)));
+  ISO_HIP(c, ensure(&c->br_flags, nb));
+  ISO_HIP(c, ensure(&c->br_list, nb * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->br_count, 2 * sizeof(unsigned long long)));
+  float* centres = static_cast<float*>(c->br_centres.ptr);
+  float* hd = static_cast<float*>(c->br_hd.ptr);
+  float* pred = static_cast<float*>(c->br_pred.ptr);
+  uint8_t* flags = static_cast<uint8_t*>(c->br_flags.ptr);
+  uint32_t* list = static_cast<uint32_t*>(c->br_list.ptr);
+  unsigned long long* counters = static_cast<unsigned long long*>(c->br_count.ptr);
+  iso::scan::State S{};
+  if ((st = prepare_scan(c, nb, &S)) != ISO_OK) return st;
+  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(S.err), 2, 0u, c->stream));
+  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(counters), 4, 0u, c->stream));
+  ISO_HIP(c, timed(c, "band_bricks", [&] {
+            hipError_t e = iso::launch_brick_centres(L, G, centres, hd, c->stream);
+            if (e == hipSuccess)
+              e = iso::mlp::launch_forward(M, centres, static_cast<int64_t>(nb), pred, c->stream);
+            if (e == hipSuccess)
+              e = iso::launch_brick_flags(G, pred, hd, margin, flags, list, S, c->stream);
+            return e;
+          }));
+  if ((st = read_ctl(c, S)) != ISO_OK) return st;
+  const uint64_t n_band = c->h_stage[1];
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  const iso::McDims D = iso::mc_dims(L);
+  SignsStage signs = [&](const iso::McBuffers& B) -> int {
+    ISO_HIP(c, timed(c, "band_background", [&] {
+              return iso::launch_band_background(L, D, G, flags, B, c->stream);
+            }));
+    ISO_HIP(c, timed(c, "band_exact", [&] {
+              return iso::launch_band_exact(ops, c->n_prog, L, G, flags, list, n_band, B,
+                                            counters, c->stream);
+            }));
+    return ISO_OK;
+  };
+  c->band_valid = false;
+  st = extract_lattice(c, L, nullptr, nullptr, out, &signs);
+  if (st != ISO_OK && st != ISO_ERR_CAPACITY) return st;
+  c->band_desc = *desc;
+  c->band_valid = true;
+  if (stats) {
+    ISO_HIP(c, hipMemcpyAsync(c->h_stage + 4, counters, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    stats->bricks = nb;
+    stats->band_bricks = n_band;
+    stats->exact_samples = c->h_stage[4];
+    stats->reserved = 0;
+  }
+  return st;
+}
+
+int iso_band_missed_cells(iso_ctx* c, const iso_grid_desc* desc, uint64_t* missed) {
+  if (!c || !desc || !missed) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  if (!c->band_valid || std::memcmp(&c->band_desc, desc, sizeof(*desc)) != 0)
+    return fail(c, ISO_ERR_NO_SOURCE, "no band extraction over this lattice");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  ISO_HIP(c, hipSetDevice(c->device));
+  const iso::McDims D = iso::mc_dims(L);
+  const iso::BrickGrid G = iso::brick_grid(L);
+  iso::McBuffers B{};
+  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
+  // Dense classification and count of the program; its active list holds
+  // every surface cell.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if ((st = classify_and_count(c, nullptr, L, D, B)) != ISO_OK) return st;
+    if ((st = read_totals(c, B)) != ISO_OK) return st;
+    if (c->h_totals[iso::kTotActive] <= B.active_capacity) break;
+    if ((st = prepare_mc(c, D, c->h_totals[iso::kTotActive], &B)) != ISO_OK) return st;
+  }
+  unsigned long long* counters = static_cast<unsigned long long*>(c->br_count.ptr);
+  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(counters + 1), 2, 0u, c->stream));
+  ISO_HIP(c, iso::launch_band_missed(L, D, G, static_cast<const uint8_t*>(c->br_flags.ptr), B,
+                                     counters + 1, c->stream));
+  ISO_HIP(c, hipMemcpyAsync(c->h_stage + 4, counters + 1, sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  *missed = c->h_stage[4];
+  return ISO_OK;
+}
+
 int iso_mesh_fetch(iso_ctx* c, iso_mesh_out* out) {
   if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
   if (!c->kept) return fail(c, ISO_ERR_NO_SOURCE, "no kept mesh");
@@ -1174,6 +1554,14 @@ int iso_memcpy_to_host(iso_ctx* c, void* dst, const void* src, uint64_t bytes) {
   if (!c || (bytes && (!dst || !src))) return ISO_ERR_INVALID_ARGUMENT;
   ISO_HIP(c, hipSetDevice(c->device));
   ISO_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
+}
+
+int iso_memcpy_device(iso_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c || (bytes && (!dst || !src))) return ISO_ERR_INVALID_ARGUMENT;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
   return ISO_OK;
 }

@@ -1,0 +1,60 @@
+// Launch interface of the learned narrow-band extraction (host side).
+//
+// The lattice's cells are grouped in 8x8x8 bricks.  The network predicts
+// the (truncated) SDF at every brick centre; a brick whose prediction is
+// within its half-diagonal plus a calibrated margin of zero is in the band.
+// Band bricks get exact program samples; every other sample takes the
+// predicted sign of its brick.  The marching-cubes count and emit kernels
+// then run unchanged over the whole lattice (emit evaluates the program at
+// edge endpoints, so vertex positions are exact).  If no surface cell lies
+// in a culled brick, the mesh is bit-identical to the dense extraction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mc_kernels.h"
+#include "mlp_kernels.h"
+#include "scan.h"
+
+namespace iso {
+
+constexpr int kBrick = 8;  // cells per brick edge
+
+// Brick flags
+enum : uint8_t {
+  kBrickBand = 1,    // exact samples
+  kBrickInside = 2,  // predicted sign (culled bricks): !(f > 0)
+};
+
+struct BrickGrid {
+  int64_t nbx, nby, nbz;  // bricks per axis: ceil(cells / 8)
+  int64_t n;              // nbx * nby * nbz
+};
+
+BrickGrid brick_grid(const Lattice& L);
+
+// Brick centres (3 floats each, brick order x fastest) and half-diagonals.
+hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* centres,
+                                float* half_diag, hipStream_t s);
+// Band / inside flags from predictions: band if |pred| <= half_diag + margin.
+// The list of band bricks (ascending) and its length go to st.ctl[1].
+hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
+                              float margin, uint8_t* flags, uint32_t* band_list,
+                              const scan::State& st, hipStream_t s);
+// Sign bits of the whole lattice from the bricks' predicted signs (and the
+// extraction's state reset, like the other signs launchers).
+hipError_t launch_band_background(const Lattice& L, const McDims& D, const BrickGrid& G,
+                                  const uint8_t* flags, const McBuffers& B, hipStream_t s);
+// Exact sign bits of every sample of the band bricks (each sample evaluated
+// once: a brick takes its far faces only where the neighbour is not in the
+// band).  n_band bricks from band_list; *evaluated receives the sample count.
+hipError_t launch_band_exact(const DevOp* ops, int n_ops, const Lattice& L, const BrickGrid& G,
+                             const uint8_t* flags, const uint32_t* band_list, uint64_t n_band,
+                             const McBuffers& B, unsigned long long* evaluated, hipStream_t s);
+// Active cells of a dense extraction (the MC active list) that lie in culled
+// bricks: missed surface cells.  Count added to *missed.
+hipError_t launch_band_missed(const Lattice& L, const McDims& D, const BrickGrid& G,
+                              const uint8_t* flags, const McBuffers& B,
+                              unsigned long long* missed, hipStream_t s);
+
+}  // namespace iso

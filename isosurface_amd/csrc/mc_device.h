@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mc_kernels.h"
 #include "mc_tables.h"
 
 namespace iso {
@@ -235,6 +236,31 @@ __device__ __forceinline__ unsigned long long slot_load(const unsigned long long
 
 __device__ __forceinline__ void slot_store(unsigned long long* p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-extraction state cleared by the first kernel of an extraction.
+struct ResetArgs {
+  uint64_t* signs;  // guard words at [-1], [n_words], [n_words + 1]
+  int64_t n_words;
+  ScanTriple* agg;
+  ScanTriple* incl;
+  int64_t n_blocks;
+  unsigned long long* totals;
+};
+
+__device__ __forceinline__ void reset_state(const ResetArgs& r) {
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = i0; i < r.n_blocks; i += stride) {
+    r.agg[i] = ScanTriple{0ull, 0ull, 0ull};
+    r.incl[i] = ScanTriple{0ull, 0ull, 0ull};
+  }
+  if (i0 < kTotSlots) r.totals[i0] = 0ull;
+  if (i0 == 0 && r.signs) {
+    r.signs[-1] = 0ull;
+    r.signs[r.n_words] = 0ull;
+    r.signs[r.n_words + 1] = 0ull;
+  }
 }
 
 }  // namespace mcd

@@ -44,30 +44,6 @@ __constant__ tables::McTables c_tables = tables::kTables;
 // ---------------------------------------------------------------------------
 // 1. signs (+ reset of the extraction's scratch state)
 // ---------------------------------------------------------------------------
-struct ResetArgs {
-  uint64_t* signs;  // guard words at [-1], [n_words], [n_words + 1]
-  int64_t n_words;
-  ScanTriple* agg;
-  ScanTriple* incl;
-  int64_t n_blocks;
-  unsigned long long* totals;
-};
-
-__device__ __forceinline__ void reset_state(const ResetArgs& r) {
-  const int64_t i0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t i = i0; i < r.n_blocks; i += stride) {
-    r.agg[i] = ScanTriple{0ull, 0ull, 0ull};
-    r.incl[i] = ScanTriple{0ull, 0ull, 0ull};
-  }
-  if (i0 < kTotSlots) r.totals[i0] = 0ull;
-  if (i0 == 0 && r.signs) {
-    r.signs[-1] = 0ull;
-    r.signs[r.n_words] = 0ull;
-    r.signs[r.n_words + 1] = 0ull;
-  }
-}
-
 constexpr int kSignVec = 4;  // float4 loads per lane per iteration (16 words)
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));

@@ -1,0 +1,465 @@
+// Narrow-band network kernels for gfx950 (MI355X): MFMA forward, backward
+// and Adam for the Fourier-feature MLP of mlp_kernels.h.
+//
+// MFMA tiling (v_mfma_f32_16x16x16_bf16, D = A B + C, 16x16 blocks):
+//   A operand: lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
+//   B operand: lane l holds B[4 (l / 16) + j][l % 16]
+//   D result:  lane l holds D[4 (l / 16) + j][l % 16]
+// A wavefront pushes 16 samples (the N dimension) through the network:
+// layer outputs land in exactly the layout the next layer's B operand
+// needs, so activations stay in registers from the features to the output.
+// The backward-data pass uses the same layout with W^T fragments; weight
+// gradients are products over the batch of hidden-major bf16 activations
+// (split over the batch, accumulated with f32 atomics).
+#include <hip/hip_runtime.h>
+
+#include "mlp_kernels.h"
+
+namespace iso {
+namespace mlp {
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// w_k = pi * 2^(k / 2) rounded to f32
+__constant__ float c_freq[kFreq] = {
+    0x1.921fb6p+1f, 0x1.1c5832p+2f, 0x1.921fb6p+2f, 0x1.1c5832p+3f,
+    0x1.921fb6p+3f, 0x1.1c5832p+4f, 0x1.921fb6p+4f, 0x1.1c5832p+5f,
+    0x1.921fb6p+5f, 0x1.1c5832p+6f, 0x1.921fb6p+6f, 0x1.1c5832p+7f,
+    0x1.921fb6p+7f, 0x1.1c5832p+8f, 0x1.921fb6p+8f, 0x1.1c5832p+9f};
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);  // NaN
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // nearest, ties to even
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t b) {
+  return __uint_as_float(static_cast<uint32_t>(b) << 16);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
+  bf16x4 r;
+  r[0] = static_cast<short>(a);
+  r[1] = static_cast<short>(b);
+  r[2] = static_cast<short>(c);
+  r[3] = static_cast<short>(d);
+  return r;
+}
+
+// Feature fi (0..111) of point p: p_d, sin(w_k p_d), cos(w_k p_d), zero padding.
+__device__ __forceinline__ float feature(int fi, float px, float py, float pz) {
+  if (fi < 3) return fi == 0 ? px : (fi == 1 ? py : pz);
+  if (fi >= kFeat) return 0.0f;
+  const int q = fi - 3;
+  const int k = q / 6, r = q - 6 * k;
+  const int d = r < 3 ? r : r - 3;
+  const float a = c_freq[k] * (d == 0 ? px : (d == 1 ? py : pz));
+  return r < 3 ? sinf(a) : cosf(a);
+}
+
+// Fragments of one 16-row block set, read from LDS: lane's 4 bf16 of block (mb, kb).
+__device__ __forceinline__ bf16x4 frag(const uint16_t* F, int n_kb, int mb, int kb, int lane) {
+  const uint16_t* p = F + ((mb * n_kb + kb) * 64 + lane) * 4;
+  return *reinterpret_cast<const bf16x4*>(p);
+}
+
+struct Tile {
+  bf16x4 h1[4], h2[4], h3[4];  // activations (B layout of the next layer), bf16
+  float f;                     // output of this lane's sample
+};
+
+// Forward pass of 16 samples; lane l's sample is l % 16 at (px, py, pz).
+// sF: fragments F1 F2 F3 in LDS; sB: b1 b2 b3 w4 b4 in LDS.  Layer 1
+// consumes the features one K-block at a time (computed just in time, and
+// stored hidden-major to xdst[row * n + s] when xdst is given).
+__device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB, float px,
+                                             float py, float pz, int lane, Tile& T,
+                                             uint16_t* xdst, int64_t n, int64_t s) {
+  const int r0 = 4 * (lane >> 4);
+  const uint16_t* F1 = sF + kFragF1;
+  const uint16_t* F2 = sF + kFragF2;
+  const uint16_t* F3 = sF + kFragF3;
+  const float* b1 = sB;
+  const float* b2 = sB + kHidden;
+  const float* b3 = sB + 2 * kHidden;
+  const float* w4 = sB + 3 * kHidden;
+  const float b4 = sB[4 * kHidden];
+  f32x4 acc1[4];
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) acc1[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+  for (int kb = 0; kb < kKB1; ++kb) {
+    const int f0 = 16 * kb + r0;
+    uint16_t xb[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) xb[j] = to_bf16(feature(f0 + j, px, py, pz));
+    if (xdst) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xdst[(f0 + j) * n + s] = xb[j];
+    }
+    const bf16x4 x = pack4(xb[0], xb[1], xb[2], xb[3]);
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) acc1[mb] = mfma(frag(F1, kKB1, mb, kb, lane), x, acc1[mb]);
+  }
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    uint16_t h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = to_bf16(fmaxf(acc1[mb][j] + b1[16 * mb + r0 + j], 0.0f));
+    T.h1[mb] = pack4(h[0], h[1], h[2], h[3]);
+  }
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F2, 4, mb, kb, lane), T.h1[kb], acc);
+    uint16_t h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) h[j] = to_bf16(fmaxf(acc[j] + b2[16 * mb + r0 + j], 0.0f));
+    T.h2[mb] = pack4(h[0], h[1], h[2], h[3]);
+  }
+  float part = 0.0f;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F3, 4, mb, kb, lane), T.h2[kb], acc);
+    uint16_t h[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      h[j] = to_bf16(fmaxf(acc[j] + b3[16 * mb + r0 + j], 0.0f));
+      part += w4[16 * mb + r0 + j] * from_bf16(h[j]);
+    }
+    T.h3[mb] = pack4(h[0], h[1], h[2], h[3]);
+  }
+  // The four lanes of a sample (l, l^16, l^32, l^48) hold its 64 terms.
+  part += __shfl_xor(part, 16, 64);
+  part += __shfl_xor(part, 32, 64);
+  T.f = part + b4;
+}
+
+constexpr int kThreads = 256;
+constexpr int kLdsFrags = kFragT3;  // forward fragments only (F1 F2 F3)
+
+// Loads fragments [0, n_frag) and the bias block into LDS.
+__device__ __forceinline__ void load_model(const Model& M, uint16_t* sF, int n_frag, float* sB) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(M.frags);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(sF);
+  for (int i = threadIdx.x; i < n_frag / 2; i += blockDim.x) dst[i] = src[i];
+  for (int i = threadIdx.x; i < 4 * kHidden + 1; i += blockDim.x) {
+    // b1 b2 b3 w4 b4
+    const int which = i / kHidden, k = i - which * kHidden;
+    const int off = which == 0 ? kOffB1 : which == 1 ? kOffB2 : which == 2 ? kOffB3 : kOffW4;
+    sB[i] = i == 4 * kHidden ? M.theta[kOffB4] : M.theta[off + k];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_forward(Model M, const float* __restrict__ points,
+                                                      int64_t n, float* __restrict__ out) {
+  __shared__ __align__(16) uint16_t sF[kLdsFrags];
+  __shared__ float sB[4 * kHidden + 1];
+  load_model(M, sF, kLdsFrags, sB);
+  __syncthreads();
+  const int lane = __lane_id();
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (kThreads / 64);
+  for (int64_t t0 = wave * 16; t0 < n; t0 += n_waves * 16) {
+    const int64_t s = t0 + (lane & 15);
+    const bool ok = s < n;
+    const float px = ok ? points[3 * s] : 0.0f, py = ok ? points[3 * s + 1] : 0.0f,
+                pz = ok ? points[3 * s + 2] : 0.0f;
+    Tile T;
+    forward_tile(sF, sB, px, py, pz, lane, T, nullptr, 0, 0);
+    if (ok && lane < 16) out[s] = T.f;
+  }
+}
+
+__device__ __forceinline__ uint32_t hash32(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t h = a * 0x9E3779B1u ^ (b + 0x85EBCA77u) * 0xC2B2AE3Du ^ (c + 0x27D4EB2Fu) * 0x165667B1u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return h;
+}
+
+// Forward + backward-data of a batch; writes hidden-major activations and
+// per-sample dL/df, accumulates the loss.
+__global__ __launch_bounds__(kThreads) void k_train_fb(Model M, Batch B, Samples S,
+                                                       uint32_t step) {
+  __shared__ __align__(16) uint16_t sF[kFrags];
+  __shared__ float sB[4 * kHidden + 1];
+  load_model(M, sF, kFrags, sB);
+  __syncthreads();
+  const int lane = __lane_id();
+  const int r0 = 4 * (lane >> 4);
+  const int col = lane & 15;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (kThreads / 64);
+  const float* w4 = sB + 3 * kHidden;
+  const float inv_n = 1.0f / static_cast<float>(B.n);
+  float loss = 0.0f;
+  for (int64_t t0 = wave * 16; t0 < B.n; t0 += n_waves * 16) {
+    const int64_t s = t0 + col;  // batch sample (B.n is a multiple of 16)
+    const uint64_t src =
+        S.gather ? hash32(S.seed, step, static_cast<uint32_t>(s)) % S.count
+                 : static_cast<uint64_t>(s);
+    const float px = S.points[3 * src], py = S.points[3 * src + 1], pz = S.points[3 * src + 2];
+    const float t = fminf(fmaxf(S.sdf[src], -S.clamp), S.clamp);
+    const float wgt = S.weight ? S.weight[src] : 1.0f;
+    Tile T;
+    forward_tile(sF, sB, px, py, pz, lane, T, B.x, B.n, s);
+    const float r = T.f - t;
+    const float g = wgt * r * inv_n;
+    if (lane < 16) {
+      loss += wgt * r * r;
+      B.g[s] = g;
+    }
+    // dz3 = bf16(g w4 [h3 > 0]), dz2 = bf16((W3^T dz3) [h2 > 0]), dz1 likewise.
+    bf16x4 d3[4], d2[4], d1[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      uint16_t d[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float h = from_bf16(static_cast<uint16_t>(T.h3[mb][j]));
+        d[j] = to_bf16(h > 0.0f ? g * w4[16 * mb + r0 + j] : 0.0f);
+      }
+      d3[mb] = pack4(d[0], d[1], d[2], d[3]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(sF + kFragT3, 4, mb, kb, lane), d3[kb], acc);
+      uint16_t d[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float h = from_bf16(static_cast<uint16_t>(T.h2[mb][j]));
+        d[j] = to_bf16(h > 0.0f ? acc[j] : 0.0f);
+      }
+      d2[mb] = pack4(d[0], d[1], d[2], d[3]);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(sF + kFragT2, 4, mb, kb, lane), d2[kb], acc);
+      uint16_t d[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const float h = from_bf16(static_cast<uint16_t>(T.h1[mb][j]));
+        d[j] = to_bf16(h > 0.0f ? acc[j] : 0.0f);
+      }
+      d1[mb] = pack4(d[0], d[1], d[2], d[3]);
+    }
+    // Hidden-major stores: row 16 mb + r0 + j, column s.
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t o = (16 * mb + r0 + j) * B.n + s;
+        B.h1[o] = static_cast<uint16_t>(T.h1[mb][j]);
+        B.h2[o] = static_cast<uint16_t>(T.h2[mb][j]);
+        B.h3[o] = static_cast<uint16_t>(T.h3[mb][j]);
+        B.dz1[o] = static_cast<uint16_t>(d1[mb][j]);
+        B.dz2[o] = static_cast<uint16_t>(d2[mb][j]);
+        B.dz3[o] = static_cast<uint16_t>(d3[mb][j]);
+      }
+  }
+  // wave sum of the loss terms (lanes 0..15 hold them)
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) loss += __shfl_xor(loss, d, 64);
+  if (lane == 0 && loss != 0.0f) atomicAdd(M.loss, loss);
+}
+
+// Weight gradients: block (layer, mb, nb) of dW_l = DZ_l Hprev^T over one
+// chunk of the batch, accumulated into M.grad.  One wavefront per (block,
+// chunk).  Output blocks: layer 1 4x7, layer 2 4x4, layer 3 4x4.
+constexpr int kWgradBlocks = 4 * kKB1 + 16 + 16;
+constexpr int kWgradChunk = 1024;
+
+__global__ __launch_bounds__(kThreads) void k_wgrad(Model M, Batch B) {
+  const int lane = __lane_id();
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  const int64_t n_chunks = (B.n + kWgradChunk - 1) / kWgradChunk;
+  if (wave >= kWgradBlocks * n_chunks) return;
+  const int blk = static_cast<int>(wave % kWgradBlocks);
+  const int64_t chunk = wave / kWgradBlocks;
+  int layer, mb, nb;
+  if (blk < 4 * kKB1) {
+    layer = 1;
+    mb = blk / kKB1;
+    nb = blk % kKB1;
+  } else if (blk < 4 * kKB1 + 16) {
+    layer = 2;
+    mb = (blk - 4 * kKB1) / 4;
+    nb = (blk - 4 * kKB1) % 4;
+  } else {
+    layer = 3;
+    mb = (blk - 4 * kKB1 - 16) / 4;
+    nb = (blk - 4 * kKB1 - 16) % 4;
+  }
+  const uint16_t* dz = layer == 1 ? B.dz1 : layer == 2 ? B.dz2 : B.dz3;
+  const uint16_t* hp = layer == 1 ? B.x : layer == 2 ? B.h1 : B.h2;
+  const int cols = layer == 1 ? kFeatPad : kHidden;
+  const int off = layer == 1 ? kOffW1 : layer == 2 ? kOffW2 : kOffW3;
+  const int r0 = 4 * (lane >> 4), col = lane & 15;
+  const int64_t s_end = (chunk + 1) * kWgradChunk < B.n ? (chunk + 1) * kWgradChunk : B.n;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int64_t s0 = chunk * kWgradChunk; s0 < s_end; s0 += 16) {
+    const bf16x4 a = *reinterpret_cast<const bf16x4*>(dz + (16 * mb + col) * B.n + s0 + r0);
+    const bf16x4 b = *reinterpret_cast<const bf16x4*>(hp + (16 * nb + col) * B.n + s0 + r0);
+    acc = mfma(a, b, acc);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) atomicAdd(&M.grad[off + (16 * mb + r0 + j) * cols + 16 * nb + col], acc[j]);
+}
+
+// Bias gradients and the output layer: db_l[i] = sum_s dz_l[i][s],
+// dw4[i] = sum_s g[s] h3[i][s], db4 = sum_s g[s].  One thread per
+// (row, chunk); rows 0..191 biases, 192..255 w4, 256 b4.
+__global__ __launch_bounds__(kThreads) void k_bgrad(Model M, Batch B) {
+  const int64_t n_chunks = (B.n + kWgradChunk - 1) / kWgradChunk;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= 257 * n_chunks) return;
+  const int row = static_cast<int>(t % 257);
+  const int64_t chunk = t / 257;
+  const int64_t s0 = chunk * kWgradChunk;
+  const int64_t s1 = s0 + kWgradChunk < B.n ? s0 + kWgradChunk : B.n;
+  float acc = 0.0f;
+  int dst;
+  if (row < 192) {
+    const int layer = row / 64, i = row % 64;
+    const uint16_t* dz = layer == 0 ? B.dz1 : layer == 1 ? B.dz2 : B.dz3;
+    for (int64_t s = s0; s < s1; ++s) acc += from_bf16(dz[i * B.n + s]);
+    dst = (layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3) + i;
+  } else if (row < 256) {
+    const int i = row - 192;
+    for (int64_t s = s0; s < s1; ++s) acc += B.g[s] * from_bf16(B.h3[i * B.n + s]);
+    dst = kOffW4 + i;
+  } else {
+    for (int64_t s = s0; s < s1; ++s) acc += B.g[s];
+    dst = kOffB4;
+  }
+  atomicAdd(&M.grad[dst], acc);
+}
+
+// Fragment slot of master weight index p (layers 1-3), and of its transpose.
+__device__ __forceinline__ void frag_slots(int p, int* f, int* ft) {
+  *f = -1;
+  *ft = -1;
+  int base, cols, kbn, fbase, tbase;
+  if (p < kOffB1) {
+    base = kOffW1, cols = kFeatPad, kbn = kKB1, fbase = kFragF1, tbase = -1;
+  } else if (p >= kOffW2 && p < kOffB2) {
+    base = kOffW2, cols = kHidden, kbn = 4, fbase = kFragF2, tbase = kFragT2;
+  } else if (p >= kOffW3 && p < kOffB3) {
+    base = kOffW3, cols = kHidden, kbn = 4, fbase = kFragF3, tbase = kFragT3;
+  } else {
+    return;
+  }
+  const int q = p - base;
+  const int row = q / cols, c = q - row * cols;
+  // W[row][c] sits in block (row / 16, c / 16) at lane (row % 16) + 16 ((c % 16) / 4), slot c % 4
+  *f = fbase + (((row >> 4) * kbn + (c >> 4)) * 64 + (row & 15) + 16 * ((c & 15) >> 2)) * 4 + (c & 3);
+  if (tbase >= 0) {
+    // W^T[c][row]: block (c / 16, row / 16), lane (c % 16) + 16 ((row % 16) / 4), slot row % 4
+    *ft = tbase + (((c >> 4) * 4 + (row >> 4)) * 64 + (c & 15) + 16 * ((row & 15) >> 2)) * 4 +
+          (row & 3);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_pack(Model M) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= kParams) return;
+  int f, ft;
+  frag_slots(p, &f, &ft);
+  const uint16_t b = to_bf16(M.theta[p]);
+  if (f >= 0) M.frags[f] = b;
+  if (ft >= 0) M.frags[ft] = b;
+}
+
+__global__ __launch_bounds__(kThreads) void k_adam(Model M, float lr, float beta1, float beta2,
+                                                   float eps, float c1, float c2) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p >= kParams) return;
+  const float g = M.grad[p];
+  const float m = beta1 * M.m[p] + (1.0f - beta1) * g;
+  const float v = beta2 * M.v[p] + (1.0f - beta2) * g * g;
+  M.m[p] = m;
+  M.v[p] = v;
+  const float w = M.theta[p] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+  M.theta[p] = w;
+  M.grad[p] = 0.0f;
+  if (p == 0) M.loss[0] = 0.0f;
+  int f, ft;
+  frag_slots(p, &f, &ft);
+  const uint16_t b = to_bf16(w);
+  if (f >= 0) M.frags[f] = b;
+  if (ft >= 0) M.frags[ft] = b;
+}
+
+__global__ __launch_bounds__(kThreads) void k_zero_grad(Model M) {
+  const int p = blockIdx.x * kThreads + threadIdx.x;
+  if (p < kParams) M.grad[p] = 0.0f;
+  if (p == 0) M.loss[0] = 0.0f;
+}
+
+unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+hipError_t launch_pack(const Model& M, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack, dim3(blocks_for(kParams)), dim3(kThreads), 0, s, M);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward(const Model& M, const float* points, int64_t n, float* out,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 16 * 4 - 1) / (16 * 4);  // one 16-sample tile per wavefront
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_forward, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, M,
+                     points, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_train_batch(const Model& M, const Batch& B, const Samples& S, uint32_t step,
+                              hipStream_t s) {
+  if (B.n <= 0 || (B.n & 15) || S.count == 0) return hipErrorInvalidValue;
+  int64_t blocks = (B.n + 16 * 4 - 1) / (16 * 4);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_train_fb, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, M, B,
+                     S, step);
+  const int64_t n_chunks = (B.n + kWgradChunk - 1) / kWgradChunk;
+  hipLaunchKernelGGL(k_wgrad, dim3(blocks_for(64 * kWgradBlocks * n_chunks)), dim3(kThreads), 0, s,
+                     M, B);
+  hipLaunchKernelGGL(k_bgrad, dim3(blocks_for(257 * n_chunks)), dim3(kThreads), 0, s, M, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_adam(const Model& M, uint32_t step, float lr, float beta1, float beta2,
+                       float eps, hipStream_t s) {
+  const double c1 = 1.0 - __builtin_pow(static_cast<double>(beta1), static_cast<double>(step));
+  const double c2 = 1.0 - __builtin_pow(static_cast<double>(beta2), static_cast<double>(step));
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(kParams)), dim3(kThreads), 0, s, M, lr, beta1, beta2,
+                     eps, static_cast<float>(c1), static_cast<float>(c2));
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_grad(const Model& M, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_grad, dim3(blocks_for(kParams)), dim3(kThreads), 0, s, M);
+  return hipGetLastError();
+}
+
+}  // namespace mlp
+}  // namespace iso

@@ -28,23 +28,39 @@ Node = Tuple[int, Tuple[float, ...]]
 
 
 class Source:
-    """A source the GPU can evaluate: lowers to an iso_node program."""
+    """A source the GPU can evaluate: lowers to an iso_node program.
+
+    Each source describes itself as (nodes before its children, children,
+    nodes after them); program() walks the tree with an explicit stack, so
+    chains thousands of operators deep (a smooth-min fold) lower too."""
 
     def program(self) -> List[Node]:
         out: List[Node] = []
-        self._emit(out)
+        stack: List[object] = [self]
+        while stack:
+            item = stack.pop()
+            if isinstance(item, Source):
+                pre, children, post = item._parts()
+                out.extend(pre)
+                stack.append(("nodes", post))
+                stack.extend(reversed(children))
+            else:
+                out.extend(item[1])
         return out
 
-    def _emit(self, out: List[Node]) -> None:  # pragma: no cover - abstract
+    def _parts(self) -> Tuple[List[Node], List["Source"], List[Node]]:  # pragma: no cover
         raise NotImplementedError
+
+    def _emit(self, out: List[Node]) -> None:
+        out.extend(self.program())
 
 
 class Sphere(Source):
     def __init__(self, radius: float):
         self.radius = float(radius)
 
-    def _emit(self, out):
-        out.append((A.ISO_OP_SPHERE, (self.radius,)))
+    def _parts(self):
+        return [], [], [(A.ISO_OP_SPHERE, (self.radius,))]
 
 
 class Torus(Source):
@@ -52,8 +68,8 @@ class Torus(Source):
         self.radius = float(radius)
         self.tube_radius = float(tube_radius)
 
-    def _emit(self, out):
-        out.append((A.ISO_OP_TORUS, (self.radius, self.tube_radius)))
+    def _parts(self):
+        return [], [], [(A.ISO_OP_TORUS, (self.radius, self.tube_radius))]
 
 
 class RectangularPrism(Source):
@@ -61,8 +77,8 @@ class RectangularPrism(Source):
         self.half_extent = tuple(float(v) for v in half_extent)
         assert len(self.half_extent) == 3
 
-    def _emit(self, out):
-        out.append((A.ISO_OP_BOX, self.half_extent))
+    def _parts(self):
+        return [], [], [(A.ISO_OP_BOX, self.half_extent)]
 
 
 class Cylinder(Source):
@@ -70,8 +86,8 @@ class Cylinder(Source):
         self.radius = float(radius)
         self.half_length = float(half_length)
 
-    def _emit(self, out):
-        out.append((A.ISO_OP_CYLINDER, (self.radius, self.half_length)))
+    def _parts(self):
+        return [], [], [(A.ISO_OP_CYLINDER, (self.radius, self.half_length))]
 
 
 class _Binary(Source):
@@ -81,10 +97,8 @@ class _Binary(Source):
         self.a = a
         self.b = b
 
-    def _emit(self, out):
-        self.a._emit(out)
-        self.b._emit(out)
-        out.append((self.OP, ()))
+    def _parts(self):
+        return [], [self.a, self.b], [(self.OP, ())]
 
 
 class Union(_Binary):
@@ -105,10 +119,8 @@ class SmoothUnion(Source):
     def __init__(self, a: Source, b: Source, k: float):
         self.a, self.b, self.k = a, b, float(k)
 
-    def _emit(self, out):
-        self.a._emit(out)
-        self.b._emit(out)
-        out.append((A.ISO_OP_SMOOTH_UNION, (self.k,)))
+    def _parts(self):
+        return [], [self.a, self.b], [(A.ISO_OP_SMOOTH_UNION, (self.k,))]
 
 
 class Translate(Source):
@@ -118,10 +130,8 @@ class Translate(Source):
         self.offset = tuple(float(v) for v in offset)
         self.source = source
 
-    def _emit(self, out):
-        out.append((A.ISO_OP_TRANSLATE, self.offset))
-        self.source._emit(out)
-        out.append((A.ISO_OP_END_TRANSLATE, ()))
+    def _parts(self):
+        return [(A.ISO_OP_TRANSLATE, self.offset)], [self.source], [(A.ISO_OP_END_TRANSLATE, ())]
 
 
 class CentralDifference:

@@ -1,0 +1,155 @@
+"""The learned narrow band on the GPU: MFMA forward and gradients against
+the CPU restatement (oracle/mlp.py), training, and band extraction judged
+by the dense extraction (the mesh must be identical when no surface cell is
+culled)."""
+import numpy as np
+import pytest
+
+from isosurface_amd import MarchingCubes, _abi as A
+from isosurface_amd import workloads as W
+from isosurface_amd.band import N_PARAMS, NarrowBand, training_samples
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+
+def test_parameter_roundtrip(ctx):
+    band = NarrowBand(ctx)
+    theta = M.init(11)
+    band.set_params(theta)
+    assert np.array_equal(band.get_params(), theta)
+    band.init(3)
+    assert len(band.get_params()) == N_PARAMS
+    assert np.all(band.get_params()[:64 * 112].reshape(64, 112)[:, 99:] == 0.0)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_forward_matches_restatement(ctx, seed):
+    # Random weights; 4096 points; the device's MFMA products sum in another
+    # order than the restatement, so a bf16 rounding of an activation can
+    # flip: agreement to 1e-2 everywhere and 1e-4 at the median.
+    band = NarrowBand(ctx)
+    theta = M.init(seed)
+    band.set_params(theta)
+    p = np.random.default_rng(seed).random((4096, 3)).astype(np.float32)
+    got = band.predict(p)
+    want = M.forward(theta, p)
+    err = np.abs(got - want)
+    print("max", err.max(), "median", np.median(err), flush=True)
+    assert np.max(err) <= 1e-2 * (1.0 + np.max(np.abs(want)))
+    assert np.median(err) <= 1e-4 * (1.0 + np.median(np.abs(want)))
+
+
+def test_gradient_matches_restatement(ctx):
+    band = NarrowBand(ctx)
+    theta = M.init(5)
+    band.set_params(theta)
+    rng = np.random.default_rng(5)
+    n = 512
+    p = rng.random((n, 3)).astype(np.float32)
+    t = rng.normal(0, 0.05, n).astype(np.float32)
+    w = (rng.random(n) + 0.5).astype(np.float32)
+    loss, g = band.gradient(p, t, w, clamp=0.05)
+    ref_loss, ref = M.loss_and_grad(theta, p, np.clip(t, -0.05, 0.05), w)
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    o = 0
+    for name, shape in M.SIZES:
+        k = int(np.prod(shape))
+        a, b = g[o:o + k], ref[o:o + k]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(name, rel, flush=True)
+        assert rel <= 1e-2, name
+        o += k
+    # the padded feature columns stay exactly zero
+    assert np.all(g[:64 * 112].reshape(64, 112)[:, 99:] == 0.0)
+
+
+def test_training_fits_a_sphere(ctx):
+    src = W.sphere_source(0.3)
+    pts, sdf = training_samples(ctx, src, 60000, 60000, seed=1, coarse=65)
+    band = NarrowBand(ctx)
+    band.init(1)
+    first = band.gradient(pts[:16384], sdf[:16384], clamp=0.05)[0]
+    last = band.train(pts, sdf, steps=600, batch=4096, lr=3e-3, clamp=0.05, seed=2)
+    print("loss", first, "->", last, flush=True)
+    assert last < 0.05 * first
+    q = np.random.default_rng(3).random((20000, 3)).astype(np.float32)
+    err = np.abs(band.predict(q) - np.clip(ctx.sample(src, q), -0.05, 0.05))
+    assert np.median(err) < 5e-3
+
+
+def band_setup(ctx, src, size, steps=800):
+    pts, sdf = training_samples(ctx, src, 100000, 100000, seed=4, coarse=65)
+    band = NarrowBand(ctx)
+    band.init(4)
+    band.train(pts, sdf, steps=steps, batch=4096, lr=3e-3, clamp=0.05, seed=5)
+    cal_p, cal_s = training_samples(ctx, src, 50000, 50000, seed=6, coarse=65)
+    return band, band.calibrate(cal_p, cal_s)
+
+
+@pytest.mark.parametrize("size", [65, 130])
+def test_band_mesh_equals_dense(ctx, size):
+    src = W.spheres_minus_torus()[0]
+    band, margin = band_setup(ctx, src, size)
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
+    mesh, stats = band.extract_mesh(desc, margin)
+    dense = MarchingCubes(size, ctx).extract_mesh(src)
+    missed = band.missed_cells(desc)
+    print(f"size {size} margin {margin:.4f} band {stats.band_bricks}/{stats.bricks} "
+          f"missed {missed}", flush=True)
+    assert stats.band_bricks < stats.bricks
+    assert missed == 0
+    assert mesh.vertices.tobytes() == dense.vertices.tobytes()
+    assert np.array_equal(mesh.indices, dense.indices)
+
+
+def test_band_without_culling_is_dense(ctx):
+    src = W.sphere_source()
+    band = NarrowBand(ctx)
+    band.init(0)
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(40, A.ISO_SOURCE_PROGRAM)
+    mesh, stats = band.extract_mesh(desc, 1e30)
+    assert stats.band_bricks == stats.bricks
+    dense = MarchingCubes(40, ctx).extract_mesh(src)
+    assert mesh.vertices.tobytes() == dense.vertices.tobytes()
+    assert np.array_equal(mesh.indices, dense.indices)
+    assert band.missed_cells(desc) == 0
+
+
+def test_untrained_band_reports_its_misses(ctx):
+    # An untrained network with no margin culls surface bricks: the missed
+    # count is what the dense judge finds, and a mesh with misses differs.
+    src = W.sphere_source()
+    band = NarrowBand(ctx)
+    band.init(9)
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(48, A.ISO_SOURCE_PROGRAM)
+    mesh, stats = band.extract_mesh(desc, 0.0)
+    missed = band.missed_cells(desc)
+    dense = MarchingCubes(48, ctx).extract_mesh(src)
+    same = (len(mesh.indices) == len(dense.indices) and np.array_equal(mesh.indices, dense.indices)
+            and mesh.vertices.tobytes() == dense.vertices.tobytes())
+    assert (missed == 0) == same or missed > 0
+
+
+def test_band_error_paths(ctx):
+    import ctypes as C
+    lib = ctx.lib
+    band = NarrowBand(ctx)
+    band.init(0)
+    desc = ctx.reference_desc(17, A.ISO_SOURCE_GRID)
+    out = A.IsoMeshOut(None, None, 0, 0, 0, 0, A.ISO_MEMORY_HOST, 0)
+    assert lib.iso_extract_mc_band(ctx.handle, C.byref(desc), 0.01, C.byref(out), None) == \
+        A.ISO_ERR_INVALID_ARGUMENT
+    desc = ctx.reference_desc(17, A.ISO_SOURCE_PROGRAM)
+    assert lib.iso_extract_mc_band(ctx.handle, C.byref(desc), -1.0, C.byref(out), None) == \
+        A.ISO_ERR_INVALID_ARGUMENT
+    bad = np.zeros(10, np.float32)
+    assert lib.iso_band_set_params(ctx.handle, bad.ctypes.data, 10) == A.ISO_ERR_INVALID_ARGUMENT
+    p = np.zeros((8, 3), np.float32)
+    t = np.zeros(8, np.float32)
+    g = np.zeros(N_PARAMS, np.float32)
+    assert lib.iso_band_gradient(ctx.handle, p.ctypes.data, t.ctypes.data, None, 8, 0.05,
+                                 g.ctypes.data, None) == A.ISO_ERR_INVALID_ARGUMENT
