@@ -270,20 +270,22 @@ def batched_chunks(ctx: Context, steps: int) -> dict:
 
 def learned_band(ctx: Context, size: int, reps: int) -> dict:
     """The learned narrow band against the dense path at size^3 cells."""
-    from isosurface_amd.band import NarrowBand, training_samples
+    from isosurface_amd.band import NarrowBand, brick_half_diagonal, training_samples
     src = W.smooth_spheres(1024, k=0.02, seed=5)
+    upper = (1.0, 1.0, (size + 1) / size)  # the lattice's extra z layer
     t0 = time.perf_counter()
-    pts, sdf = training_samples(ctx, src, 1_000_000, 1_000_000, seed=11, coarse=257)
-    cal_p, cal_s = training_samples(ctx, src, 500_000, 500_000, seed=12, coarse=257)
+    pts, sdf = training_samples(ctx, src, 1_000_000, 1_000_000, seed=11, coarse=257, upper=upper)
+    cal_p, cal_s = training_samples(ctx, src, 500_000, 500_000, seed=12, coarse=257, upper=upper)
     t_samples = time.perf_counter() - t0
     band = NarrowBand(ctx)
     band.init(7)
-    steps = 3000
+    steps = 10000
     t0 = time.perf_counter()
-    loss = band.train(pts, sdf, steps=steps, batch=16384, lr=3e-3, lr_min=5e-5, clamp=0.05,
-                      seed=13)
+    weight = (1.0 + 9.0 * np.exp(-np.abs(sdf) / 0.01)).astype(np.float32)
+    loss = band.train(pts, sdf, weight, steps=steps, batch=16384, lr=3e-3, lr_min=5e-5,
+                      clamp=0.05, seed=13)
     t_train = time.perf_counter() - t0
-    margin = band.calibrate(cal_p, cal_s)
+    margin = band.calibrate(cal_p, cal_s, brick_half_diagonal(size + 1))
     ctx.upload_program(src.program())
     desc = ctx.reference_desc(size + 1, A.ISO_SOURCE_PROGRAM)
     cells = size * size * (size + 1)

@@ -250,6 +250,10 @@ typedef struct iso_band_train_params {
   float lr_min;
   float clamp;      /* targets are clamp(sdf, -clamp, clamp) */
   uint32_t seed;    /* batch sampling */
+  float overshoot;  /* loss weight (>= 1) of the part of an error that takes the
+                       prediction beyond its target, away from zero, or across
+                       zero: 1 is plain least squares, larger values train a
+                       prediction that errs toward the band */
 } iso_band_train_params;
 
 typedef struct iso_band_stats {
@@ -266,22 +270,26 @@ int iso_band_init(iso_ctx* ctx, uint32_t seed);
 int iso_band_set_params(iso_ctx* ctx, const float* theta, uint64_t n);
 int iso_band_get_params(iso_ctx* ctx, float* theta, uint64_t n);
 /* Trains with Adam on n samples (points: 3 floats each; weight may be null)
- * in `memory`; *last_loss receives the mean weighted squared error / 2 of
- * the last batch. */
+ * in `memory`.  Per sample, with r = f - t and e the distance from f to the
+ * interval between 0 and t (signed like r), the loss is
+ * w (r^2 + (overshoot - 1) e^2) / 2, averaged over the batch; *last_loss
+ * receives it for the last batch. */
 int iso_band_train(iso_ctx* ctx, const float* points, const float* sdf, const float* weight,
                    uint64_t n, int32_t memory, const iso_band_train_params* params,
                    float* last_loss);
 /* Loss and gradient of exactly these n samples (host memory, n a multiple
  * of 16), without updating the network. */
 int iso_band_gradient(iso_ctx* ctx, const float* points, const float* sdf, const float* weight,
-                      uint64_t n, float clamp, float* grad, float* loss);
+                      uint64_t n, float clamp, float overshoot, float* grad, float* loss);
 /* Network output at n points (3 floats each) in `memory`. */
 int iso_band_predict(iso_ctx* ctx, const float* points, uint64_t n, float* out, int32_t memory);
 /* Band extraction of the current program over `desc` (ISO_SOURCE_PROGRAM). */
 int iso_extract_mc_band(iso_ctx* ctx, const iso_grid_desc* desc, float margin, iso_mesh_out* out,
                         iso_band_stats* stats);
-/* Surface cells of the dense extraction of desc that lie in bricks the last
- * iso_extract_mc_band over the same desc culled. */
+/* Cells whose marching-cubes case differs between the last
+ * iso_extract_mc_band over desc and the dense extraction of desc (missed,
+ * spurious or altered surface cells): zero means the band's mesh is the
+ * dense mesh.  Valid until the next extraction in the context. */
 int iso_band_missed_cells(iso_ctx* ctx, const iso_grid_desc* desc, uint64_t* missed);
 
 /* After an ISO_MEMORY_HOST extraction returned ISO_ERR_CAPACITY, the mesh is

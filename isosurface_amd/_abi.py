@@ -89,7 +89,8 @@ ISO_BAND_PARAMS = 15617
 
 class IsoBandTrainParams(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
-                ("lr_min", C.c_float), ("clamp", C.c_float), ("seed", C.c_uint32)]
+                ("lr_min", C.c_float), ("clamp", C.c_float), ("seed", C.c_uint32),
+                ("overshoot", C.c_float)]
 
 
 class IsoBandStats(C.Structure):
@@ -131,7 +132,7 @@ SIGNATURES = [
                                  C.c_int32, C.POINTER(IsoBandTrainParams),
                                  C.POINTER(C.c_float)]),
     ("iso_band_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
-                                    C.c_float, C.c_void_p, C.POINTER(C.c_float)]),
+                                    C.c_float, C.c_float, C.c_void_p, C.POINTER(C.c_float)]),
     ("iso_band_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32]),
     ("iso_extract_mc_band", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float,
                                       C.POINTER(IsoMeshOut), C.POINTER(IsoBandStats)]),

@@ -4,7 +4,7 @@
     band.init(seed)
     pts, sdf = training_samples(ctx, source, ...)       # made by the plain path
     band.train(pts, sdf, steps=..., batch=16384)
-    margin = band.calibrate(held_out_pts, held_out_sdf)  # max |f_hat - clamp(f)|
+    margin = band.calibrate(pts2, sdf2, half_diag)     # held-out cull condition
     mesh, stats = band.extract_mesh(desc, margin)        # exact SDF only in the band
     band.missed_cells(desc)                              # judged by the dense path
 """
@@ -47,7 +47,10 @@ class NarrowBand:
 
     def train(self, points: np.ndarray, sdf: np.ndarray, weight: Optional[np.ndarray] = None,
               steps: int = 2000, batch: int = 16384, lr: float = 3e-3, lr_min: float = 1e-4,
-              clamp: float = 0.05, seed: int = 1) -> float:
+              clamp: float = 0.05, seed: int = 1, overshoot: float = 1.0) -> float:
+        """Adam on (points, sdf); `overshoot` > 1 weighs the part of each error
+        that lies beyond the interval between 0 and the target more, so the
+        network errs toward the band (see iso_band_train)."""
         points = np.ascontiguousarray(points, np.float32)
         sdf = np.ascontiguousarray(sdf, np.float32)
         wp = None
@@ -55,14 +58,15 @@ class NarrowBand:
             weight = np.ascontiguousarray(weight, np.float32)
             wp = weight.ctypes.data
         self.clamp = clamp
-        prm = A.IsoBandTrainParams(steps, batch, lr, lr_min, clamp, seed)
+        prm = A.IsoBandTrainParams(steps, batch, lr, lr_min, clamp, seed, overshoot)
         loss = C.c_float(0.0)
         self.ctx.check(self.lib.iso_band_train(self.ctx.handle, points.ctypes.data, sdf.ctypes.data,
                                                wp, len(sdf), A.ISO_MEMORY_HOST, C.byref(prm),
                                                C.byref(loss)), "iso_band_train")
         return float(loss.value)
 
-    def gradient(self, points, sdf, weight=None, clamp: float = 0.05) -> Tuple[float, np.ndarray]:
+    def gradient(self, points, sdf, weight=None, clamp: float = 0.05,
+                 overshoot: float = 1.0) -> Tuple[float, np.ndarray]:
         points = np.ascontiguousarray(points, np.float32)
         sdf = np.ascontiguousarray(sdf, np.float32)
         wp = None
@@ -73,7 +77,7 @@ class NarrowBand:
         loss = C.c_float(0.0)
         self.ctx.check(self.lib.iso_band_gradient(self.ctx.handle, points.ctypes.data,
                                                   sdf.ctypes.data, wp, len(sdf), clamp,
-                                                  grad.ctypes.data, C.byref(loss)),
+                                                  overshoot, grad.ctypes.data, C.byref(loss)),
                        "iso_band_gradient")
         return float(loss.value), grad
 
@@ -85,11 +89,20 @@ class NarrowBand:
                        "iso_band_predict")
         return out
 
-    def calibrate(self, points: np.ndarray, sdf: np.ndarray, safety: float = 1.25) -> float:
-        """Margin: the largest |prediction - truncated target| on held-out
-        samples, times `safety`."""
-        err = np.abs(self.predict(points) - np.clip(sdf, -self.clamp, self.clamp))
-        return float(np.max(err)) * safety
+    def calibrate(self, points: np.ndarray, sdf: np.ndarray, half_diag: float,
+                  safety: float = 1.1, slack: float = 2e-4) -> float:
+        """The smallest margin m under which, on held-out samples, the cull
+        test |f_hat| > half_diag + m keeps every sample that may be surface
+        (|f| <= half_diag) and never culls a sample whose predicted sign is
+        wrong; times `safety`, plus `slack`.  (A brick is culled from its
+        centre's prediction, so the samples stand in for brick centres.)"""
+        f_hat = self.predict(points).astype(np.float64)
+        f = np.asarray(sdf, np.float64)
+        must_keep = (np.abs(f) <= half_diag) | ((f_hat > 0) != (f > 0))
+        if not np.any(must_keep):
+            return slack
+        m = max(0.0, float(np.max(np.abs(f_hat[must_keep]) - half_diag)))
+        return m * safety + slack
 
     def extract_mesh(self, desc: A.IsoGridDesc, margin: float,
                      vertex_capacity: int = 0, index_capacity: int = 0):
@@ -120,13 +133,20 @@ class NarrowBand:
         return int(n.value)
 
 
+def brick_half_diagonal(size: int) -> float:
+    """Half-diagonal of an 8x8x8-cell brick of MarchingCubes::new(size)."""
+    return 0.5 * 8.0 * float(np.sqrt(3.0)) * float(np.float32(1.0) / np.float32(size - 1))
+
+
 def training_samples(ctx: Context, source: Source, n_uniform: int, n_surface: int,
-                     seed: int = 0, jitter: float = 0.01, coarse: int = 129):
-    """(points, sdf) generated by the plain path: uniform points in [0, 1]^3
-    plus vertices of a coarse plain GPU extraction, jittered by
+                     seed: int = 0, jitter: float = 0.01, coarse: int = 129,
+                     upper=(1.0, 1.0, 1.0)):
+    """(points, sdf) generated by the plain path: uniform points in
+    [0, upper] plus vertices of a coarse plain GPU extraction, jittered by
     N(0, jitter); targets are the program's exact values (GPU)."""
     rng = np.random.default_rng(seed)
-    pts = [rng.random((n_uniform, 3), dtype=np.float32)]
+    pts = [(rng.random((n_uniform, 3), dtype=np.float32) *
+            np.asarray(upper, np.float32)).astype(np.float32)]
     if n_surface:
         mesh = MarchingCubes(coarse, ctx).extract_mesh(source)
         pick = rng.integers(0, len(mesh.vertices), n_surface)

@@ -54,6 +54,8 @@ struct iso_ctx {
 
   // scratch
   DevBuf signs, seg, tplanes, agg, incl, active, totals, vout, iout, pts, vals;
+  DevBuf pvals;  // program samples of the current extraction (emit reads them)
+  DevBuf judge;  // dense sign words that judge a band extraction
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
@@ -268,7 +270,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_v,       &c->nn_grad,    &c->nn_frags,  &c->nn_loss,  &c->nn_x,
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
-                    &c->br_flags,   &c->br_list,    &c->br_count};
+                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
@@ -308,6 +310,7 @@ int iso_program_upload(iso_ctx* c, const iso_node* nodes, uint32_t n_nodes) {
                             hipMemcpyHostToDevice, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
   c->n_prog = static_cast<int>(ops.size());
+  c->band_valid = false;  // a band extraction is judged against its own program
   return ISO_OK;
 }
 
@@ -433,21 +436,23 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
 
 // signs (with state reset) + count/scan; samples from `grid` (device) or,
 // when it is null, from the context's program.
-// A replacement for the signs stage (the learned band's).
-using SignsStage = std::function<int(const iso::McBuffers&)>;
+// A replacement for the signs stage (the learned band's); `values`, when
+// not null, receives the exact samples it evaluates.
+using SignsStage = std::function<int(const iso::McBuffers&, float* values)>;
 
 int classify_and_count(iso_ctx* c, const float* grid, const iso::Lattice& L, const iso::McDims& D,
-                       const iso::McBuffers& B, const SignsStage* signs = nullptr) {
+                       const iso::McBuffers& B, const SignsStage* signs = nullptr,
+                       float* values = nullptr) {
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   if (signs) {
-    const int st = (*signs)(B);
+    const int st = (*signs)(B, values);
     if (st != ISO_OK) return st;
   } else if (grid) {
     ISO_HIP(c, timed(c, "mc_signs_grid",
                      [&] { return iso::launch_signs_grid(grid, L, D, B, c->stream); }));
   } else {
     ISO_HIP(c, timed(c, "mc_signs_program", [&] {
-              return iso::launch_signs_program(ops, c->n_prog, L, D, B, c->stream);
+              return iso::launch_signs_program(ops, c->n_prog, L, D, B, values, c->stream);
             }));
   }
   ISO_HIP(c, timed(c, "mc_count", [&] { return iso::launch_count(L, D, B, c->stream); }));
@@ -467,9 +472,19 @@ int read_totals(iso_ctx* c, const iso::McBuffers& B) {
 // Marching cubes over lattice L whose samples come from `grid` (device) or,
 // when grid is null, from the context's program; with `positions` the
 // vertices interpolate between per-point positions.  Writes the mesh to out.
+int prepare_scan(iso_ctx* c, uint64_t n, iso::scan::State* st);
+
+// The brick flags of a narrow-band extraction, for its emit stage.
+struct BandEmit {
+  const uint8_t* flags;
+  int64_t nbx, nby;
+};
+
 int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const float* positions,
-                    iso_mesh_out* out, const SignsStage* signs = nullptr) {
+                    iso_mesh_out* out, const SignsStage* signs = nullptr,
+                    const BandEmit* band = nullptr) {
   int st = ISO_OK;
+  c->band_valid = false;  // the context's sign words are about to change
   const iso::McDims D = iso::mc_dims(L);
   if (D.n_segs >= (int64_t(1) << 26))
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");
@@ -477,6 +492,20 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
   if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   uint64_t nv = 0, ni = 0;
+  // A program source stores its samples during classification and emit
+  // reads them (the same bits a re-evaluation would give); without the
+  // memory for them, emit evaluates the program at the edge endpoints.  A
+  // narrow band stores only its exact bricks' samples.
+  float* values = nullptr;
+  if (!grid && ensure(&c->pvals, static_cast<size_t>(D.n_points) * sizeof(float)) == hipSuccess)
+    values = static_cast<float*>(c->pvals.ptr);
+  auto emit = [&]() -> hipError_t {
+    if (band && values)
+      return iso::launch_emit_band(values, band->flags, band->nbx, band->nby, ops, c->n_prog, L,
+                                   D, B, c->stream);
+    const float* src = grid ? grid : (band ? nullptr : values);
+    return iso::launch_emit(src, ops, c->n_prog, positions, L, D, B, c->stream);
+  };
 
   if (out->memory == ISO_MEMORY_DEVICE) {
     B.vertices = out->vertices;
@@ -484,10 +513,8 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
     B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
     B.index_capacity = out->indices ? out->index_capacity : 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-      if ((st = classify_and_count(c, grid, L, D, B, signs)) != ISO_OK) return st;
-      ISO_HIP(c, timed(c, "mc_emit", [&] {
-                return iso::launch_emit(grid, ops, c->n_prog, positions, L, D, B, c->stream);
-              }));
+      if ((st = classify_and_count(c, grid, L, D, B, signs, values)) != ISO_OK) return st;
+      ISO_HIP(c, timed(c, "mc_emit", emit));
       if ((st = read_totals(c, B)) != ISO_OK) return st;
       const uint64_t n_act = c->h_totals[iso::kTotActive];
       if (n_act <= B.active_capacity) break;
@@ -505,12 +532,12 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
   }
 
   // Host memory: size the context's device buffers from the scan totals.
-  if ((st = classify_and_count(c, grid, L, D, B, signs)) != ISO_OK) return st;
+  if ((st = classify_and_count(c, grid, L, D, B, signs, values)) != ISO_OK) return st;
   if ((st = read_totals(c, B)) != ISO_OK) return st;
   const uint64_t n_act = c->h_totals[iso::kTotActive];
   if (n_act > B.active_capacity) {
     if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
-    if ((st = classify_and_count(c, grid, L, D, B, signs)) != ISO_OK) return st;
+    if ((st = classify_and_count(c, grid, L, D, B, signs, values)) != ISO_OK) return st;
     if ((st = read_totals(c, B)) != ISO_OK) return st;
   }
   nv = c->h_totals[iso::kTotVertices];
@@ -524,9 +551,7 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
   B.indices = static_cast<uint32_t*>(c->iout.ptr);
   B.vertex_capacity = nv;
   B.index_capacity = ni;
-  ISO_HIP(c, timed(c, "mc_emit", [&] {
-            return iso::launch_emit(grid, ops, c->n_prog, positions, L, D, B, c->stream);
-          }));
+  ISO_HIP(c, timed(c, "mc_emit", emit));
   if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
       (ni && (!out->indices || out->index_capacity < ni))) {
     ISO_HIP(c, hipStreamSynchronize(c->stream));
@@ -611,7 +636,9 @@ int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mes
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   ISO_HIP(c, timed(c, "dc_sample",
                    [&] { return iso::launch_dc_sample(ops, c->n_prog, L, B, c->stream); }));
-  ISO_HIP(c, timed(c, "dc_dual", [&] { return iso::launch_dc_dual(L, B, c->stream); }));
+  iso::scan::State S{};
+  if ((st = prepare_scan(c, n_cells, &S)) != ISO_OK) return st;
+  ISO_HIP(c, timed(c, "dc_dual", [&] { return iso::launch_dc_dual(L, B, S, c->stream); }));
   ISO_HIP(c, timed(c, "dc_feature", [&] {
             return iso::launch_dc_feature(ops, c->n_prog, L, epsilon, B, c->stream);
           }));
@@ -621,6 +648,8 @@ int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mes
   ISO_HIP(c, hipMemcpyAsync(c->h_dc, B.counters, iso::kDcSlots * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->h_dc[iso::kDcScanErr])
+    return fail(c, ISO_ERR_DEVICE, "the mixed-cell scan did not complete");
   if (c->h_dc[iso::kDcStalled])
     return fail(c, ISO_ERR_DEVICE, "a feature point's SVD did not converge");
   return st;
@@ -1024,6 +1053,7 @@ int iso_extract_batch(iso_ctx* c, iso_batch_out* out) {
   if (!out->vertex_offsets || !out->index_offsets)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "offset arrays are required");
   c->kept = false;
+  c->band_valid = false;
   ISO_HIP(c, hipSetDevice(c->device));
   const iso::BatchPlan& P = c->plan;
   iso::McDims D{};
@@ -1282,9 +1312,11 @@ int iso_band_train(iso_ctx* c, const float* points, const float* sdf, const floa
   if (!c || !points || !sdf || !prm || n == 0)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument or no samples");
   if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
-  if (prm->batch == 0 || (prm->batch & 15u) || prm->steps == 0 || !(prm->lr > 0.0f) ||
-      !(prm->clamp > 0.0f))
+  if (prm->batch == 0 || (prm->batch & 15u) || prm->steps == 0)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "batch must be a positive multiple of 16");
+  if (!(prm->lr > 0.0f) || !(prm->clamp > 0.0f) || !(prm->overshoot >= 1.0f) ||
+      !(prm->overshoot < 1.0e6f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lr and clamp must be positive, overshoot in [1, 1e6)");
   if (memory != ISO_MEMORY_HOST && memory != ISO_MEMORY_DEVICE)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown memory kind");
   ISO_HIP(c, hipSetDevice(c->device));
@@ -1304,6 +1336,7 @@ int iso_band_train(iso_ctx* c, const float* points, const float* sdf, const floa
   S.weight = static_cast<const float*>(dw);
   S.count = n;
   S.clamp = prm->clamp;
+  S.overshoot = prm->overshoot;
   S.seed = prm->seed;
   S.gather = 1;
   ISO_HIP(c, iso::mlp::launch_zero_grad(M, c->stream));
@@ -1334,9 +1367,11 @@ int iso_band_train(iso_ctx* c, const float* points, const float* sdf, const floa
 }
 
 int iso_band_gradient(iso_ctx* c, const float* points, const float* sdf, const float* weight,
-                      uint64_t n, float clamp, float* grad, float* loss) {
-  if (!c || !points || !sdf || !grad || n == 0 || (n & 15u) || !(clamp > 0.0f))
+                      uint64_t n, float clamp, float overshoot, float* grad, float* loss) {
+  if (!c || !points || !sdf || !grad || n == 0 || (n & 15u))
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "n must be a positive multiple of 16");
+  if (!(clamp > 0.0f) || !(overshoot >= 1.0f) || !(overshoot < 1.0e6f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "clamp must be positive, overshoot in [1, 1e6)");
   if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
   ISO_HIP(c, hipSetDevice(c->device));
   int st = ISO_OK;
@@ -1356,6 +1391,7 @@ int iso_band_gradient(iso_ctx* c, const float* points, const float* sdf, const f
   S.weight = static_cast<const float*>(dw);
   S.count = n;
   S.clamp = clamp;
+  S.overshoot = overshoot;
   S.seed = 0;
   S.gather = 0;
   ISO_HIP(c, iso::mlp::launch_zero_grad(M, c->stream));
@@ -1449,18 +1485,19 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, iso
   const uint64_t n_band = c->h_stage[1];
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   const iso::McDims D = iso::mc_dims(L);
-  SignsStage signs = [&](const iso::McBuffers& B) -> int {
+  SignsStage signs = [&](const iso::McBuffers& B, float* values) -> int {
     ISO_HIP(c, timed(c, "band_background", [&] {
               return iso::launch_band_background(L, D, G, flags, B, c->stream);
             }));
     ISO_HIP(c, timed(c, "band_exact", [&] {
-              return iso::launch_band_exact(ops, c->n_prog, L, G, flags, list, n_band, B,
+              return iso::launch_band_exact(ops, c->n_prog, L, G, flags, list, n_band, B, values,
                                             counters, c->stream);
             }));
     return ISO_OK;
   };
   c->band_valid = false;
-  st = extract_lattice(c, L, nullptr, nullptr, out, &signs);
+  const BandEmit band{flags, G.nbx, G.nby};
+  st = extract_lattice(c, L, nullptr, nullptr, out, &signs, &band);
   if (st != ISO_OK && st != ISO_ERR_CAPACITY) return st;
   c->band_desc = *desc;
   c->band_valid = true;
@@ -1485,21 +1522,18 @@ int iso_band_missed_cells(iso_ctx* c, const iso_grid_desc* desc, uint64_t* misse
   if (st != ISO_OK) return st;
   ISO_HIP(c, hipSetDevice(c->device));
   const iso::McDims D = iso::mc_dims(L);
-  const iso::BrickGrid G = iso::brick_grid(L);
+  // The band extraction's sign words are still in the context (any other
+  // extraction clears band_valid); the dense ones go to a buffer of their own.
   iso::McBuffers B{};
   if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
-  // Dense classification and count of the program; its active list holds
-  // every surface cell.
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    if ((st = classify_and_count(c, nullptr, L, D, B)) != ISO_OK) return st;
-    if ((st = read_totals(c, B)) != ISO_OK) return st;
-    if (c->h_totals[iso::kTotActive] <= B.active_capacity) break;
-    if ((st = prepare_mc(c, D, c->h_totals[iso::kTotActive], &B)) != ISO_OK) return st;
-  }
+  ISO_HIP(c, ensure(&c->judge, (D.n_sign_words + 3) * sizeof(uint64_t)));
+  iso::McBuffers J = B;
+  J.signs = static_cast<uint64_t*>(c->judge.ptr) + 1;
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  ISO_HIP(c, iso::launch_signs_program(ops, c->n_prog, L, D, J, nullptr, c->stream));
   unsigned long long* counters = static_cast<unsigned long long*>(c->br_count.ptr);
   ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(counters + 1), 2, 0u, c->stream));
-  ISO_HIP(c, iso::launch_band_missed(L, D, G, static_cast<const uint8_t*>(c->br_flags.ptr), B,
-                                     counters + 1, c->stream));
+  ISO_HIP(c, iso::launch_band_diff(L, D, B.signs, J.signs, counters + 1, c->stream));
   ISO_HIP(c, hipMemcpyAsync(c->h_stage + 4, counters + 1, sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));

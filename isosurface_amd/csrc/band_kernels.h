@@ -5,8 +5,8 @@
 // within its half-diagonal plus a calibrated margin of zero is in the band.
 // Band bricks get exact program samples; every other sample takes the
 // predicted sign of its brick.  The marching-cubes count and emit kernels
-// then run unchanged over the whole lattice (emit evaluates the program at
-// edge endpoints, so vertex positions are exact).  If no surface cell lies
+// then run unchanged over the whole lattice (emit reads the stored exact
+// samples, so vertex positions are exact).  If no surface cell lies
 // in a culled brick, the mesh is bit-identical to the dense extraction.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -18,7 +18,7 @@
 
 namespace iso {
 
-constexpr int kBrick = 8;  // cells per brick edge
+constexpr int kBrick = 1 << kBandBrickShift;  // cells per brick edge (8)
 
 // Brick flags
 enum : uint8_t {
@@ -45,16 +45,19 @@ hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float
 // extraction's state reset, like the other signs launchers).
 hipError_t launch_band_background(const Lattice& L, const McDims& D, const BrickGrid& G,
                                   const uint8_t* flags, const McBuffers& B, hipStream_t s);
-// Exact sign bits of every sample of the band bricks (each sample evaluated
-// once: a brick takes its far faces only where the neighbour is not in the
-// band).  n_band bricks from band_list; *evaluated receives the sample count.
+// Exact sign bits of every sample of the band bricks' closed sample boxes,
+// each sample evaluated once (by the band brick latest in brick order among
+// those holding it); values[f] receives the sample when values is not null.
+// n_band bricks from band_list; *evaluated receives the sample count.
 hipError_t launch_band_exact(const DevOp* ops, int n_ops, const Lattice& L, const BrickGrid& G,
                              const uint8_t* flags, const uint32_t* band_list, uint64_t n_band,
-                             const McBuffers& B, unsigned long long* evaluated, hipStream_t s);
-// Active cells of a dense extraction (the MC active list) that lie in culled
-// bricks: missed surface cells.  Count added to *missed.
-hipError_t launch_band_missed(const Lattice& L, const McDims& D, const BrickGrid& G,
-                              const uint8_t* flags, const McBuffers& B,
-                              unsigned long long* missed, hipStream_t s);
+                             const McBuffers& B, float* values, unsigned long long* evaluated,
+                             hipStream_t s);
+// Cells of lattice L whose marching-cubes case differs between the sign
+// arrays signs_a and signs_b (each with the extraction's guard words); the
+// count is added to *differing.
+hipError_t launch_band_diff(const Lattice& L, const McDims& D, const uint64_t* signs_a,
+                            const uint64_t* signs_b, unsigned long long* differing,
+                            hipStream_t s);
 
 }  // namespace iso

@@ -37,11 +37,14 @@ __global__ __launch_bounds__(256) void k_brick_centres(Lattice L, BrickGrid G,
   const int64_t b = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= G.n) return;
   const BrickBox B = brick_box(L, G, b);
+  // Every brick is tested with the half-diagonal of a full brick (bricks
+  // cut by the lattice's far faces are smaller), so one calibration holds
+  // for all of them.
   float d2 = 0.0f;
   for (int a = 0; a < 3; ++a) {
     const float mid = 0.5f * static_cast<float>(B.lo[a] + B.hi[a]);
     centres[3 * b + a] = L.origin[a] + mid * L.spacing[a];
-    const float ext = static_cast<float>(B.hi[a] - B.lo[a]) * L.spacing[a];
+    const float ext = static_cast<float>(kBrick) * L.spacing[a];
     d2 += ext * ext;
   }
   half_diag[b] = 0.5f * sqrtf(d2);
@@ -104,66 +107,119 @@ __global__ __launch_bounds__(256) void k_band_background(Lattice L, BrickGrid G,
   }
 }
 
-__device__ __forceinline__ bool in_band(const uint8_t* flags, const BrickGrid& G, int64_t bx,
-                                        int64_t by, int64_t bz) {
-  if (bx >= G.nbx || by >= G.nby || bz >= G.nbz) return false;
-  return (flags[(bz * G.nby + by) * G.nbx + bx] & kBrickBand) != 0;
+// Neighbour bricks (bit (dz+1)*9 + (dy+1)*3 + dx+1) that also hold a sample
+// of class (cx, cy, cz) — per axis 0: on the brick's low face, 1: inside,
+// 2: on its high face — and come later in brick order.  The band brick
+// latest in brick order among those holding a sample evaluates it, so each
+// exact sample is evaluated once.
+__device__ __forceinline__ uint32_t later_holders(int cx, int cy, int cz) {
+  uint32_t m = 0;
+  for (int dz = cz == 0 ? -1 : 0; dz <= (cz == 2 ? 1 : 0); ++dz)
+    for (int dy = cy == 0 ? -1 : 0; dy <= (cy == 2 ? 1 : 0); ++dy)
+      for (int dx = cx == 0 ? -1 : 0; dx <= (cx == 2 ? 1 : 0); ++dx)
+        if (dz > 0 || (dz == 0 && (dy > 0 || (dy == 0 && dx > 0))))
+          m |= 1u << ((dz + 1) * 9 + (dy + 1) * 3 + dx + 1);
+  return m;
 }
 
-// One lane per (band brick, sample row): row (ly, lz) of the brick's
-// 9 x 9 rows, x across the brick.  A brick evaluates its far face along an
-// axis only where the neighbour along that axis is not in the band, so every
-// sample of the band's closed bricks is evaluated (the band brick with the
-// largest coordinate sum among those holding it always takes it).
-constexpr int kRows = (kBrick + 1) * (kBrick + 1);
+constexpr int kExactWaves = 4;                      // waves per workgroup, one brick each
+constexpr int kBoxRows = (kBrick + 1) * (kBrick + 1);  // x rows of a brick's sample box
 
-__global__ __launch_bounds__(256) void k_band_exact(const DevOp* __restrict__ ops, int n_ops,
-                                                    Lattice L, BrickGrid G,
-                                                    const uint8_t* __restrict__ flags,
-                                                    const uint32_t* __restrict__ band_list,
-                                                    uint64_t n_band, uint64_t* __restrict__ S,
-                                                    unsigned long long* evaluated) {
-  const uint64_t n_items = n_band * kRows;
+// One wavefront per band brick: its lanes walk the brick's closed sample
+// box flat (x fastest), evaluate the samples the brick owns, store their
+// values, and gather the sign bits per x row in LDS; then one lane per row
+// merges the row into the sign words (at most two words, OR for inside,
+// AND-NOT for outside, over the background the bricks' predictions wrote).
+__global__ __launch_bounds__(64 * kExactWaves) void k_band_exact(
+    const DevOp* __restrict__ ops, int n_ops, Lattice L, BrickGrid G,
+    const uint8_t* __restrict__ flags, const uint32_t* __restrict__ band_list, uint64_t n_band,
+    uint64_t* __restrict__ S, float* __restrict__ values, unsigned long long* evaluated) {
+  __shared__ uint32_t s_in[kExactWaves][kBoxRows];
+  __shared__ uint32_t s_own[kExactWaves][kBoxRows];
+  const int lane = __lane_id();
+  const int wv = threadIdx.x >> 6;
   uint32_t count = 0;
-  for (uint64_t it = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; it < n_items;
-       it += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const int64_t b = band_list[it / kRows];
-    const int row = static_cast<int>(it % kRows);
-    const int ly = row % (kBrick + 1), lz = row / (kBrick + 1);
-    const int64_t bx = b % G.nbx, by = (b / G.nbx) % G.nby, bz = b / (G.nbx * G.nby);
-    const int64_t y = kBrick * by + ly, z = kBrick * bz + lz;
-    if (y > L.ny - 1 || z > L.nz - 1) continue;
-    if (ly == kBrick && in_band(flags, G, bx, by + 1, bz)) continue;
-    if (lz == kBrick && in_band(flags, G, bx, by, bz + 1)) continue;
-    const int64_t x0 = kBrick * bx;
-    int64_t x1 = x0 + kBrick;  // inclusive far face unless the +x neighbour takes it
-    if (in_band(flags, G, bx + 1, by, bz)) x1 -= 1;
-    if (x1 > L.nx - 1) x1 = L.nx - 1;
-    const float py = L.origin[1] + static_cast<float>(y) * L.spacing[1];
-    const float pz = L.origin[2] + static_cast<float>(z) * L.spacing[2];
-    const int64_t f0 = (z * L.ny + y) * L.nx + x0;
-    const int64_t w0 = f0 >> 6;
-    uint64_t set0 = 0, clr0 = 0, set1 = 0, clr1 = 0;
-    for (int64_t x = x0; x <= x1; ++x) {
-      const float px = L.origin[0] + static_cast<float>(x) * L.spacing[0];
-      const float v = sdf_eval(ops, n_ops, px, py, pz);
-      const int64_t f = f0 + (x - x0);
-      const uint64_t bit = 1ull << (f & 63);
-      const bool inside = !(v > 0.0f);  // classify_corners, distance.rs:52-54
-      if ((f >> 6) == w0) {
-        if (inside) set0 |= bit;
-        else clr0 |= bit;
-      } else {
-        if (inside) set1 |= bit;
-        else clr1 |= bit;
-      }
-      ++count;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kExactWaves; base < n_band;
+       base += static_cast<uint64_t>(gridDim.x) * kExactWaves) {
+    const uint64_t item = base + wv;
+    const bool live = item < n_band;  // uniform per wave
+    for (int r = lane; r < kBoxRows; r += 64) {
+      s_in[wv][r] = 0;
+      s_own[wv][r] = 0;
     }
-    unsigned long long* W = reinterpret_cast<unsigned long long*>(S);
-    if (set0) atomicOr(&W[w0], set0);
-    if (clr0) atomicAnd(&W[w0], ~clr0);
-    if (set1) atomicOr(&W[w0 + 1], set1);
-    if (clr1) atomicAnd(&W[w0 + 1], ~clr1);
+    int64_t lo[3] = {0, 0, 0};
+    int ext[3] = {0, 0, 0};
+    uint32_t nb = 0;
+    if (live) {
+      const int64_t b = band_list[item];
+      const int64_t bc[3] = {b % G.nbx, (b / G.nbx) % G.nby, b / (G.nbx * G.nby)};
+      const int64_t nbk[3] = {G.nbx, G.nby, G.nbz};
+      const int64_t n[3] = {L.nx, L.ny, L.nz};
+      // Band flags of the 26 neighbours (and the brick itself, bit 13).
+      bool band = false;
+      if (lane < 27) {
+        const int64_t q[3] = {bc[0] + lane % 3 - 1, bc[1] + (lane / 3) % 3 - 1,
+                              bc[2] + lane / 9 - 1};
+        if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < nbk[0] && q[1] < nbk[1] &&
+            q[2] < nbk[2])
+          band = (flags[(q[2] * G.nby + q[1]) * G.nbx + q[0]] & kBrickBand) != 0;
+      }
+      nb = static_cast<uint32_t>(__ballot(band));
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = kBrick * bc[a];
+        int64_t hi = lo[a] + kBrick < n[a] - 1 ? lo[a] + kBrick : n[a] - 1;
+        // the high face goes to the next brick along the axis when it is in the band
+        const int up = a == 0 ? 14 : a == 1 ? 16 : 22;
+        if (hi == lo[a] + kBrick && (nb >> up) & 1u) hi -= 1;
+        ext[a] = static_cast<int>(hi - lo[a] + 1);
+      }
+    }
+    __syncthreads();
+    if (live) {
+      const int exy = ext[0] * ext[1];
+      const int total = exy * ext[2];
+      for (int t = lane; t < total; t += 64) {
+        const int lz = t / exy;
+        const int rem = t - lz * exy;
+        const int ly = rem / ext[0];
+        const int lx = rem - ly * ext[0];
+        const int cx = lx == 0 ? 0 : lx == kBrick ? 2 : 1;
+        const int cy = ly == 0 ? 0 : ly == kBrick ? 2 : 1;
+        const int cz = lz == 0 ? 0 : lz == kBrick ? 2 : 1;
+        if (later_holders(cx, cy, cz) & nb) continue;
+        const int64_t x = lo[0] + lx, y = lo[1] + ly, z = lo[2] + lz;
+        const float v = sdf_eval(ops, n_ops, L.origin[0] + static_cast<float>(x) * L.spacing[0],
+                                 L.origin[1] + static_cast<float>(y) * L.spacing[1],
+                                 L.origin[2] + static_cast<float>(z) * L.spacing[2]);
+        if (values) values[(z * L.ny + y) * L.nx + x] = v;
+        const int r = ly + (kBrick + 1) * lz;
+        if (!(v > 0.0f)) atomicOr(&s_in[wv][r], 1u << lx);  // classify_corners, distance.rs:52-54
+        atomicOr(&s_own[wv][r], 1u << lx);
+        ++count;
+      }
+    }
+    __syncthreads();
+    if (live) {
+      for (int r = lane; r < kBoxRows; r += 64) {
+        const uint64_t own = s_own[wv][r];
+        if (!own) continue;
+        const uint64_t in = s_in[wv][r] & own;
+        const uint64_t out = own & ~in;
+        const int ly = r % (kBrick + 1), lz = r / (kBrick + 1);
+        const int64_t f0 = ((lo[2] + lz) * L.ny + lo[1] + ly) * L.nx + lo[0];
+        const int64_t w0 = f0 >> 6;
+        const int sh = static_cast<int>(f0 & 63);
+        unsigned long long* W = reinterpret_cast<unsigned long long*>(S);
+        if (in << sh) atomicOr(&W[w0], in << sh);
+        if (out << sh) atomicAnd(&W[w0], ~(out << sh));
+        if (sh) {
+          const uint64_t in1 = in >> (64 - sh), out1 = out >> (64 - sh);
+          if (in1) atomicOr(&W[w0 + 1], in1);
+          if (out1) atomicAnd(&W[w0 + 1], ~out1);
+        }
+      }
+    }
+    __syncthreads();
   }
   unsigned long long c = count;
 #pragma unroll
@@ -171,32 +227,39 @@ __global__ __launch_bounds__(256) void k_band_exact(const DevOp* __restrict__ op
   if (__lane_id() == 0 && c) atomicAdd(evaluated, c);
 }
 
-__global__ __launch_bounds__(256) void k_band_missed(Lattice L, McDims D, BrickGrid G,
-                                                     const uint8_t* __restrict__ flags,
-                                                     const uint32_t* __restrict__ active,
-                                                     uint64_t capacity,
-                                                     const unsigned long long* __restrict__ totals,
-                                                     unsigned long long* missed) {
-  uint64_t n = totals[kTotActive];
-  if (n > capacity) n = capacity;
-  const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
-  const uint32_t cyn = static_cast<uint32_t>(L.ny - 1);
+// Cells whose marching-cubes case differs between two sign arrays of the
+// same lattice (a band extraction's and the dense one's): a cell differs
+// when any of its eight corner bits does.  One thread per 64-cell segment.
+__global__ __launch_bounds__(256) void k_band_diff(Lattice L, McDims D,
+                                                   const uint64_t* __restrict__ Sa,
+                                                   const uint64_t* __restrict__ Sb,
+                                                   unsigned long long* differing) {
+  const int64_t nx = L.nx, nxy = L.nx * L.ny;
+  const int64_t spr = D.segs_per_row;
+  const int64_t cyn = L.ny - 1;
   uint32_t count = 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    const uint32_t rec = active[i];
-    const uint32_t g = rec >> 6;
-    const uint32_t row = g / spr;
-    const uint32_t w = g - row * spr;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < D.n_segs;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t row = g / spr, w = g - (g / spr) * spr;
     const int64_t cz = row / cyn, cy = row - (row / cyn) * cyn;
-    const int64_t cx = 64 * static_cast<int64_t>(w) + (rec & 63u);
-    const int64_t b = ((cz / kBrick) * G.nby + cy / kBrick) * G.nbx + cx / kBrick;
-    if (!(flags[b] & kBrickBand)) ++count;
+    const int64_t f00 = cz * nxy + cy * nx + 64 * w;
+    uint64_t m = 0, n = 0;
+    const int64_t offs[4] = {0, nx, nxy, nxy + nx};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      uint64_t na, nb;
+      const uint64_t a = row_word(Sa, f00 + offs[k], &na);
+      const uint64_t b = row_word(Sb, f00 + offs[k], &nb);
+      m |= a ^ b;
+      n |= na ^ nb;
+    }
+    // cell i has corners at x = i and i + 1
+    count += __popcll((m | (m >> 1) | (n << 63)) & seg_valid(nx, w));
   }
   unsigned long long c = count;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
-  if (__lane_id() == 0 && c) atomicAdd(missed, c);
+  if (__lane_id() == 0 && c) atomicAdd(differing, c);
 }
 
 unsigned grid_of(uint64_t n, uint64_t cap) {
@@ -253,18 +316,23 @@ hipError_t launch_band_background(const Lattice& L, const McDims& D, const Brick
 
 hipError_t launch_band_exact(const DevOp* ops, int n_ops, const Lattice& L, const BrickGrid& G,
                              const uint8_t* flags, const uint32_t* band_list, uint64_t n_band,
-                             const McBuffers& B, unsigned long long* evaluated, hipStream_t s) {
+                             const McBuffers& B, float* values, unsigned long long* evaluated,
+                             hipStream_t s) {
   if (n_band == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_band_exact, dim3(grid_of(n_band * kRows, 1u << 20)), dim3(256), 0, s, ops,
-                     n_ops, L, G, flags, band_list, n_band, B.signs, evaluated);
+  if (!ops || !flags || !band_list || !B.signs || !evaluated) return hipErrorInvalidValue;
+  uint64_t blocks = (n_band + kExactWaves - 1) / kExactWaves;
+  if (blocks > (1u << 20)) blocks = 1u << 20;
+  hipLaunchKernelGGL(k_band_exact, dim3(static_cast<unsigned>(blocks)), dim3(64 * kExactWaves), 0,
+                     s, ops, n_ops, L, G, flags, band_list, n_band, B.signs, values, evaluated);
   return hipGetLastError();
 }
 
-hipError_t launch_band_missed(const Lattice& L, const McDims& D, const BrickGrid& G,
-                              const uint8_t* flags, const McBuffers& B,
-                              unsigned long long* missed, hipStream_t s) {
-  hipLaunchKernelGGL(k_band_missed, dim3(grid_of(B.active_capacity, 16384)), dim3(256), 0, s, L,
-                     D, G, flags, B.active, B.active_capacity, B.totals, missed);
+hipError_t launch_band_diff(const Lattice& L, const McDims& D, const uint64_t* signs_a,
+                            const uint64_t* signs_b, unsigned long long* differing,
+                            hipStream_t s) {
+  if (!signs_a || !signs_b || !differing) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_band_diff, dim3(grid_of(static_cast<uint64_t>(D.n_segs), 16384)),
+                     dim3(256), 0, s, L, D, signs_a, signs_b, differing);
   return hipGetLastError();
 }
 

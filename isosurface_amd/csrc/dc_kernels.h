@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "mc_kernels.h"
+#include "scan.h"
 #include "sdf_program.h"
 
 namespace iso {
@@ -18,7 +19,8 @@ namespace iso {
 enum : int {
   kDcMixed = 0,    // number of mixed primal cells listed
   kDcStalled = 1,  // nonzero when a QEF's SVD hit the sweep bound
-  kDcSlots = 2,
+  kDcScanErr = 2,  // nonzero when the mixed-cell scan's look-back gave up
+  kDcSlots = 3,
 };
 
 struct DcBuffers {
@@ -32,8 +34,10 @@ struct DcBuffers {
 // Samples the program over the primal lattice and clears the counters.
 hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const DcBuffers& B,
                             hipStream_t s);
-// Dual values, centre dual vertices and the mixed-cell list.
-hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, hipStream_t s);
+// Dual values, centre dual vertices and the mixed-cell list (ascending, by a
+// single-pass scan whose state `st` holds scan::blocks_for(cells, 1) slots;
+// its err word is redirected to counters[kDcScanErr]).
+hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, hipStream_t s);
 // Feature points of the listed mixed cells (normals by central differences).
 hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
                              const DcBuffers& B, hipStream_t s);

@@ -7,14 +7,14 @@
 //   2. dual     one thread per primal cell: dual value
 //               lerp(v0, v6, 0.5) = 0.5*v0 + 0.5*v6, and either the centre
 //               lerp(c0, c6, 0.5) as dual vertex (cube index 0 or 255) or an
-//               entry in the mixed-cell list;
+//               entry in the mixed-cell list, in ascending cell order by a
+//               single-pass scan;
 //   3. feature  one thread per listed mixed cell: corner normals by central
 //               differences of the program (source.rs:82-94), normalised or
 //               zero (marching_cubes_impl.rs:58-69), then TangentPlanes and
 //               the QEF solve (feature/mod.rs:91-148, feature/qef.rs:36-68)
 //               with the reference's f64 SVD.
-// The list order is whatever the atomics give, but every result is written
-// to its cell's own slot, so the output is deterministic.
+// Every result is written to its cell's own slot.
 #include <hip/hip_runtime.h>
 
 #include "dc_kernels.h"
@@ -46,67 +46,57 @@ __global__ __launch_bounds__(256) void k_dc_sample(const DevOp* __restrict__ ops
   }
 }
 
-__global__ __launch_bounds__(256) void k_dc_dual(Lattice L, int64_t n_cells,
-                                                 const float* __restrict__ samples,
-                                                 float* __restrict__ dual_values,
-                                                 float* __restrict__ dual_points,
-                                                 uint32_t* __restrict__ mixed,
-                                                 unsigned long long* __restrict__ counters) {
-  const int64_t nx = L.nx, ny = L.ny;
-  const uint32_t cnx = static_cast<uint32_t>(nx - 1);
-  const uint32_t cnxy = static_cast<uint32_t>((nx - 1) * (ny - 1));
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  const int lane = __lane_id();
-  // Every lane of a wave runs the same number of iterations, so the ballot
-  // below sees the whole wave.
-  const int64_t start = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t wave_start = start - lane;
-  for (int64_t base = wave_start; base < n_cells; base += stride) {
-    const int64_t c = base + lane;
-    bool is_mixed = false;
-    if (c < n_cells) {
-      const uint32_t cc = static_cast<uint32_t>(c);
-      const uint32_t z = cc / cnxy;
-      const uint32_t r = cc - z * cnxy;
-      const uint32_t y = r / cnx;
-      const uint32_t x = r - y * cnx;
-      float v[8];
-      uint32_t cube = 0;
+// Scan load: one primal cell's dual value and, unless mixed, its centre.
+struct DualLoad {
+  Lattice L;
+  const float* samples;
+  float* dual_values;
+  float* dual_points;
+  __device__ unsigned long long operator()(uint64_t c) const {
+    const int64_t nx = L.nx, ny = L.ny;
+    const uint32_t cnx = static_cast<uint32_t>(nx - 1);
+    const uint32_t cnxy = static_cast<uint32_t>((nx - 1) * (ny - 1));
+    const uint32_t cc = static_cast<uint32_t>(c);
+    const uint32_t z = cc / cnxy;
+    const uint32_t r = cc - z * cnxy;
+    const uint32_t y = r / cnx;
+    const uint32_t x = r - y * cnx;
+    float v[8];
+    uint32_t cube = 0;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int64_t px = x + tables::kCorner[i][0], py = y + tables::kCorner[i][1],
-                      pz = z + tables::kCorner[i][2];
-        v[i] = samples[(pz * ny + py) * nx + px];
-        if (!(v[i] > 0.0f)) cube |= 1u << i;  // classify_corners, distance.rs:52-54
-      }
-      // Signed::lerp (distance.rs:56-58) with factor 0.5: (1 - 0.5) * a + 0.5 * b
-      dual_values[c] = 0.5f * v[0] + 0.5f * v[6];
-      is_mixed = cube != 0u && cube != 255u;
-      if (!is_mixed) {
-        // Vec3::lerp (vector.rs:325-333) of corners 0 and 6
-        const float c0x = lattice_coord(L, 0, x), c0y = lattice_coord(L, 1, y),
-                    c0z = lattice_coord(L, 2, z);
-        const float c6x = lattice_coord(L, 0, x + 1), c6y = lattice_coord(L, 1, y + 1),
-                    c6z = lattice_coord(L, 2, z + 1);
-        dual_points[3 * c + 0] = 0.5f * c0x + 0.5f * c6x;
-        dual_points[3 * c + 1] = 0.5f * c0y + 0.5f * c6y;
-        dual_points[3 * c + 2] = 0.5f * c0z + 0.5f * c6z;
-      }
+    for (int i = 0; i < 8; ++i) {
+      const int64_t px = x + tables::kCorner[i][0], py = y + tables::kCorner[i][1],
+                    pz = z + tables::kCorner[i][2];
+      v[i] = samples[(pz * ny + py) * nx + px];
+      if (!(v[i] > 0.0f)) cube |= 1u << i;  // classify_corners, distance.rs:52-54
     }
-    // One atomic per wave reserves the wave's slots in the mixed-cell list.
-    const uint64_t ballot = __ballot(is_mixed);
-    if (ballot) {
-      unsigned long long first = 0;
-      const int leader = __builtin_ctzll(ballot);
-      if (lane == leader) first = atomicAdd(&counters[kDcMixed], static_cast<unsigned long long>(__popcll(ballot)));
-      first = __shfl(first, leader, 64);
-      if (is_mixed) {
-        const uint64_t below = ballot & ((1ull << lane) - 1ull);
-        mixed[first + __popcll(below)] = static_cast<uint32_t>(c);
-      }
+    // Signed::lerp (distance.rs:56-58) with factor 0.5: (1 - 0.5) * a + 0.5 * b
+    dual_values[c] = 0.5f * v[0] + 0.5f * v[6];
+    const bool is_mixed = cube != 0u && cube != 255u;
+    if (!is_mixed) {
+      // Vec3::lerp (vector.rs:325-333) of corners 0 and 6
+      const float c0x = lattice_coord(L, 0, x), c0y = lattice_coord(L, 1, y),
+                  c0z = lattice_coord(L, 2, z);
+      const float c6x = lattice_coord(L, 0, x + 1), c6y = lattice_coord(L, 1, y + 1),
+                  c6z = lattice_coord(L, 2, z + 1);
+      dual_points[3 * c + 0] = 0.5f * c0x + 0.5f * c6x;
+      dual_points[3 * c + 1] = 0.5f * c0y + 0.5f * c6y;
+      dual_points[3 * c + 2] = 0.5f * c0z + 0.5f * c6z;
     }
+    return is_mixed ? 1ull : 0ull;
   }
-}
+};
+
+// Scan sink: the mixed cells' list slots and, at the last cell, the count.
+struct DualSink {
+  uint64_t n;
+  uint32_t* mixed;
+  unsigned long long* counters;
+  __device__ void operator()(uint64_t c, unsigned long long v, unsigned long long ex) const {
+    if (v) mixed[ex] = static_cast<uint32_t>(c);
+    if (c == n - 1) counters[kDcMixed] = ex + v;
+  }
+};
 
 // CentralDifference::sample_normal (source.rs:82-94) at p, then
 // normalised().unwrap_or_default() (marching_cubes_impl.rs:63-67).
@@ -176,14 +166,13 @@ hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const
   return hipGetLastError();
 }
 
-hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, hipStream_t s) {
-  const int64_t n_cells = (L.nx - 1) * (L.ny - 1) * (L.nz - 1);
-  int64_t blocks = (n_cells + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_dc_dual, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L, n_cells,
-                     B.samples, B.dual_values, B.dual_points, B.mixed, B.counters);
-  return hipGetLastError();
+hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, hipStream_t s) {
+  const uint64_t n_cells = static_cast<uint64_t>((L.nx - 1) * (L.ny - 1) * (L.nz - 1));
+  if (n_cells == 0 || n_cells > 0xffffffffull || !st.slot || !st.ctl) return hipErrorInvalidValue;
+  st.err = B.counters + kDcScanErr;
+  const DualLoad load{L, B.samples, B.dual_values, B.dual_points};
+  const DualSink sink{n_cells, B.mixed, B.counters};
+  return scan::launch<1>(n_cells, load, sink, st, s);
 }
 
 hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,

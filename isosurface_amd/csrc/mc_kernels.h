@@ -28,6 +28,7 @@ struct McDims {
 constexpr int kCountBlock = 1024;  // threads per count workgroup (16 waves)
 constexpr int kCountRounds = 1;    // segments per count thread
 constexpr int kCountSegs = kCountBlock * kCountRounds;  // segments per count workgroup
+constexpr int kBandBrickShift = 3;  // narrow-band bricks: 8 x 8 x 8 cells
 
 // Running totals of the decoupled look-back scan.
 struct ScanTriple {
@@ -93,13 +94,21 @@ struct McBuffers {
 // reset the per-extraction state (guard words, totals, look-back words).
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,
                              const McBuffers& B, hipStream_t s);
+// With `values` (n_points floats) the program's samples are stored too.
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
-                                const McBuffers& B, hipStream_t s);
+                                const McBuffers& B, float* values, hipStream_t s);
 hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
 // With `positions` (3 floats per lattice point, requires `grid`) vertices
 // interpolate between those positions instead of lattice coordinates.
 hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const float* positions,
                        const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
+// Emit of a narrow-band extraction: corner values from `values` (stored by
+// the band's exact sampling) for cells of bricks flagged exact (bit 0 of
+// brick_flags, bricks of 2^kBandBrickShift cells, nbx x nby per z layer),
+// the program elsewhere.
+hipError_t launch_emit_band(const float* values, const uint8_t* brick_flags, int64_t nbx,
+                            int64_t nby, const DevOp* ops, int n_ops, const Lattice& L,
+                            const McDims& D, const McBuffers& B, hipStream_t s);
 hipError_t launch_sample_lattice(const DevOp* ops, int n_ops, const Lattice& L, float* out,
                                  hipStream_t s);
 

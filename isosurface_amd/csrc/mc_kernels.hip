@@ -129,10 +129,13 @@ __global__ __launch_bounds__(256) void k_signs_grid(const float* __restrict__ g,
   }
 }
 
+// Program signs; with `values` the samples are also stored (x fastest), so
+// emit reads the endpoint values instead of evaluating the program again.
 __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__ ops, int n_ops,
                                                        Lattice L, int64_t n_points,
                                                        int64_t n_words,
-                                                       uint64_t* __restrict__ S, ResetArgs R) {
+                                                       uint64_t* __restrict__ S,
+                                                       float* __restrict__ values, ResetArgs R) {
   reset_state(R);
   const int lane = __lane_id();
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -151,6 +154,7 @@ __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__
       const float v = sdf_eval(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
                                lattice_coord(L, 2, z));
       inside = !(v > 0.0f);
+      if (values) values[f] = v;
     }
     const uint64_t m = __ballot(inside);
     if (lane == 0) S[w] = m;
@@ -450,6 +454,9 @@ struct EmitArgs {
   uint32_t* indices;
   uint64_t vcap, icap;
   const unsigned long long* totals;
+  // kEmitBand: brick flags (bit 0: exact samples), bricks per x and y row
+  const uint8_t* brick_flags;
+  int64_t nbx, nby;
 };
 
 // Where an extraction's sample values and vertex positions come from.
@@ -457,6 +464,8 @@ enum EmitSource : int {
   kEmitProgram = 0,    // values from the SDF program, positions from the lattice
   kEmitGrid = 1,       // values from a dense grid, positions from the lattice
   kEmitPositions = 2,  // values from a dense grid, positions per lattice point
+  kEmitBand = 3,       // values stored for the exact bricks of a narrow band, the
+                       // SDF program elsewhere; positions from the lattice
 };
 
 __device__ __forceinline__ float space_coord(const CellSpace& C, int axis, int64_t i) {
@@ -466,7 +475,7 @@ __device__ __forceinline__ float space_coord(const CellSpace& C, int axis, int64
 template <int kSrc>
 __device__ __forceinline__ float space_value(const CellSpace& C, int64_t x, int64_t y,
                                              int64_t z) {
-  if (kSrc != kEmitProgram) {
+  if (kSrc != kEmitProgram && kSrc != kEmitBand) {
     return C.grid[(z * C.ny + y) * C.nx + x];
   } else {
     return sdf_eval(C.ops, C.n_ops, space_coord(C, 0, x), space_coord(C, 1, y),
@@ -546,7 +555,15 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     const uint64_t tp0 = tp[0], tp1 = tp[1], tp2 = tp[2];
     const uint32_t tseg = A.seg[g].y;
     float cv[8];
-    if (kSrc != kEmitProgram) {
+    // A band cell's corners all lie in its brick's closed sample box, whose
+    // samples were evaluated exactly and stored; a culled brick's cells are
+    // only active when a predicted sign was wrong, and evaluate the program.
+    bool stored = kSrc != kEmitProgram;
+    if (kSrc == kEmitBand)
+      stored = (A.brick_flags[((cz >> kBandBrickShift) * A.nby + (cy >> kBandBrickShift)) * A.nbx +
+                              (cx >> kBandBrickShift)] &
+                1u) != 0;
+    if (stored) {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         cv[k] = C.grid[((cz + tables::kCorner[k][2]) * ny + cy + tables::kCorner[k][1]) * nx + cx +
@@ -627,8 +644,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
                       uz = cz + tables::kCorner[u][2];
         const int64_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
                       vz = cz + tables::kCorner[v][2];
-        const float va = kSrc != kEmitProgram ? cv[u] : space_value<kSrc>(C, ux, uy, uz);
-        const float vb = kSrc != kEmitProgram ? cv[v] : space_value<kSrc>(C, vx, vy, vz);
+        const float va = stored ? cv[u] : space_value<kSrc>(C, ux, uy, uz);
+        const float vb = stored ? cv[v] : space_value<kSrc>(C, vx, vy, vz);
         float pa[3], pb[3], p[3];
         space_position<kSrc>(C, ux, uy, uz, pa);
         space_position<kSrc>(C, vx, vy, vz, pb);
@@ -777,13 +794,13 @@ hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& 
 }
 
 hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
-                                const McBuffers& B, hipStream_t s) {
+                                const McBuffers& B, float* values, hipStream_t s) {
   if (!ops) return hipErrorInvalidValue;
   int64_t blocks = grid_for(D.n_sign_words);
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
   if (blocks < min_blocks) blocks = min_blocks;
   hipLaunchKernelGGL(k_signs_program, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
-                     n_ops, L, D.n_points, D.n_sign_words, B.signs, reset_args(D, B));
+                     n_ops, L, D.n_points, D.n_sign_words, B.signs, values, reset_args(D, B));
   return hipGetLastError();
 }
 
@@ -822,6 +839,8 @@ EmitArgs emit_args(const McBuffers& B) {
   A.vcap = B.vertex_capacity;
   A.icap = B.index_capacity;
   A.totals = B.totals;
+  A.brick_flags = nullptr;
+  A.nbx = A.nby = 0;
   return A;
 }
 
@@ -869,6 +888,20 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const flo
     auto k = k_emit<kEmitProgram, SingleSpace>;
     hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_band(const float* values, const uint8_t* brick_flags, int64_t nbx,
+                            int64_t nby, const DevOp* ops, int n_ops, const Lattice& L,
+                            const McDims& D, const McBuffers& B, hipStream_t s) {
+  if (!values || !brick_flags || !ops || nbx < 1 || nby < 1) return hipErrorInvalidValue;
+  EmitArgs A = emit_args(B);
+  A.brick_flags = brick_flags;
+  A.nbx = nbx;
+  A.nby = nby;
+  const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, values, nullptr);
+  auto k = k_emit<kEmitBand, SingleSpace>;
+  hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
   return hipGetLastError();
 }
 

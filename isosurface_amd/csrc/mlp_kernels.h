@@ -70,6 +70,7 @@ struct Samples {
   const float* weight;
   uint64_t count;
   float clamp;  // targets are clamp(sdf, -clamp, clamp)
+  float overshoot;  // weight of the error beyond the interval between 0 and the target
   uint32_t seed;
   int32_t gather;  // 0: batch sample i is sample i (exact gradients for tests)
 };

@@ -217,10 +217,14 @@ __global__ __launch_bounds__(kThreads) void k_train_fb(Model M, Batch B, Samples
     const float wgt = S.weight ? S.weight[src] : 1.0f;
     Tile T;
     forward_tile(sF, sB, px, py, pz, lane, T, B.x, B.n, s);
+    // Least squares plus (overshoot - 1) e^2, e the part of the error that
+    // lies outside the interval between 0 and the target.
     const float r = T.f - t;
-    const float g = wgt * r * inv_n;
+    const float e = T.f - fminf(fmaxf(T.f, fminf(t, 0.0f)), fmaxf(t, 0.0f));
+    const float ow = S.overshoot - 1.0f;
+    const float g = wgt * (r + ow * e) * inv_n;
     if (lane < 16) {
-      loss += wgt * r * r;
+      loss += wgt * (r * r + ow * e * e);
       B.g[s] = g;
     }
     // dz3 = bf16(g w4 [h3 > 0]), dz2 = bf16((W3^T dz3) [h2 > 0]), dz1 likewise.
@@ -325,12 +329,15 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(Model M, Batch B) {
 }
 
 // Bias gradients and the output layer: db_l[i] = sum_s dz_l[i][s],
-// dw4[i] = sum_s g[s] h3[i][s], db4 = sum_s g[s].  One thread per
-// (row, chunk); rows 0..191 biases, 192..255 w4, 256 b4.
+// dw4[i] = sum_s g[s] h3[i][s], db4 = sum_s g[s].  One wavefront per
+// (row, chunk of samples): lanes stride over the chunk (coalesced reads of
+// the hidden-major rows), then a wave reduction and one atomic per wave.
+// Rows 0..191 biases, 192..255 w4, 256 b4.
 __global__ __launch_bounds__(kThreads) void k_bgrad(Model M, Batch B) {
   const int64_t n_chunks = (B.n + kWgradChunk - 1) / kWgradChunk;
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (t >= 257 * n_chunks) return;
+  const int64_t t = (static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x) >> 6;
+  if (t >= 257 * n_chunks) return;  // uniform per wave
+  const int lane = __lane_id();
   const int row = static_cast<int>(t % 257);
   const int64_t chunk = t / 257;
   const int64_t s0 = chunk * kWgradChunk;
@@ -339,18 +346,20 @@ __global__ __launch_bounds__(kThreads) void k_bgrad(Model M, Batch B) {
   int dst;
   if (row < 192) {
     const int layer = row / 64, i = row % 64;
-    const uint16_t* dz = layer == 0 ? B.dz1 : layer == 1 ? B.dz2 : B.dz3;
-    for (int64_t s = s0; s < s1; ++s) acc += from_bf16(dz[i * B.n + s]);
+    const uint16_t* dz = (layer == 0 ? B.dz1 : layer == 1 ? B.dz2 : B.dz3) + i * B.n;
+    for (int64_t s = s0 + lane; s < s1; s += 64) acc += from_bf16(dz[s]);
     dst = (layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3) + i;
   } else if (row < 256) {
-    const int i = row - 192;
-    for (int64_t s = s0; s < s1; ++s) acc += B.g[s] * from_bf16(B.h3[i * B.n + s]);
-    dst = kOffW4 + i;
+    const uint16_t* h3 = B.h3 + (row - 192) * B.n;
+    for (int64_t s = s0 + lane; s < s1; s += 64) acc += B.g[s] * from_bf16(h3[s]);
+    dst = kOffW4 + row - 192;
   } else {
-    for (int64_t s = s0; s < s1; ++s) acc += B.g[s];
+    for (int64_t s = s0 + lane; s < s1; s += 64) acc += B.g[s];
     dst = kOffB4;
   }
-  atomicAdd(&M.grad[dst], acc);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, 64);
+  if (lane == 0) atomicAdd(&M.grad[dst], acc);
 }
 
 // Fragment slot of master weight index p (layers 1-3), and of its transpose.
@@ -443,7 +452,7 @@ hipError_t launch_train_batch(const Model& M, const Batch& B, const Samples& S, 
   const int64_t n_chunks = (B.n + kWgradChunk - 1) / kWgradChunk;
   hipLaunchKernelGGL(k_wgrad, dim3(blocks_for(64 * kWgradBlocks * n_chunks)), dim3(kThreads), 0, s,
                      M, B);
-  hipLaunchKernelGGL(k_bgrad, dim3(blocks_for(257 * n_chunks)), dim3(kThreads), 0, s, M, B);
+  hipLaunchKernelGGL(k_bgrad, dim3(blocks_for(64 * 257 * n_chunks)), dim3(kThreads), 0, s, M, B);
   return hipGetLastError();
 }
 

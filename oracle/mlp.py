@@ -16,8 +16,10 @@ Network (parameter vector layout identical to the device's):
             (64 units each), f = w4 . h3 + b4
   rounding  x, W1..W3, h1..h3 in bfloat16 (round to nearest even);
             matrix products accumulate in float32; biases, w4, b4 float32
-  loss      L = sum_s wgt_s * (f_s - t_s)^2 / (2 * n) over a batch of n
-  backward  g_s = wgt_s (f_s - t_s) / n; dz3 = bf16(g w4 * [h3 > 0]);
+  loss      L = sum_s wgt_s * (r_s^2 + (a - 1) e_s^2) / (2 * n) over a batch of
+            n, r = f - t, e = f - clip(f, min(0, t), max(0, t)) (the error
+            beyond the interval between 0 and t), a the overshoot weight
+  backward  g_s = wgt_s (r_s + (a - 1) e_s) / n; dz3 = bf16(g w4 * [h3 > 0]);
             dz2 = bf16((W3^T dz3) * [h2 > 0]); dz1 = bf16((W2^T dz2) * [h1 > 0]);
             dW_l = sum_s dz_l h_(l-1)^T (bf16 operands, float32 sums),
             db_l = sum_s dz_l, dw4 = sum_s g h3, db4 = sum_s g
@@ -112,14 +114,24 @@ def forward(theta: np.ndarray, p: np.ndarray, keep: bool = False):
     return f
 
 
-def loss_and_grad(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray):
+def excess(f: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """The part of f - t beyond the interval between 0 and t (0 inside it)."""
+    return f - np.clip(f, np.minimum(t, 0.0), np.maximum(t, 0.0))
+
+
+def loss_and_grad(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray,
+                  overshoot: float = 1.0):
     """Loss and gradient with the device's rounding points (see module doc)."""
     P = unpack(np.asarray(theta, np.float32))
     n = len(p)
     f, (x, h1, h2, h3) = forward(theta, p, keep=True)
+    t = np.asarray(t, np.float32)
     r = (f - t).astype(np.float32)
-    loss = float(np.sum(wgt.astype(np.float64) * r.astype(np.float64) ** 2) / (2 * n))
-    g = (wgt * r / np.float32(n)).astype(np.float32)
+    e = excess(f, t).astype(np.float32)
+    ow = np.float32(overshoot) - np.float32(1.0)
+    loss = float(np.sum(wgt.astype(np.float64) * (r.astype(np.float64) ** 2 +
+                                                  float(ow) * e.astype(np.float64) ** 2)) / (2 * n))
+    g = (wgt * (r + ow * e) * (np.float32(1.0) / np.float32(n))).astype(np.float32)
     dz3 = bf16(g[:, None] * P["w4"][None, :] * (h3 > 0))
     dz2 = bf16(_mm(dz3, bf16(P["W3"])) * (h2 > 0))
     dz1 = bf16(_mm(dz2, bf16(P["W2"])) * (h1 > 0))
@@ -143,7 +155,8 @@ def forward64(theta: np.ndarray, p: np.ndarray) -> np.ndarray:
     return h3 @ P["w4"] + P["b4"][0]
 
 
-def grad64(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray) -> np.ndarray:
+def grad64(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray,
+           overshoot: float = 1.0) -> np.ndarray:
     """Exact float64 gradient of the same loss (no rounding), by backpropagation."""
     P = {k: v.astype(np.float64) for k, v in unpack(np.asarray(theta, np.float64)).items()}
     n = len(p)
@@ -155,7 +168,7 @@ def grad64(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray) -> 
     z3 = h2 @ P["W3"].T + P["b3"]
     h3 = np.maximum(z3, 0.0)
     f = h3 @ P["w4"] + P["b4"][0]
-    g = wgt * (f - t) / n
+    g = wgt * ((f - t) + (overshoot - 1.0) * excess(f, t)) / n
     dz3 = g[:, None] * P["w4"][None, :] * (z3 > 0)
     dz2 = (dz3 @ P["W3"]) * (z2 > 0)
     dz1 = (dz2 @ P["W2"]) * (z1 > 0)
@@ -165,9 +178,11 @@ def grad64(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray) -> 
     return np.concatenate([G[nm].reshape(-1) for nm, _ in SIZES])
 
 
-def loss64(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray) -> float:
+def loss64(theta: np.ndarray, p: np.ndarray, t: np.ndarray, wgt: np.ndarray,
+           overshoot: float = 1.0) -> float:
     f = forward64(theta, p)
-    return float(np.sum(wgt * (f - t) ** 2) / (2 * len(p)))
+    return float(np.sum(wgt * ((f - t) ** 2 + (overshoot - 1.0) * excess(f, t) ** 2)) /
+                 (2 * len(p)))
 
 
 def adam(theta, m, v, grad, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):

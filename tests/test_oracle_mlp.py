@@ -50,14 +50,36 @@ def test_parameter_layout():
     assert np.array_equal(M.pack(P), theta)
 
 
-@pytest.mark.parametrize("seed", [0, 1, 2])
-def test_float64_backprop_matches_finite_differences(seed):
+def test_excess_closed_forms():
+    t = np.array([0.3, 0.3, 0.3, 0.3, -0.2, -0.2, -0.2, 0.0, 0.0])
+    f = np.array([0.1, 0.5, -0.4, 0.3, -0.1, -0.5, 0.25, 0.7, -0.7])
+    want = np.array([0.0, 0.2, -0.4, 0.0, 0.0, -0.3, 0.25, 0.7, -0.7])
+    assert np.allclose(M.excess(f, t), want, atol=1e-15)
+
+
+def test_overshoot_one_is_least_squares():
+    rng = np.random.default_rng(4)
+    theta = M.init(4)
+    p = rng.random((64, 3)).astype(np.float32)
+    t = rng.normal(0, 0.05, 64).astype(np.float32)
+    w = np.ones(64, np.float32)
+    f = M.forward64(theta.astype(np.float64), p)
+    assert M.loss64(theta, p, t, w, 1.0) == pytest.approx(np.sum((f - t) ** 2) / 128, rel=1e-12)
+    # the extra term only counts errors beyond the interval [min(0,t), max(0,t)]
+    extra = M.loss64(theta, p, t, w, 3.0) - M.loss64(theta, p, t, w, 1.0)
+    e = np.where(t > 0, np.where(f > t, f - t, np.minimum(f, 0.0)),
+                 np.where(f < t, f - t, np.maximum(f, 0.0)))
+    assert extra == pytest.approx(2.0 * np.sum(e ** 2) / 128, rel=1e-9)
+
+
+@pytest.mark.parametrize("seed,overshoot", [(0, 1.0), (1, 1.0), (2, 1.0), (3, 5.0), (4, 5.0)])
+def test_float64_backprop_matches_finite_differences(seed, overshoot):
     rng = np.random.default_rng(seed)
     theta = M.init(seed).astype(np.float64)
     p = rng.random((24, 3)).astype(np.float32)
     t = rng.normal(0, 0.05, 24)
     w = rng.random(24) + 0.5
-    g = M.grad64(theta, p, t, w)
+    g = M.grad64(theta, p, t, w, overshoot)
     idx = rng.choice(np.nonzero(np.abs(g) > 1e-6)[0], 60, replace=False)
     eps = 1e-6
     for i in idx:
@@ -65,7 +87,7 @@ def test_float64_backprop_matches_finite_differences(seed):
         tm = theta.copy()
         tp[i] += eps
         tm[i] -= eps
-        fd = (M.loss64(tp, p, t, w) - M.loss64(tm, p, t, w)) / (2 * eps)
+        fd = (M.loss64(tp, p, t, w, overshoot) - M.loss64(tm, p, t, w, overshoot)) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-5 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
 
 
