@@ -268,24 +268,42 @@ def batched_chunks(ctx: Context, steps: int) -> dict:
             "triangles": int(out.index_count) // 3, "parity_vs_oracle": parity}
 
 
-def learned_band(ctx: Context, size: int, reps: int) -> dict:
+def brick_centres(size: int) -> np.ndarray:
+    """Centres of the 8^3-cell bricks of MarchingCubes::new(size) (x fastest),
+    as the band kernels place them."""
+    sp = np.float32(1.0) / np.float32(size - 1)
+    axes = []
+    for n in (size, size, size + 1):
+        lo = np.arange(0, n - 1, 8)
+        hi = np.minimum(lo + 8, n - 1)
+        axes.append(((0.5 * (lo + hi)).astype(np.float32) * sp).astype(np.float32))
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return np.ascontiguousarray(np.stack([x, y, z], -1).reshape(-1, 3), np.float32)
+
+
+def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
     """The learned narrow band against the dense path at size^3 cells."""
     from isosurface_amd.band import NarrowBand, brick_half_diagonal, training_samples
     src = W.smooth_spheres(1024, k=0.02, seed=5)
     upper = (1.0, 1.0, (size + 1) / size)  # the lattice's extra z layer
+    hd = brick_half_diagonal(size + 1)
     t0 = time.perf_counter()
     pts, sdf = training_samples(ctx, src, 1_000_000, 1_000_000, seed=11, coarse=257, upper=upper)
     cal_p, cal_s = training_samples(ctx, src, 500_000, 500_000, seed=12, coarse=257, upper=upper)
     t_samples = time.perf_counter() - t0
     band = NarrowBand(ctx)
+    centres = brick_centres(size + 1)
+    # The same network before training: its calibrated margin keeps every brick.
     band.init(7)
-    steps = 10000
+    m0 = band.calibrate(cal_p, cal_s, hd)
+    untrained_fraction = float(np.mean(~(np.abs(band.predict(centres)) > hd + m0)))
+    overshoot, lr = 8.0, 2e-3
     t0 = time.perf_counter()
     weight = (1.0 + 9.0 * np.exp(-np.abs(sdf) / 0.01)).astype(np.float32)
-    loss = band.train(pts, sdf, weight, steps=steps, batch=16384, lr=3e-3, lr_min=5e-5,
-                      clamp=0.05, seed=13)
+    loss = band.train(pts, sdf, weight, steps=steps, batch=16384, lr=lr, lr_min=5e-5,
+                      clamp=0.05, seed=13, overshoot=overshoot)
     t_train = time.perf_counter() - t0
-    margin = band.calibrate(cal_p, cal_s, brick_half_diagonal(size + 1))
+    margin = band.calibrate(cal_p, cal_s, hd)
     ctx.upload_program(src.program())
     desc = ctx.reference_desc(size + 1, A.ISO_SOURCE_PROGRAM)
     cells = size * size * (size + 1)
@@ -296,48 +314,71 @@ def learned_band(ctx: Context, size: int, reps: int) -> dict:
         ctx.check(st, "iso_extract_mc")
     nv, ni = int(out.vertex_count), int(out.index_count)
     probe.free()
-    dense = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    out = dense.out()
-    ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out)), "dense")
-    ctx.synchronize()
-    t_dense = time.perf_counter() - t0
-    dd = digest(ctx, dense, out)
-    dense.free()
-    meshb = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
-    stats = A.IsoBandStats()
+    # Room for a band mesh that differs from the dense one.
+    mesh = DeviceMesh(ctx, 2 * max(nv, 1), 2 * max(ni, 3))
     times = []
     for _ in range(max(1, reps)):
         ctx.synchronize()
         t0 = time.perf_counter()
-        outb = meshb.out()
-        ctx.check(ctx.lib.iso_extract_mc_band(ctx.handle, C.byref(desc), margin, C.byref(outb),
-                                              C.byref(stats)), "band")
+        out = mesh.out()
+        ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out)), "dense")
         ctx.synchronize()
         times.append(time.perf_counter() - t0)
-    t_band = min(times)
-    same = (int(outb.vertex_count), int(outb.index_count)) == (nv, ni) and \
-        digest(ctx, meshb, outb) == dd
-    meshb.free()
-    missed = C.c_uint64(0)
-    ctx.check(ctx.lib.iso_band_missed_cells(ctx.handle, C.byref(desc), C.byref(missed)), "missed")
+    t_dense = min(times)
+    dd = digest(ctx, mesh, out)
+
+    def run(flags: int, mode: str) -> dict:
+        stats = A.IsoBandStats()
+        times = []
+        for _ in range(max(1, reps)):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            outb = mesh.out()
+            st = ctx.lib.iso_extract_mc_band(ctx.handle, C.byref(desc), margin, flags,
+                                             C.byref(outb), C.byref(stats))
+            ctx.synchronize()
+            times.append(time.perf_counter() - t0)
+            if st not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+                ctx.check(st, "band")
+        t_band = min(times)
+        same = st == A.ISO_OK and \
+            (int(outb.vertex_count), int(outb.index_count)) == (nv, ni) and \
+            digest(ctx, mesh, outb) == dd
+        missed = C.c_uint64(0)
+        ctx.check(ctx.lib.iso_band_missed_cells(ctx.handle, C.byref(desc), C.byref(missed)),
+                  "missed")
+        r = {"band_ms": 1e3 * t_band, "speedup": t_dense / t_band,
+             "band_cells_per_s": cells / t_band, "band_bricks": int(stats.band_bricks),
+             "band_fraction": int(stats.band_bricks) / max(int(stats.bricks), 1),
+             "exact_samples": int(stats.exact_samples),
+             "centre_samples": int(stats.centre_samples),
+             "missed_cells": int(missed.value), "identical_to_dense": same}
+        log(f"band [{mode}]: {1e3 * t_band:.1f} ms, x{t_dense / t_band:.2f}, band "
+            f"{r['band_fraction']:.3f}, missed {missed.value}, identical {same}")
+        return r
+
+    learned = run(0, "learned")
+    certified = run(A.ISO_BAND_CERTIFY, "learned+certificate")
+    exact_only = run(A.ISO_BAND_NO_NETWORK, "certificate only")
+    mesh.free()
     res = {
         "workload": (f"MarchingCubes::new({size + 1}) ({cells} cells) on the smooth-min (k=0.02) "
                      "of 1024 seeded spheres, radii U[0.01,0.05], program source"),
-        "dense_ms": 1e3 * t_dense, "band_ms": 1e3 * t_band, "speedup": t_dense / t_band,
-        "dense_cells_per_s": cells / t_dense, "band_cells_per_s": cells / t_band,
-        "bricks": int(stats.bricks), "band_bricks": int(stats.band_bricks),
-        "band_fraction": int(stats.band_bricks) / max(int(stats.bricks), 1),
-        "exact_samples": int(stats.exact_samples), "margin": margin,
-        "missed_cells": int(missed.value), "identical_to_dense": same,
+        "dense_ms": 1e3 * t_dense, "dense_cells_per_s": cells / t_dense,
+        "bricks": int(centres.shape[0]), "margin": margin, "half_diagonal": hd,
         "vertices": nv, "triangles": ni // 3,
-        "training": {"samples": len(sdf), "steps": steps, "batch": 16384, "final_loss": loss,
-                     "train_ms": 1e3 * t_train, "sample_generation_ms": 1e3 * t_samples},
+        **learned,
+        "untrained_network_band_fraction": untrained_fraction,
+        "learned_with_certificate": certified,
+        "certificate_without_network": exact_only,
+        "training": {"samples": len(sdf), "steps": steps, "batch": 16384, "lr": lr,
+                     "overshoot": overshoot, "final_loss": loss, "train_ms": 1e3 * t_train,
+                     "sample_generation_ms": 1e3 * t_samples},
     }
-    log(f"learned band: dense {1e3 * t_dense:.1f} ms, band {1e3 * t_band:.1f} ms, "
-        f"x{t_dense / t_band:.2f}, band {res['band_fraction']:.3f}, missed {missed.value}, "
-        f"identical {same}")
+    log(f"learned band: dense {1e3 * t_dense:.1f} ms; learned x{learned['speedup']:.2f} "
+        f"(band {learned['band_fraction']:.3f}, untrained {untrained_fraction:.3f}); with "
+        f"certificate x{certified['speedup']:.2f}; certificate alone "
+        f"x{exact_only['speedup']:.2f}")
     return res
 
 
@@ -352,6 +393,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-side", action="store_true",
                     help="skip the batched, dual-contouring and octree workloads")
+    ap.add_argument("--band-steps", type=int, default=100000,
+                    help="training steps of the learned-band network")
     ap.add_argument("--band-size", type=int, default=1024,
                     help="cells per axis of the learned-band workload (0: skip it)")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -491,7 +534,7 @@ def main() -> int:
 
     if not args.no_side:
         if args.band_size > 0:
-            result["learned_band"] = learned_band(ctx, args.band_size, 3)
+            result["learned_band"] = learned_band(ctx, args.band_size, 2, args.band_steps)
         result["batched_chunks"] = batched_chunks(ctx, max(2, args.program_steps))
         result.update(side_workloads(ctx, max(1, args.program_steps), not args.no_cpu_baseline))
 

@@ -257,11 +257,20 @@ typedef struct iso_band_train_params {
 } iso_band_train_params;
 
 typedef struct iso_band_stats {
-  uint64_t bricks;         /* bricks of the lattice */
-  uint64_t band_bricks;    /* bricks evaluated exactly */
-  uint64_t exact_samples;  /* program evaluations of the band (without emit's) */
-  uint64_t reserved;
+  uint64_t bricks;          /* bricks of the lattice */
+  uint64_t band_bricks;     /* bricks evaluated exactly */
+  uint64_t exact_samples;   /* program evaluations of the band (without emit's) */
+  uint64_t centre_samples;  /* program evaluations of the certificate */
 } iso_band_stats;
+
+/* iso_extract_mc_band flags.  ISO_BAND_CERTIFY: each brick the network
+ * keeps in the band is evaluated at its centre, and leaves the band (with
+ * the exact sign) when |f| exceeds its half-diagonal — exact for every
+ * program, all of which are 1-Lipschitz; the network's culls stay
+ * unverified.  ISO_BAND_NO_NETWORK: no network and no margin, every brick is
+ * certified (the method without the network, for comparison). */
+#define ISO_BAND_CERTIFY 1u
+#define ISO_BAND_NO_NETWORK 2u
 
 /* Deterministic He-uniform initialisation of the context's network. */
 int iso_band_init(iso_ctx* ctx, uint32_t seed);
@@ -283,9 +292,10 @@ int iso_band_gradient(iso_ctx* ctx, const float* points, const float* sdf, const
                       uint64_t n, float clamp, float overshoot, float* grad, float* loss);
 /* Network output at n points (3 floats each) in `memory`. */
 int iso_band_predict(iso_ctx* ctx, const float* points, uint64_t n, float* out, int32_t memory);
-/* Band extraction of the current program over `desc` (ISO_SOURCE_PROGRAM). */
-int iso_extract_mc_band(iso_ctx* ctx, const iso_grid_desc* desc, float margin, iso_mesh_out* out,
-                        iso_band_stats* stats);
+/* Band extraction of the current program over `desc` (ISO_SOURCE_PROGRAM);
+ * flags: ISO_BAND_* (0: the network and the margin alone decide). */
+int iso_extract_mc_band(iso_ctx* ctx, const iso_grid_desc* desc, float margin, uint32_t flags,
+                        iso_mesh_out* out, iso_band_stats* stats);
 /* Cells whose marching-cubes case differs between the last
  * iso_extract_mc_band over desc and the dense extraction of desc (missed,
  * spurious or altered surface cells): zero means the band's mesh is the

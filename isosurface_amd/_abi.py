@@ -85,6 +85,8 @@ class IsoBatchOut(C.Structure):
 
 
 ISO_BAND_PARAMS = 15617
+ISO_BAND_CERTIFY = 1
+ISO_BAND_NO_NETWORK = 2
 
 
 class IsoBandTrainParams(C.Structure):
@@ -95,7 +97,7 @@ class IsoBandTrainParams(C.Structure):
 
 class IsoBandStats(C.Structure):
     _fields_ = [("bricks", C.c_uint64), ("band_bricks", C.c_uint64),
-                ("exact_samples", C.c_uint64), ("reserved", C.c_uint64)]
+                ("exact_samples", C.c_uint64), ("centre_samples", C.c_uint64)]
 
 
 class IsoKernelStat(C.Structure):
@@ -134,7 +136,7 @@ SIGNATURES = [
     ("iso_band_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                     C.c_float, C.c_float, C.c_void_p, C.POINTER(C.c_float)]),
     ("iso_band_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32]),
-    ("iso_extract_mc_band", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float,
+    ("iso_extract_mc_band", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float, C.c_uint32,
                                       C.POINTER(IsoMeshOut), C.POINTER(IsoBandStats)]),
     ("iso_band_missed_cells", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc),
                                         C.POINTER(C.c_uint64)]),

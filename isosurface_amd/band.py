@@ -104,8 +104,11 @@ class NarrowBand:
         m = max(0.0, float(np.max(np.abs(f_hat[must_keep]) - half_diag)))
         return m * safety + slack
 
-    def extract_mesh(self, desc: A.IsoGridDesc, margin: float,
+    def extract_mesh(self, desc: A.IsoGridDesc, margin: float, flags: int = 0,
                      vertex_capacity: int = 0, index_capacity: int = 0):
+        """Band extraction; flags: A.ISO_BAND_CERTIFY (the network's band
+        candidates are certified by an exact centre sample) or
+        A.ISO_BAND_NO_NETWORK (every brick certified, no network)."""
         vcap = max(int(vertex_capacity), 1)
         icap = max(int(index_capacity), 3)
         verts = np.empty((vcap, 3), np.float32)
@@ -113,8 +116,8 @@ class NarrowBand:
         out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
                            A.ISO_MEMORY_HOST, 0)
         stats = A.IsoBandStats()
-        st = self.lib.iso_extract_mc_band(self.ctx.handle, C.byref(desc), margin, C.byref(out),
-                                          C.byref(stats))
+        st = self.lib.iso_extract_mc_band(self.ctx.handle, C.byref(desc), margin, flags,
+                                          C.byref(out), C.byref(stats))
         if st == A.ISO_ERR_CAPACITY:
             nv, ni = int(out.vertex_count), int(out.index_count)
             verts = np.empty((max(nv, 1), 3), np.float32)

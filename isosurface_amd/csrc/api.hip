@@ -1437,8 +1437,8 @@ int iso_band_predict(iso_ctx* c, const float* points, uint64_t n, float* out, in
   return ISO_OK;
 }
 
-int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, iso_mesh_out* out,
-                        iso_band_stats* stats) {
+int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uint32_t flags,
+                        iso_mesh_out* out, iso_band_stats* stats) {
   if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
   iso::Lattice L;
   int st = check_lattice(c, desc, &L);
@@ -1446,15 +1446,19 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, iso
   if (desc->source != ISO_SOURCE_PROGRAM)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "band extraction needs a program source");
   if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
-  if (!c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
+  if (flags & ~(ISO_BAND_CERTIFY | ISO_BAND_NO_NETWORK))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown band flags");
+  const bool use_network = !(flags & ISO_BAND_NO_NETWORK);
+  const bool certify = (flags & (ISO_BAND_CERTIFY | ISO_BAND_NO_NETWORK)) != 0;
+  if (use_network && !c->nn_ready) return fail(c, ISO_ERR_NO_SOURCE, "no network initialised");
   if (!(margin >= 0.0f) || !(margin < 3.0e38f))
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "margin must be finite and non-negative");
   if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
   c->kept = false;
   ISO_HIP(c, hipSetDevice(c->device));
-  iso::mlp::Model M;
-  if ((st = nn_model(c, &M)) != ISO_OK) return st;
+  iso::mlp::Model M{};
+  if (use_network && (st = nn_model(c, &M)) != ISO_OK) return st;
   const iso::BrickGrid G = iso::brick_grid(L);
   const uint64_t nb = static_cast<uint64_t>(G.n);
   ISO_HIP(c, ensure(&c->br_centres, 3 * nb * sizeof(float)));
@@ -1462,53 +1466,55 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, iso
   ISO_HIP(c, ensure(&c->br_pred, nb * sizeof(float)));
   ISO_HIP(c, ensure(&c->br_flags, nb));
   ISO_HIP(c, ensure(&c->br_list, nb * sizeof(uint32_t)));
-  ISO_HIP(c, ensure(&c->br_count, 2 * sizeof(unsigned long long)));
+  ISO_HIP(c, ensure(&c->br_count, 4 * sizeof(unsigned long long)));
   float* centres = static_cast<float*>(c->br_centres.ptr);
   float* hd = static_cast<float*>(c->br_hd.ptr);
   float* pred = static_cast<float*>(c->br_pred.ptr);
-  uint8_t* flags = static_cast<uint8_t*>(c->br_flags.ptr);
+  uint8_t* bflags = static_cast<uint8_t*>(c->br_flags.ptr);
   uint32_t* list = static_cast<uint32_t*>(c->br_list.ptr);
   unsigned long long* counters = static_cast<unsigned long long*>(c->br_count.ptr);
   iso::scan::State S{};
   if ((st = prepare_scan(c, nb, &S)) != ISO_OK) return st;
   ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(S.err), 2, 0u, c->stream));
-  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(counters), 4, 0u, c->stream));
+  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(counters), 8, 0u, c->stream));
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   ISO_HIP(c, timed(c, "band_bricks", [&] {
             hipError_t e = iso::launch_brick_centres(L, G, centres, hd, c->stream);
-            if (e == hipSuccess)
+            if (e == hipSuccess && use_network)
               e = iso::mlp::launch_forward(M, centres, static_cast<int64_t>(nb), pred, c->stream);
             if (e == hipSuccess)
-              e = iso::launch_brick_flags(G, pred, hd, margin, flags, list, S, c->stream);
+              e = iso::launch_brick_flags(G, use_network ? pred : nullptr, hd, centres,
+                                          certify ? ops : nullptr, c->n_prog, margin, bflags,
+                                          list, counters + 2, S, c->stream);
             return e;
           }));
   if ((st = read_ctl(c, S)) != ISO_OK) return st;
   const uint64_t n_band = c->h_stage[1];
-  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   const iso::McDims D = iso::mc_dims(L);
   SignsStage signs = [&](const iso::McBuffers& B, float* values) -> int {
     ISO_HIP(c, timed(c, "band_background", [&] {
-              return iso::launch_band_background(L, D, G, flags, B, c->stream);
+              return iso::launch_band_background(L, D, G, bflags, B, c->stream);
             }));
     ISO_HIP(c, timed(c, "band_exact", [&] {
-              return iso::launch_band_exact(ops, c->n_prog, L, G, flags, list, n_band, B, values,
+              return iso::launch_band_exact(ops, c->n_prog, L, G, bflags, list, n_band, B, values,
                                             counters, c->stream);
             }));
     return ISO_OK;
   };
   c->band_valid = false;
-  const BandEmit band{flags, G.nbx, G.nby};
+  const BandEmit band{bflags, G.nbx, G.nby};
   st = extract_lattice(c, L, nullptr, nullptr, out, &signs, &band);
   if (st != ISO_OK && st != ISO_ERR_CAPACITY) return st;
   c->band_desc = *desc;
   c->band_valid = true;
   if (stats) {
-    ISO_HIP(c, hipMemcpyAsync(c->h_stage + 4, counters, sizeof(unsigned long long),
+    ISO_HIP(c, hipMemcpyAsync(c->h_stage + 4, counters, 3 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, c->stream));
     ISO_HIP(c, hipStreamSynchronize(c->stream));
     stats->bricks = nb;
     stats->band_bricks = n_band;
     stats->exact_samples = c->h_stage[4];
-    stats->reserved = 0;
+    stats->centre_samples = c->h_stage[6];
   }
   return st;
 }

@@ -15,6 +15,7 @@
 #include "mc_kernels.h"
 #include "mlp_kernels.h"
 #include "scan.h"
+#include "sdf_program.h"
 
 namespace iso {
 
@@ -36,11 +37,22 @@ BrickGrid brick_grid(const Lattice& L);
 // Brick centres (3 floats each, brick order x fastest) and half-diagonals.
 hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* centres,
                                 float* half_diag, hipStream_t s);
-// Band / inside flags from predictions: band if |pred| <= half_diag + margin.
-// The list of band bricks (ascending) and its length go to st.ctl[1].
+// f32 evaluation-error allowance of the Lipschitz certificate.
+constexpr float kCertifyRelSlack = 1.0e-4f;
+constexpr float kCertifyAbsSlack = 2.0e-5f;
+
+// Band / inside flags.  With predictions, a brick is a band candidate if
+// |pred| <= half_diag + margin, else it takes the predicted sign; without
+// them (pred null) every brick is a candidate.  With certify_ops, each
+// candidate's centre is evaluated exactly (count added to *certified) and a
+// candidate with |f| > half_diag (plus the f32 allowance) leaves the band
+// with the exact sign.  The list of band bricks (ascending) and its length
+// go to st.ctl[1].
 hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
+                              const float* centres, const DevOp* certify_ops, int n_ops,
                               float margin, uint8_t* flags, uint32_t* band_list,
-                              const scan::State& st, hipStream_t s);
+                              unsigned long long* certified, const scan::State& st,
+                              hipStream_t s);
 // Sign bits of the whole lattice from the bricks' predicted signs (and the
 // extraction's state reset, like the other signs launchers).
 hipError_t launch_band_background(const Lattice& L, const McDims& D, const BrickGrid& G,

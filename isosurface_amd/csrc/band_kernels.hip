@@ -51,15 +51,43 @@ __global__ __launch_bounds__(256) void k_brick_centres(Lattice L, BrickGrid G,
 }
 
 struct FlagLoad {
-  const float* pred;
+  const float* pred;       // network predictions; null: every brick is a candidate
   const float* half_diag;
+  const float* centres;
+  const DevOp* ops;        // non-null: certify the candidates
+  int n_ops;
   float margin;
   uint8_t* flags;
+  unsigned long long* certified;  // centre evaluations
   __device__ unsigned long long operator()(uint64_t i) const {
-    const float p = pred[i];
-    // A NaN prediction is never trusted: the brick stays in the band.
-    const bool band = !(fabsf(p) > half_diag[i] + margin);
-    flags[i] = static_cast<uint8_t>((band ? kBrickBand : 0) | (!(p > 0.0f) ? kBrickInside : 0));
+    const float hd = half_diag[i];
+    bool band = true, inside = false;
+    if (pred) {
+      const float p = pred[i];
+      // A NaN prediction is never trusted: the brick stays in the band.
+      band = !(fabsf(p) > hd + margin);
+      inside = !(p > 0.0f);
+    }
+    const bool evaluate = band && ops != nullptr;
+    if (ops) {
+      const uint64_t m = __ballot(evaluate);
+      if (evaluate && __lane_id() == __builtin_ctzll(m))
+        atomicAdd(certified, static_cast<unsigned long long>(__popcll(m)));
+    }
+    if (evaluate) {
+      // Every program is 1-Lipschitz (exact primitive distances, min, max,
+      // negation, translation and the polynomial smooth minimum, whose
+      // gradient is a convex combination of its operands'), so a centre
+      // value farther than the half-diagonal from zero (plus a bound on the
+      // f32 evaluation error) proves the brick's closed sample box holds
+      // that sign everywhere.
+      const float v = sdf_eval(ops, n_ops, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]);
+      if (fabsf(v) > hd * (1.0f + kCertifyRelSlack) + kCertifyAbsSlack) {
+        band = false;
+        inside = !(v > 0.0f);
+      }
+    }
+    flags[i] = static_cast<uint8_t>((band ? kBrickBand : 0) | (inside ? kBrickInside : 0));
     return band ? 1ull : 0ull;
   }
 };
@@ -288,10 +316,16 @@ hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* cen
 }
 
 hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
+                              const float* centres, const DevOp* certify_ops, int n_ops,
                               float margin, uint8_t* flags, uint32_t* band_list,
-                              const scan::State& st, hipStream_t s) {
-  FlagLoad load{pred, half_diag, margin, flags};
+                              unsigned long long* certified, const scan::State& st,
+                              hipStream_t s) {
+  if (!pred && !certify_ops) return hipErrorInvalidValue;
+  if (certify_ops && (!centres || !certified)) return hipErrorInvalidValue;
+  FlagLoad load{pred, half_diag, centres, certify_ops, n_ops, margin, flags, certified};
   FlagSink sink{band_list};
+  // Certificates evaluate the program per brick: one brick per thread.
+  if (certify_ops) return scan::launch<1>(static_cast<uint64_t>(G.n), load, sink, st, s);
   return scan::launch<scan::kItems>(static_cast<uint64_t>(G.n), load, sink, st, s);
 }
 

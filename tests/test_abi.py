@@ -33,21 +33,30 @@ def test_library_exports_every_declared_symbol():
     assert {n for n, _, _ in A.SIGNATURES} == set(declared_functions())
 
 
+STRUCTS = [("iso_node", A.IsoNode), ("iso_grid_desc", A.IsoGridDesc),
+           ("iso_mesh_out", A.IsoMeshOut), ("iso_kernel_stat", A.IsoKernelStat),
+           ("iso_chunk_desc", A.IsoChunkDesc), ("iso_batch_out", A.IsoBatchOut),
+           ("iso_band_train_params", A.IsoBandTrainParams), ("iso_band_stats", A.IsoBandStats)]
+
+
 def test_struct_layouts_match_c(tmp_path):
+    # sizeof and every field's offset of each ctypes mirror against the C header
+    lines, want = [], []
+    for cname, cls in STRUCTS:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(C.sizeof(cls))
+        for field in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {field[0]}));')
+            want.append(getattr(cls, field[0]).offset)
     src = tmp_path / "layout.c"
-    src.write_text(
-        '#include "isosurface.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-        'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(iso_node),'
-        ' sizeof(iso_grid_desc), sizeof(iso_mesh_out), sizeof(iso_kernel_stat),'
-        ' offsetof(iso_grid_desc, source), offsetof(iso_mesh_out, memory));return 0;}\n')
+    src.write_text('#include "isosurface.h"\n#include <stdio.h>\n#include <stddef.h>\n'
+                   'int main(void){' + "".join(lines) + 'return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-pedantic", f"-I{ROOT / 'include'}",
                     str(src), "-o", str(exe)], check=True)
     got = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True,
                                        check=True).stdout.split()))
-    assert got == [C.sizeof(A.IsoNode), C.sizeof(A.IsoGridDesc), C.sizeof(A.IsoMeshOut),
-                   C.sizeof(A.IsoKernelStat), A.IsoGridDesc.source.offset,
-                   A.IsoMeshOut.memory.offset]
+    assert got == want
 
 
 def test_status_strings():
