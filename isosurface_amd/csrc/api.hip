@@ -53,7 +53,7 @@ struct iso_ctx {
   int64_t gnx = 0, gny = 0, gnz = 0;
 
   // scratch
-  DevBuf signs, seg, tplanes, agg, incl, active, totals, vout, iout, pts, vals;
+  DevBuf signs, seg, tplanes, vplanes, agg, incl, active, totals, vout, iout, pts, vals;
   DevBuf pvals;  // program samples of the current extraction (emit reads them)
   DevBuf judge;  // dense sign words that judge a band extraction
   // dual contouring: primal samples, dual values and points, mixed list, counters
@@ -258,7 +258,7 @@ int iso_ctx_destroy(iso_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
-  DevBuf* bufs[] = {&c->prog,       &c->grid_owned, &c->signs,     &c->seg,      &c->tplanes,
+  DevBuf* bufs[] = {&c->prog,       &c->grid_owned, &c->signs,     &c->seg,      &c->tplanes, &c->vplanes,
                     &c->agg,        &c->incl,       &c->active,    &c->totals,
                     &c->vout,       &c->iout,       &c->pts,       &c->vals,     &c->dc_samples,
                     &c->dc_values,  &c->dc_points,  &c->dc_mixed,  &c->dc_counters,
@@ -411,6 +411,7 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
   ISO_HIP(c, ensure(&c->signs, (D.n_sign_words + 3) * sizeof(uint64_t)));
   ISO_HIP(c, ensure(&c->seg, D.n_segs * sizeof(uint2)));
   ISO_HIP(c, ensure(&c->tplanes, 3 * D.n_segs * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->vplanes, 4 * D.n_segs * sizeof(uint64_t)));
   ISO_HIP(c, ensure(&c->agg, D.n_blocks * sizeof(iso::ScanTriple)));
   ISO_HIP(c, ensure(&c->incl, D.n_blocks * sizeof(iso::ScanTriple)));
   ISO_HIP(c, ensure(&c->totals, iso::kTotSlots * sizeof(unsigned long long)));
@@ -426,6 +427,7 @@ int prepare_mc(iso_ctx* c, const iso::McDims& D, uint64_t min_active, iso::McBuf
   B->signs = static_cast<uint64_t*>(c->signs.ptr) + 1;
   B->seg = static_cast<uint2*>(c->seg.ptr);
   B->tplanes = static_cast<uint64_t*>(c->tplanes.ptr);
+  B->vplanes = static_cast<uint64_t*>(c->vplanes.ptr);
   B->agg = static_cast<iso::ScanTriple*>(c->agg.ptr);
   B->incl = static_cast<iso::ScanTriple*>(c->incl.ptr);
   B->active = static_cast<uint32_t*>(c->active.ptr);
@@ -1085,6 +1087,7 @@ int iso_extract_batch(iso_ctx* c, iso_batch_out* out) {
     // A graph replays only with exactly the buffers it was captured with.
     const iso::McBuffers& G = c->g_buffers;
     const bool replay = graph && c->b_graph && G.seg == B.seg && G.tplanes == B.tplanes &&
+                        G.vplanes == B.vplanes &&
                         G.agg == B.agg && G.incl == B.incl && G.active == B.active &&
                         G.active_capacity == B.active_capacity && G.totals == B.totals &&
                         G.vertices == B.vertices && G.indices == B.indices &&

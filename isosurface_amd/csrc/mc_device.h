@@ -210,6 +210,48 @@ __device__ __forceinline__ uint32_t owned_count(const SegWords& s, uint64_t m, b
   return n;
 }
 
+// Adds the 1-bit-per-cell mask m to the bit-sliced per-cell counter p.
+__device__ __forceinline__ void sliced_add(uint64_t* p, int n_planes, uint64_t m) {
+#pragma unroll
+  for (int k = 0; k < n_planes; ++k) {
+    const uint64_t carry = p[k] & m;
+    p[k] ^= m;
+    m = carry;
+  }
+}
+
+// Per-cell owned crossed-edge counts of the cells in `valid`, bit-sliced in
+// four planes (a cell owns at most 12 edges): the same edges owned_count
+// counts, so owned_count(s, m, ...) == sum_k 2^k popc(p[k] & m) for m in valid.
+__device__ __forceinline__ void owned_planes(const SegWords& s, uint64_t valid, bool y0, bool z0,
+                                             bool first, uint64_t* p) {
+  const uint64_t* c = s.c;
+  p[0] = p[1] = p[2] = p[3] = 0;
+  sliced_add(p, 4, valid & (c[6] ^ c[7]));
+  sliced_add(p, 4, valid & (c[5] ^ c[6]));
+  sliced_add(p, 4, valid & (c[2] ^ c[6]));
+  if (y0) {
+    sliced_add(p, 4, valid & (c[4] ^ c[5]));
+    sliced_add(p, 4, valid & (c[1] ^ c[5]));
+  }
+  if (z0) {
+    sliced_add(p, 4, valid & (c[2] ^ c[3]));
+    sliced_add(p, 4, valid & (c[1] ^ c[2]));
+  }
+  if (y0 && z0) sliced_add(p, 4, valid & (c[0] ^ c[1]));
+  if (first) {
+    const uint64_t v0 = valid & 1ull;
+    sliced_add(p, 4, v0 & (c[4] ^ c[7]));
+    sliced_add(p, 4, v0 & (c[3] ^ c[7]));
+    if (z0) sliced_add(p, 4, v0 & (c[0] ^ c[3]));
+    if (y0) sliced_add(p, 4, v0 & (c[0] ^ c[4]));
+  }
+}
+
+__device__ __forceinline__ uint32_t planes_count(const uint64_t* p, uint64_t m) {
+  return popc64(p[0] & m) + 2 * popc64(p[1] & m) + 4 * popc64(p[2] & m) + 8 * popc64(p[3] & m);
+}
+
 // Cube index of the cell whose low corner sits at (dx, dy, dz) of the 3x3x3
 // sign neighbourhood `cube` (bit dx + 3 dy + 9 dz).
 __device__ __forceinline__ uint32_t case_in_cube(uint32_t cube, int dx, int dy, int dz) {

@@ -79,6 +79,7 @@ struct McBuffers {
   uint64_t* signs;          // n_sign_words; signs[-1], signs[n], signs[n+1] are guard words
   uint2* seg;               // n_segs: global first (vertex, triangle) id of each segment
   uint64_t* tplanes;        // 3 * n_segs: per-cell triangle counts, bit-sliced
+  uint64_t* vplanes;        // 4 * n_segs: per-cell owned-vertex counts (0..12), bit-sliced
   ScanTriple* agg;          // n_blocks: look-back aggregates, each word status << 62 | value
   ScanTriple* incl;         // n_blocks: look-back inclusive prefixes, same encoding
   uint32_t* active;         // active-cell list: (segment << 6) | bit, cell order

@@ -44,67 +44,12 @@ __constant__ tables::McTables c_tables = tables::kTables;
 // ---------------------------------------------------------------------------
 // 1. signs (+ reset of the extraction's scratch state)
 // ---------------------------------------------------------------------------
-constexpr int kSignVec = 4;  // float4 loads per lane per iteration (16 words)
+constexpr int kSignWords = 16;  // sign words per wavefront iteration (1024 samples)
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// Moves bit i of a 16-bit value to bit 4i.
-__device__ __forceinline__ uint64_t spread4(uint64_t x) {
-  x &= 0xFFFFull;
-  x = (x | (x << 24)) & 0x000000FF000000FFull;
-  x = (x | (x << 12)) & 0x000F000F000F000Full;
-  x = (x | (x << 6)) & 0x0303030303030303ull;
-  x = (x | (x << 3)) & 0x1111111111111111ull;
-  return x;
-}
-
-// 16-byte loads: lane l of a wavefront reads samples 4l..4l+3 of a
-// 256-sample chunk, so ballot j holds component j of every lane and word k
-// of the chunk interleaves the 16-lane slices [16k, 16k+16) of the ballots.
-__global__ __launch_bounds__(256) void k_signs_grid4(const f32x4* __restrict__ g4,
-                                                     int64_t n_points, int64_t n_words,
-                                                     uint64_t* __restrict__ S, ResetArgs R) {
-  reset_state(R);
-  const int lane = __lane_id();
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
-  const float* g = reinterpret_cast<const float*>(g4);
-  constexpr int64_t kSpan = 256 * kSignVec;  // samples per wavefront iteration
-  for (int64_t base = wave * kSpan; base < n_points; base += n_waves * kSpan) {
-    f32x4 v[kSignVec];
-#pragma unroll
-    for (int j = 0; j < kSignVec; ++j) {
-      const int64_t f = base + 256 * j + 4 * lane;
-      if (f + 3 < n_points) {
-        v[j] = __builtin_nontemporal_load(g4 + (f >> 2));
-      } else {
-        v[j].x = f + 0 < n_points ? g[f + 0] : 1.0f;
-        v[j].y = f + 1 < n_points ? g[f + 1] : 1.0f;
-        v[j].z = f + 2 < n_points ? g[f + 2] : 1.0f;
-        v[j].w = 1.0f;
-      }
-    }
-    uint64_t mine = 0;
-#pragma unroll
-    for (int j = 0; j < kSignVec; ++j) {
-      const uint64_t b0 = __ballot(!(v[j].x > 0.0f));
-      const uint64_t b1 = __ballot(!(v[j].y > 0.0f));
-      const uint64_t b2 = __ballot(!(v[j].z > 0.0f));
-      const uint64_t b3 = __ballot(!(v[j].w > 0.0f));
-      if ((lane >> 2) == j) {
-        const int sh = 16 * (lane & 3);
-        mine = spread4(b0 >> sh) | (spread4(b1 >> sh) << 1) | (spread4(b2 >> sh) << 2) |
-               (spread4(b3 >> sh) << 3);
-      }
-    }
-    const int64_t w = (base >> 6) + lane;
-    if (lane < 4 * kSignVec && w < n_words) S[w] = mine;
-  }
-}
-
-// Grids that are not 16-byte aligned: one float per lane per ballot.
-constexpr int kSignUnroll = 8;
-
+// Lane l of a wavefront reads samples 64j + l of a 1024-sample chunk (one
+// coalesced 256-byte load per j), so the ballot of load j is sign word j
+// itself: no bit interleaving, a compare and a lane select per word.  Works
+// for any 4-byte aligned grid.
 __global__ __launch_bounds__(256) void k_signs_grid(const float* __restrict__ g,
                                                     int64_t n_points, int64_t n_words,
                                                     uint64_t* __restrict__ S, ResetArgs R) {
@@ -112,20 +57,24 @@ __global__ __launch_bounds__(256) void k_signs_grid(const float* __restrict__ g,
   const int lane = __lane_id();
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
-  for (int64_t w0 = wave * kSignUnroll; w0 < n_words; w0 += n_waves * kSignUnroll) {
-    float v[kSignUnroll];
+  for (int64_t w0 = wave * kSignWords; w0 < n_words; w0 += n_waves * kSignWords) {
+    const float* p = g + w0 * 64 + lane;
+    float v[kSignWords];
+    if ((w0 + kSignWords) * 64 <= n_points) {  // uniform: the whole chunk is in the grid
 #pragma unroll
-    for (int j = 0; j < kSignUnroll; ++j) {
-      const int64_t f = (w0 + j) * 64 + lane;
-      v[j] = f < n_points ? g[f] : 1.0f;
+      for (int j = 0; j < kSignWords; ++j) v[j] = __builtin_nontemporal_load(p + 64 * j);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kSignWords; ++j)
+        v[j] = (w0 + j) * 64 + lane < n_points ? p[64 * j] : 1.0f;
     }
     uint64_t mine = 0;
 #pragma unroll
-    for (int j = 0; j < kSignUnroll; ++j) {
-      const uint64_t m = __ballot(!(v[j] > 0.0f));
+    for (int j = 0; j < kSignWords; ++j) {
+      const uint64_t m = __ballot(!(v[j] > 0.0f));  // classify_corners, distance.rs:52-54
       if (lane == j) mine = m;
     }
-    if (lane < kSignUnroll && w0 + lane < n_words) S[w0 + lane] = mine;
+    if (lane < kSignWords && w0 + lane < n_words) S[w0 + lane] = mine;
   }
 }
 
@@ -288,7 +237,11 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
       const uint64_t w11 = row_word(S, f00 + nx * ny + nx, &n11);
       const SegWords sw = seg_words(w00, n00, w10, n10, w01, n01, w11, n11);
       const uint64_t valid = seg_valid(nx, w);
-      nv = owned_count(sw, valid, cy == 0, cz == 0, w == 0);
+      uint64_t vp[4];
+      owned_planes(sw, valid, cy == 0, cz == 0, w == 0, vp);
+      nv = planes_count(vp, ~0ull);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) B.vplanes[4 * g + k] = vp[k];
       act[r] = mixed_cells(sw) & valid;
       na = popc64(act[r]);
       uint64_t p0 = 0, p1 = 0, p2 = 0;  // per-cell triangle counts, bit-sliced
@@ -448,6 +401,7 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
 struct EmitArgs {
   const uint2* seg;
   const uint64_t* tplanes;
+  const uint64_t* vplanes;
   const uint32_t* active;
   uint64_t active_capacity;
   float* vertices;
@@ -468,15 +422,18 @@ enum EmitSource : int {
                        // SDF program elsewhere; positions from the lattice
 };
 
-__device__ __forceinline__ float space_coord(const CellSpace& C, int axis, int64_t i) {
+// Lattice coordinate i (< 2^31) along an axis: the same float as a 64-bit
+// index would give (both conversions round the integer once).
+__device__ __forceinline__ float space_coord(const CellSpace& C, int axis, int32_t i) {
   return C.origin[axis] + static_cast<float>(i) * C.spacing[axis];
 }
 
+// Sample value at lattice point (x, y, z), flat index f.
 template <int kSrc>
-__device__ __forceinline__ float space_value(const CellSpace& C, int64_t x, int64_t y,
-                                             int64_t z) {
+__device__ __forceinline__ float space_value(const CellSpace& C, int32_t x, int32_t y, int32_t z,
+                                             int64_t f) {
   if (kSrc != kEmitProgram && kSrc != kEmitBand) {
-    return C.grid[(z * C.ny + y) * C.nx + x];
+    return C.grid[f];
   } else {
     return sdf_eval(C.ops, C.n_ops, space_coord(C, 0, x), space_coord(C, 1, y),
                     space_coord(C, 2, z));
@@ -486,10 +443,10 @@ __device__ __forceinline__ float space_value(const CellSpace& C, int64_t x, int6
 constexpr int kEmitThreads = 256;
 
 template <int kSrc>
-__device__ __forceinline__ void space_position(const CellSpace& C, int64_t x, int64_t y,
-                                               int64_t z, float* p) {
+__device__ __forceinline__ void space_position(const CellSpace& C, int32_t x, int32_t y,
+                                               int32_t z, int64_t f, float* p) {
   if (kSrc == kEmitPositions) {
-    const float* q = C.positions + 3 * ((z * C.ny + y) * C.nx + x);
+    const float* q = C.positions + 3 * f;
     p[0] = q[0];
     p[1] = q[1];
     p[2] = q[2];
@@ -531,9 +488,12 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     const uint32_t vbase = A.seg[C.seg_off].x;
     const uint32_t row = lg / spr;
     const uint32_t w = lg - row * spr;
-    const int64_t cz = row / cyn;
-    const int64_t cy = row - cz * cyn;
-    const int64_t cx = 64 * static_cast<int64_t>(w) + b;
+    const int32_t cz = static_cast<int32_t>(row / cyn);
+    const int32_t cy = static_cast<int32_t>(row - static_cast<uint32_t>(cz) * cyn);
+    const int32_t cx = static_cast<int32_t>(64 * w) + b;
+    // Flat index of the cell's corner 0; the other corners and the sign
+    // neighbourhood are fixed offsets of it (nx and nxy are per lattice).
+    const int64_t fc = (static_cast<int64_t>(cz) * ny + cy) * nx + cx;
 
     // Loads that depend only on the cell's position go out together with the
     // sign words below, so the cell costs one dependent memory round trip
@@ -541,14 +501,16 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     // that can own one of its edges (segment of (x-rx, y-ry, z-rz)), the
     // cell's triangle planes and first triangle id, and for a grid source
     // its eight corner values.
+    // Segment of the cell at (x-rx, y-ry, z-rz): rz layers and ry rows of
+    // segments back, and one segment back for x-1 at bit 0.
+    const uint32_t layer_segs = cyn * spr;
     uint32_t obase_pre[8];
 #pragma unroll
     for (int o = 0; o < 8; ++o) {
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
       obase_pre[o] = 0;
       if ((rx && cx < 1) || (ry && cy < 1) || (rz && cz < 1)) continue;
-      const int64_t ow = (rx && b == 0) ? static_cast<int64_t>(w) - 1 : static_cast<int64_t>(w);
-      const uint64_t og = C.seg_off + (static_cast<uint64_t>(cz - rz) * cyn + (cy - ry)) * spr + ow;
+      const uint32_t og = g - (rz ? layer_segs : 0u) - (ry ? spr : 0u) - ((rx && b == 0) ? 1u : 0u);
       obase_pre[o] = A.seg[og].x;
     }
     const uint64_t* tp = A.tplanes + 3 * static_cast<uint64_t>(g);
@@ -566,7 +528,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     if (stored) {
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        cv[k] = C.grid[((cz + tables::kCorner[k][2]) * ny + cy + tables::kCorner[k][1]) * nx + cx +
+        cv[k] = C.grid[fc + tables::kCorner[k][2] * nxy + tables::kCorner[k][1] * nx +
                        tables::kCorner[k][0]];
     }
 
@@ -580,9 +542,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     for (int dz = 0; dz < 3; ++dz) {
 #pragma unroll
       for (int dy = 0; dy < 3; ++dy) {
-        const int64_t y = cy - 1 + dy, z = cz - 1 + dz;
-        if (y >= 0 && z >= 0) {
-          const int64_t q = z * nxy + y * nx + cx - 1;  // sample x-1 (>= -1)
+        if (cy - 1 + dy >= 0 && cz - 1 + dz >= 0) {
+          const int64_t q = fc - 1 + (dy - 1) * nx + (dz - 1) * nxy;  // sample x-1 (>= -1)
           const int64_t wq = q >> 6;
           const int sq = static_cast<int>(q & 63);
           const uint64_t lo = S[wq], hi = S[wq + 1];
@@ -606,28 +567,24 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
       needed |= 1u << (rx | (ry << 1) | (rz << 2));
     }
 
-    // First vertex id, cube index and owned crossed edges of each needed owner.
+    // First vertex id, cube index and owned crossed edges of each needed
+    // owner: the owner segment's first id plus the owned vertices of its
+    // cells before the owner, from the segment's bit-sliced counts.
 #pragma unroll 1
     for (int o = 0; o < 8; ++o) {
       if (!(needed & (1u << o))) continue;
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
-      const int64_t ox = cx - rx, oy = cy - ry, oz = cz - rz;
       const uint32_t oc = case_in_cube(cube, 1 - rx, 1 - ry, 1 - rz);
-      const uint32_t oown = owned_edges(ox, oy, oz) & s_t.edge_mask[oc];
-      // Owner's segment words: point rows oy..oy+1 and oz..oz+1 of its
-      // segment (w, or w-1 for x-1 at bit 0), re-read from the sign array.
+      const uint32_t oown = owned_edges(cx - rx, cy - ry, cz - rz) & s_t.edge_mask[oc];
       const bool pseg = rx && b == 0;
-      const int64_t ow = pseg ? static_cast<int64_t>(w) - 1 : static_cast<int64_t>(w);
       const int ob = pseg ? 63 : b - rx;
-      const int64_t f00 = oz * nxy + oy * nx + 64 * ow;  // re-read: cache hits
-      uint64_t n00, n10, n01, n11;
-      const uint64_t w00 = row_word(S, f00, &n00);
-      const uint64_t w10 = row_word(S, f00 + nx, &n10);
-      const uint64_t w01 = row_word(S, f00 + nxy, &n01);
-      const uint64_t w11 = row_word(S, f00 + nxy + nx, &n11);
-      const SegWords os = seg_words(w00, n00, w10, n10, w01, n01, w11, n11);
-      const uint32_t pre =
-          owned_count(os, seg_valid(nx, ow) & low_mask(ob), oy == 0, oz == 0, ow == 0);
+      const uint32_t og = g - (rz ? layer_segs : 0u) - (ry ? spr : 0u) - (pseg ? 1u : 0u);
+      const ulonglong2* vq =
+          reinterpret_cast<const ulonglong2*>(A.vplanes + 4 * static_cast<uint64_t>(og));
+      const ulonglong2 v01 = vq[0], v23 = vq[1];
+      const uint64_t lm = low_mask(ob);
+      const uint32_t pre = popc64(v01.x & lm) + 2 * popc64(v01.y & lm) +
+                           4 * popc64(v23.x & lm) + 8 * popc64(v23.y & lm);
       s_obase[o][tid] = obase_pre[o] + pre;
       s_oinfo[o][tid] = oc | (oown << 8);
     }
@@ -640,15 +597,19 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
         const int e = __builtin_ctz(todo);
         const uint32_t id = base + static_cast<uint32_t>(__popc(mine & s_t.before[c0][e]));
         const int u = tables::kEdgeCorners[e][0], v = tables::kEdgeCorners[e][1];
-        const int64_t ux = cx + tables::kCorner[u][0], uy = cy + tables::kCorner[u][1],
+        const int32_t ux = cx + tables::kCorner[u][0], uy = cy + tables::kCorner[u][1],
                       uz = cz + tables::kCorner[u][2];
-        const int64_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
+        const int32_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
                       vz = cz + tables::kCorner[v][2];
-        const float va = stored ? cv[u] : space_value<kSrc>(C, ux, uy, uz);
-        const float vb = stored ? cv[v] : space_value<kSrc>(C, vx, vy, vz);
+        const int64_t fu = fc + tables::kCorner[u][2] * nxy + tables::kCorner[u][1] * nx +
+                           tables::kCorner[u][0];
+        const int64_t fv = fc + tables::kCorner[v][2] * nxy + tables::kCorner[v][1] * nx +
+                           tables::kCorner[v][0];
+        const float va = stored ? cv[u] : space_value<kSrc>(C, ux, uy, uz, fu);
+        const float vb = stored ? cv[v] : space_value<kSrc>(C, vx, vy, vz, fv);
         float pa[3], pb[3], p[3];
-        space_position<kSrc>(C, ux, uy, uz, pa);
-        space_position<kSrc>(C, vx, vy, vz, pb);
+        space_position<kSrc>(C, ux, uy, uz, fu, pa);
+        space_position<kSrc>(C, vx, vy, vz, fv, pb);
         crossing(va, vb, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], p);
         if (id < A.vcap) {
           A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
@@ -778,18 +739,10 @@ hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& 
   const ResetArgs R = reset_args(D, B);
   // At least enough threads to clear the look-back words in one sweep.
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
-  if ((reinterpret_cast<uintptr_t>(grid) & 15u) == 0) {
-    int64_t blocks = grid_for((D.n_points + 256 * kSignVec - 1) / (256 * kSignVec));
-    if (blocks < min_blocks) blocks = min_blocks;
-    hipLaunchKernelGGL(k_signs_grid4, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
-                       reinterpret_cast<const f32x4*>(grid), D.n_points, D.n_sign_words, B.signs,
-                       R);
-  } else {
-    int64_t blocks = grid_for((D.n_sign_words + kSignUnroll - 1) / kSignUnroll);
-    if (blocks < min_blocks) blocks = min_blocks;
-    hipLaunchKernelGGL(k_signs_grid, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, grid,
-                       D.n_points, D.n_sign_words, B.signs, R);
-  }
+  int64_t blocks = grid_for((D.n_sign_words + kSignWords - 1) / kSignWords);
+  if (blocks < min_blocks) blocks = min_blocks;
+  hipLaunchKernelGGL(k_signs_grid, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, grid,
+                     D.n_points, D.n_sign_words, B.signs, R);
   return hipGetLastError();
 }
 
@@ -832,6 +785,7 @@ EmitArgs emit_args(const McBuffers& B) {
   EmitArgs A;
   A.seg = B.seg;
   A.tplanes = B.tplanes;
+  A.vplanes = B.vplanes;
   A.active = B.active;
   A.active_capacity = B.active_capacity;
   A.vertices = B.vertices;

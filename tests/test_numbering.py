@@ -158,11 +158,51 @@ def owned_count_planes(c, m, y0, z0, first):
     return n
 
 
-@pytest.mark.parametrize("seed", range(30))
+def owned_planes(c, valid, y0, z0, first):
+    """The count kernel's per-cell owned-vertex counts, bit-sliced in four
+    planes by a ripple-carry adder over the owned-edge masks."""
+    p = [0, 0, 0, 0]
+
+    def add(m):
+        for k in range(4):
+            carry = p[k] & m
+            p[k] ^= m
+            m = carry
+
+    for m in (c[6] ^ c[7], c[5] ^ c[6], c[2] ^ c[6]):
+        add(valid & m)
+    if y0:
+        add(valid & (c[4] ^ c[5]))
+        add(valid & (c[1] ^ c[5]))
+    if z0:
+        add(valid & (c[2] ^ c[3]))
+        add(valid & (c[1] ^ c[2]))
+    if y0 and z0:
+        add(valid & (c[0] ^ c[1]))
+    if first:
+        v0 = valid & 1
+        add(v0 & (c[4] ^ c[7]))
+        add(v0 & (c[3] ^ c[7]))
+        if z0:
+            add(v0 & (c[0] ^ c[3]))
+        if y0:
+            add(v0 & (c[0] ^ c[4]))
+    return p
+
+
+def planes_count(p, m):
+    return sum((1 << k) * _popc(p[k] & m) for k in range(4))
+
+
+@pytest.mark.parametrize("seed", range(31))
 def test_bit_sliced_owned_counts_match_per_cell_rule(seed):
     rng = np.random.RandomState(500 + seed)
     nx, ny, nz = int(rng.randint(2, 200)), int(rng.randint(2, 4)), int(rng.randint(2, 4))
     s = ~(rng.randn(nz, ny, nx).astype(np.float32) > 0)
+    if seed == 30:
+        # a 3-D checkerboard crosses every edge: the corner cell owns all 12
+        z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        s = ((x + y + z) % 2) == 0
 
     def rowbits(y, z, x0):
         v = 0
@@ -193,3 +233,5 @@ def test_bit_sliced_owned_counts_match_per_cell_rule(seed):
                                    if s[cz + dz, cy + dy, x + dx])
                         want += _popc(owned(x, cy, cz) & MASK[case])
                     assert owned_count_planes(c, valid & lm, cy == 0, cz == 0, w == 0) == want
+                    planes = owned_planes(c, valid, cy == 0, cz == 0, w == 0)
+                    assert planes_count(planes, lm) == want
