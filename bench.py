@@ -12,11 +12,12 @@ digests of the oracle's mesh).
 
 Printed JSON line:
   value         cells/s of the timed steps (wall clock, synchronised)
-  roofline      the dominant kernel (sign classification of the FP32 grid) —
-                algorithmic bytes per launch / its mean HIP-event duration
-                over the timed steps, against MI355X's 8 TB/s HBM peak
-  pipeline      every kernel of the step, and the step's algorithmic bytes
-                (4 B per sample + 12 B per vertex + 12 B per triangle)
+  roofline      the classify + interpolate + emit stage (its three kernels):
+                algorithmic bytes (4 B per sample + 12 B per vertex + 12 B
+                per triangle) / the sum of the kernels' mean HIP-event
+                durations over the timed steps, against MI355X's 8 TB/s HBM
+                peak; traffic from the committed counter summary
+  kernels       each kernel's own algorithmic bytes, duration and fraction
   program_path  the same extraction sampling the CSG program on the GPU
                 instead of reading the grid (VALU-bound, reported aside)
   batched_chunks
@@ -109,6 +110,25 @@ def kernel_pass(ctx, desc, mesh, steps):
     stats = ctx.kernel_stats()
     ctx.reset_kernel_stats()
     return {k: {"launches": n, "mean_us": 1e3 * ms / max(n, 1)} for k, (n, ms) in stats.items()}
+
+
+def measured_traffic():
+    """HBM bytes per launch of the stage's kernels, as measured by the
+    committed FETCH_SIZE / WRITE_SIZE counter passes (scaled by a 64 MiB
+    copy; see the summary's header), or None when the summary is absent."""
+    path = ROOT / "profiles" / "mc_grid_257_hbm_traffic.md"
+    if not path.exists():
+        return None
+    out = {"kernels": {}}
+    for line in path.read_text().splitlines():
+        parts = line.strip().split()
+        if len(parts) == 2 and parts[0].startswith("traffic_bytes."):
+            name = parts[0][len("traffic_bytes."):].rstrip(":")
+            if name == "stage":
+                out["stage"] = int(parts[1])
+            else:
+                out["kernels"][name] = int(parts[1])
+    return out if "stage" in out else None
 
 
 def digest(ctx, mesh: DeviceMesh, out) -> tuple:
@@ -466,12 +486,25 @@ def main() -> int:
     log(f"grid path: {ms_per_step:.4f} ms/step, {value / 1e9:.2f} G cells/s")
 
     kernels = kernel_pass(ctx, desc, mesh, args.steps)
-    dom_name = "mc_signs_grid"
-    dom = kernels[dom_name]
-    dom_bytes = 4 * n + n // 8  # read every sample once, write one bit per sample
-    achieved = dom_bytes / (dom["mean_us"] * 1e-6) / 1e9
-    kernel_us = sum(k["mean_us"] for k in kernels.values())
-    pipe_bytes = 4 * n + 12 * nv + 4 * ni
+    # The classify + interpolate + emit stage is the unit of the roofline:
+    # algorithmic bytes 4 per sample read + 12 per vertex + 12 per triangle
+    # written, over the summed mean durations of its three kernels.
+    stage = ["mc_signs_grid", "mc_count", "mc_emit"]
+    stage_us = sum(kernels[k]["mean_us"] for k in stage)
+    stage_bytes = 4 * n + 12 * nv + 4 * ni
+    achieved = stage_bytes / (stage_us * 1e-6) / 1e9
+    segs = (desc.nx - 1 + 63) // 64 * (desc.ny - 1) * (desc.nz - 1)
+    kernel_bytes = {
+        "mc_signs_grid": 4 * n + n // 8,           # every sample once, one bit out
+        "mc_count": n // 8 + 8 * segs,              # sign bits in, first ids out
+        "mc_emit": 8 * nv + 12 * nv + 4 * ni,       # endpoint samples in, mesh out
+    }
+    per_kernel = {
+        k: {"mean_us": round(kernels[k]["mean_us"], 3), "algorithmic_bytes": b,
+            "achieved_GBps": b / (kernels[k]["mean_us"] * 1e-6) / 1e9,
+            "frac": b / (kernels[k]["mean_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS}
+        for k, b in kernel_bytes.items()}
+    traffic = measured_traffic()
     log(f"kernels: {json.dumps(kernels)}")
 
     # Program path: the CSG sampled by the extraction itself (no grid).
@@ -513,22 +546,18 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom_name,
+            "kernel": "k_signs_grid + k_count + k_emit (classify, count and scan, "
+                      "interpolate and emit: one extraction)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "algorithmic_bytes": dom_bytes,
-            "mean_us": dom["mean_us"],
+            "traffic": traffic.get("stage") if traffic else None,
+            "algorithmic_bytes": stage_bytes,
+            "mean_us": stage_us,
         },
-        "pipeline": {
-            "kernels_us": {k: round(v["mean_us"], 2) for k, v in kernels.items()},
-            "kernel_us_per_step": kernel_us,
-            "algorithmic_bytes": pipe_bytes,
-            "achieved_GBps": pipe_bytes / (kernel_us * 1e-6) / 1e9,
-            "frac_of_hbm_peak": pipe_bytes / (kernel_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-        },
+        "kernels": per_kernel,
+        "traffic_per_kernel": traffic.get("kernels") if traffic else None,
         "program_path": program_path,
     }
 

@@ -71,19 +71,56 @@ constexpr uint64_t pack_of_axis_min(int half) {
 constexpr uint64_t kOfAxisMin0 = pack_of_axis_min(0);
 constexpr uint64_t kOfAxisMin1 = pack_of_axis_min(1);
 
+// Owner of each edge of a cell that has lower neighbours along every axis,
+// per case: need[c] = owner slots (rx | ry << 1 | rz << 2 of the cell at
+// (x-rx, y-ry, z-rz)) its crossed edges use, ix[c][j] = slot | (the edge's
+// local id in the owner) << 3 for entry j of its triangle list.
+struct InteriorTables {
+  uint8_t ix[256][16];
+  uint8_t need[256];
+};
+
+constexpr InteriorTables make_interior() {
+  InteriorTables t{};
+  const EdgeGeom g = make_geom();
+  const tables::McTables& T = tables::kTables;
+  uint8_t slot[12] = {}, owner_edge[12] = {};
+  for (int e = 0; e < 12; ++e) {
+    const int axis = g.axis[e];
+    const int r[3] = {axis != 0 && g.minp[e][0] == 0 ? 1 : 0,
+                      axis != 1 && g.minp[e][1] == 0 ? 1 : 0,
+                      axis != 2 && g.minp[e][2] == 0 ? 1 : 0};
+    slot[e] = static_cast<uint8_t>(r[0] | (r[1] << 1) | (r[2] << 2));
+    owner_edge[e] = static_cast<uint8_t>(
+        g.of_axis_min[axis][(g.minp[e][0] | r[0]) | ((g.minp[e][1] | r[1]) << 1) |
+                            ((g.minp[e][2] | r[2]) << 2)]);
+  }
+  for (int c = 0; c < 256; ++c) {
+    uint32_t need = 0;
+    for (int e = 0; e < 12; ++e)
+      if ((T.edge_mask[c] >> e) & 1u) need |= 1u << slot[e];
+    t.need[c] = static_cast<uint8_t>(need);
+    for (int j = 0; j < 16; ++j) {
+      const int e = T.tri[c][j];
+      t.ix[c][j] = e < 0 ? 0 : static_cast<uint8_t>(slot[e] | (owner_edge[e] << 3));
+    }
+  }
+  return t;
+}
+
 __device__ __forceinline__ int edge_of_axis_min(int axis, int rel) {
   const int k = 8 * axis + rel;
   const uint64_t w = k < 16 ? kOfAxisMin0 : kOfAxisMin1;
   return static_cast<int>((w >> (4 * (k & 15))) & 15u);
 }
 
-__device__ __forceinline__ void load_tables(tables::McTables* dst,
-                                            const tables::McTables* from) {
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(from);
-  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
-  constexpr int kWords = sizeof(tables::McTables) / 4;
-  static_assert(sizeof(tables::McTables) % 4 == 0, "table size");
-  for (int i = threadIdx.x; i < kWords; i += blockDim.x) d[i] = src[i];
+// Copies a table into LDS with 16-byte accesses.
+template <class T>
+__device__ __forceinline__ void load_table16(T* dst, const T* from) {
+  static_assert(sizeof(T) % 16 == 0, "table size");
+  const uint4* src = reinterpret_cast<const uint4*>(from);
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  for (int i = threadIdx.x; i < static_cast<int>(sizeof(T) / 16); i += blockDim.x) d[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------

@@ -39,7 +39,8 @@ namespace {
 
 using namespace mcd;
 
-__constant__ tables::McTables c_tables = tables::kTables;
+__constant__ __align__(16) tables::McTables c_tables = tables::kTables;
+__constant__ __align__(16) InteriorTables c_interior = make_interior();
 
 // ---------------------------------------------------------------------------
 // 1. signs (+ reset of the extraction's scratch state)
@@ -459,13 +460,16 @@ __device__ __forceinline__ void space_position(const CellSpace& C, int32_t x, in
 
 template <int kSrc, class Space>
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
-  __shared__ tables::McTables s_t;
+  __shared__ __align__(16) tables::McTables s_t;
   // Per thread: first vertex id and (cube index | owned crossed edges << 8)
   // of each of the eight cells that can own one of its edges, indexed
   // rx | ry<<1 | rz<<2 for the cell at (x-rx, y-ry, z-rz).
   __shared__ uint32_t s_obase[8][kEmitThreads];
   __shared__ uint32_t s_oinfo[8][kEmitThreads];
-  load_tables(&s_t, &c_tables);
+  // Owners of the edges of cells with lower neighbours along every axis.
+  __shared__ __align__(16) InteriorTables s_in;
+  load_table16(&s_t, &c_tables);
+  load_table16(&s_in, &c_interior);
   __syncthreads();
 
   const int tid = threadIdx.x;
@@ -556,15 +560,18 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     const uint32_t emask = s_t.edge_mask[c0];
 
     // Which owner cells the cell's crossed edges need.
-    uint32_t needed = 0;
-    for (uint32_t m = emask; m; m &= m - 1) {
-      const int e = __builtin_ctz(m);
-      const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
-      const int axis = static_cast<int>(gm >> 3);
-      const int rx = (axis != 0 && (gm & 1u) == 0 && cx >= 1) ? 1 : 0;
-      const int ry = (axis != 1 && (gm & 2u) == 0 && cy >= 1) ? 1 : 0;
-      const int rz = (axis != 2 && (gm & 4u) == 0 && cz >= 1) ? 1 : 0;
-      needed |= 1u << (rx | (ry << 1) | (rz << 2));
+    const bool interior = cx >= 1 && cy >= 1 && cz >= 1;
+    uint32_t needed = interior ? s_in.need[c0] : 0u;
+    if (!interior) {
+      for (uint32_t m = emask; m; m &= m - 1) {
+        const int e = __builtin_ctz(m);
+        const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
+        const int axis = static_cast<int>(gm >> 3);
+        const int rx = (axis != 0 && (gm & 1u) == 0 && cx >= 1) ? 1 : 0;
+        const int ry = (axis != 1 && (gm & 2u) == 0 && cy >= 1) ? 1 : 0;
+        const int rz = (axis != 2 && (gm & 4u) == 0 && cz >= 1) ? 1 : 0;
+        needed |= 1u << (rx | (ry << 1) | (rz << 2));
+      }
     }
 
     // First vertex id, cube index and owned crossed edges of each needed
@@ -625,17 +632,24 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
         tseg + popc64(tp0 & lm) + 2 * popc64(tp1 & lm) + 4 * popc64(tp2 & lm);
     const uint32_t nidx = 3u * s_t.tri_count[c0];
     for (uint32_t j = 0; j < nidx; ++j) {
-      const int e = s_t.tri[c0][j];
-      const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
-      const int axis = static_cast<int>(gm >> 3);
-      const int px = gm & 1, py = (gm >> 1) & 1, pz = (gm >> 2) & 1;
-      // Owner: one cell back along each non-axis direction where the edge
-      // lies on this cell's low face and a lower cell exists.
-      const int rx = (axis != 0 && px == 0 && cx >= 1) ? 1 : 0;
-      const int ry = (axis != 1 && py == 0 && cy >= 1) ? 1 : 0;
-      const int rz = (axis != 2 && pz == 0 && cz >= 1) ? 1 : 0;
-      const int o = rx | (ry << 1) | (rz << 2);
-      const int eo = edge_of_axis_min(axis, (px | rx) | ((py | ry) << 1) | ((pz | rz) << 2));
+      int o, eo;
+      if (interior) {
+        const uint32_t ix = s_in.ix[c0][j];
+        o = static_cast<int>(ix & 7u);
+        eo = static_cast<int>(ix >> 3);
+      } else {
+        const int e = s_t.tri[c0][j];
+        const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
+        const int axis = static_cast<int>(gm >> 3);
+        const int px = gm & 1, py = (gm >> 1) & 1, pz = (gm >> 2) & 1;
+        // Owner: one cell back along each non-axis direction where the edge
+        // lies on this cell's low face and a lower cell exists.
+        const int rx = (axis != 0 && px == 0 && cx >= 1) ? 1 : 0;
+        const int ry = (axis != 1 && py == 0 && cy >= 1) ? 1 : 0;
+        const int rz = (axis != 2 && pz == 0 && cz >= 1) ? 1 : 0;
+        o = rx | (ry << 1) | (rz << 2);
+        eo = edge_of_axis_min(axis, (px | rx) | ((py | ry) << 1) | ((pz | rz) << 2));
+      }
       const uint32_t info = s_oinfo[o][tid];
       const uint32_t oc = info & 0xffu, oown = info >> 8;
       const uint32_t id =
