@@ -108,6 +108,27 @@ constexpr InteriorTables make_interior() {
   return t;
 }
 
+// Corner mask (bit per corner) of each edge's two endpoints, eight bits per
+// edge: edges 0-7 in the first word, 8-11 in the second.
+constexpr uint64_t pack_edge_corners(int half) {
+  uint64_t r = 0;
+  for (int i = 0; i < 8; ++i) {
+    const int e = 8 * half + i;
+    if (e >= 12) break;
+    r |= static_cast<uint64_t>((1u << tables::kEdgeCorners[e][0]) |
+                               (1u << tables::kEdgeCorners[e][1]))
+         << (8 * i);
+  }
+  return r;
+}
+constexpr uint64_t kEdgeCornerMask0 = pack_edge_corners(0);
+constexpr uint64_t kEdgeCornerMask1 = pack_edge_corners(1);
+
+__device__ __forceinline__ uint32_t edge_corner_mask(int e) {
+  const uint64_t w = e < 8 ? kEdgeCornerMask0 : kEdgeCornerMask1;
+  return static_cast<uint32_t>((w >> (8 * (e & 7))) & 0xFFu);
+}
+
 __device__ __forceinline__ int edge_of_axis_min(int axis, int rel) {
   const int k = 8 * axis + rel;
   const uint64_t w = k < 16 ? kOfAxisMin0 : kOfAxisMin1;

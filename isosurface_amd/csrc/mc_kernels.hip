@@ -442,6 +442,7 @@ __device__ __forceinline__ float space_value(const CellSpace& C, int32_t x, int3
 }
 
 constexpr int kEmitThreads = 256;
+constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X
 
 template <int kSrc>
 __device__ __forceinline__ void space_position(const CellSpace& C, int32_t x, int32_t y,
@@ -476,8 +477,19 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
   uint64_t n_act = A.totals[kTotActive];
   if (n_act > A.active_capacity) n_act = A.active_capacity;
 
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kEmitThreads + tid; i < n_act;
-       i += static_cast<uint64_t>(gridDim.x) * kEmitThreads) {
+  // Workgroups are dealt to the eight XCDs round-robin; each XCD takes a
+  // contiguous eighth of the active list (neighbouring cells, which share
+  // sign words, corner values and segment entries, stay in one L2), and
+  // each workgroup a contiguous chunk of that.
+  const uint32_t nb = gridDim.x;
+  const uint32_t xcd = blockIdx.x % kXcds, k_in = blockIdx.x / kXcds;
+  const uint32_t per = nb / kXcds, rem = nb % kXcds;
+  const uint64_t vblock = static_cast<uint64_t>(xcd) * per + (xcd < rem ? xcd : rem) + k_in;
+  const uint64_t chunk = (n_act + nb - 1) / nb;
+  const uint64_t first = vblock * chunk;
+  const uint64_t last = first + chunk < n_act ? first + chunk : n_act;
+
+  for (uint64_t i = first + tid; i < last; i += kEmitThreads) {
     const uint32_t rec = A.active[i];
     const uint32_t g = rec >> 6;  // global segment
     const int b = static_cast<int>(rec & 63u);
@@ -500,17 +512,16 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     const int64_t fc = (static_cast<int64_t>(cz) * ny + cy) * nx + cx;
 
     // Loads that depend only on the cell's position go out together with the
-    // sign words below, so the cell costs one dependent memory round trip
-    // before the owners' segment words: the first vertex id of every cell
-    // that can own one of its edges (segment of (x-rx, y-ry, z-rz)), the
-    // cell's triangle planes and first triangle id, and for a grid source
-    // its eight corner values.
-    // Segment of the cell at (x-rx, y-ry, z-rz): rz layers and ry rows of
-    // segments back, and one segment back for x-1 at bit 0.
+    // sign words below: the first vertex id of every cell that can own one
+    // of its edges (segment of (x-rx, y-ry, z-rz): rz layers and ry rows of
+    // segments back, and one segment back for x-1 at bit 0; slot 7, a step
+    // back along all three axes, holds no edge of the cell), and the cell's
+    // triangle planes and first triangle id.
     const uint32_t layer_segs = cyn * spr;
     uint32_t obase_pre[8];
+    obase_pre[7] = 0;
 #pragma unroll
-    for (int o = 0; o < 8; ++o) {
+    for (int o = 0; o < 7; ++o) {
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
       obase_pre[o] = 0;
       if ((rx && cx < 1) || (ry && cy < 1) || (rz && cz < 1)) continue;
@@ -520,7 +531,6 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     const uint64_t* tp = A.tplanes + 3 * static_cast<uint64_t>(g);
     const uint64_t tp0 = tp[0], tp1 = tp[1], tp2 = tp[2];
     const uint32_t tseg = A.seg[g].y;
-    float cv[8];
     // A band cell's corners all lie in its brick's closed sample box, whose
     // samples were evaluated exactly and stored; a culled brick's cells are
     // only active when a predicted sign was wrong, and evaluate the program.
@@ -529,12 +539,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
       stored = (A.brick_flags[((cz >> kBandBrickShift) * A.nby + (cy >> kBandBrickShift)) * A.nbx +
                               (cx >> kBandBrickShift)] &
                 1u) != 0;
-    if (stored) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k)
-        cv[k] = C.grid[fc + tables::kCorner[k][2] * nxy + tables::kCorner[k][1] * nx +
-                       tables::kCorner[k][0]];
-    }
+    const bool interior = cx >= 1 && cy >= 1 && cz >= 1;
 
     // 3x3x3 sign neighbourhood: every cell that can own one of this cell's
     // edges lies inside it.  Row (dy, dz) contributes the bits of samples
@@ -558,9 +563,23 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     }
     const uint32_t c0 = case_in_cube(cube, 1, 1, 1);
     const uint32_t emask = s_t.edge_mask[c0];
+    // Vertices this cell owns, and the corner values their crossings read
+    // (only the endpoints of those edges are loaded).
+    const uint32_t mine = owned_edges(cx, cy, cz) & emask;
+    float cv[8];
+    if (stored) {
+      uint32_t need_c = 0;
+      for (uint32_t m = mine; m; m &= m - 1) need_c |= edge_corner_mask(__builtin_ctz(m));
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        cv[k] = 0.0f;
+        if ((need_c >> k) & 1u)
+          cv[k] = C.grid[fc + tables::kCorner[k][2] * nxy + tables::kCorner[k][1] * nx +
+                         tables::kCorner[k][0]];
+      }
+    }
 
     // Which owner cells the cell's crossed edges need.
-    const bool interior = cx >= 1 && cy >= 1 && cz >= 1;
     uint32_t needed = interior ? s_in.need[c0] : 0u;
     if (!interior) {
       for (uint32_t m = emask; m; m &= m - 1) {
@@ -578,7 +597,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     // owner: the owner segment's first id plus the owned vertices of its
     // cells before the owner, from the segment's bit-sliced counts.
 #pragma unroll 1
-    for (int o = 0; o < 8; ++o) {
+    for (int o = 0; o < 7; ++o) {
       if (!(needed & (1u << o))) continue;
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
       const uint32_t oc = case_in_cube(cube, 1 - rx, 1 - ry, 1 - rz);
@@ -597,7 +616,6 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     }
 
     // Vertices this cell owns: positions along its own edge directions.
-    const uint32_t mine = owned_edges(cx, cy, cz) & emask;
     if (mine) {
       const uint32_t base = s_obase[0][tid];
       for (uint32_t todo = mine; todo; todo &= todo - 1) {
