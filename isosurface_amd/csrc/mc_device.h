@@ -129,6 +129,30 @@ __device__ __forceinline__ uint32_t edge_corner_mask(int e) {
   return static_cast<uint32_t>((w >> (8 * (e & 7))) & 0xFFu);
 }
 
+// The tables emit stages in LDS (the triangle lists themselves stay in
+// constant memory: cells away from the low faces read ix instead).
+struct EmitTables {
+  uint16_t before[256][12];   // edges first used before edge e (by case)
+  uint16_t edge_mask[256];    // edges crossed by the case
+  uint8_t tri_count[256];     // triangles emitted by the case
+  uint8_t need[256];          // InteriorTables::need
+  uint8_t ix[256][16];        // InteriorTables::ix
+};
+
+constexpr EmitTables make_emit_tables() {
+  EmitTables t{};
+  const tables::McTables& T = tables::kTables;
+  const InteriorTables in = make_interior();
+  for (int c = 0; c < 256; ++c) {
+    for (int e = 0; e < 12; ++e) t.before[c][e] = T.before[c][e];
+    t.edge_mask[c] = T.edge_mask[c];
+    t.tri_count[c] = T.tri_count[c];
+    t.need[c] = in.need[c];
+    for (int j = 0; j < 16; ++j) t.ix[c][j] = in.ix[c][j];
+  }
+  return t;
+}
+
 __device__ __forceinline__ int edge_of_axis_min(int axis, int rel) {
   const int k = 8 * axis + rel;
   const uint64_t w = k < 16 ? kOfAxisMin0 : kOfAxisMin1;

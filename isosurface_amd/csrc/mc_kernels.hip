@@ -40,7 +40,7 @@ namespace {
 using namespace mcd;
 
 __constant__ __align__(16) tables::McTables c_tables = tables::kTables;
-__constant__ __align__(16) InteriorTables c_interior = make_interior();
+__constant__ __align__(16) EmitTables c_emit = make_emit_tables();
 
 // ---------------------------------------------------------------------------
 // 1. signs (+ reset of the extraction's scratch state)
@@ -461,16 +461,14 @@ __device__ __forceinline__ void space_position(const CellSpace& C, int32_t x, in
 
 template <int kSrc, class Space>
 __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
-  __shared__ __align__(16) tables::McTables s_t;
-  // Per thread: first vertex id and (cube index | owned crossed edges << 8)
-  // of each of the eight cells that can own one of its edges, indexed
-  // rx | ry<<1 | rz<<2 for the cell at (x-rx, y-ry, z-rz).
-  __shared__ uint32_t s_obase[8][kEmitThreads];
-  __shared__ uint32_t s_oinfo[8][kEmitThreads];
-  // Owners of the edges of cells with lower neighbours along every axis.
-  __shared__ __align__(16) InteriorTables s_in;
-  load_table16(&s_t, &c_tables);
-  load_table16(&s_in, &c_interior);
+  // LDS stays near 20 KB per workgroup so eight fit a CU: the tables, and
+  // per thread the first vertex id and the cube index of each of the seven
+  // cells that can own one of its edges, slot rx | ry<<1 | rz<<2 for the
+  // cell at (x-rx, y-ry, z-rz).
+  __shared__ __align__(16) EmitTables s_t;
+  __shared__ uint32_t s_obase[7][kEmitThreads];
+  __shared__ uint8_t s_ocase[7][kEmitThreads];
+  load_table16(&s_t, &c_emit);
   __syncthreads();
 
   const int tid = threadIdx.x;
@@ -580,7 +578,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     }
 
     // Which owner cells the cell's crossed edges need.
-    uint32_t needed = interior ? s_in.need[c0] : 0u;
+    uint32_t needed = interior ? s_t.need[c0] : 0u;
     if (!interior) {
       for (uint32_t m = emask; m; m &= m - 1) {
         const int e = __builtin_ctz(m);
@@ -601,7 +599,6 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
       if (!(needed & (1u << o))) continue;
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
       const uint32_t oc = case_in_cube(cube, 1 - rx, 1 - ry, 1 - rz);
-      const uint32_t oown = owned_edges(cx - rx, cy - ry, cz - rz) & s_t.edge_mask[oc];
       const bool pseg = rx && b == 0;
       const int ob = pseg ? 63 : b - rx;
       const uint32_t og = g - (rz ? layer_segs : 0u) - (ry ? spr : 0u) - (pseg ? 1u : 0u);
@@ -612,7 +609,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
       const uint32_t pre = popc64(v01.x & lm) + 2 * popc64(v01.y & lm) +
                            4 * popc64(v23.x & lm) + 8 * popc64(v23.y & lm);
       s_obase[o][tid] = obase_pre[o] + pre;
-      s_oinfo[o][tid] = oc | (oown << 8);
+      s_ocase[o][tid] = static_cast<uint8_t>(oc);
     }
 
     // Vertices this cell owns: positions along its own edge directions.
@@ -652,11 +649,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     for (uint32_t j = 0; j < nidx; ++j) {
       int o, eo;
       if (interior) {
-        const uint32_t ix = s_in.ix[c0][j];
+        const uint32_t ix = s_t.ix[c0][j];
         o = static_cast<int>(ix & 7u);
         eo = static_cast<int>(ix >> 3);
       } else {
-        const int e = s_t.tri[c0][j];
+        const int e = c_tables.tri[c0][j];
         const uint32_t gm = static_cast<uint32_t>(kEdgeGeomPacked >> (5 * e)) & 31u;
         const int axis = static_cast<int>(gm >> 3);
         const int px = gm & 1, py = (gm >> 1) & 1, pz = (gm >> 2) & 1;
@@ -668,8 +665,9 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
         o = rx | (ry << 1) | (rz << 2);
         eo = edge_of_axis_min(axis, (px | rx) | ((py | ry) << 1) | ((pz | rz) << 2));
       }
-      const uint32_t info = s_oinfo[o][tid];
-      const uint32_t oc = info & 0xffu, oown = info >> 8;
+      const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
+      const uint32_t oc = s_ocase[o][tid];
+      const uint32_t oown = owned_edges(cx - rx, cy - ry, cz - rz) & s_t.edge_mask[oc];
       const uint32_t id =
           s_obase[o][tid] + static_cast<uint32_t>(__popc(oown & s_t.before[oc][eo]));
       const uint64_t ix = 3 * tbase + j;
