@@ -538,6 +538,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
                               (cx >> kBandBrickShift)] &
                 1u) != 0;
     const bool interior = cx >= 1 && cy >= 1 && cz >= 1;
+    const bool deep = cx >= 2 && cy >= 2 && cz >= 2;  // so are all its owners
 
     // 3x3x3 sign neighbourhood: every cell that can own one of this cell's
     // edges lies inside it.  Row (dy, dz) contributes the bits of samples
@@ -613,31 +614,40 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     }
 
     // Vertices this cell owns: positions along its own edge directions.
-    if (mine) {
-      const uint32_t base = s_obase[0][tid];
+    // (A cell away from the low faces owns only e5, e6 and e10, whose corner
+    // values are picked at compile time; others loop over their edges.)
+    const uint32_t vbase_cell = s_obase[0][tid];
+    auto vertex = [&](int e, int u, int v, float cu, float cvv) {
+      const uint32_t id = vbase_cell + static_cast<uint32_t>(__popc(mine & s_t.before[c0][e]));
+      const int32_t ux = cx + tables::kCorner[u][0], uy = cy + tables::kCorner[u][1],
+                    uz = cz + tables::kCorner[u][2];
+      const int32_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
+                    vz = cz + tables::kCorner[v][2];
+      const int64_t fu = fc + tables::kCorner[u][2] * nxy + tables::kCorner[u][1] * nx +
+                         tables::kCorner[u][0];
+      const int64_t fv = fc + tables::kCorner[v][2] * nxy + tables::kCorner[v][1] * nx +
+                         tables::kCorner[v][0];
+      const float va = stored ? cu : space_value<kSrc>(C, ux, uy, uz, fu);
+      const float vb = stored ? cvv : space_value<kSrc>(C, vx, vy, vz, fv);
+      float pa[3], pb[3], p[3];
+      space_position<kSrc>(C, ux, uy, uz, fu, pa);
+      space_position<kSrc>(C, vx, vy, vz, fv, pb);
+      crossing(va, vb, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], p);
+      if (id < A.vcap) {
+        A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
+        A.vertices[3 * static_cast<uint64_t>(id) + 1] = p[1];
+        A.vertices[3 * static_cast<uint64_t>(id) + 2] = p[2];
+      }
+    };
+    if (interior) {
+      if (mine & (1u << 5)) vertex(5, 5, 6, cv[5], cv[6]);
+      if (mine & (1u << 6)) vertex(6, 6, 7, cv[6], cv[7]);
+      if (mine & (1u << 10)) vertex(10, 2, 6, cv[2], cv[6]);
+    } else {
       for (uint32_t todo = mine; todo; todo &= todo - 1) {
         const int e = __builtin_ctz(todo);
-        const uint32_t id = base + static_cast<uint32_t>(__popc(mine & s_t.before[c0][e]));
         const int u = tables::kEdgeCorners[e][0], v = tables::kEdgeCorners[e][1];
-        const int32_t ux = cx + tables::kCorner[u][0], uy = cy + tables::kCorner[u][1],
-                      uz = cz + tables::kCorner[u][2];
-        const int32_t vx = cx + tables::kCorner[v][0], vy = cy + tables::kCorner[v][1],
-                      vz = cz + tables::kCorner[v][2];
-        const int64_t fu = fc + tables::kCorner[u][2] * nxy + tables::kCorner[u][1] * nx +
-                           tables::kCorner[u][0];
-        const int64_t fv = fc + tables::kCorner[v][2] * nxy + tables::kCorner[v][1] * nx +
-                           tables::kCorner[v][0];
-        const float va = stored ? cv[u] : space_value<kSrc>(C, ux, uy, uz, fu);
-        const float vb = stored ? cv[v] : space_value<kSrc>(C, vx, vy, vz, fv);
-        float pa[3], pb[3], p[3];
-        space_position<kSrc>(C, ux, uy, uz, fu, pa);
-        space_position<kSrc>(C, vx, vy, vz, fv, pb);
-        crossing(va, vb, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], p);
-        if (id < A.vcap) {
-          A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
-          A.vertices[3 * static_cast<uint64_t>(id) + 1] = p[1];
-          A.vertices[3 * static_cast<uint64_t>(id) + 2] = p[2];
-        }
+        vertex(e, u, v, cv[u], cv[v]);
       }
     }
 
@@ -665,9 +675,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
         o = rx | (ry << 1) | (rz << 2);
         eo = edge_of_axis_min(axis, (px | rx) | ((py | ry) << 1) | ((pz | rz) << 2));
       }
+      // Owners with no coordinate 0 own e5, e6 and e10 only.
       const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
       const uint32_t oc = s_ocase[o][tid];
-      const uint32_t oown = owned_edges(cx - rx, cy - ry, cz - rz) & s_t.edge_mask[oc];
+      const uint32_t oedges = deep ? 0x460u : owned_edges(cx - rx, cy - ry, cz - rz);
+      const uint32_t oown = oedges & s_t.edge_mask[oc];
       const uint32_t id =
           s_obase[o][tid] + static_cast<uint32_t>(__popc(oown & s_t.before[oc][eo]));
       const uint64_t ix = 3 * tbase + j;
