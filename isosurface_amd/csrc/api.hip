@@ -57,7 +57,7 @@ struct iso_ctx {
   DevBuf pvals;  // program samples of the current extraction (emit reads them)
   DevBuf judge;  // dense sign words that judge a band extraction
   // dual contouring: primal samples, dual values and points, mixed list, counters
-  DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_counters;
+  DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
   // octree marching cubes: nodes, dense primal lattice, cubes, edge table, scan state
   DevBuf oc_keys, oc_values, oc_leaf, oc_dense, oc_primal, oc_plevel, oc_corner, oc_case,
@@ -261,7 +261,7 @@ int iso_ctx_destroy(iso_ctx* c) {
   DevBuf* bufs[] = {&c->prog,       &c->grid_owned, &c->signs,     &c->seg,      &c->tplanes, &c->vplanes,
                     &c->agg,        &c->incl,       &c->active,    &c->totals,
                     &c->vout,       &c->iout,       &c->pts,       &c->vals,     &c->dc_samples,
-                    &c->dc_values,  &c->dc_points,  &c->dc_mixed,  &c->dc_counters,
+                    &c->dc_values,  &c->dc_points,  &c->dc_mixed,  &c->dc_flags,  &c->dc_counters,
                     &c->oc_keys,    &c->oc_values,  &c->oc_leaf,   &c->oc_dense, &c->oc_primal,
                     &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
                     &c->oc_towner,  &c->oc_tid,     &c->sc_slot,   &c->sc_ctl,
@@ -628,12 +628,14 @@ int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mes
   ISO_HIP(c, ensure(&c->dc_values, n_cells * sizeof(float)));
   ISO_HIP(c, ensure(&c->dc_points, 3 * n_cells * sizeof(float)));
   ISO_HIP(c, ensure(&c->dc_mixed, n_cells * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->dc_flags, n_cells));
   ISO_HIP(c, ensure(&c->dc_counters, iso::kDcSlots * sizeof(unsigned long long)));
   iso::DcBuffers B;
   B.samples = static_cast<float*>(c->dc_samples.ptr);
   B.dual_values = static_cast<float*>(c->dc_values.ptr);
   B.dual_points = static_cast<float*>(c->dc_points.ptr);
   B.mixed = static_cast<uint32_t*>(c->dc_mixed.ptr);
+  B.mixed_flag = static_cast<uint8_t*>(c->dc_flags.ptr);
   B.counters = static_cast<unsigned long long*>(c->dc_counters.ptr);
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   ISO_HIP(c, timed(c, "dc_sample",

@@ -28,15 +28,17 @@ struct DcBuffers {
   float* dual_values;   // one per primal cell, z-major
   float* dual_points;   // 3 per primal cell
   uint32_t* mixed;      // mixed primal cells, capacity = number of cells
+  uint8_t* mixed_flag;  // one per primal cell
   unsigned long long* counters;  // kDcSlots
 };
 
 // Samples the program over the primal lattice and clears the counters.
 hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const DcBuffers& B,
                             hipStream_t s);
-// Dual values, centre dual vertices and the mixed-cell list (ascending, by a
-// single-pass scan whose state `st` holds scan::blocks_for(cells, 1) slots;
-// its err word is redirected to counters[kDcScanErr]).
+// Dual values, centre dual vertices and mixed flags (thread per cell), then
+// the mixed-cell list (ascending, by a single-pass scan over the flags whose
+// state `st` holds scan::blocks_for(cells, 1) slots; its err word is
+// redirected to counters[kDcScanErr]).
 hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, hipStream_t s);
 // Feature points of the listed mixed cells (normals by central differences).
 hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,

@@ -46,13 +46,14 @@ __global__ __launch_bounds__(256) void k_dc_sample(const DevOp* __restrict__ ops
   }
 }
 
-// Scan load: one primal cell's dual value and, unless mixed, its centre.
-struct DualLoad {
+// One primal cell: its dual value and, unless mixed, its centre; returns
+// whether it is mixed.
+struct DualCell {
   Lattice L;
   const float* samples;
   float* dual_values;
   float* dual_points;
-  __device__ unsigned long long operator()(uint64_t c) const {
+  __device__ bool operator()(uint64_t c) const {
     const int64_t nx = L.nx, ny = L.ny;
     const uint32_t cnx = static_cast<uint32_t>(nx - 1);
     const uint32_t cnxy = static_cast<uint32_t>((nx - 1) * (ny - 1));
@@ -83,8 +84,23 @@ struct DualLoad {
       dual_points[3 * c + 1] = 0.5f * c0y + 0.5f * c6y;
       dual_points[3 * c + 2] = 0.5f * c0z + 0.5f * c6z;
     }
-    return is_mixed ? 1ull : 0ull;
+    return is_mixed;
   }
+};
+
+// Thread per primal cell (coalesced): dual values, centres, mixed flags.
+__global__ __launch_bounds__(256) void k_dc_dual(DualCell cell, uint64_t n_cells,
+                                                 uint8_t* __restrict__ mixed_flag) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; c < n_cells;
+       c += stride)
+    mixed_flag[c] = cell(c) ? 1u : 0u;
+}
+
+// Scan load: the mixed flag.
+struct MixedLoad {
+  const uint8_t* flag;
+  __device__ unsigned long long operator()(uint64_t c) const { return flag[c]; }
 };
 
 // Scan sink: the mixed cells' list slots and, at the last cell, the count.
@@ -169,10 +185,18 @@ hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const
 hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, hipStream_t s) {
   const uint64_t n_cells = static_cast<uint64_t>((L.nx - 1) * (L.ny - 1) * (L.nz - 1));
   if (n_cells == 0 || n_cells > 0xffffffffull || !st.slot || !st.ctl) return hipErrorInvalidValue;
+  if (!B.mixed_flag) return hipErrorInvalidValue;
   st.err = B.counters + kDcScanErr;
-  const DualLoad load{L, B.samples, B.dual_values, B.dual_points};
+  const DualCell cell{L, B.samples, B.dual_values, B.dual_points};
+  uint64_t blocks = (n_cells + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_dc_dual, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, cell, n_cells,
+                     B.mixed_flag);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const MixedLoad load{B.mixed_flag};
   const DualSink sink{n_cells, B.mixed, B.counters};
-  return scan::launch<1>(n_cells, load, sink, st, s);
+  return scan::launch<scan::kItems>(n_cells, load, sink, st, s);
 }
 
 hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,

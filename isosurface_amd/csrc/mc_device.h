@@ -308,10 +308,11 @@ __device__ __forceinline__ void sliced_add(uint64_t* p, int n_planes, uint64_t m
 __device__ __forceinline__ void owned_planes(const SegWords& s, uint64_t valid, bool y0, bool z0,
                                              bool first, uint64_t* p) {
   const uint64_t* c = s.c;
-  p[0] = p[1] = p[2] = p[3] = 0;
-  sliced_add(p, 4, valid & (c[6] ^ c[7]));
-  sliced_add(p, 4, valid & (c[5] ^ c[6]));
-  sliced_add(p, 4, valid & (c[2] ^ c[6]));
+  // e6, e5, e10 (every cell): a full adder of three masks
+  const uint64_t a = valid & (c[6] ^ c[7]), b = valid & (c[5] ^ c[6]), d = valid & (c[2] ^ c[6]);
+  p[0] = a ^ b ^ d;
+  p[1] = (a & b) | (d & (a ^ b));
+  p[2] = p[3] = 0;
   if (y0) {
     sliced_add(p, 4, valid & (c[4] ^ c[5]));
     sliced_add(p, 4, valid & (c[1] ^ c[5]));

@@ -595,22 +595,43 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     // First vertex id, cube index and owned crossed edges of each needed
     // owner: the owner segment's first id plus the owned vertices of its
     // cells before the owner, from the segment's bit-sliced counts.
+    // Owners 2k and 2k+1 (x and x-1 of the same row and layer) share a
+    // segment unless the cell is first in its own: one planes load per pair.
+#pragma unroll
+    for (int o = 0; o < 7; ++o) s_obase[o][tid] = obase_pre[o];
 #pragma unroll 1
-    for (int o = 0; o < 7; ++o) {
-      if (!(needed & (1u << o))) continue;
-      const int rx = o & 1, ry = (o >> 1) & 1, rz = (o >> 2) & 1;
-      const uint32_t oc = case_in_cube(cube, 1 - rx, 1 - ry, 1 - rz);
-      const bool pseg = rx && b == 0;
-      const int ob = pseg ? 63 : b - rx;
-      const uint32_t og = g - (rz ? layer_segs : 0u) - (ry ? spr : 0u) - (pseg ? 1u : 0u);
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t pair = (needed >> (2 * k)) & (k == 3 ? 1u : 3u);
+      if (!pair) continue;
+      const int ry = k & 1, rz = k >> 1;
+      const uint32_t og = g - (rz ? layer_segs : 0u) - (ry ? spr : 0u);
       const ulonglong2* vq =
           reinterpret_cast<const ulonglong2*>(A.vplanes + 4 * static_cast<uint64_t>(og));
       const ulonglong2 v01 = vq[0], v23 = vq[1];
-      const uint64_t lm = low_mask(ob);
-      const uint32_t pre = popc64(v01.x & lm) + 2 * popc64(v01.y & lm) +
-                           4 * popc64(v23.x & lm) + 8 * popc64(v23.y & lm);
-      s_obase[o][tid] = obase_pre[o] + pre;
-      s_ocase[o][tid] = static_cast<uint8_t>(oc);
+      if (pair & 1u) {  // the cell (x, y-ry, z-rz)
+        const uint64_t lm = low_mask(b);
+        s_obase[2 * k][tid] += popc64(v01.x & lm) + 2 * popc64(v01.y & lm) +
+                               4 * popc64(v23.x & lm) + 8 * popc64(v23.y & lm);
+        s_ocase[2 * k][tid] = static_cast<uint8_t>(case_in_cube(cube, 1, 1 - ry, 1 - rz));
+      }
+      if (pair & 2u) {  // the cell (x-1, y-ry, z-rz)
+        uint64_t p0 = v01.x, p1 = v01.y, p2 = v23.x, p3 = v23.y;
+        int ob = b - 1;
+        if (b == 0) {  // last cell of the previous segment
+          const ulonglong2* wq = reinterpret_cast<const ulonglong2*>(
+              A.vplanes + 4 * static_cast<uint64_t>(og - 1));
+          const ulonglong2 w01 = wq[0], w23 = wq[1];
+          p0 = w01.x;
+          p1 = w01.y;
+          p2 = w23.x;
+          p3 = w23.y;
+          ob = 63;
+        }
+        const uint64_t lm = low_mask(ob);
+        s_obase[2 * k + 1][tid] +=
+            popc64(p0 & lm) + 2 * popc64(p1 & lm) + 4 * popc64(p2 & lm) + 8 * popc64(p3 & lm);
+        s_ocase[2 * k + 1][tid] = static_cast<uint8_t>(case_in_cube(cube, 0, 1 - ry, 1 - rz));
+      }
     }
 
     // Vertices this cell owns: positions along its own edge directions.

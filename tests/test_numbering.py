@@ -161,7 +161,9 @@ def owned_count_planes(c, m, y0, z0, first):
 def owned_planes(c, valid, y0, z0, first):
     """The count kernel's per-cell owned-vertex counts, bit-sliced in four
     planes by a ripple-carry adder over the owned-edge masks."""
-    p = [0, 0, 0, 0]
+    # e6, e5, e10 of every cell: a full adder of the three masks
+    a, b, d = valid & (c[6] ^ c[7]), valid & (c[5] ^ c[6]), valid & (c[2] ^ c[6])
+    p = [a ^ b ^ d, (a & b) | (d & (a ^ b)), 0, 0]
 
     def add(m):
         for k in range(4):
@@ -169,8 +171,6 @@ def owned_planes(c, valid, y0, z0, first):
             p[k] ^= m
             m = carry
 
-    for m in (c[6] ^ c[7], c[5] ^ c[6], c[2] ^ c[6]):
-        add(valid & m)
     if y0:
         add(valid & (c[4] ^ c[5]))
         add(valid & (c[1] ^ c[5]))
