@@ -468,24 +468,29 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
   __shared__ __align__(16) EmitTables s_t;
   __shared__ uint32_t s_obase[7][kEmitThreads];
   __shared__ uint8_t s_ocase[7][kEmitThreads];
-  load_table16(&s_t, &c_emit);
-  __syncthreads();
 
   const int tid = threadIdx.x;
   uint64_t n_act = A.totals[kTotActive];
   if (n_act > A.active_capacity) n_act = A.active_capacity;
 
-  // Workgroups are dealt to the eight XCDs round-robin; each XCD takes a
-  // contiguous eighth of the active list (neighbouring cells, which share
-  // sign words, corner values and segment entries, stay in one L2), and
-  // each workgroup a contiguous chunk of that.
-  const uint32_t nb = gridDim.x;
+  // The active list in 256-cell chunks, one per thread of a workgroup; a
+  // grid larger than the chunk count leaves its spare workgroups idle before
+  // they stage any table.  Workgroups are dealt to the eight XCDs
+  // round-robin; the working ones of each XCD take a contiguous share of the
+  // chunks (neighbouring cells, which share sign words, corner values and
+  // segment entries, stay in one L2).
+  const uint64_t n_chunks = (n_act + kEmitThreads - 1) / kEmitThreads;
+  if (blockIdx.x >= n_chunks) return;
+  const uint32_t nw = n_chunks < gridDim.x ? static_cast<uint32_t>(n_chunks) : gridDim.x;
   const uint32_t xcd = blockIdx.x % kXcds, k_in = blockIdx.x / kXcds;
-  const uint32_t per = nb / kXcds, rem = nb % kXcds;
+  const uint32_t per = nw / kXcds, rem = nw % kXcds;
   const uint64_t vblock = static_cast<uint64_t>(xcd) * per + (xcd < rem ? xcd : rem) + k_in;
-  const uint64_t chunk = (n_act + nb - 1) / nb;
-  const uint64_t first = vblock * chunk;
-  const uint64_t last = first + chunk < n_act ? first + chunk : n_act;
+  const uint64_t chunks_per = (n_chunks + nw - 1) / nw;
+  const uint64_t first = vblock * chunks_per * kEmitThreads;
+  uint64_t last = first + chunks_per * kEmitThreads;
+  if (last > n_act) last = n_act;
+  load_table16(&s_t, &c_emit);
+  __syncthreads();
 
   for (uint64_t i = first + tid; i < last; i += kEmitThreads) {
     const uint32_t rec = A.active[i];
@@ -865,17 +870,16 @@ EmitArgs emit_args(const McBuffers& B) {
 // at most `per_cu` resident workgroups per CU (0: as many as fit).
 template <class K>
 unsigned emit_blocks(const McBuffers& B, int per_cu, K kernel) {
-  if (per_cu <= 0) {
-    int fit = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, kernel, kEmitThreads, 0) !=
-            hipSuccess ||
-        fit < 1)
-      fit = 4;
-    per_cu = fit;
-  }
+  (void)kernel;
+  // One 256-cell chunk of the active list per workgroup (the list's length
+  // is only known on the device: workgroups past it exit at once), or at
+  // most per_cu resident workgroups per CU, each taking several chunks.
   int64_t blocks = (static_cast<int64_t>(B.active_capacity) + kEmitThreads - 1) / kEmitThreads;
-  const int64_t cap = static_cast<int64_t>(per_cu) * device_cus();
-  if (blocks > cap) blocks = cap;
+  if (per_cu > 0) {
+    const int64_t cap = static_cast<int64_t>(per_cu) * device_cus();
+    if (blocks > cap) blocks = cap;
+  }
+  if (blocks > 0x7fffffff) blocks = 0x7fffffff;
   if (blocks < 1) blocks = 1;
   return static_cast<unsigned>(blocks);
 }
