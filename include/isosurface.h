@@ -113,6 +113,13 @@ int iso_program_upload(iso_ctx* ctx, const iso_node* nodes, uint32_t n_nodes);
 /* Samples the current program at n points (xyz interleaved, host memory):
  * ScalarSource::sample_scalar evaluated on the GPU. */
 int iso_program_sample(iso_ctx* ctx, const float* points, uint64_t n, float* values);
+/* CentralDifference::new_with_epsilon(program, epsilon).sample_normal at n
+ * points (3 floats each) in `memory` (source.rs:82-94): per axis
+ * (f(p + e) - f(p - e)) / (2 epsilon), not normalised — what the normal
+ * sinks IndexedInterleavedNormals / OnlyInterleavedNormals interleave with
+ * each vertex (extractor.rs:46-69,97-127). */
+int iso_sample_normals(iso_ctx* ctx, const float* points, uint64_t n, float epsilon,
+                       float* normals, int32_t memory);
 
 /* ---- sources: dense sample grids ---------------------------------------- */
 
@@ -167,6 +174,13 @@ typedef struct iso_mesh_out {
  * IndexedVertices extractor (marching_cubes.rs:59-82, extractor.rs:72-94).
  * Vertex order, positions and index lists are the reference's. */
 int iso_extract_mc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
+
+/* PointCloud::<Signed>::new(size).extract(source, extractor)
+ * (point_cloud.rs:50-63): one point per mixed cell (cube index neither 0
+ * nor 255) in the lattice's cell order, corners[0].lerp(corners[6], 0.5);
+ * vertices only (index_count = 0).  Same sources, capacity and fetch rules
+ * as iso_extract_mc. */
+int iso_extract_point_cloud(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
 
 /* Dual contouring with MinimiseQEF feature placement and CentralDifference
  * normals of the current program (step `epsilon`, the reference's default is

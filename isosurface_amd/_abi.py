@@ -115,6 +115,10 @@ SIGNATURES = [
     ("iso_program_validate", C.c_int, [C.POINTER(IsoNode), C.c_uint32, C.c_char_p, C.c_size_t]),
     ("iso_program_upload", C.c_int, [C.c_void_p, C.POINTER(IsoNode), C.c_uint32]),
     ("iso_program_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("iso_sample_normals", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_float, C.c_void_p,
+                                     C.c_int32]),
+    ("iso_extract_point_cloud", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc),
+                                          C.POINTER(IsoMeshOut)]),
     ("iso_program_sample_lattice", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_void_p,
                                              C.c_int32]),
     ("iso_grid_upload", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64]),

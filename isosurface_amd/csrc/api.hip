@@ -56,6 +56,7 @@ struct iso_ctx {
   DevBuf signs, seg, tplanes, vplanes, agg, incl, active, totals, vout, iout, pts, vals;
   DevBuf pvals;  // program samples of the current extraction (emit reads them)
   DevBuf judge;  // dense sign words that judge a band extraction
+  DevBuf nrm;    // normals of iso_sample_normals (host-memory calls)
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
@@ -270,7 +271,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_v,       &c->nn_grad,    &c->nn_frags,  &c->nn_loss,  &c->nn_x,
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
-                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge};
+                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge,  &c->nrm};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
@@ -330,6 +331,39 @@ int iso_program_sample(iso_ctx* c, const float* points, uint64_t n, float* value
           }));
   ISO_HIP(c, hipMemcpyAsync(values, c->vals.ptr, n * sizeof(float), hipMemcpyDeviceToHost,
                             c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
+}
+
+int iso_sample_normals(iso_ctx* c, const float* points, uint64_t n, float epsilon, float* normals,
+                       int32_t memory) {
+  if (!c || (n && (!points || !normals))) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null buffer");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (!(epsilon > 0.0f) || !(epsilon < 3.0e38f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "epsilon must be positive and finite");
+  if (memory != ISO_MEMORY_HOST && memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown memory kind");
+  if (n == 0) return ISO_OK;
+  if (n > (uint64_t(1) << 38)) return fail(c, ISO_ERR_INVALID_ARGUMENT, "too many points");
+  ISO_HIP(c, hipSetDevice(c->device));
+  const float* dp = points;
+  float* dn = normals;
+  if (memory == ISO_MEMORY_HOST) {
+    ISO_HIP(c, ensure(&c->pts, n * 3 * sizeof(float)));
+    ISO_HIP(c, ensure(&c->nrm, n * 3 * sizeof(float)));
+    ISO_HIP(c, hipMemcpyAsync(c->pts.ptr, points, n * 3 * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
+    dp = static_cast<const float*>(c->pts.ptr);
+    dn = static_cast<float*>(c->nrm.ptr);
+  }
+  ISO_HIP(c, timed(c, "sample_normals", [&] {
+            return iso::launch_sample_normals(static_cast<const iso::DevOp*>(c->prog.ptr),
+                                              c->n_prog, dp, n, epsilon, dn, c->stream);
+          }));
+  if (memory == ISO_MEMORY_HOST)
+    ISO_HIP(c, hipMemcpyAsync(normals, dn, n * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
   collect_timing(c);
   return ISO_OK;
@@ -594,6 +628,72 @@ int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
   c->kept = false;
   ISO_HIP(c, hipSetDevice(c->device));
   return extract_lattice(c, L, from_grid ? c->grid : nullptr, nullptr, out);
+}
+
+int iso_extract_point_cloud(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  const bool from_grid = desc->source == ISO_SOURCE_GRID;
+  if (from_grid) {
+    if (!c->grid) return fail(c, ISO_ERR_NO_SOURCE, "no grid bound");
+    if (c->gnx != L.nx || c->gny != L.ny || c->gnz != L.nz)
+      return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice dims differ from the bound grid");
+  } else if (c->n_prog == 0) {
+    return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  }
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+  c->band_valid = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+  const iso::McDims D = iso::mc_dims(L);
+  if (D.n_segs >= (int64_t(1) << 26))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");
+  iso::McBuffers B{};
+  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
+  // Classification and the count's active list (the mixed cells in cell
+  // order) are all a point cloud needs.
+  const float* grid = from_grid ? c->grid : nullptr;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if ((st = classify_and_count(c, grid, L, D, B)) != ISO_OK) return st;
+    if ((st = read_totals(c, B)) != ISO_OK) return st;
+    if (c->h_totals[iso::kTotActive] <= B.active_capacity) break;
+    if ((st = prepare_mc(c, D, c->h_totals[iso::kTotActive], &B)) != ISO_OK) return st;
+  }
+  const uint64_t n = c->h_totals[iso::kTotActive];
+  out->vertex_count = n;
+  out->index_count = 0;
+  if (out->memory == ISO_MEMORY_DEVICE) {
+    if (n > (out->vertices ? out->vertex_capacity : 0))
+      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    ISO_HIP(c, timed(c, "point_cloud", [&] {
+              return iso::launch_point_cloud(L, D, B, out->vertices, n, c->stream);
+            }));
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    return ISO_OK;
+  }
+  ISO_HIP(c, ensure(&c->vout, (n ? n : 1) * 3 * sizeof(float)));
+  ISO_HIP(c, timed(c, "point_cloud", [&] {
+            return iso::launch_point_cloud(L, D, B, static_cast<float*>(c->vout.ptr), n,
+                                           c->stream);
+          }));
+  if (n && (!out->vertices || out->vertex_capacity < n)) {
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    collect_timing(c);
+    c->kept = true;
+    c->kept_vertices = n;
+    c->kept_indices = 0;
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small; points kept for iso_mesh_fetch");
+  }
+  if (n)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, c->vout.ptr, n * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
 }
 
 int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mesh_out* out) {

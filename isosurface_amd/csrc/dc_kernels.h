@@ -40,6 +40,10 @@ hipError_t launch_dc_sample(const DevOp* ops, int n_ops, const Lattice& L, const
 // state `st` holds scan::blocks_for(cells, 1) slots; its err word is
 // redirected to counters[kDcScanErr]).
 hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, hipStream_t s);
+// CentralDifference::sample_normal of the program at n points (3 floats
+// each, device), unnormalised, into normals (3 floats each, device).
+hipError_t launch_sample_normals(const DevOp* ops, int n_ops, const float* points, uint64_t n,
+                                 float epsilon, float* normals, hipStream_t s);
 // Feature points of the listed mixed cells (normals by central differences).
 hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
                              const DcBuffers& B, hipStream_t s);

@@ -128,6 +128,26 @@ __device__ __forceinline__ qef::Vec3f central_normal(const DevOp* ops, int n_ops
   return qef::normalised_or_zero(qef::Vec3f{vx / e2, vy / e2, vz / e2});
 }
 
+// CentralDifference::sample_normal (source.rs:82-94) as the normal sinks use
+// it (extractor.rs:46-69,97-127): not normalised.
+__global__ __launch_bounds__(256) void k_sample_normals(const DevOp* __restrict__ ops, int n_ops,
+                                                        const float* __restrict__ pts, uint64_t n,
+                                                        float eps, float* __restrict__ out) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+  const float vx = sdf_eval(ops, n_ops, px + eps, py + 0.0f, pz + 0.0f) -
+                   sdf_eval(ops, n_ops, px - eps, py - 0.0f, pz - 0.0f);
+  const float vy = sdf_eval(ops, n_ops, px + 0.0f, py + eps, pz + 0.0f) -
+                   sdf_eval(ops, n_ops, px - 0.0f, py - eps, pz - 0.0f);
+  const float vz = sdf_eval(ops, n_ops, px + 0.0f, py + 0.0f, pz + eps) -
+                   sdf_eval(ops, n_ops, px - 0.0f, py - 0.0f, pz - eps);
+  const float e2 = 2.0f * eps;
+  out[3 * i + 0] = vx / e2;
+  out[3 * i + 1] = vy / e2;
+  out[3 * i + 2] = vz / e2;
+}
+
 constexpr int kFeatureThreads = 128;
 
 __global__ __launch_bounds__(kFeatureThreads) void k_dc_feature(
@@ -197,6 +217,15 @@ hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, 
   const MixedLoad load{B.mixed_flag};
   const DualSink sink{n_cells, B.mixed, B.counters};
   return scan::launch<scan::kItems>(n_cells, load, sink, st, s);
+}
+
+hipError_t launch_sample_normals(const DevOp* ops, int n_ops, const float* points, uint64_t n,
+                                 float epsilon, float* normals, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (!ops || !points || !normals) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_sample_normals, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                     s, ops, n_ops, points, n, epsilon, normals);
+  return hipGetLastError();
 }
 
 hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,

@@ -103,6 +103,10 @@ hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, h
 // interpolate between those positions instead of lattice coordinates.
 hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const float* positions,
                        const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
+// PointCloud points (3 floats each, up to `capacity`) of the mixed cells in
+// the active list, after launch_count.
+hipError_t launch_point_cloud(const Lattice& L, const McDims& D, const McBuffers& B,
+                              float* points, uint64_t capacity, hipStream_t s);
 // Emit of a narrow-band extraction: corner values from `values` (stored by
 // the band's exact sampling) for cells of bricks flagged exact (bit 0 of
 // brick_flags, bricks of 2^kBandBrickShift cells, nbx x nby per z layer),

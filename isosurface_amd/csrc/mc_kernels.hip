@@ -714,6 +714,36 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
   }
 }
 
+// PointCloud::extract (point_cloud.rs:50-63): per mixed cell, in cell order,
+// corners[0].lerp(corners[6], 0.5) (vector.rs:325-333, 0.5 * a + 0.5 * b per
+// component).  One thread per entry of the active list, which holds exactly
+// the mixed cells in cell order.
+__global__ __launch_bounds__(256) void k_point_cloud(Lattice L, McDims D,
+                                                     const uint32_t* __restrict__ active,
+                                                     uint64_t active_capacity,
+                                                     const unsigned long long* __restrict__ totals,
+                                                     float* __restrict__ points, uint64_t capacity) {
+  uint64_t n = totals[kTotActive];
+  if (n > active_capacity) n = active_capacity;
+  if (n > capacity) n = capacity;
+  const uint32_t spr = static_cast<uint32_t>(D.segs_per_row);
+  const uint32_t cyn = static_cast<uint32_t>(L.ny - 1);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t rec = active[i];
+    const uint32_t g = rec >> 6;
+    const uint32_t row = g / spr;
+    const uint32_t w = g - row * spr;
+    const int64_t cz = row / cyn, cy = row - (row / cyn) * cyn;
+    const int64_t cx = 64 * static_cast<int64_t>(w) + (rec & 63u);
+    const float c0[3] = {lattice_coord(L, 0, cx), lattice_coord(L, 1, cy), lattice_coord(L, 2, cz)};
+    const float c6[3] = {lattice_coord(L, 0, cx + 1), lattice_coord(L, 1, cy + 1),
+                         lattice_coord(L, 2, cz + 1)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) points[3 * i + a] = 0.5f * c0[a] + 0.5f * c6[a];
+  }
+}
+
 // Per chunk first vertex / index of the batch output (n_chunks + 1 entries).
 __global__ __launch_bounds__(256) void k_batch_offsets(const uint2* __restrict__ seg,
                                                        const uint32_t* __restrict__ seg_start,
@@ -909,6 +939,17 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const flo
     auto k = k_emit<kEmitProgram, SingleSpace>;
     hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_point_cloud(const Lattice& L, const McDims& D, const McBuffers& B,
+                              float* points, uint64_t capacity, hipStream_t s) {
+  if (!points || capacity == 0) return hipSuccess;
+  int64_t blocks = (static_cast<int64_t>(B.active_capacity) + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_point_cloud, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, L, D,
+                     B.active, B.active_capacity, B.totals, points, capacity);
   return hipGetLastError();
 }
 

@@ -81,6 +81,52 @@ class OnlyVertices(Extractor):
         pass
 
 
+class IndexedInterleavedNormals(Extractor):
+    """xyz and normal per vertex (6 f32) and u32 indices (extractor.rs:97-127);
+    the normal is source.sample_normal(v) of a CentralDifference source,
+    evaluated on the GPU (iso_sample_normals)."""
+
+    def __init__(self, vertices: Optional[list] = None, indices: Optional[list] = None,
+                 source: Optional[CentralDifference] = None, context: Optional["Context"] = None):
+        if not isinstance(source, CentralDifference):
+            raise TypeError("the normal sinks need a HermiteSource: a CentralDifference source")
+        self.vertices = vertices if vertices is not None else []
+        self.indices = indices if indices is not None else []
+        self.source = source
+        self.ctx = context
+
+    def extract_vertex(self, v):
+        self.extract_vertices(np.asarray([v], dtype=np.float32))
+
+    def extract_index(self, index):
+        self.indices.append(int(index) & 0xFFFFFFFF)
+
+    def extract_vertices(self, vertices):
+        v = np.asarray(vertices, dtype=np.float32).reshape(-1, 3)
+        if not len(v):
+            return
+        ctx = self.ctx or default_context()
+        n = ctx.sample_normals(self.source, v)
+        self.vertices.extend(np.concatenate([v, n], axis=1).reshape(-1).tolist())
+
+    def extract_indices(self, indices):
+        self.indices.extend(np.asarray(indices, dtype=np.uint32).tolist())
+
+
+class OnlyInterleavedNormals(IndexedInterleavedNormals):
+    """xyz and normal per vertex, faces discarded (extractor.rs:46-69)."""
+
+    def __init__(self, vertices: Optional[list] = None,
+                 source: Optional[CentralDifference] = None, context: Optional["Context"] = None):
+        super().__init__(vertices, None, source, context)
+
+    def extract_index(self, index):
+        pass
+
+    def extract_indices(self, indices):
+        pass
+
+
 @dataclass
 class Mesh:
     vertices: np.ndarray  # (V, 3) float32
@@ -147,6 +193,30 @@ class Context:
         self.check(self.lib.iso_program_sample(self.handle, pts.ctypes.data, len(pts),
                                                out.ctypes.data), "iso_program_sample")
         return out
+
+    def sample_normals(self, source: CentralDifference, points: np.ndarray) -> np.ndarray:
+        """CentralDifference::sample_normal at each point (not normalised), on the GPU."""
+        self.upload_program(source.program())
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(pts)
+        self.check(self.lib.iso_sample_normals(self.handle, pts.ctypes.data, len(pts),
+                                               C.c_float(source.epsilon), out.ctypes.data,
+                                               A.ISO_MEMORY_HOST), "iso_sample_normals")
+        return out
+
+    def extract_point_cloud(self, desc: A.IsoGridDesc, capacity: int = 0) -> np.ndarray:
+        """iso_extract_point_cloud into a host array (grown on ISO_ERR_CAPACITY)."""
+        cap = max(int(capacity), 1)
+        pts = np.empty((cap, 3), dtype=np.float32)
+        out = A.IsoMeshOut(pts.ctypes.data, None, cap, 0, 0, 0, A.ISO_MEMORY_HOST, 0)
+        st = self.lib.iso_extract_point_cloud(self.handle, C.byref(desc), C.byref(out))
+        if st == A.ISO_ERR_CAPACITY:
+            n = int(out.vertex_count)
+            pts = np.empty((max(n, 1), 3), dtype=np.float32)
+            out = A.IsoMeshOut(pts.ctypes.data, None, max(n, 1), 0, 0, 0, A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.handle, C.byref(out))
+        self.check(st, "iso_extract_point_cloud")
+        return pts[:int(out.vertex_count)].copy()
 
     def reference_desc(self, size: int, source_kind: int) -> A.IsoGridDesc:
         d = A.IsoGridDesc()
@@ -332,6 +402,33 @@ class MarchingCubes:
     def extract(self, source, extractor: Extractor) -> None:
         """MarchingCubes::extract (marching_cubes.rs:59-82)."""
         _replay(self.extract_mesh(source), extractor)
+
+
+class PointCloud:
+    """PointCloud::<Signed>::new(size) (point_cloud.rs:29-42) on the GPU: one
+    vertex per mixed cell, its centre, in cell order; no faces."""
+
+    def __init__(self, size: int, context: Optional[Context] = None):
+        if size < 2:
+            raise ValueError("size must be at least 2")
+        self.size = int(size)
+        self.ctx = context or default_context()
+        self._last_count = 0
+
+    def extract_points(self, source) -> np.ndarray:
+        if isinstance(source, Source):
+            self.ctx.upload_program(source.program())
+            desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_PROGRAM)
+        else:
+            desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_GRID)
+            self.ctx.upload_grid(sample_lattice_host(source, desc))
+        pts = self.ctx.extract_point_cloud(desc, self._last_count)
+        self._last_count = len(pts)
+        return pts
+
+    def extract(self, source, extractor: Extractor) -> None:
+        """PointCloud::extract (point_cloud.rs:50-63): vertices only."""
+        extractor.extract_vertices(self.extract_points(source))
 
 
 class MinimiseQEF:

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <fstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "isosurface.hpp"
@@ -50,6 +51,25 @@ int main(int argc, char** argv) {
     dump(out + ".host.f32", hv);
     dump(out + ".host.u32", hi);
     std::printf("host source: %zu vertices, %zu triangles\n", hv.size() / 3, hi.size() / 3);
+
+    // PointCloud::new(65).extract(&source, &mut OnlyVertices) and the normal
+    // sink IndexedInterleavedNormals over CentralDifference::new_with_epsilon.
+    std::vector<float> pts;
+    isosurface::OnlyVertices psink(pts);
+    isosurface::PointCloud pc(65, ctx);
+    pc.extract(isosurface::DemoSource(isosurface::Sphere{0.35f}), psink);
+    dump(out + ".points.f32", pts);
+    std::printf("point cloud: %zu points\n", pts.size() / 3);
+
+    const auto demo = isosurface::DemoSource(isosurface::Sphere{0.35f});
+    isosurface::CentralDifference<std::remove_cv_t<decltype(demo)>> cd{demo, 1e-4f};
+    std::vector<float> nv;
+    std::vector<uint32_t> ni;
+    isosurface::IndexedInterleavedNormals<std::remove_cv_t<decltype(demo)>> nsink(nv, &ni, cd, ctx);
+    mc.extract(demo, nsink);
+    dump(out + ".normals.f32", nv);
+    dump(out + ".normals.u32", ni);
+    std::printf("normal sink: %zu vertices\n", nv.size() / 6);
 
     // Error channel: a malformed program surfaces as isosurface::Error.
     try {

@@ -173,6 +173,12 @@ class Extractor {
   virtual ~Extractor() = default;
   virtual void extract_vertex(Vec3 v) = 0;
   virtual void extract_index(size_t index) = 0;
+  // Bulk form (n vertices, xyz interleaved): the same calls in order, so a
+  // sink that needs device work per vertex can batch it.
+  virtual void extract_vertices(const float* xyz, uint64_t n) {
+    for (uint64_t v = 0; v < n; ++v)
+      extract_vertex(Vec3{xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]});
+  }
 };
 
 class IndexedVertices : public Extractor {
@@ -290,8 +296,7 @@ class MarchingCubes {
       st = iso_mesh_fetch(ctx_.get(), &out);
     }
     ctx_.check(st, "iso_extract_mc");
-    for (uint64_t v = 0; v < out.vertex_count; ++v)
-      extractor.extract_vertex(Vec3{vertices_[3 * v], vertices_[3 * v + 1], vertices_[3 * v + 2]});
+    extractor.extract_vertices(vertices_.data(), out.vertex_count);
     for (uint64_t i = 0; i < out.index_count; ++i) extractor.extract_index(indices_[i]);
   }
 
@@ -299,6 +304,97 @@ class MarchingCubes {
   Context& ctx_;
   std::vector<float> vertices_;      // reused across calls, like the
   std::vector<uint32_t> indices_;    // reference's layer buffers
+};
+
+// ---- PointCloud (point_cloud.rs:29-63) ----------------------------------------
+
+class PointCloud {
+ public:
+  PointCloud(size_t size, Context& ctx) : size_(size), ctx_(ctx) {
+    if (size < 2) throw Error(ISO_ERR_INVALID_ARGUMENT, "size must be at least 2");
+  }
+
+  // PointCloud::extract: one vertex per mixed cell (its centre), cell order,
+  // no faces.  Program sources only here (grids: Context + the C ABI).
+  template <class S>
+  void extract(const S& source, Extractor& extractor) {
+    static_assert(is_lowerable<S>::value, "PointCloud::extract takes a lowerable source");
+    Program prog;
+    source.lower(prog);
+    ctx_.check(iso_program_upload(ctx_.get(), prog.data(), static_cast<uint32_t>(prog.size())),
+               "iso_program_upload");
+    iso_grid_desc desc;
+    ctx_.check(iso_grid_desc_reference(static_cast<int64_t>(size_), ISO_SOURCE_PROGRAM, &desc),
+               "iso_grid_desc_reference");
+    iso_mesh_out out{};
+    out.memory = ISO_MEMORY_HOST;
+    out.vertices = points_.data();
+    out.vertex_capacity = points_.size() / 3;
+    int st = iso_extract_point_cloud(ctx_.get(), &desc, &out);
+    if (st == ISO_ERR_CAPACITY) {
+      points_.resize(3 * out.vertex_count);
+      out.vertices = points_.data();
+      out.vertex_capacity = out.vertex_count;
+      st = iso_mesh_fetch(ctx_.get(), &out);
+    }
+    ctx_.check(st, "iso_extract_point_cloud");
+    extractor.extract_vertices(points_.data(), out.vertex_count);
+  }
+
+ private:
+  size_t size_;
+  Context& ctx_;
+  std::vector<float> points_;
+};
+
+// ---- HermiteSource and the normal sinks (source.rs:52-94, extractor.rs) -------
+
+// CentralDifference::new_with_epsilon(source, epsilon): lowers like its
+// source; sample_normal runs on the GPU.
+template <class S>
+struct CentralDifference {
+  S source;
+  float epsilon = 0.000001f;
+  void lower(Program& p) const { source.lower(p); }
+};
+
+// IndexedInterleavedNormals (extractor.rs:97-127): xyz then the source's
+// normal per vertex, and the indices.  OnlyInterleavedNormals
+// (extractor.rs:46-69) is the same without indices (keep_indices = false).
+template <class S>
+class IndexedInterleavedNormals : public Extractor {
+ public:
+  IndexedInterleavedNormals(std::vector<float>& vertices, std::vector<uint32_t>* indices,
+                            const CentralDifference<S>& source, Context& ctx)
+      : vertices_(vertices), indices_(indices), source_(source), ctx_(ctx) {}
+  void extract_vertex(Vec3 v) override {
+    const float xyz[3] = {v.x, v.y, v.z};
+    extract_vertices(xyz, 1);
+  }
+  void extract_index(size_t index) override {
+    if (indices_) indices_->push_back(static_cast<uint32_t>(index));
+  }
+  void extract_vertices(const float* xyz, uint64_t n) override {
+    if (n == 0) return;
+    Program prog;
+    source_.lower(prog);
+    ctx_.check(iso_program_upload(ctx_.get(), prog.data(), static_cast<uint32_t>(prog.size())),
+               "iso_program_upload");
+    std::vector<float> normals(3 * n);
+    ctx_.check(iso_sample_normals(ctx_.get(), xyz, n, source_.epsilon, normals.data(),
+                                  ISO_MEMORY_HOST),
+               "iso_sample_normals");
+    for (uint64_t v = 0; v < n; ++v) {
+      for (int a = 0; a < 3; ++a) vertices_.push_back(xyz[3 * v + a]);
+      for (int a = 0; a < 3; ++a) vertices_.push_back(normals[3 * v + a]);
+    }
+  }
+
+ private:
+  std::vector<float>& vertices_;
+  std::vector<uint32_t>* indices_;
+  CentralDifference<S> source_;
+  Context& ctx_;
 };
 
 }  // namespace isosurface

@@ -104,6 +104,74 @@ int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso
   return ISO_OK;
 }
 
+// PointCloud::extract (point_cloud.rs:50-63) driving the same traversal:
+// per cell in order, classify_corners (marching_cubes_impl.rs:26-37) and,
+// for a mixed cube, corners[0].lerp(corners[6], 0.5) (vector.rs:325-333).
+int or_extract_point_cloud(const iso_node* prog, uint32_t n, const float* grid,
+                           const iso_grid_desc* desc, or_mesh* out) {
+  std::memset(out, 0, sizeof(*out));
+  const iso_grid_desc& d = *desc;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return ISO_ERR_INVALID_ARGUMENT;
+  orc::Program t;
+  if (!grid && !orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
+  const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
+  auto value = [&](int64_t x, int64_t y, int64_t z) -> float {
+    if (grid) return grid[(z * ny + y) * nx + x];
+    return orc::sample(t, F3{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)});
+  };
+  std::vector<float> layers[2] = {std::vector<float>(nx * ny), std::vector<float>(nx * ny)};
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) layers[0][y * nx + x] = value(x, y, 0);
+  std::vector<float> pts;
+  for (int64_t z = 0; z + 1 < nz; ++z) {
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) layers[1][y * nx + x] = value(x, y, z + 1);
+    for (int64_t y = 0; y + 1 < ny; ++y) {
+      for (int64_t x = 0; x + 1 < nx; ++x) {
+        unsigned cube = 0;
+        for (int i = 0; i < 8; ++i) {
+          const float v = layers[kCorner[i][2]][(y + kCorner[i][1]) * nx + x + kCorner[i][0]];
+          if (!(v > 0.0f)) cube |= 1u << i;  // Signed::is_positive (distance.rs:52-54)
+        }
+        if (cube == 0 || cube == 255) continue;
+        const F3 c0{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
+        const F3 c6{orc::coord(d, 0, x + 1), orc::coord(d, 1, y + 1), orc::coord(d, 2, z + 1)};
+        const float f = 0.5f, of = 1.0f - f;
+        pts.push_back(of * c0.x + f * c6.x);
+        pts.push_back(of * c0.y + f * c6.y);
+        pts.push_back(of * c0.z + f * c6.z);
+      }
+    }
+    std::swap(layers[0], layers[1]);
+  }
+  out->n_vertices = pts.size() / 3;
+  out->vertices = static_cast<float*>(std::malloc((pts.size() ? pts.size() : 1) * sizeof(float)));
+  if (!out->vertices) return ISO_ERR_DEVICE;
+  std::memcpy(out->vertices, pts.data(), pts.size() * sizeof(float));
+  return ISO_OK;
+}
+
+// CentralDifference::sample_normal (source.rs:82-94), not normalised.
+int or_sample_normals(const iso_node* prog, uint32_t n, const float* points, uint64_t n_points,
+                      float epsilon, float* out) {
+  orc::Program t;
+  if (!orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
+  for (uint64_t i = 0; i < n_points; ++i) {
+    const F3 q{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    const float vx = orc::sample(t, F3{q.x + epsilon, q.y + 0.0f, q.z + 0.0f}) -
+                     orc::sample(t, F3{q.x - epsilon, q.y - 0.0f, q.z - 0.0f});
+    const float vy = orc::sample(t, F3{q.x + 0.0f, q.y + epsilon, q.z + 0.0f}) -
+                     orc::sample(t, F3{q.x - 0.0f, q.y - epsilon, q.z - 0.0f});
+    const float vz = orc::sample(t, F3{q.x + 0.0f, q.y + 0.0f, q.z + epsilon}) -
+                     orc::sample(t, F3{q.x - 0.0f, q.y - 0.0f, q.z - epsilon});
+    const float e2 = 2.0f * epsilon;
+    out[3 * i + 0] = vx / e2;
+    out[3 * i + 1] = vy / e2;
+    out[3 * i + 2] = vz / e2;
+  }
+  return ISO_OK;
+}
+
 int or_edge_points_f64(const iso_node* prog, uint32_t n, const iso_grid_desc* desc,
                        const int64_t* keys, uint64_t n_keys, double* out_xyz) {
   orc::Program t;

@@ -47,6 +47,14 @@ int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* des
 int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid,
                   const iso_grid_desc* desc, int want_cases, or_mesh* out);
 
+/* PointCloud::extract over a lattice (samples from `prog` or `grid`): the
+ * vertices are the centres of the mixed cells in cell order. */
+int or_extract_point_cloud(const iso_node* prog, uint32_t n, const float* grid,
+                           const iso_grid_desc* desc, or_mesh* out);
+/* CentralDifference::sample_normal (not normalised) of a program at points. */
+int or_sample_normals(const iso_node* prog, uint32_t n, const float* points, uint64_t n_points,
+                      float epsilon, float* out);
+
 /* f64 crossing point of lattice edges (FP64 twin): for each key
  * (x, y, z, axis) the zero of the linear interpolant between the f64 SDF
  * values at the edge's endpoints. */

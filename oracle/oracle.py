@@ -66,6 +66,12 @@ def load():
         lib.or_extract_mc.restype = C.c_int
         lib.or_extract_mc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
                                       C.POINTER(IsoGridDesc), C.c_int, C.POINTER(OrMesh)]
+        lib.or_extract_point_cloud.restype = C.c_int
+        lib.or_extract_point_cloud.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
+                                               C.POINTER(IsoGridDesc), C.POINTER(OrMesh)]
+        lib.or_sample_normals.restype = C.c_int
+        lib.or_sample_normals.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p, C.c_uint64,
+                                          C.c_float, C.c_void_p]
         lib.or_edge_points_f64.restype = C.c_int
         lib.or_edge_points_f64.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.POINTER(IsoGridDesc),
                                            C.c_void_p, C.c_uint64, C.c_void_p]
@@ -178,6 +184,39 @@ def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=Fal
     finally:
         lib.or_mesh_free(C.byref(m))
     return OracleMesh(v, i, k, c)
+
+
+def extract_point_cloud(program=None, grid=None, desc: IsoGridDesc = None) -> np.ndarray:
+    """PointCloud::extract restated on the CPU: (n, 3) float32 cell centres."""
+    lib = load()
+    arr = nodes(program) if program is not None else None
+    n = len(program) if program is not None else 0
+    gptr = None
+    if grid is not None:
+        grid = np.ascontiguousarray(grid, dtype=np.float32)
+        gptr = grid.ctypes.data
+    m = OrMesh()
+    desc = as_desc(desc)
+    st = lib.or_extract_point_cloud(arr, n, gptr, C.byref(desc), C.byref(m))
+    assert st == 0, st
+    try:
+        nv = int(m.n_vertices)
+        v = np.ctypeslib.as_array(m.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else \
+            np.zeros((0, 3), np.float32)
+    finally:
+        lib.or_mesh_free(C.byref(m))
+    return v
+
+
+def sample_normals(program, points: np.ndarray, epsilon: float) -> np.ndarray:
+    """CentralDifference::sample_normal (not normalised) at each point."""
+    lib = load()
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    out = np.empty_like(pts)
+    st = lib.or_sample_normals(nodes(program), len(program), pts.ctypes.data, len(pts),
+                               C.c_float(epsilon), out.ctypes.data)
+    assert st == 0, st
+    return out
 
 
 def edge_points_f64(program, desc: IsoGridDesc, keys: np.ndarray) -> np.ndarray:

@@ -29,3 +29,16 @@ def test_cpp_mirror_matches_oracle(tmp_path):
         assert len(v) == 3 * GOLD["vertices"] and len(i) == 3 * GOLD["triangles"]
         assert hashlib.sha256(v.tobytes()).hexdigest() == GOLD["vertices_sha256"], kind
         assert hashlib.sha256(i.tobytes()).hexdigest() == GOLD["indices_sha256"], kind
+    # PointCloud and the interleaved-normal sink through the same mirror
+    from isosurface_amd import workloads as W
+    from oracle import oracle as O
+    src = W.sphere_source(0.35)
+    pts = np.fromfile(f"{out}.points.f32", dtype="<f4").reshape(-1, 3)
+    want = O.extract_point_cloud(src.program(), desc=O.reference_desc(65, 0))
+    assert pts.tobytes() == want.tobytes()
+    nv = np.fromfile(f"{out}.normals.f32", dtype="<f4").reshape(-1, 6)
+    ni = np.fromfile(f"{out}.normals.u32", dtype="<u4")
+    v = np.fromfile(f"{out}.program.f32", dtype="<f4").reshape(-1, 3)
+    assert nv[:, :3].tobytes() == v.tobytes()
+    assert ni.tobytes() == np.fromfile(f"{out}.program.u32", dtype="<u4").tobytes()
+    assert nv[:, 3:].tobytes() == O.sample_normals(src.program(), v, 1e-4).tobytes()
