@@ -167,8 +167,19 @@ typedef struct iso_mesh_out {
   uint64_t vertex_count;    /* out */
   uint64_t index_count;     /* out */
   int32_t memory;           /* ISO_MEMORY_* of vertices / indices */
-  int32_t reserved;
+  int32_t flags;            /* ISO_MESH_* (0: none) */
 } iso_mesh_out;
+
+/* iso_mesh_out flags.  ISO_MESH_GRAPH (iso_extract_mc into device memory):
+ * the extraction's launches are captured into a HIP graph and replayed by
+ * later calls that repeat the lattice, the source (the same grid binding, or
+ * the same uploaded program) and the output buffers — one launch instead of
+ * three for callers that extract repeatedly into the same buffers.  The
+ * graph reads the grid's and the program's memory when it runs, so new
+ * samples in the same grid buffer are extracted correctly.  Ignored for
+ * host memory, while kernel timing is on, and for a context on the null
+ * stream (which cannot be captured).  Other entry points ignore the field. */
+#define ISO_MESH_GRAPH 1
 
 /* Marching cubes over the lattice: replaces MarchingCubes::extract with an
  * IndexedVertices extractor (marching_cubes.rs:59-82, extractor.rs:72-94).

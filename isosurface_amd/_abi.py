@@ -61,7 +61,7 @@ class IsoMeshOut(C.Structure):
         ("vertex_count", C.c_uint64),
         ("index_count", C.c_uint64),
         ("memory", C.c_int32),
-        ("reserved", C.c_int32),
+        ("flags", C.c_int32),
     ]
 
 
@@ -84,6 +84,7 @@ class IsoBatchOut(C.Structure):
     ]
 
 
+ISO_MESH_GRAPH = 1
 ISO_BAND_PARAMS = 15617
 ISO_BAND_CERTIFY = 1
 ISO_BAND_NO_NETWORK = 2

@@ -37,6 +37,17 @@ struct PendingTiming {
 
 }  // namespace
 
+// What a captured single-lattice extraction was captured for: every kernel
+// argument of its launches derives from these.
+struct MeshGraphKey {
+  iso::Lattice L;
+  const float* grid;
+  const void* ops;
+  const float* values;
+  int64_t n_prog;
+  iso::McBuffers B;
+};
+
 struct iso_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -85,6 +96,9 @@ struct iso_ctx {
   // captured extraction and the buffers it was captured with
   hipGraphExec_t b_graph = nullptr;
   iso::McBuffers g_buffers{};
+  // captured single-lattice extraction (ISO_MESH_GRAPH)
+  hipGraphExec_t m_graph = nullptr;
+  MeshGraphKey m_key{};
   uint64_t active_capacity = 0;
   unsigned long long* h_totals = nullptr;  // pinned
 
@@ -273,6 +287,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
                     &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge,  &c->nrm};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
+  if (c->m_graph) (void)hipGraphExecDestroy(c->m_graph);
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
   if (c->h_dc) (void)hipHostFree(c->h_dc);
@@ -548,7 +563,36 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
     B.indices = out->indices;
     B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
     B.index_capacity = out->indices ? out->index_capacity : 0;
-    for (int attempt = 0; attempt < 2; ++attempt) {
+    // Graph replay: the launches below depend only on the key's fields
+    // (sample and program memory is read when the graph runs), so a capture
+    // replays for any later call with an equal key.
+    // (The legacy null stream cannot be captured: such contexts extract eagerly.)
+    const bool graph = (out->flags & ISO_MESH_GRAPH) && !signs && !band && !positions &&
+                       !c->timing && c->stream != nullptr;
+    auto key_of = [&](const iso::McBuffers& b) {
+      MeshGraphKey k;
+      std::memset(&k, 0, sizeof(k));
+      k.L = L;
+      k.grid = grid;
+      k.ops = grid ? nullptr : ops;
+      k.values = values;
+      k.n_prog = grid ? 0 : c->n_prog;
+      k.B = b;
+      return k;
+    };
+    bool replayed = false;
+    if (graph && c->m_graph) {
+      const MeshGraphKey k = key_of(B);
+      if (std::memcmp(&k, &c->m_key, sizeof(k)) == 0) {
+        ISO_HIP(c, hipGraphLaunch(c->m_graph, c->stream));
+        if ((st = read_totals(c, B)) != ISO_OK) return st;
+        const uint64_t n_act = c->h_totals[iso::kTotActive];
+        replayed = n_act <= B.active_capacity;
+        // A longer active-cell list changes the key: extract and capture anew.
+        if (!replayed && (st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+      }
+    }
+    for (int attempt = 0; attempt < 2 && !replayed; ++attempt) {
       if ((st = classify_and_count(c, grid, L, D, B, signs, values)) != ISO_OK) return st;
       ISO_HIP(c, timed(c, "mc_emit", emit));
       if ((st = read_totals(c, B)) != ISO_OK) return st;
@@ -564,6 +608,30 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
     if (c->h_totals[iso::kTotErrors] & 1ull) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
     if (nv > B.vertex_capacity || ni > B.index_capacity)
       return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    if (graph && !replayed) {
+      if (c->m_graph) {
+        (void)hipGraphExecDestroy(c->m_graph);
+        c->m_graph = nullptr;
+      }
+      hipGraph_t g = nullptr;
+      ISO_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t le = hipSuccess;
+      st = classify_and_count(c, grid, L, D, B, nullptr, values);
+      if (st == ISO_OK) le = emit();
+      const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (st != ISO_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (le != hipSuccess && g) (void)hipGraphDestroy(g);
+      ISO_HIP(c, le);
+      ISO_HIP(c, ce);
+      const hipError_t ie = hipGraphInstantiate(&c->m_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) c->m_graph = nullptr;
+      ISO_HIP(c, ie);
+      c->m_key = key_of(B);
+    }
     return ISO_OK;
   }
 

@@ -246,3 +246,72 @@ def test_error_against_f64_oracle_is_stated(ctx):
           f"mean = {err.mean():.3e}", flush=True)
     # measured on the CPU oracle at 129: max 2.9e-6, mean 6.2e-9 (domain [0,1])
     assert err.max() < 1e-5 and err.mean() < 1e-7
+
+
+def _fetch_device_mesh(ctx, dv, di, out):
+    nv, ni = int(out.vertex_count), int(out.index_count)
+    verts = np.empty((nv, 3), np.float32)
+    idx = np.empty(ni, np.uint32)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, verts.ctypes.data, dv, verts.nbytes)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, idx.ctypes.data, di, idx.nbytes)
+    return verts, idx
+
+
+def test_graph_replay_matches_oracle_and_reads_new_samples(ctx):
+    # ISO_MESH_GRAPH: the first call captures signs + count + emit, later
+    # calls with the same grid binding and buffers replay it.  New samples
+    # written into the bound grid (eager work between replays) must be
+    # extracted, so every replay is compared with the oracle.  Eager calls
+    # in between leave the capture alone; a program source recaptures.
+    size = 65
+    gdesc = ctx.reference_desc(size, A.ISO_SOURCE_GRID)
+    pdesc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
+    n = gdesc.nx * gdesc.ny * gdesc.nz
+    sources = [W.sphere_source(), W.spheres_minus_torus()[0], W.boxes_union()]
+    refs = [O.extract_mc(s.program(), desc=O.reference_desc(size)) for s in sources]
+    vcap = max(len(r.vertices) for r in refs)
+    icap = max(len(r.indices) for r in refs)
+    dgrid, dv, di = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 4 * n, C.byref(dgrid)), "alloc")
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 12 * vcap, C.byref(dv)), "alloc")
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 4 * icap, C.byref(di)), "alloc")
+    zeros = np.zeros(max(3 * vcap, icap), np.uint32)
+
+    def clear():
+        ctx.lib.iso_memcpy_to_device(ctx.handle, dv, zeros.ctypes.data, 12 * vcap)
+        ctx.lib.iso_memcpy_to_device(ctx.handle, di, zeros.ctypes.data, 4 * icap)
+
+    def run(desc, flags, ref):
+        clear()
+        out = A.IsoMeshOut(dv, di, vcap, icap, 0, 0, A.ISO_MEMORY_DEVICE, flags)
+        ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out)), "extract")
+        verts, idx = _fetch_device_mesh(ctx, dv, di, out)
+        assert verts.tobytes() == ref.vertices.tobytes()
+        assert np.array_equal(idx, ref.indices)
+
+    try:
+        ctx.check(ctx.lib.iso_grid_bind_device(ctx.handle, dgrid, gdesc.nx, gdesc.ny, gdesc.nz),
+                  "bind")
+        for rep in range(2):
+            for s, ref in zip(sources, refs):
+                ctx.upload_program(s.program())
+                ctx.check(ctx.lib.iso_program_sample_lattice(ctx.handle, C.byref(gdesc), dgrid,
+                                                             A.ISO_MEMORY_DEVICE), "sample")
+                run(gdesc, A.ISO_MESH_GRAPH, ref)
+                run(gdesc, A.ISO_MESH_GRAPH, ref)
+                if rep == 1:
+                    run(gdesc, 0, ref)
+        for s, ref in zip(sources, refs):
+            ctx.upload_program(s.program())
+            run(pdesc, A.ISO_MESH_GRAPH, ref)
+            run(pdesc, A.ISO_MESH_GRAPH, ref)
+        # Too-small output buffers are reported (and nothing is captured
+        # for them); the full buffers then extract and replay as before.
+        small = A.IsoMeshOut(dv, di, 100, 300, 0, 0, A.ISO_MEMORY_DEVICE, A.ISO_MESH_GRAPH)
+        assert ctx.lib.iso_extract_mc(ctx.handle, C.byref(pdesc), C.byref(small)) == \
+            A.ISO_ERR_CAPACITY
+        run(pdesc, A.ISO_MESH_GRAPH, refs[-1])
+    finally:
+        ctx.upload_grid(np.ones((2, 2, 2), np.float32))  # unbind dgrid before freeing it
+        for p in (dgrid, dv, di):
+            ctx.lib.iso_device_free(ctx.handle, p)
