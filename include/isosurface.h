@@ -120,6 +120,13 @@ int iso_program_sample(iso_ctx* ctx, const float* points, uint64_t n, float* val
  * each vertex (extractor.rs:46-69,97-127). */
 int iso_sample_normals(iso_ctx* ctx, const float* points, uint64_t n, float epsilon,
                        float* normals, int32_t memory);
+/* VectorSource::sample_vector of the current program at n points (3 floats
+ * in and 3 out per point, host memory; source.rs:32-38 with the shapes'
+ * implementations, implicit/sphere.rs:41-55, torus.rs:48-91,
+ * cylinder.rs:50-71, rectangular_prism.rs:43-73, csg.rs:41-106): the signed
+ * distance along each axis, f32::MAX where that axis line misses the shape.
+ * ISO_ERR_PROGRAM if the program has a smooth union (no directed form). */
+int iso_program_sample_vector(iso_ctx* ctx, const float* points, uint64_t n, float* values);
 
 /* ---- sources: dense sample grids ---------------------------------------- */
 
@@ -129,6 +136,10 @@ int iso_grid_upload(iso_ctx* ctx, const float* values, int64_t nx, int64_t ny, i
 /* Uses caller-owned device memory as the current grid (no copy). */
 int iso_grid_bind_device(iso_ctx* ctx, const float* device_values, int64_t nx, int64_t ny,
                          int64_t nz);
+/* Copies a dense grid of directed distances (3 floats per point — the
+ * PrimalGrid<Directed> values a VectorSource gives, x fastest) from host
+ * memory; iso_extract_mc_directed of a grid source reads it. */
+int iso_grid_upload_directed(iso_ctx* ctx, const float* xyz, int64_t nx, int64_t ny, int64_t nz);
 
 /* ---- extraction ----------------------------------------------------------- */
 
@@ -185,6 +196,16 @@ typedef struct iso_mesh_out {
  * IndexedVertices extractor (marching_cubes.rs:59-82, extractor.rs:72-94).
  * Vertex order, positions and index lists are the reference's. */
 int iso_extract_mc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
+
+/* MarchingCubes::<Directed>::extract with IndexedVertices
+ * (marching_cubes.rs:59-82 over distance.rs:72-103): a lattice point is
+ * inside when no component of its directed distance is positive, and each
+ * crossing interpolates the component along the edge's axis
+ * ((p_a - p_b).abs().max_component_index()).  Values from the current
+ * program's sample_vector (ISO_SOURCE_PROGRAM; ISO_ERR_PROGRAM for a smooth
+ * union) or from iso_grid_upload_directed (ISO_SOURCE_GRID).  Same outputs,
+ * capacity and fetch rules as iso_extract_mc (ISO_MESH_GRAPH is ignored). */
+int iso_extract_mc_directed(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
 
 /* PointCloud::<Signed>::new(size).extract(source, extractor)
  * (point_cloud.rs:50-63): one point per mixed cell (cube index neither 0

@@ -8,10 +8,11 @@ Source / Extractor / MarchingCubes / DualContouring /
 LinearHashedMarchingCubes API; the C++ mirror for compiled hosts is
 isosurface_amd/host/isosurface.hpp.
 """
-from .engine import (BatchedMarchingCubes, Context, DualContouring, Extractor,
+from .engine import (BatchedMarchingCubes, Context, Directed, DualContouring, Extractor,
                      IndexedInterleavedNormals, IndexedVertices, LinearHashedMarchingCubes,
                      MarchingCubes, Mesh, MinimiseQEF, OnlyInterleavedNormals, OnlyVertices,
-                     PointCloud, chunk_desc, default_context, sample_lattice_host)
+                     PointCloud, Signed, chunk_desc, default_context, sample_lattice_host,
+                     sample_lattice_host_vector)
 from .sources import (CentralDifference, Cylinder, DemoSource, Difference, Intersection,
                       RectangularPrism,
                       SmoothUnion, Source, Sphere, Torus, Translate, Union, smooth_union_all,
@@ -20,7 +21,8 @@ from .sources import (CentralDifference, Cylinder, DemoSource, Difference, Inter
 __all__ = [
     "BatchedMarchingCubes", "chunk_desc", "CentralDifference", "DualContouring", "LinearHashedMarchingCubes", "MinimiseQEF",
     "Context", "Extractor", "IndexedVertices", "MarchingCubes", "Mesh", "OnlyVertices",
-    "IndexedInterleavedNormals", "OnlyInterleavedNormals", "PointCloud",
+    "IndexedInterleavedNormals", "OnlyInterleavedNormals", "PointCloud", "Signed", "Directed",
+    "sample_lattice_host_vector",
     "default_context", "sample_lattice_host", "Cylinder", "DemoSource", "Difference",
     "Intersection", "RectangularPrism", "SmoothUnion", "Source", "Sphere", "Torus",
     "Translate", "Union", "smooth_union_all", "to_nodes", "union_all",

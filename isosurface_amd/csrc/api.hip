@@ -57,6 +57,7 @@ struct iso_ctx {
   // current program
   DevBuf prog;
   int n_prog = 0;
+  bool prog_directed = false;  // the program has a directed form (no smooth union)
 
   // current grid
   DevBuf grid_owned;
@@ -68,6 +69,10 @@ struct iso_ctx {
   DevBuf pvals;  // program samples of the current extraction (emit reads them)
   DevBuf judge;  // dense sign words that judge a band extraction
   DevBuf nrm;    // normals of iso_sample_normals (host-memory calls)
+  // directed values: the uploaded grid, and a program's values of the
+  // current extraction (3 floats per lattice point)
+  DevBuf dgrid, dvals;
+  int64_t dgnx = 0, dgny = 0, dgnz = 0;
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
@@ -285,7 +290,8 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_v,       &c->nn_grad,    &c->nn_frags,  &c->nn_loss,  &c->nn_x,
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
-                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge,  &c->nrm};
+                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge,  &c->nrm,
+                    &c->dgrid,      &c->dvals};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   if (c->m_graph) (void)hipGraphExecDestroy(c->m_graph);
   for (DevBuf* b : bufs) release(b);
@@ -326,6 +332,9 @@ int iso_program_upload(iso_ctx* c, const iso_node* nodes, uint32_t n_nodes) {
                             hipMemcpyHostToDevice, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
   c->n_prog = static_cast<int>(ops.size());
+  c->prog_directed = true;
+  for (uint32_t i = 0; i < n_nodes; ++i)
+    if (nodes[i].op == ISO_OP_SMOOTH_UNION) c->prog_directed = false;
   c->band_valid = false;  // a band extraction is judged against its own program
   return ISO_OK;
 }
@@ -423,6 +432,20 @@ int iso_grid_upload(iso_ctx* c, const float* values, int64_t nx, int64_t ny, int
   c->gnx = nx;
   c->gny = ny;
   c->gnz = nz;
+  return ISO_OK;
+}
+
+int iso_grid_upload_directed(iso_ctx* c, const float* xyz, int64_t nx, int64_t ny, int64_t nz) {
+  if (!c || !xyz || nx < 1 || ny < 1 || nz < 1)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "bad grid upload arguments");
+  const size_t bytes = static_cast<size_t>(nx) * ny * nz * 3 * sizeof(float);
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, ensure(&c->dgrid, bytes));
+  ISO_HIP(c, hipMemcpyAsync(c->dgrid.ptr, xyz, bytes, hipMemcpyHostToDevice, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  c->dgnx = nx;
+  c->dgny = ny;
+  c->dgnz = nz;
   return ISO_OK;
 }
 
@@ -533,7 +556,7 @@ struct BandEmit {
 
 int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const float* positions,
                     iso_mesh_out* out, const SignsStage* signs = nullptr,
-                    const BandEmit* band = nullptr) {
+                    const BandEmit* band = nullptr, const float* directed = nullptr) {
   int st = ISO_OK;
   c->band_valid = false;  // the context's sign words are about to change
   const iso::McDims D = iso::mc_dims(L);
@@ -548,9 +571,11 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
   // memory for them, emit evaluates the program at the edge endpoints.  A
   // narrow band stores only its exact bricks' samples.
   float* values = nullptr;
-  if (!grid && ensure(&c->pvals, static_cast<size_t>(D.n_points) * sizeof(float)) == hipSuccess)
+  if (!grid && !directed &&
+      ensure(&c->pvals, static_cast<size_t>(D.n_points) * sizeof(float)) == hipSuccess)
     values = static_cast<float*>(c->pvals.ptr);
   auto emit = [&]() -> hipError_t {
+    if (directed) return iso::launch_emit_directed(directed, L, D, B, c->stream);
     if (band && values)
       return iso::launch_emit_band(values, band->flags, band->nbx, band->nby, ops, c->n_prog, L,
                                    D, B, c->stream);
@@ -696,6 +721,64 @@ int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
   c->kept = false;
   ISO_HIP(c, hipSetDevice(c->device));
   return extract_lattice(c, L, from_grid ? c->grid : nullptr, nullptr, out);
+}
+
+int iso_extract_mc_directed(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  const bool from_grid = desc->source == ISO_SOURCE_GRID;
+  if (from_grid) {
+    if (!c->dgrid.ptr || c->dgnx == 0) return fail(c, ISO_ERR_NO_SOURCE, "no directed grid uploaded");
+    if (c->dgnx != L.nx || c->dgny != L.ny || c->dgnz != L.nz)
+      return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice dims differ from the directed grid");
+  } else if (c->n_prog == 0) {
+    return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  } else if (!c->prog_directed) {
+    return fail(c, ISO_ERR_PROGRAM, "the program has no directed form (smooth union)");
+  }
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+  const iso::McDims D = iso::mc_dims(L);
+  float* vals = static_cast<float*>(c->dgrid.ptr);
+  if (!from_grid) {
+    ISO_HIP(c, ensure(&c->dvals, static_cast<size_t>(D.n_points) * 3 * sizeof(float)));
+    vals = static_cast<float*>(c->dvals.ptr);
+  }
+  const iso::DevOp* ops = from_grid ? nullptr : static_cast<const iso::DevOp*>(c->prog.ptr);
+  const SignsStage signs = [&](const iso::McBuffers& B, float*) -> int {
+    ISO_HIP(c, timed(c, "mc_signs_directed", [&] {
+              return iso::launch_signs_directed(ops, c->n_prog, L, D, B, vals, c->stream);
+            }));
+    return ISO_OK;
+  };
+  return extract_lattice(c, L, nullptr, nullptr, out, &signs, nullptr, vals);
+}
+
+int iso_program_sample_vector(iso_ctx* c, const float* points, uint64_t n, float* values) {
+  if (!c || (n && (!points || !values))) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null buffer");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (!c->prog_directed)
+    return fail(c, ISO_ERR_PROGRAM, "the program has no directed form (smooth union)");
+  if (n == 0) return ISO_OK;
+  ISO_HIP(c, hipSetDevice(c->device));
+  ISO_HIP(c, ensure(&c->pts, n * 3 * sizeof(float)));
+  ISO_HIP(c, ensure(&c->nrm, n * 3 * sizeof(float)));
+  ISO_HIP(c, hipMemcpyAsync(c->pts.ptr, points, n * 3 * sizeof(float), hipMemcpyHostToDevice,
+                            c->stream));
+  ISO_HIP(c, timed(c, "sample_vector_points", [&] {
+            return iso::launch_sample_vector_points(static_cast<const iso::DevOp*>(c->prog.ptr),
+                                                    c->n_prog, static_cast<const float*>(c->pts.ptr),
+                                                    n, static_cast<float*>(c->nrm.ptr), c->stream);
+          }));
+  ISO_HIP(c, hipMemcpyAsync(values, c->nrm.ptr, n * 3 * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return ISO_OK;
 }
 
 int iso_extract_point_cloud(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {

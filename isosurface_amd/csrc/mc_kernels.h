@@ -103,6 +103,17 @@ hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, h
 // interpolate between those positions instead of lattice coordinates.
 hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const float* positions,
                        const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s);
+// MarchingCubes::<Directed>: sign bits from directed values (3 floats per
+// lattice point, x fastest) — evaluated from `ops` and stored into `values`,
+// or, with ops null, read from `values` — and emit interpolating along each
+// edge's axis component.
+hipError_t launch_signs_directed(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
+                                 const McBuffers& B, float* values, hipStream_t s);
+hipError_t launch_emit_directed(const float* values, const Lattice& L, const McDims& D,
+                                const McBuffers& B, hipStream_t s);
+// Directed distances of a program at points (3 floats in and out per point).
+hipError_t launch_sample_vector_points(const DevOp* ops, int n_ops, const float* pts, uint64_t n,
+                                       float* out, hipStream_t s);
 // PointCloud points (3 floats each, up to `capacity`) of the mixed cells in
 // the active list, after launch_count.
 hipError_t launch_point_cloud(const Lattice& L, const McDims& D, const McBuffers& B,

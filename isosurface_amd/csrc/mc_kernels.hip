@@ -111,6 +111,49 @@ __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__
   }
 }
 
+// Directed signs (MarchingCubes::<Directed>): a lattice point is inside when
+// no component of its directed distance is positive (Directed::is_positive,
+// distance.rs:76-79).  From a program (`ops` set: the values are evaluated
+// and stored, 3 floats per point, for emit) or from stored values.
+__global__ __launch_bounds__(256) void k_signs_directed(const DevOp* __restrict__ ops, int n_ops,
+                                                        Lattice L, int64_t n_points,
+                                                        int64_t n_words,
+                                                        uint64_t* __restrict__ S,
+                                                        float* __restrict__ values, ResetArgs R) {
+  reset_state(R);
+  const int lane = __lane_id();
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
+  const uint32_t nx = static_cast<uint32_t>(L.nx);
+  const uint32_t nxy = static_cast<uint32_t>(L.nx * L.ny);
+  for (int64_t w = wave; w < n_words; w += n_waves) {
+    const int64_t f = w * 64 + lane;
+    bool inside = false;
+    if (f < n_points) {
+      float v[3];
+      if (ops) {
+        const uint32_t ff = static_cast<uint32_t>(f);
+        const uint32_t z = ff / nxy;
+        const uint32_t r = ff - z * nxy;
+        const uint32_t y = r / nx;
+        const uint32_t x = r - y * nx;
+        sdf_eval_vector(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
+                        lattice_coord(L, 2, z), v);
+        values[3 * f + 0] = v[0];
+        values[3 * f + 1] = v[1];
+        values[3 * f + 2] = v[2];
+      } else {
+        v[0] = values[3 * f + 0];
+        v[1] = values[3 * f + 1];
+        v[2] = values[3 * f + 2];
+      }
+      inside = !(v[0] > 0.0f || v[1] > 0.0f || v[2] > 0.0f);
+    }
+    const uint64_t m = __ballot(inside);
+    if (lane == 0) S[w] = m;
+  }
+}
+
 // Batched chunks: one wavefront per sign word of the concatenated word
 // numbering; the word's chunk is found by binary search, so a wavefront
 // always evaluates a single chunk's program.  Thread i < n_chunks also
@@ -421,6 +464,8 @@ enum EmitSource : int {
   kEmitPositions = 2,  // values from a dense grid, positions per lattice point
   kEmitBand = 3,       // values stored for the exact bricks of a narrow band, the
                        // SDF program elsewhere; positions from the lattice
+  kEmitDirected = 4,   // directed values (3 floats per point, in `grid`),
+                       // positions from the lattice
 };
 
 // Lattice coordinate i (< 2^31) along an axis: the same float as a 64-bit
@@ -537,7 +582,7 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     // A band cell's corners all lie in its brick's closed sample box, whose
     // samples were evaluated exactly and stored; a culled brick's cells are
     // only active when a predicted sign was wrong, and evaluate the program.
-    bool stored = kSrc != kEmitProgram;
+    bool stored = kSrc != kEmitProgram && kSrc != kEmitDirected;
     if (kSrc == kEmitBand)
       stored = (A.brick_flags[((cz >> kBandBrickShift) * A.nby + (cy >> kBandBrickShift)) * A.nbx +
                               (cx >> kBandBrickShift)] &
@@ -653,11 +698,23 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
                          tables::kCorner[u][0];
       const int64_t fv = fc + tables::kCorner[v][2] * nxy + tables::kCorner[v][1] * nx +
                          tables::kCorner[v][0];
-      const float va = stored ? cu : space_value<kSrc>(C, ux, uy, uz, fu);
-      const float vb = stored ? cvv : space_value<kSrc>(C, vx, vy, vz, fv);
       float pa[3], pb[3], p[3];
       space_position<kSrc>(C, ux, uy, uz, fu, pa);
       space_position<kSrc>(C, vx, vy, vz, fv, pb);
+      float va, vb;
+      if (kSrc == kEmitDirected) {
+        // Directed::find_crossing_point (distance.rs:89-102): the component
+        // of the axis (p_a - p_b).abs().max_component_index() (vector.rs:
+        // 255-263) — the edge's own axis on any non-degenerate lattice.
+        const float dx = __builtin_fabsf(pa[0] - pb[0]), dy = __builtin_fabsf(pa[1] - pb[1]),
+                    dz = __builtin_fabsf(pa[2] - pb[2]);
+        const int axis = (dx > dy && dx > dz) ? 0 : (dy > dz ? 1 : 2);
+        va = C.grid[3 * fu + axis];
+        vb = C.grid[3 * fv + axis];
+      } else {
+        va = stored ? cu : space_value<kSrc>(C, ux, uy, uz, fu);
+        vb = stored ? cvv : space_value<kSrc>(C, vx, vy, vz, fv);
+      }
       crossing(va, vb, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], p);
       if (id < A.vcap) {
         A.vertices[3 * static_cast<uint64_t>(id) + 0] = p[0];
@@ -782,6 +839,15 @@ __global__ __launch_bounds__(256) void k_sample_lattice(const DevOp* __restrict_
     out[f] = sdf_eval(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
                       lattice_coord(L, 2, z));
   }
+}
+
+__global__ __launch_bounds__(256) void k_sample_vector_points(const DevOp* __restrict__ ops,
+                                                              int n_ops,
+                                                              const float* __restrict__ pts,
+                                                              uint64_t n,
+                                                              float* __restrict__ out) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) sdf_eval_vector(ops, n_ops, pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], out + 3 * i);
 }
 
 __global__ __launch_bounds__(256) void k_sample_points(const DevOp* __restrict__ ops, int n_ops,
@@ -942,6 +1008,27 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const flo
   return hipGetLastError();
 }
 
+hipError_t launch_signs_directed(const DevOp* ops, int n_ops, const Lattice& L, const McDims& D,
+                                 const McBuffers& B, float* values, hipStream_t s) {
+  if (!values) return hipErrorInvalidValue;
+  int64_t blocks = grid_for(D.n_sign_words);
+  const int64_t min_blocks = (D.n_blocks + 255) / 256;
+  if (blocks < min_blocks) blocks = min_blocks;
+  hipLaunchKernelGGL(k_signs_directed, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
+                     n_ops, L, D.n_points, D.n_sign_words, B.signs, values, reset_args(D, B));
+  return hipGetLastError();
+}
+
+hipError_t launch_emit_directed(const float* values, const Lattice& L, const McDims& D,
+                                const McBuffers& B, hipStream_t s) {
+  if (!values) return hipErrorInvalidValue;
+  const EmitArgs A = emit_args(B);
+  const SingleSpace sp = single_space(L, D, B.signs, nullptr, 0, values, nullptr);
+  auto k = k_emit<kEmitDirected, SingleSpace>;
+  hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
+  return hipGetLastError();
+}
+
 hipError_t launch_point_cloud(const Lattice& L, const McDims& D, const McBuffers& B,
                               float* points, uint64_t capacity, hipStream_t s) {
   if (!points || capacity == 0) return hipSuccess;
@@ -1045,6 +1132,15 @@ hipError_t launch_sample_lattice(const DevOp* ops, int n_ops, const Lattice& L, 
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_sample_lattice, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
                      n_ops, L, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_vector_points(const DevOp* ops, int n_ops, const float* pts, uint64_t n,
+                                       float* out, hipStream_t s) {
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sample_vector_points, dim3(blocks), dim3(256), 0, s, ops, n_ops, pts, n,
+                     out);
   return hipGetLastError();
 }
 

@@ -120,4 +120,146 @@ __device__ __forceinline__ float sdf_eval(const DevOp* __restrict__ ops, int n, 
   return top;
 }
 
+// ---------------------------------------------------------------------------
+// Directed distances: VectorSource::sample_vector (source.rs:32-38), the
+// signed distance to the surface along each cardinal axis, or f32::MAX where
+// the axis line through the point misses the shape.  Each restates the
+// reference's f32 operation order; a = |p| componentwise.
+// ---------------------------------------------------------------------------
+constexpr float kDirectedMiss = 3.40282347e+38f;  // f32::MAX
+
+__device__ __forceinline__ void prim_vector(uint32_t code, const float* prm, float x, float y,
+                                            float z, float* out) {
+  const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y), az = __builtin_fabsf(z);
+  if (code == kSphere) {  // implicit/sphere.rs:41-55
+    const float r2 = prm[0] * prm[0];
+    const float l_yz = r2 - (ay * ay + az * az);
+    const float l_xz = r2 - (ax * ax + az * az);
+    const float l_xy = r2 - (ax * ax + ay * ay);
+    out[0] = l_yz < 0.0f ? kDirectedMiss : ax - __builtin_sqrtf(l_yz);
+    out[1] = l_xz < 0.0f ? kDirectedMiss : ay - __builtin_sqrtf(l_xz);
+    out[2] = l_xy < 0.0f ? kDirectedMiss : az - __builtin_sqrtf(l_xy);
+  } else if (code == kTorus) {  // implicit/torus.rs:48-91
+    const float R = prm[0], r = prm[1];
+    const float l = __builtin_sqrtf(ax * ax + ay * ay);
+    const float l_xy = l - R;
+    const float tz = __builtin_sqrtf(r * r - az * az);
+    const float rx = R + tz, ry = R - tz;
+    if (az > r || ay > R + tz)
+      out[0] = kDirectedMiss;
+    else if (ax == 0.0f)
+      out[0] = __builtin_fabsf(ay - R) - r;
+    else
+      out[0] = fmaxf(ax - __builtin_sqrtf(rx * rx - ay * ay), __builtin_sqrtf(ry * ry - ay * ay) - ax);
+    if (az > r || ax > R + tz)
+      out[1] = kDirectedMiss;
+    else if (ay == 0.0f)
+      out[1] = __builtin_fabsf(ax - R) - r;
+    else
+      out[1] = fmaxf(ay - __builtin_sqrtf(rx * rx - ax * ax), __builtin_sqrtf(ry * ry - ax * ax) - ay);
+    out[2] = __builtin_fabsf(l_xy) > r ? kDirectedMiss : az - __builtin_sqrtf(r * r - l_xy * l_xy);
+  } else if (code == kBox) {  // implicit/rectangular_prism.rs:43-73
+    const float hx = prm[0], hy = prm[1], hz = prm[2];
+    const float mx = ((ay - hy) > 0.0f || (az - hz) > 0.0f) ? kDirectedMiss : -kDirectedMiss;
+    const float my = ((ax - hx) > 0.0f || (az - hz) > 0.0f) ? kDirectedMiss : -kDirectedMiss;
+    const float mz = ((ax - hx) > 0.0f || (ay - hy) > 0.0f) ? kDirectedMiss : -kDirectedMiss;
+    const bool in = ax < hx && ay < hy && az < hz;
+    const float cx = in ? fmaxf(ax, hx) : fminf(ax, hx);
+    const float cy = in ? fmaxf(ay, hy) : fminf(ay, hy);
+    const float cz = in ? fmaxf(az, hz) : fminf(az, hz);
+    out[0] = fmaxf(ax - cx, mx);
+    out[1] = fmaxf(ay - cy, my);
+    out[2] = fmaxf(az - cz, mz);
+  } else {  // kCylinder, implicit/cylinder.rs:50-71
+    const float r = prm[0], h = prm[1];
+    out[0] = (az > h || ay > r) ? kDirectedMiss : ax - __builtin_sqrtf(r * r - ay * ay);
+    out[1] = (az > h || ax > r) ? kDirectedMiss : ay - __builtin_sqrtf(r * r - ax * ax);
+    out[2] = (ax * ax + ay * ay) > r * r ? kDirectedMiss : az - h;
+  }
+}
+
+// sdf_eval's stack machine over Directed values: the CSG operators act
+// componentwise (csg.rs:41-45, 74-78, 102-106: min, max, and b.max(-a)).
+// Programs with a smooth union have no directed form; the host rejects them
+// before launch.
+__device__ __forceinline__ void sdf_eval_vector(const DevOp* __restrict__ ops, int n, float px,
+                                                float py, float pz, float* out) {
+  float top[3] = {0.0f, 0.0f, 0.0f};
+  float st[kMaxValueStack][3];
+  int sp = 0;
+  float cx = px, cy = py, cz = pz;
+  float sx[kMaxPointStack], sy[kMaxPointStack], sz[kMaxPointStack];
+  int psp = 0;
+  bool have_top = false;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t code = ops[i].code;
+    if (code <= kCylinder) {
+      float qx = cx, qy = cy, qz = cz;
+      if (ops[i].translated) {
+        qx = cx - ops[i].off[0];
+        qy = cy - ops[i].off[1];
+        qz = cz - ops[i].off[2];
+      }
+      float v[3];
+      prim_vector(code, ops[i].prm, qx, qy, qz, v);
+      const uint32_t mode = ops[i].combine;
+      if (mode == kPush) {
+        if (have_top) {
+          st[sp][0] = top[0];
+          st[sp][1] = top[1];
+          st[sp][2] = top[2];
+          ++sp;
+        }
+        top[0] = v[0];
+        top[1] = v[1];
+        top[2] = v[2];
+        have_top = true;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (mode == kMin)
+            top[k] = fminf(top[k], v[k]);
+          else if (mode == kMax)
+            top[k] = fmaxf(top[k], v[k]);
+          else if (mode == kDiff)
+            top[k] = fmaxf(v[k], -top[k]);
+          else  // kDiffRev
+            top[k] = fmaxf(top[k], -v[k]);
+        }
+      }
+    } else if (code == kPushTranslate) {
+      sx[psp] = cx;
+      sy[psp] = cy;
+      sz[psp] = cz;
+      ++psp;
+      cx = cx - ops[i].off[0];
+      cy = cy - ops[i].off[1];
+      cz = cz - ops[i].off[2];
+    } else if (code == kPopTranslate) {
+      --psp;
+      cx = sx[psp];
+      cy = sy[psp];
+      cz = sz[psp];
+    } else {
+      --sp;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float t = top[k];
+        const float u = st[sp][k];
+        if (code == kUnion)
+          top[k] = fminf(u, t);
+        else if (code == kIntersection)
+          top[k] = fmaxf(u, t);
+        else if (code == kDifference)  // a = u, b = t
+          top[k] = fmaxf(t, -u);
+        else  // kDifferenceRev: b = u, a = t
+          top[k] = fmaxf(u, -t);
+      }
+    }
+  }
+  out[0] = top[0];
+  out[1] = top[1];
+  out[2] = top[2];
+}
+
 }  // namespace iso

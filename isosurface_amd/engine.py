@@ -194,6 +194,24 @@ class Context:
                                                out.ctypes.data), "iso_program_sample")
         return out
 
+    def sample_vector(self, source: Source, points: np.ndarray) -> np.ndarray:
+        """VectorSource::sample_vector at each point, on the GPU: (n, 3) float32."""
+        self.upload_program(source.program())
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty_like(pts)
+        self.check(self.lib.iso_program_sample_vector(self.handle, pts.ctypes.data, len(pts),
+                                                      out.ctypes.data), "iso_program_sample_vector")
+        return out
+
+    def upload_grid_directed(self, values: np.ndarray) -> None:
+        """Directed values (nz, ny, nx, 3) for iso_extract_mc_directed of a grid."""
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        nz, ny, nx, three = values.shape
+        if three != 3:
+            raise ValueError("directed grids hold 3 floats per point")
+        self.check(self.lib.iso_grid_upload_directed(self.handle, values.ctypes.data, nx, ny, nz),
+                   "iso_grid_upload_directed")
+
     def sample_normals(self, source: CentralDifference, points: np.ndarray) -> np.ndarray:
         """CentralDifference::sample_normal at each point (not normalised), on the GPU."""
         self.upload_program(source.program())
@@ -225,15 +243,17 @@ class Context:
         return d
 
     def extract_mc(self, desc: A.IsoGridDesc, vertex_capacity: int = 0,
-                   index_capacity: int = 0) -> Mesh:
-        """iso_extract_mc into host arrays (grown on ISO_ERR_CAPACITY)."""
+                   index_capacity: int = 0, directed: bool = False) -> Mesh:
+        """iso_extract_mc (iso_extract_mc_directed with directed=True) into
+        host arrays (grown on ISO_ERR_CAPACITY)."""
         vcap = max(int(vertex_capacity), 1)
         icap = max(int(index_capacity), 3)
         verts = np.empty((vcap, 3), dtype=np.float32)
         idx = np.empty(icap, dtype=np.uint32)
         out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
                            A.ISO_MEMORY_HOST, 0)
-        st = self.lib.iso_extract_mc(self.handle, C.byref(desc), C.byref(out))
+        fn = self.lib.iso_extract_mc_directed if directed else self.lib.iso_extract_mc
+        st = fn(self.handle, C.byref(desc), C.byref(out))
         if st == A.ISO_ERR_CAPACITY:
             nv, ni = int(out.vertex_count), int(out.index_count)
             verts = np.empty((max(nv, 1), 3), dtype=np.float32)
@@ -378,14 +398,44 @@ def sample_lattice_host(source, desc: A.IsoGridDesc) -> np.ndarray:
                                                                          desc.nx))
 
 
-class MarchingCubes:
-    """MarchingCubes::<Signed>::new(size) (marching_cubes.rs:46-50) on the GPU."""
+def sample_lattice_host_vector(source, desc: A.IsoGridDesc) -> np.ndarray:
+    """Samples an arbitrary host VectorSource (`sample_vector(x, y, z)` over
+    numpy arrays, returning the three components) over the lattice:
+    (nz, ny, nx, 3) float32, the reference's coordinates."""
+    def axis(n, o, s):
+        return (np.float32(o) + np.arange(n, dtype=np.float32) * np.float32(s)).astype(np.float32)
 
-    def __init__(self, size: int, context: Optional[Context] = None):
+    xs = axis(desc.nx, desc.origin[0], desc.spacing[0])
+    ys = axis(desc.ny, desc.origin[1], desc.spacing[1])
+    zs = axis(desc.nz, desc.origin[2], desc.spacing[2])
+    z, y, x = np.meshgrid(zs, ys, xs, indexing="ij")
+    vx, vy, vz = source.sample_vector(x, y, z)
+    v = np.stack([np.broadcast_to(np.asarray(c, dtype=np.float32), x.shape) for c in (vx, vy, vz)],
+                 axis=-1)
+    return np.ascontiguousarray(v.reshape(desc.nz, desc.ny, desc.nx, 3))
+
+
+class Signed:
+    """The scalar distance (distance.rs:40-70): MarchingCubes(size) default."""
+
+
+class Directed:
+    """The per-axis distance (distance.rs:43-103): MarchingCubes(size,
+    distance=Directed) samples sources with sample_vector."""
+
+
+class MarchingCubes:
+    """MarchingCubes::<D>::new(size) (marching_cubes.rs:38-50) on the GPU,
+    D = Signed (default) or Directed."""
+
+    def __init__(self, size: int, context: Optional[Context] = None, distance=Signed):
         if size < 2:
             raise ValueError("size must be at least 2")
+        if distance not in (Signed, Directed):
+            raise TypeError("distance must be Signed or Directed")
         self.size = int(size)
         self.ctx = context or default_context()
+        self.directed = distance is Directed
         self._last_counts = (0, 0)
 
     def extract_mesh(self, source) -> Mesh:
@@ -394,8 +444,11 @@ class MarchingCubes:
             desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_PROGRAM)
         else:
             desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_GRID)
-            self.ctx.upload_grid(sample_lattice_host(source, desc))
-        mesh = self.ctx.extract_mc(desc, *self._last_counts)
+            if self.directed:
+                self.ctx.upload_grid_directed(sample_lattice_host_vector(source, desc))
+            else:
+                self.ctx.upload_grid(sample_lattice_host(source, desc))
+        mesh = self.ctx.extract_mc(desc, *self._last_counts, directed=self.directed)
         self._last_counts = (len(mesh.vertices), len(mesh.indices))
         return mesh
 

@@ -3,7 +3,8 @@
 // and, for a source the GPU cannot evaluate, the same sphere written as a
 // host lambda (sampled on the host, uploaded, extracted on the GPU).
 // Writes the meshes as raw little-endian arrays for the test suite:
-//   <out>.program.f32 / .program.u32 and <out>.host.f32 / .host.u32
+//   <out>.program.f32 / .program.u32 and <out>.host.f32 / .host.u32, and the
+//   MarchingCubes::<Directed> meshes <out>.directed.* / .directed_host.*
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -19,6 +20,18 @@ struct HostSphere {  // a user source: Sphere(0.35) behind DemoSource, on the ho
   isosurface::Signed sample_scalar(isosurface::Vec3 p) const {
     const float x = p.x - 0.5f, y = p.y - 0.5f, z = p.z - 0.5f;
     return {std::sqrt((x * x + y * y) + z * z) - 0.35f};
+  }
+};
+
+struct HostDirectedSphere {  // the same sphere's VectorSource form (sphere.rs:41-55)
+  isosurface::Directed sample_vector(isosurface::Vec3 p) const {
+    const float ax = std::fabs(p.x - 0.5f), ay = std::fabs(p.y - 0.5f), az = std::fabs(p.z - 0.5f);
+    const float r2 = 0.35f * 0.35f;
+    const float l_yz = r2 - (ay * ay + az * az), l_xz = r2 - (ax * ax + az * az),
+                l_xy = r2 - (ax * ax + ay * ay);
+    const float big = 3.40282347e+38f;
+    return {{l_yz < 0.0f ? big : ax - std::sqrt(l_yz), l_xz < 0.0f ? big : ay - std::sqrt(l_xz),
+             l_xy < 0.0f ? big : az - std::sqrt(l_xy)}};
   }
 };
 
@@ -70,6 +83,22 @@ int main(int argc, char** argv) {
     dump(out + ".normals.f32", nv);
     dump(out + ".normals.u32", ni);
     std::printf("normal sink: %zu vertices\n", nv.size() / 6);
+
+    // MarchingCubes::<Directed>::new(65), for the lowered and the host source.
+    isosurface::MarchingCubes<isosurface::Directed> dmc(65, ctx);
+    std::vector<float> dv;
+    std::vector<uint32_t> di;
+    isosurface::IndexedVertices dsink(dv, di);
+    dmc.extract(isosurface::DemoSource(isosurface::Sphere{0.35f}), dsink);
+    dump(out + ".directed.f32", dv);
+    dump(out + ".directed.u32", di);
+    std::vector<float> dhv;
+    std::vector<uint32_t> dhi;
+    isosurface::IndexedVertices dhsink(dhv, dhi);
+    dmc.extract(HostDirectedSphere{}, dhsink);
+    dump(out + ".directed_host.f32", dhv);
+    dump(out + ".directed_host.u32", dhi);
+    std::printf("directed: %zu vertices, %zu triangles\n", dv.size() / 3, di.size() / 3);
 
     // Error channel: a malformed program surfaces as isosurface::Error.
     try {

@@ -43,6 +43,12 @@ struct Signed {
   float value;
 };
 
+// Distance along each cardinal axis (distance.rs:43-45): what a VectorSource
+// returns from sample_vector.
+struct Directed {
+  Vec3 value;
+};
+
 class Error : public std::runtime_error {
  public:
   Error(int status, const std::string& what) : std::runtime_error(what), status_(status) {}
@@ -240,9 +246,14 @@ class Context {
 
 // ---- MarchingCubes ------------------------------------------------------------
 
+template <class D = Signed>
 class MarchingCubes {
+  static_assert(std::is_same_v<D, Signed> || std::is_same_v<D, Directed>,
+                "MarchingCubes<D>: D is Signed or Directed");
+  static constexpr bool kDirected = std::is_same_v<D, Directed>;
+
  public:
-  // MarchingCubes::<Signed>::new(size) (marching_cubes.rs:46-50).
+  // MarchingCubes::<D>::new(size) (marching_cubes.rs:38-50).
   MarchingCubes(size_t size, Context& ctx) : size_(size), ctx_(ctx) {
     if (size < 2) throw Error(ISO_ERR_INVALID_ARGUMENT, "size must be at least 2");
   }
@@ -261,7 +272,8 @@ class MarchingCubes {
     } else {
       ctx_.check(iso_grid_desc_reference(static_cast<int64_t>(size_), ISO_SOURCE_GRID, &desc),
                  "iso_grid_desc_reference");
-      std::vector<float> grid(static_cast<size_t>(desc.nx * desc.ny * desc.nz));
+      const size_t per = kDirected ? 3 : 1;
+      std::vector<float> grid(static_cast<size_t>(desc.nx * desc.ny * desc.nz) * per);
       size_t i = 0;
       for (int64_t z = 0; z < desc.nz; ++z)
         for (int64_t y = 0; y < desc.ny; ++y)
@@ -269,10 +281,21 @@ class MarchingCubes {
             const Vec3 p{desc.origin[0] + static_cast<float>(x) * desc.spacing[0],
                          desc.origin[1] + static_cast<float>(y) * desc.spacing[1],
                          desc.origin[2] + static_cast<float>(z) * desc.spacing[2]};
-            grid[i++] = source.sample_scalar(p).value;
+            if constexpr (kDirected) {  // Sample<Directed> is sample_vector (sampler.rs:43-47)
+              const Vec3 v = source.sample_vector(p).value;
+              grid[i++] = v.x;
+              grid[i++] = v.y;
+              grid[i++] = v.z;
+            } else {
+              grid[i++] = source.sample_scalar(p).value;
+            }
           }
-      ctx_.check(iso_grid_upload(ctx_.get(), grid.data(), desc.nx, desc.ny, desc.nz),
-                 "iso_grid_upload");
+      if constexpr (kDirected)
+        ctx_.check(iso_grid_upload_directed(ctx_.get(), grid.data(), desc.nx, desc.ny, desc.nz),
+                   "iso_grid_upload_directed");
+      else
+        ctx_.check(iso_grid_upload(ctx_.get(), grid.data(), desc.nx, desc.ny, desc.nz),
+                   "iso_grid_upload");
     }
     run(desc, extractor);
   }
@@ -285,7 +308,8 @@ class MarchingCubes {
     out.indices = indices_.data();
     out.vertex_capacity = vertices_.size() / 3;
     out.index_capacity = indices_.size();
-    int st = iso_extract_mc(ctx_.get(), &desc, &out);
+    int st = kDirected ? iso_extract_mc_directed(ctx_.get(), &desc, &out)
+                       : iso_extract_mc(ctx_.get(), &desc, &out);
     if (st == ISO_ERR_CAPACITY) {
       vertices_.resize(3 * out.vertex_count);
       indices_.resize(out.index_count);

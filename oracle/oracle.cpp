@@ -49,26 +49,24 @@ int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* des
   return ISO_OK;
 }
 
-// MarchingCubes::extract (marching_cubes.rs:59-82) driving
-// PrimalGrid::traverse (traversal/primal_grid.rs:39-89).
-int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso_grid_desc* desc,
-                  int want_cases, or_mesh* out) {
-  std::memset(out, 0, sizeof(*out));
-  const iso_grid_desc& d = *desc;
-  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return ISO_ERR_INVALID_ARGUMENT;
-  orc::Program t;
-  if (!grid && !orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
+}  // extern "C"
 
+namespace {
+
+// MarchingCubes<D>::extract (marching_cubes.rs:59-82) driving
+// PrimalGrid::traverse (traversal/primal_grid.rs:39-89); value(x, y, z, p)
+// is the Sample<D> call at lattice point (x, y, z), position p.
+template <class D, class ValueAt>
+void extract_mc_t(const iso_grid_desc& d, ValueAt value, int want_cases, or_mesh* out) {
   const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
   struct Sample {
     F3 p;
-    float v;
+    typename D::Value v;
   };
   std::vector<Sample> layers[2] = {std::vector<Sample>(nx * ny), std::vector<Sample>(nx * ny)};
   auto sample = [&](int64_t x, int64_t y, int64_t z) -> Sample {
     const F3 p{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
-    const float v = grid ? grid[(z * ny + y) * nx + x] : orc::sample(t, p);
-    return {p, v};
+    return {p, value(x, y, z, p)};
   };
 
   orc::MeshBuilder mb;
@@ -81,7 +79,7 @@ int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso
 
   orc::Pt keys[8];
   F3 corners[8];
-  float values[8];
+  typename D::Value values[8];
   for (int64_t z = 0; z + 1 < nz; ++z) {
     // Cache layer z+1 (primal_grid.rs:60-70)
     for (int64_t y = 0; y < ny; ++y)
@@ -95,12 +93,62 @@ int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso
           corners[i] = s.p;
           values[i] = s.v;
         }
-        orc::march(mb, keys, corners, values);  // marching_cubes.rs:66-79
+        orc::march<D>(mb, keys, corners, values);  // marching_cubes.rs:66-79
       }
     }
     std::swap(layers[0], layers[1]);
   }
   orc::finish(mb, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid, const iso_grid_desc* desc,
+                  int want_cases, or_mesh* out) {
+  std::memset(out, 0, sizeof(*out));
+  const iso_grid_desc& d = *desc;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return ISO_ERR_INVALID_ARGUMENT;
+  orc::Program t;
+  if (!grid && !orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
+  extract_mc_t<orc::SignedD>(
+      d,
+      [&](int64_t x, int64_t y, int64_t z, F3 p) {
+        return grid ? grid[(z * d.ny + y) * d.nx + x] : orc::sample(t, p);
+      },
+      want_cases, out);
+  return ISO_OK;
+}
+
+// MarchingCubes::<Directed>::extract: Sampler<S: VectorSource> yields
+// sample_vector (sampler.rs:43-47); grid3 holds 3 floats per lattice point.
+int or_extract_mc_directed(const iso_node* prog, uint32_t n, const float* grid3,
+                           const iso_grid_desc* desc, int want_cases, or_mesh* out) {
+  std::memset(out, 0, sizeof(*out));
+  const iso_grid_desc& d = *desc;
+  if (d.nx < 2 || d.ny < 2 || d.nz < 2) return ISO_ERR_INVALID_ARGUMENT;
+  orc::Program t;
+  if (!grid3 && (!orc::load_program(prog, n, &t) || !orc::has_vector_form(t)))
+    return ISO_ERR_PROGRAM;
+  extract_mc_t<orc::DirectedD>(
+      d,
+      [&](int64_t x, int64_t y, int64_t z, F3 p) {
+        if (!grid3) return orc::sample_vector(t, p);
+        const float* g = grid3 + 3 * ((z * d.ny + y) * d.nx + x);
+        return F3{g[0], g[1], g[2]};
+      },
+      want_cases, out);
+  return ISO_OK;
+}
+
+int or_sample_vector(const iso_node* prog, uint32_t n, float x, float y, float z, float* out) {
+  orc::Program t;
+  if (!orc::load_program(prog, n, &t) || !orc::has_vector_form(t)) return ISO_ERR_PROGRAM;
+  const F3 v = orc::sample_vector(t, F3{x, y, z});
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
   return ISO_OK;
 }
 

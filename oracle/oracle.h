@@ -47,6 +47,15 @@ int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* des
 int or_extract_mc(const iso_node* prog, uint32_t n, const float* grid,
                   const iso_grid_desc* desc, int want_cases, or_mesh* out);
 
+/* Directed distance of a program at one point (VectorSource::sample_vector);
+ * ISO_ERR_PROGRAM for a program with no directed form (smooth union). */
+int or_sample_vector(const iso_node* prog, uint32_t n, float x, float y, float z, float* out);
+/* MarchingCubes::<Directed>::extract with IndexedVertices: values from
+ * `prog`'s sample_vector (grid3 == NULL) or from grid3 (3 floats per
+ * lattice point, x fastest). */
+int or_extract_mc_directed(const iso_node* prog, uint32_t n, const float* grid3,
+                           const iso_grid_desc* desc, int want_cases, or_mesh* out);
+
 /* PointCloud::extract over a lattice (samples from `prog` or `grid`): the
  * vertices are the centres of the mixed cells in cell order. */
 int or_extract_point_cloud(const iso_node* prog, uint32_t n, const float* grid,

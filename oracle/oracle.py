@@ -67,6 +67,9 @@ def load():
         lib.or_extract_mc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
                                       C.POINTER(IsoGridDesc), C.c_int, C.POINTER(OrMesh)]
         lib.or_extract_point_cloud.restype = C.c_int
+        lib.or_extract_mc_directed.argtypes = lib.or_extract_mc.argtypes
+        lib.or_sample_vector.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_float, C.c_float,
+                                         C.c_float, C.c_void_p]
         lib.or_extract_point_cloud.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
                                                C.POINTER(IsoGridDesc), C.POINTER(OrMesh)]
         lib.or_sample_normals.restype = C.c_int
@@ -158,8 +161,10 @@ class OracleMesh:
         self.cases = cases        # per-cell cube index (or None)
 
 
-def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=False) -> OracleMesh:
-    """MarchingCubes::extract + IndexedVertices, restated on the CPU."""
+def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=False,
+               directed=False) -> OracleMesh:
+    """MarchingCubes::extract + IndexedVertices, restated on the CPU; with
+    directed=True, MarchingCubes::<Directed> (grid: 3 floats per point)."""
     lib = load()
     arr = nodes(program) if program is not None else None
     n = len(program) if program is not None else 0
@@ -169,8 +174,10 @@ def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=Fal
         gptr = grid.ctypes.data
     m = OrMesh()
     desc = as_desc(desc)
-    st = lib.or_extract_mc(arr, n, gptr, C.byref(desc), 1 if want_cases else 0, C.byref(m))
-    assert st == 0, st
+    fn = lib.or_extract_mc_directed if directed else lib.or_extract_mc
+    st = fn(arr, n, gptr, C.byref(desc), 1 if want_cases else 0, C.byref(m))
+    if st != 0:
+        raise ValueError(f"oracle extraction failed: status {st}")
     try:
         nv, ni = int(m.n_vertices), int(m.n_indices)
         v = np.ctypeslib.as_array(m.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else \
@@ -184,6 +191,15 @@ def extract_mc(program=None, grid=None, desc: IsoGridDesc = None, want_cases=Fal
     finally:
         lib.or_mesh_free(C.byref(m))
     return OracleMesh(v, i, k, c)
+
+
+def sample_vector(program, x, y, z) -> np.ndarray:
+    """VectorSource::sample_vector of a program at one point: 3 float32."""
+    out = np.zeros(3, np.float32)
+    st = load().or_sample_vector(nodes(program), len(program), x, y, z, out.ctypes.data)
+    if st != 0:
+        raise ValueError("program has no directed form")
+    return out
 
 
 def extract_point_cloud(program=None, grid=None, desc: IsoGridDesc = None) -> np.ndarray:

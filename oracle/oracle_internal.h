@@ -174,6 +174,102 @@ T eval(const Program& t, int id, V3<T> p) {
 
 inline float sample(const Program& p, F3 q) { return eval<float>(p, p.root, q); }
 
+// ---------------------------------------------------------------------------
+// Directed distances (VectorSource::sample_vector, source.rs:32-38): per
+// axis, the signed distance to the surface along that axis, or f32::MAX.
+// ---------------------------------------------------------------------------
+constexpr float kF32Max = 3.40282347e+38f;
+
+inline bool has_vector_form(const Program& t) {
+  for (const TreeNode& n : t.nodes)
+    if (n.op == ISO_OP_SMOOTH_UNION) return false;  // no VectorSource counterpart
+  return true;
+}
+
+inline F3 eval_vector(const Program& t, int id, F3 p) {
+  const TreeNode& n = t.nodes[id];
+  switch (n.op) {
+    case ISO_OP_SPHERE: {
+      // sphere.rs:41-55 (Vec2::len_sq is x*x + y*y)
+      const F3 a{std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)};
+      const float r2 = n.p[0] * n.p[0];
+      const float l_yz = r2 - (a.y * a.y + a.z * a.z);
+      const float l_xz = r2 - (a.x * a.x + a.z * a.z);
+      const float l_xy = r2 - (a.x * a.x + a.y * a.y);
+      return {l_yz < 0.0f ? kF32Max : a.x - std::sqrt(l_yz),
+              l_xz < 0.0f ? kF32Max : a.y - std::sqrt(l_xz),
+              l_xy < 0.0f ? kF32Max : a.z - std::sqrt(l_xy)};
+    }
+    case ISO_OP_TORUS: {
+      // torus.rs:48-91
+      const F3 a{std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)};
+      const float radius = n.p[0], tube = n.p[1];
+      const float l = std::sqrt(a.x * a.x + a.y * a.y);
+      const float l_xy = l - radius;
+      const float tube_at_z = std::sqrt(tube * tube - a.z * a.z);
+      const float rx = radius + tube_at_z, ry = radius - tube_at_z;
+      F3 out;
+      if (a.z > tube || a.y > radius + tube_at_z)
+        out.x = kF32Max;
+      else if (a.x == 0.0f)
+        out.x = std::fabs(a.y - radius) - tube;
+      else
+        out.x = fmx(a.x - std::sqrt(rx * rx - a.y * a.y), std::sqrt(ry * ry - a.y * a.y) - a.x);
+      if (a.z > tube || a.x > radius + tube_at_z)
+        out.y = kF32Max;
+      else if (a.y == 0.0f)
+        out.y = std::fabs(a.x - radius) - tube;
+      else
+        out.y = fmx(a.y - std::sqrt(rx * rx - a.x * a.x), std::sqrt(ry * ry - a.x * a.x) - a.y);
+      out.z = std::fabs(l_xy) > tube ? kF32Max : a.z - std::sqrt(tube * tube - l_xy * l_xy);
+      return out;
+    }
+    case ISO_OP_BOX: {
+      // rectangular_prism.rs:43-73
+      const F3 a{std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)};
+      const F3 h{n.p[0], n.p[1], n.p[2]};
+      const F3 mask{(((a.y - h.y) > 0.0f || (a.z - h.z) > 0.0f) ? 1.0f : -1.0f) * kF32Max,
+                    (((a.x - h.x) > 0.0f || (a.z - h.z) > 0.0f) ? 1.0f : -1.0f) * kF32Max,
+                    (((a.x - h.x) > 0.0f || (a.y - h.y) > 0.0f) ? 1.0f : -1.0f) * kF32Max};
+      const bool inside = a.x < h.x && a.y < h.y && a.z < h.z;
+      const F3 c = inside ? F3{fmx(a.x, h.x), fmx(a.y, h.y), fmx(a.z, h.z)}
+                          : F3{fmn(a.x, h.x), fmn(a.y, h.y), fmn(a.z, h.z)};
+      return {fmx(a.x - c.x, mask.x), fmx(a.y - c.y, mask.y), fmx(a.z - c.z, mask.z)};
+    }
+    case ISO_OP_CYLINDER: {
+      // cylinder.rs:50-71
+      const F3 a{std::fabs(p.x), std::fabs(p.y), std::fabs(p.z)};
+      const float r = n.p[0], hl = n.p[1];
+      return {(a.z > hl || a.y > r) ? kF32Max : a.x - std::sqrt(r * r - a.y * a.y),
+              (a.z > hl || a.x > r) ? kF32Max : a.y - std::sqrt(r * r - a.x * a.x),
+              (a.x * a.x + a.y * a.y) > r * r ? kF32Max : a.z - hl};
+    }
+    case ISO_OP_UNION: {
+      // csg.rs:41-45: a.min(b) componentwise (vector.rs:305-311)
+      const F3 a = eval_vector(t, n.a, p), b = eval_vector(t, n.b, p);
+      return {fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)};
+    }
+    case ISO_OP_INTERSECTION: {
+      // csg.rs:74-78: a.max(b) componentwise (vector.rs:315-321)
+      const F3 a = eval_vector(t, n.a, p), b = eval_vector(t, n.b, p);
+      return {fmx(a.x, b.x), fmx(a.y, b.y), fmx(a.z, b.z)};
+    }
+    case ISO_OP_DIFFERENCE: {
+      // csg.rs:102-106: b.max(-a)
+      const F3 b = eval_vector(t, n.b, p), a = eval_vector(t, n.a, p);
+      return {fmx(b.x, -a.x), fmx(b.y, -a.y), fmx(b.z, -a.z)};
+    }
+    case ISO_OP_TRANSLATE: {
+      // DemoSource::sample_vector: q = p - 0.5 (examples/common/sources.rs:45-50)
+      const F3 q{p.x - n.p[0], p.y - n.p[1], p.z - n.p[2]};
+      return eval_vector(t, n.a, q);
+    }
+  }
+  return {0.0f, 0.0f, 0.0f};
+}
+
+inline F3 sample_vector(const Program& p, F3 q) { return eval_vector(p, p.root, q); }
+
 // Lattice coordinates (primal_grid.rs:45-67: x as f32 * one_over_size).
 inline float coord(const iso_grid_desc& d, int axis, int64_t i) {
   return d.origin[axis] + static_cast<float>(i) * d.spacing[axis];
@@ -194,6 +290,33 @@ inline F3 find_crossing_point(float a, float b, F3 pa, F3 pb) {
   float s = 1.0f - t;
   return {pa.x * s + pb.x * t, pa.y * s + pb.y * t, pa.z * s + pb.z * t};
 }
+
+// The two Distance types (distance.rs:20-103) as the cell callback uses them.
+struct SignedD {
+  using Value = float;
+  // Signed::is_positive (distance.rs:52-54)
+  static bool is_positive(float v) { return v > 0.0f; }
+  static F3 crossing(float a, float b, F3 pa, F3 pb) { return find_crossing_point(a, b, pa, pb); }
+};
+
+struct DirectedD {
+  using Value = F3;
+  // Directed::is_positive (distance.rs:76-79): any component positive
+  static bool is_positive(F3 v) { return v.x > 0.0f || v.y > 0.0f || v.z > 0.0f; }
+  // Directed::find_crossing_point (distance.rs:89-102): the axis is
+  // (p_a - p_b).abs().max_component_index() (vector.rs:255-263); the
+  // interpolation is Signed's on that component.
+  static F3 crossing(F3 a, F3 b, F3 pa, F3 pb) {
+    const F3 d{std::fabs(pa.x - pb.x), std::fabs(pa.y - pb.y), std::fabs(pa.z - pb.z)};
+    const int axis = (d.x > d.y && d.x > d.z) ? 0 : (d.y > d.z ? 1 : 2);
+    const float av = axis == 0 ? a.x : (axis == 1 ? a.y : a.z);
+    const float bv = axis == 0 ? b.x : (axis == 1 ? b.y : b.z);
+    const float delta = bv - av;
+    const float t = (delta == 0.0f) ? 0.5f : -av / delta;
+    const float s = 1.0f - t;
+    return {pa.x * s + pb.x * t, pa.y * s + pb.y * t, pa.z * s + pb.z * t};
+  }
+};
 
 // ---------------------------------------------------------------------------
 // IndexCache + MeshTopology (index_cache.rs, mesh.rs) + IndexedVertices
@@ -291,19 +414,24 @@ inline GridKey grid_key(const Pt keys[8], int edge) {
   return {a, b};
 }
 
-// The marching_cubes.rs:66-79 cell callback: classify, find_edge_crossings,
-// march_cube with GridKey dedup.
-inline void march(MeshBuilder& mb, const Pt keys[8], const F3 corners[8], const float values[8]) {
+// The marching_cubes.rs:66-79 cell callback: classify_corners
+// (marching_cubes_impl.rs:26-37), find_edge_crossings (:39-56), march_cube
+// with GridKey dedup; D is the Distance type (SignedD or DirectedD).
+template <class D = SignedD>
+inline void march(MeshBuilder& mb, const Pt keys[8], const F3 corners[8],
+                  const typename D::Value values[8]) {
   using iso::tables::kEdgeCorners;
   using iso::tables::kTables;
-  const int cube = classify(values);
+  int cube = 0;
+  for (int i = 0; i < 8; ++i)
+    if (!D::is_positive(values[i])) cube |= 1 << i;
   if (mb.want_cases) mb.cases.push_back(static_cast<uint8_t>(cube));
   F3 verts[12];
   const uint16_t mask = kTables.edge_mask[cube];
   for (int e = 0; e < 12; ++e) {
     if (mask & (1u << e)) {
       const int u = kEdgeCorners[e][0], v = kEdgeCorners[e][1];
-      verts[e] = find_crossing_point(values[u], values[v], corners[u], corners[v]);
+      verts[e] = D::crossing(values[u], values[v], corners[u], corners[v]);
     }
   }
   for (int i = 0; i < 5; ++i) {  // march_cube (marching_cubes_impl.rs:102-117)

@@ -42,3 +42,10 @@ def test_cpp_mirror_matches_oracle(tmp_path):
     assert nv[:, :3].tobytes() == v.tobytes()
     assert ni.tobytes() == np.fromfile(f"{out}.program.u32", dtype="<u4").tobytes()
     assert nv[:, 3:].tobytes() == O.sample_normals(src.program(), v, 1e-4).tobytes()
+    # MarchingCubes::<Directed>: the lowered sphere and the host VectorSource
+    ref = O.extract_mc(src.program(), desc=O.reference_desc(65, 0), directed=True)
+    for kind in ("directed", "directed_host"):
+        dv = np.fromfile(f"{out}.{kind}.f32", dtype="<f4")
+        di = np.fromfile(f"{out}.{kind}.u32", dtype="<u4")
+        assert dv.tobytes() == ref.vertices.tobytes(), kind
+        assert np.array_equal(di, ref.indices), kind
