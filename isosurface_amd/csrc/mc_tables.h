@@ -124,5 +124,100 @@ constexpr McTables make_tables() {
 
 inline constexpr McTables kTables = make_tables();
 
+// The polygon loops of each case that ExtendedMarchingCubes fans around a
+// feature vertex (reference EDGE_LOOPS, marching_cubes_tables.rs:337-594,
+// read by extended_marching_cubes.rs:135-155 as: loop count, the loops'
+// sizes, then their edges).  Same data, our encoding: per case the loop
+// count, then per loop its size and its edges, as hex digits.  Three rows of
+// the reference's text carry entries after the last loop that its code never
+// reads (cases 167, 181, 229); they are not kept.  27 cases list an edge in
+// two loops (a hexagon split into two quads along a diagonal): the vertex is
+// shared and the diagonal is fanned twice.  The test suite pins the decoded
+// table against SHA-256 of the reference's text.
+inline constexpr const char* kLoopsEncoded[256] = {
+    "0", "13083", "13019", "141983", "1312a", "23083312a",
+    "142a90", "15a9832", "133b2", "1408b2", "23190323b", "1598b21",
+    "143ba1", "158ba10", "15ba903", "148ba9", "13478", "144730",
+    "230193847", "1573194", "2312a3847", "243047312a", "242a903847", "16732a94",
+    "2384733b2", "152047b", "339013847323b", "24219b4b947", "243ba13784", "161047ba",
+    "23478503ba9", "1547ba9", "13954", "239543083", "140154", "1531548",
+    "2312a3954", "33308312a3495", "15402a5", "242a5344835", "23954323b", "2408b23495",
+    "240154323b", "1621548b", "243ba13954", "234955108ba", "165403ba", "15548ba",
+    "147895", "1557309", "1515780", "143157", "2478953a12", "23a12509573",
+    "162a5780", "152a573", "24789533b2", "1620957b", "2323b580157", "15b2157",
+    "243ba145789", "1757ba109", "17ba57803", "1457ba", "13a65", "2308335a6",
+    "2390135a6", "24198335a6", "141265", "2412653308", "1502659", "16265983",
+    "2323b3a65", "2408b23a65", "33019323b35a6", "235a652198b", "15513b6", "165108b6",
+    "243b6045906", "156598b", "235a63478", "244730365a", "3319035a63847", "23a65594731",
+    "2412653478", "24265143047", "23847559026", "177326594", "333b237843a65", "235a657b204",
+    "430193478323b35a6", "34219b447b935a6", "238475b6513", "1751047b6", "34065943b603847", "24947b4659b",
+    "1449a6", "2449a63083", "156401a", "161a6483", "1526491", "23308591264",
+    "142640", "1583264", "24a6493b23", "242b804a649", "233b251a640", "17648b21a",
+    "163b6491", "178b64910", "153b640", "148b64", "1589a67", "1609a673",
+    "2480174a671", "15a6731", "16126789", "172673091", "1578026", "147326",
+    "2323b56789a", "17209a67b", "3480174a6713b23", "24b21741a67", "178913b67", "230913b67",
+    "2403b647806", "137b6", "1376b", "233083b76", "230193b76", "2419833b76",
+    "23a1236b7", "3312a330836b7", "242a9036b7", "236b7532a98", "142376", "1562087",
+    "2423763019", "16621987", "151376a", "24a17648710", "16a90376", "1576a98",
+    "146b84", "15046b3", "246b843901", "166b3194", "246b8432a1", "2312a5b3046",
+    "2446b842a90", "17a946b32", "1546238", "144620", "23190538462", "1519462",
+    "1613846a", "15a1046", "1746a9038", "1446a9", "23495376b", "3308334953b76",
+    "240154376b", "23b76548315", "339543a12376b", "436b7312a30833495", "2376b5a5402", "34532a4483536b7",
+    "2423763549", "23954587620", "24376240154", "176215487", "2395456a137", "34087146a173954",
+    "1740376a5", "2448a5476a8", "15b8956", "24095646b30", "160156b8", "156b315",
+    "2312a556b89", "34b3064956032a1", "17b802a56", "246b354a532", "16238956", "1595620",
+    "171562380", "141562", "17138956a", "2456094a106", "23038356a", "13a56",
+    "145ab7", "245ab73830", "245ab73190", "24ab7541983", "157512b", "2308352b751",
+    "162b7590", "17759832b", "15375a2", "165a2087", "239015a2375", "179875a21",
+    "143751", "1508751", "1590375", "147598", "15ab845", "16045ab3",
+    "23019545ab8", "17ab31945", "162b8451", "1704512b3", "1702b8459", "2394532b3",
+    "24235a44538", "155a204", "3435a2484533019", "16a21945", "1584513", "140451",
+    "24035948453", "13945", "159ab74", "230835749ab", "161ab740", "1731ab748",
+    "242b91449b7", "3412b94749b3830", "15b7402", "24b74243248", "1623749a", "179a20874",
+    "1737401a2", "231a23874", "1549137", "24871044917", "143740", "13487",
+    "1489ab", "15309ab", "1501ab8", "1431ab", "1512b89", "242b914309b",
+    "1402b8", "1332b", "152389a", "14209a", "24238a41a80", "131a2",
+    "141389", "13091", "13038", "0",
+};
+
+struct LoopTables {
+  uint8_t count[256];        // loops of the case
+  uint8_t size[256][4];      // vertices of each loop
+  uint8_t total[256];        // sum of the sizes: the fan triangles of the case
+  int8_t edge[256][12];      // loop edges, loop after loop, -1 padded
+  uint16_t before[256][12];  // edges first listed before edge e (by case)
+};
+
+constexpr LoopTables make_loop_tables() {
+  LoopTables t{};
+  for (int c = 0; c < 256; ++c) {
+    const char* s = kLoopsEncoded[c];
+    for (int k = 0; k < 12; ++k) t.edge[c][k] = -1;
+    for (int k = 0; k < 4; ++k) t.size[c][k] = 0;
+    for (int e = 0; e < 12; ++e) t.before[c][e] = 0;
+    int n = 0;
+    const int loops = hex_digit(s[n++]);
+    t.count[c] = static_cast<uint8_t>(loops);
+    int m = 0;
+    uint16_t seen = 0;
+    for (int l = 0; l < loops; ++l) {
+      const int size = hex_digit(s[n++]);
+      t.size[c][l] = static_cast<uint8_t>(size);
+      for (int k = 0; k < size; ++k) {
+        const int e = hex_digit(s[n++]);
+        t.edge[c][m++] = static_cast<int8_t>(e);
+        if (!(seen & (1u << e))) {
+          t.before[c][e] = seen;
+          seen = static_cast<uint16_t>(seen | (1u << e));
+        }
+      }
+    }
+    t.total[c] = static_cast<uint8_t>(m);
+  }
+  return t;
+}
+
+inline constexpr LoopTables kLoops = make_loop_tables();
+
 }  // namespace tables
 }  // namespace iso

@@ -58,46 +58,13 @@ namespace {
 // is the Sample<D> call at lattice point (x, y, z), position p.
 template <class D, class ValueAt>
 void extract_mc_t(const iso_grid_desc& d, ValueAt value, int want_cases, or_mesh* out) {
-  const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
-  struct Sample {
-    F3 p;
-    typename D::Value v;
-  };
-  std::vector<Sample> layers[2] = {std::vector<Sample>(nx * ny), std::vector<Sample>(nx * ny)};
-  auto sample = [&](int64_t x, int64_t y, int64_t z) -> Sample {
-    const F3 p{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
-    return {p, value(x, y, z, p)};
-  };
-
   orc::MeshBuilder mb;
   mb.want_cases = want_cases != 0;
-  if (mb.want_cases) mb.cases.reserve((nx - 1) * (ny - 1) * (nz - 1));
-
-  // Cache layer zero (primal_grid.rs:47-53)
-  for (int64_t y = 0; y < ny; ++y)
-    for (int64_t x = 0; x < nx; ++x) layers[0][y * nx + x] = sample(x, y, 0);
-
-  orc::Pt keys[8];
-  F3 corners[8];
-  typename D::Value values[8];
-  for (int64_t z = 0; z + 1 < nz; ++z) {
-    // Cache layer z+1 (primal_grid.rs:60-70)
-    for (int64_t y = 0; y < ny; ++y)
-      for (int64_t x = 0; x < nx; ++x) layers[1][y * nx + x] = sample(x, y, z + 1);
-    // Cells of this layer (primal_grid.rs:72-85)
-    for (int64_t y = 0; y + 1 < ny; ++y) {
-      for (int64_t x = 0; x + 1 < nx; ++x) {
-        for (int i = 0; i < 8; ++i) {
-          keys[i] = {x + kCorner[i][0], y + kCorner[i][1], z + kCorner[i][2]};
-          const Sample& s = layers[kCorner[i][2]][(y + kCorner[i][1]) * nx + x + kCorner[i][0]];
-          corners[i] = s.p;
-          values[i] = s.v;
-        }
-        orc::march<D>(mb, keys, corners, values);  // marching_cubes.rs:66-79
-      }
-    }
-    std::swap(layers[0], layers[1]);
-  }
+  if (mb.want_cases) mb.cases.reserve((d.nx - 1) * (d.ny - 1) * (d.nz - 1));
+  orc::traverse_primal<D>(d, value, [&](const orc::Pt* keys, const F3* corners,
+                                        const typename D::Value* values) {
+    orc::march<D>(mb, keys, corners, values);  // marching_cubes.rs:66-79
+  });
   orc::finish(mb, out);
 }
 
@@ -256,6 +223,18 @@ void or_tri_table(int8_t out[256 * 16]) {
 
 void or_edge_mask_table(uint16_t out[256]) {
   for (int c = 0; c < 256; ++c) out[c] = kTables.edge_mask[c];
+}
+
+void or_loop_table(int8_t out[256 * 17]) {
+  using iso::tables::kLoops;
+  for (int c = 0; c < 256; ++c) {
+    int8_t* row = out + 17 * c;
+    for (int k = 0; k < 17; ++k) row[k] = -1;
+    int n = 0;
+    row[n++] = static_cast<int8_t>(kLoops.count[c]);
+    for (int l = 0; l < kLoops.count[c]; ++l) row[n++] = static_cast<int8_t>(kLoops.size[c][l]);
+    for (int k = 0; k < kLoops.total[c]; ++k) row[n++] = kLoops.edge[c][k];
+  }
 }
 
 }  // extern "C"

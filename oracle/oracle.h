@@ -56,6 +56,20 @@ int or_sample_vector(const iso_node* prog, uint32_t n, float x, float y, float z
 int or_extract_mc_directed(const iso_node* prog, uint32_t n, const float* grid3,
                            const iso_grid_desc* desc, int want_cases, or_mesh* out);
 
+/* ExtendedMarchingCubes::new(size).extract with IndexedVertices over a
+ * program lattice: Directed values from the program's sample_vector, normals
+ * from its HermiteSource (epsilon == 0; shapes, Union and translations only)
+ * or CentralDifference::new_with_epsilon(program, epsilon) (epsilon > 0).
+ * Feature vertices have edge key (-1, -1, -1, 3).  want_features != 0 puts
+ * one byte per cell (1: the cell placed a feature vertex) in `cases`. */
+int or_extract_emc(const iso_node* prog, uint32_t n, float epsilon, const iso_grid_desc* desc,
+                   int want_features, or_mesh* out);
+
+/* HermiteSource::sample_normal of a program (shapes, Union, translations),
+ * not normalised; ISO_ERR_PROGRAM for other operators. */
+int or_sample_hermite_normal(const iso_node* prog, uint32_t n, float x, float y, float z,
+                             float* out);
+
 /* PointCloud::extract over a lattice (samples from `prog` or `grid`): the
  * vertices are the centres of the mixed cells in cell order. */
 int or_extract_point_cloud(const iso_node* prog, uint32_t n, const float* grid,
@@ -102,6 +116,9 @@ void or_mesh_free(or_mesh* m);
 /* Decoded tables, for checksum tests. */
 void or_tri_table(int8_t out[256 * 16]);
 void or_edge_mask_table(uint16_t out[256]);
+/* The edge-loop table in the reference's EDGE_LOOPS row layout
+ * (count, sizes, edges, -1 padding; int8[256][17]). */
+void or_loop_table(int8_t out[256 * 17]);
 
 #ifdef __cplusplus
 }

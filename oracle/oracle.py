@@ -68,6 +68,7 @@ def load():
                                       C.POINTER(IsoGridDesc), C.c_int, C.POINTER(OrMesh)]
         lib.or_extract_point_cloud.restype = C.c_int
         lib.or_extract_mc_directed.argtypes = lib.or_extract_mc.argtypes
+        lib.or_loop_table.argtypes = [C.c_void_p]
         lib.or_sample_vector.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_float, C.c_float,
                                          C.c_float, C.c_void_p]
         lib.or_extract_point_cloud.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
@@ -202,6 +203,45 @@ def sample_vector(program, x, y, z) -> np.ndarray:
     return out
 
 
+def hermite_normal(program, x, y, z) -> np.ndarray:
+    """HermiteSource::sample_normal of a program at one point (not normalised)."""
+    lib = load()
+    lib.or_sample_hermite_normal.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_float,
+                                             C.c_float, C.c_float, C.c_void_p]
+    out = np.zeros(3, np.float32)
+    if lib.or_sample_hermite_normal(nodes(program), len(program), x, y, z, out.ctypes.data):
+        raise ValueError("program has no HermiteSource form")
+    return out
+
+
+def extract_emc(program, desc: IsoGridDesc, epsilon: float = 0.0, want_features=False):
+    """ExtendedMarchingCubes::extract + IndexedVertices, restated on the CPU.
+    Returns an OracleMesh; with want_features, .cases holds one byte per cell
+    (1: the cell placed a feature vertex)."""
+    lib = load()
+    lib.or_extract_emc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_float,
+                                   C.POINTER(IsoGridDesc), C.c_int, C.POINTER(OrMesh)]
+    m = OrMesh()
+    desc = as_desc(desc)
+    st = lib.or_extract_emc(nodes(program), len(program), epsilon, C.byref(desc),
+                            1 if want_features else 0, C.byref(m))
+    if st != 0:
+        raise ValueError(f"oracle extraction failed: status {st}")
+    try:
+        nv, ni = int(m.n_vertices), int(m.n_indices)
+        v = np.ctypeslib.as_array(m.vertices, shape=(nv * 3,)).reshape(nv, 3).copy() if nv else \
+            np.zeros((0, 3), np.float32)
+        i = np.ctypeslib.as_array(m.indices, shape=(ni,)).copy() if ni else np.zeros(0, np.uint32)
+        k = np.ctypeslib.as_array(m.edge_keys, shape=(nv * 4,)).reshape(nv, 4).copy() if nv else \
+            np.zeros((0, 4), np.int64)
+        c = None
+        if want_features and m.n_cases:
+            c = np.ctypeslib.as_array(m.cases, shape=(int(m.n_cases),)).copy()
+    finally:
+        lib.or_mesh_free(C.byref(m))
+    return OracleMesh(v, i, k, c)
+
+
 def extract_point_cloud(program=None, grid=None, desc: IsoGridDesc = None) -> np.ndarray:
     """PointCloud::extract restated on the CPU: (n, 3) float32 cell centres."""
     lib = load()
@@ -250,6 +290,12 @@ def tri_table() -> np.ndarray:
     out = np.empty(256 * 16, dtype=np.int8)
     load().or_tri_table(out.ctypes.data)
     return out.reshape(256, 16)
+
+
+def loop_table() -> np.ndarray:
+    out = np.empty(256 * 17, dtype=np.int8)
+    load().or_loop_table(out.ctypes.data)
+    return out.reshape(256, 17)
 
 
 def edge_mask_table() -> np.ndarray:

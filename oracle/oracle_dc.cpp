@@ -454,13 +454,16 @@ F3 normalised_or_zero(F3 v) {
   return {v.x / l, v.y / l, v.z / l};
 }
 
-// TangentPlanes::new + MinimiseQEF::place_feature_with_tangents.
-F3 place_feature(const F3 corners[8], const F3 normals[8]) {
+// TangentPlanes::new (feature/mod.rs:91-148) over n points and normals +
+// MinimiseQEF::place_feature_with_tangents (qef.rs:36-68); `feature`, when
+// not null, receives whether the planes were classified Edge or Corner.
+F3 place_feature_n(const F3* corners, const F3* normals, int n, bool* feature) {
   F3 com{0.0f, 0.0f, 0.0f}, axis{0.0f, 0.0f, 0.0f};
   float min_angle = 3.40282347e+38f;
   float count = 0.0f;
-  for (int i = 0; i < 8; ++i) {
-    for (int j = 0; j < 8; ++j) {
+  if (feature) *feature = false;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
       const float angle = dot(normals[i], normals[j]);
       if (angle < min_angle) {
         axis = cross(normals[i], normals[j]);
@@ -476,9 +479,10 @@ F3 place_feature(const F3 corners[8], const F3 normals[8]) {
   com.y /= count;
   com.z /= count;
   if (min_angle > kFeatureAngle) return com;  // Planar
+  if (feature) *feature = true;
   axis = normalised_or_zero(axis);
   float min_c = 1.0f, max_c = -1.0f;
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < n; ++i) {
     const float c = dot(axis, normals[i]);
     min_c = std::fmin(min_c, c);
     max_c = std::fmax(max_c, c);
@@ -486,15 +490,15 @@ F3 place_feature(const F3 corners[8], const F3 normals[8]) {
   float c = std::fmax(std::fabs(min_c), std::fabs(max_c));
   c = std::sqrt(1.0f - c * c);
   const bool edge = c > kFeatureAngle;
-  double a[8][3], b[8];
-  for (int i = 0; i < 8; ++i) {
+  double a[12][3], b[12];
+  for (int i = 0; i < n; ++i) {
     a[i][0] = normals[i].x;
     a[i][1] = normals[i].y;
     a[i][2] = normals[i].z;
     const F3 r{corners[i].x - com.x, corners[i].y - com.y, corners[i].z - com.z};
     b[i] = static_cast<double>(dot(r, normals[i]));
   }
-  Svd s = svd(a, 8);
+  Svd s = svd(a, n);
   if (edge) {  // drop the smallest singular value (qef.rs:51-63)
     double smin = 1.7976931348623157e308;
     int id = 0;
@@ -512,7 +516,18 @@ F3 place_feature(const F3 corners[8], const F3 normals[8]) {
           com.z + static_cast<float>(x[2])};
 }
 
+F3 place_feature(const F3 corners[8], const F3 normals[8]) {
+  return place_feature_n(corners, normals, 8, nullptr);
+}
+
 }  // namespace
+
+namespace orc {
+F3 place_feature_planes(const F3* points, const F3* normals, int n, bool* feature) {
+  return place_feature_n(points, normals, n, feature);
+}
+F3 unit_or_zero(F3 v) { return normalised_or_zero(v); }
+}  // namespace orc
 
 // ---------------------------------------------------------------------------
 // DualContouring::extract (dual_contouring.rs:65-105)

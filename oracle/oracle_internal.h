@@ -390,6 +390,20 @@ struct MeshBuilder {
     return idx;
   }
 
+  // add_vertex(None, v) (mesh.rs:240-251): always a new vertex, no cache
+  // entry; its key is recorded as (-1, -1, -1, 3).
+  uint32_t add_vertex_unkeyed(F3 v) {
+    uint32_t idx = next_vertex++;
+    vertices.push_back(v.x);
+    vertices.push_back(v.y);
+    vertices.push_back(v.z);
+    keys.push_back(-1);
+    keys.push_back(-1);
+    keys.push_back(-1);
+    keys.push_back(3);
+    return idx;
+  }
+
   void add_edge(uint32_t u, uint32_t v, uint32_t face) {
     EdgeHandle e = u > v ? EdgeHandle{v, u} : EdgeHandle{u, v};
     edges.insert(e);
@@ -443,6 +457,54 @@ inline void march(MeshBuilder& mb, const Pt keys[8], const F3 corners[8],
     mb.add_face(a, b, c);
   }
 }
+
+// PrimalGrid::traverse (traversal/primal_grid.rs:39-89): two cached layers
+// of (position, value), cells in z-major order; value(x, y, z, p) is the
+// Sample<D> call, cell(keys, corners, values) the per-cell callback.
+template <class D, class ValueAt, class Cell>
+void traverse_primal(const iso_grid_desc& d, ValueAt value, Cell cell) {
+  using iso::tables::kCorner;
+  const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
+  struct Sample {
+    F3 p;
+    typename D::Value v;
+  };
+  std::vector<Sample> layers[2] = {std::vector<Sample>(nx * ny), std::vector<Sample>(nx * ny)};
+  auto sample = [&](int64_t x, int64_t y, int64_t z) -> Sample {
+    const F3 p{coord(d, 0, x), coord(d, 1, y), coord(d, 2, z)};
+    return {p, value(x, y, z, p)};
+  };
+  // Cache layer zero (primal_grid.rs:47-53)
+  for (int64_t y = 0; y < ny; ++y)
+    for (int64_t x = 0; x < nx; ++x) layers[0][y * nx + x] = sample(x, y, 0);
+  Pt keys[8];
+  F3 corners[8];
+  typename D::Value values[8];
+  for (int64_t z = 0; z + 1 < nz; ++z) {
+    // Cache layer z+1 (primal_grid.rs:60-70)
+    for (int64_t y = 0; y < ny; ++y)
+      for (int64_t x = 0; x < nx; ++x) layers[1][y * nx + x] = sample(x, y, z + 1);
+    // Cells of this layer (primal_grid.rs:72-85)
+    for (int64_t y = 0; y + 1 < ny; ++y) {
+      for (int64_t x = 0; x + 1 < nx; ++x) {
+        for (int i = 0; i < 8; ++i) {
+          keys[i] = {x + kCorner[i][0], y + kCorner[i][1], z + kCorner[i][2]};
+          const Sample& s = layers[kCorner[i][2]][(y + kCorner[i][1]) * nx + x + kCorner[i][0]];
+          corners[i] = s.p;
+          values[i] = s.v;
+        }
+        cell(keys, corners, values);
+      }
+    }
+    std::swap(layers[0], layers[1]);
+  }
+}
+
+// Feature placement (oracle_dc.cpp): TangentPlanes::new over n points and
+// normals + MinimiseQEF::place_feature_with_tangents; *feature = the planes
+// are not Planar.  unit_or_zero = Vec3::normalised().unwrap_or_default().
+F3 place_feature_planes(const F3* points, const F3* normals, int n, bool* feature);
+F3 unit_or_zero(F3 v);
 
 template <typename T>
 T* dup(const std::vector<T>& v) {

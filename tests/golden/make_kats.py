@@ -1,6 +1,6 @@
-"""Writes reference_kats.json and reference_vector_kats.json: the scalar
-and directed known-answer tests of the reference crate's own unit tests, as
-data.
+"""Writes reference_kats.json, reference_vector_kats.json and
+reference_normal_kats.json: the scalar, directed and normal known-answer
+tests of the reference crate's own unit tests, as data.
 
 Reads the Rust test modules of src/implicit/{sphere,torus,cylinder,
 rectangular_prism,csg}.rs (only in the build container, where the reference
@@ -9,8 +9,9 @@ the source that `x` names (as an iso_node post-order program), the point and
 the expected f32 value; and for every `assert_eq!(x.sample_vector(p).0, e)`
 the expected Vec3, with the f32 expressions the tests write (such as
 `Vec2::from_scalar(1.0 - (4.0f32 - 1.0f32).sqrt()).extend(-3.0)`) evaluated
-in f32 and f32::MAX kept as "MAX".  The JSON is committed; tests never read
-the reference itself.
+in f32 and f32::MAX kept as "MAX"; and for every
+`assert_eq!(x.sample_normal(p).normalised().unwrap(), n)` the expected unit
+normal.  The JSON is committed; tests never read the reference itself.
 
 usage: python tests/golden/make_kats.py /root/reference
 """
@@ -112,9 +113,22 @@ def vector_kats(name, text, progs):
     return out
 
 
+def normal_kats(name, text, progs):
+    out = []
+    pat = re.compile(r"assert_eq!\(\s*(\w+)\s*\.sample_normal\((Vec3::\w+\([^)]*\))\)\s*"
+                     r"\.normalised\(\)\s*\.unwrap\(\),\s*(.*?),?\s*\);", re.S)
+    for m in pat.finditer(text):
+        var, p, e = m.groups()
+        lineno = text.count("\n", 0, m.start()) + 1
+        out.append({"where": f"src/implicit/{name}:{lineno}", "program": progs[var],
+                    "point": vec3(p), "expected": vec3_expr(e)})
+    return out
+
+
 def main(ref_root: str) -> None:
     kats = []
     vkats = []
+    nkats = []
     for name in FILES:
         path = Path(ref_root) / "src" / "implicit" / name
         lines = path.read_text().splitlines()
@@ -137,12 +151,16 @@ def main(ref_root: str) -> None:
                 kats.append({"where": f"src/implicit/{name}:{lineno}", "program": progs[var],
                              "point": vec3(p), "expected": float(v)})
         vkats += vector_kats(name, "\n".join(lines), progs)
+        nkats += normal_kats(name, "\n".join(lines), progs)
     out = Path(__file__).with_name("reference_kats.json")
     out.write_text(json.dumps(kats, indent=1) + "\n")
     print(f"{len(kats)} known-answer tests -> {out}")
     vout = Path(__file__).with_name("reference_vector_kats.json")
     vout.write_text(json.dumps(vkats, indent=1) + "\n")
     print(f"{len(vkats)} directed known-answer tests -> {vout}")
+    nout = Path(__file__).with_name("reference_normal_kats.json")
+    nout.write_text(json.dumps(nkats, indent=1) + "\n")
+    print(f"{len(nkats)} normal known-answer tests -> {nout}")
 
 
 if __name__ == "__main__":
