@@ -207,6 +207,20 @@ int iso_extract_mc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
  * capacity and fetch rules as iso_extract_mc (ISO_MESH_GRAPH is ignored). */
 int iso_extract_mc_directed(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
 
+/* ExtendedMarchingCubes::new(size).extract with IndexedVertices
+ * (extended_marching_cubes.rs:59-186) over a program lattice: Directed
+ * values from the program's sample_vector; the normals at the edge
+ * crossings from its HermiteSource (epsilon == 0: shapes, unions and
+ * translations, implicit/<shape>.rs sample_normal, csg.rs:47-51) or from
+ * CentralDifference::new_with_epsilon(program, epsilon) (epsilon > 0,
+ * source.rs:52-94).  A cell whose tangent planes show a feature gets a QEF
+ * feature vertex with triangle fans over its edge loops, and
+ * flip_feature_edges rotates the fan edges between feature vertices, all as
+ * the reference does (vertex order, f32 bits and index lists).  Program
+ * sources only (ISO_ERR_INVALID_ARGUMENT for a grid); same outputs, capacity
+ * and fetch rules as iso_extract_mc. */
+int iso_extract_emc(iso_ctx* ctx, const iso_grid_desc* desc, float epsilon, iso_mesh_out* out);
+
 /* PointCloud::<Signed>::new(size).extract(source, extractor)
  * (point_cloud.rs:50-63): one point per mixed cell (cube index neither 0
  * nor 255) in the lattice's cell order, corners[0].lerp(corners[6], 0.5);

@@ -8,7 +8,8 @@ Source / Extractor / MarchingCubes / DualContouring /
 LinearHashedMarchingCubes API; the C++ mirror for compiled hosts is
 isosurface_amd/host/isosurface.hpp.
 """
-from .engine import (BatchedMarchingCubes, Context, Directed, DualContouring, Extractor,
+from .engine import (BatchedMarchingCubes, Context, Directed, DualContouring,
+                     ExtendedMarchingCubes, Extractor,
                      IndexedInterleavedNormals, IndexedVertices, LinearHashedMarchingCubes,
                      MarchingCubes, Mesh, MinimiseQEF, OnlyInterleavedNormals, OnlyVertices,
                      PointCloud, Signed, chunk_desc, default_context, sample_lattice_host,
@@ -22,6 +23,7 @@ __all__ = [
     "BatchedMarchingCubes", "chunk_desc", "CentralDifference", "DualContouring", "LinearHashedMarchingCubes", "MinimiseQEF",
     "Context", "Extractor", "IndexedVertices", "MarchingCubes", "Mesh", "OnlyVertices",
     "IndexedInterleavedNormals", "OnlyInterleavedNormals", "PointCloud", "Signed", "Directed",
+    "ExtendedMarchingCubes",
     "sample_lattice_host_vector",
     "default_context", "sample_lattice_host", "Cylinder", "DemoSource", "Difference",
     "Intersection", "RectangularPrism", "SmoothUnion", "Source", "Sphere", "Torus",

@@ -128,6 +128,8 @@ SIGNATURES = [
     ("iso_extract_mc", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.POINTER(IsoMeshOut)]),
     ("iso_extract_mc_directed", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc),
                                           C.POINTER(IsoMeshOut)]),
+    ("iso_extract_emc", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float,
+                                  C.POINTER(IsoMeshOut)]),
     ("iso_program_sample_vector", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     ("iso_grid_upload_directed", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64,
                                            C.c_int64]),

@@ -33,7 +33,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["api.hip", "mc_kernels.hip", "dc_kernels.hip", "lhmc_kernels.hip", "mlp_kernels.hip",
-           "band_kernels.hip", "program.cpp"]
+           "band_kernels.hip", "emc_kernels.hip", "program.cpp"]
 
 
 def hipcc() -> str:

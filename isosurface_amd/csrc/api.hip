@@ -12,6 +12,7 @@
 #include "../../include/isosurface.h"
 #include "band_kernels.h"
 #include "dc_kernels.h"
+#include "emc_kernels.h"
 #include "lhmc_kernels.h"
 #include "mc_kernels.h"
 #include "mlp_kernels.h"
@@ -58,6 +59,7 @@ struct iso_ctx {
   DevBuf prog;
   int n_prog = 0;
   bool prog_directed = false;  // the program has a directed form (no smooth union)
+  bool prog_hermite = false;   // analytic normals exist (shapes, unions, translations)
 
   // current grid
   DevBuf grid_owned;
@@ -73,6 +75,8 @@ struct iso_ctx {
   // current extraction (3 floats per lattice point)
   DevBuf dgrid, dvals;
   int64_t dgnx = 0, dgny = 0, dgnz = 0;
+  // extended marching cubes: per active cell state, first ids, flip records
+  DevBuf emc_cells, emc_first, emc_flips, emc_ctr;
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
@@ -291,7 +295,8 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
                     &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge,  &c->nrm,
-                    &c->dgrid,      &c->dvals};
+                    &c->dgrid,      &c->dvals,      &c->emc_cells, &c->emc_first,
+                    &c->emc_flips,  &c->emc_ctr};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   if (c->m_graph) (void)hipGraphExecDestroy(c->m_graph);
   for (DevBuf* b : bufs) release(b);
@@ -333,8 +338,13 @@ int iso_program_upload(iso_ctx* c, const iso_node* nodes, uint32_t n_nodes) {
   ISO_HIP(c, hipStreamSynchronize(c->stream));
   c->n_prog = static_cast<int>(ops.size());
   c->prog_directed = true;
-  for (uint32_t i = 0; i < n_nodes; ++i)
+  c->prog_hermite = true;
+  for (uint32_t i = 0; i < n_nodes; ++i) {
     if (nodes[i].op == ISO_OP_SMOOTH_UNION) c->prog_directed = false;
+    if (nodes[i].op == ISO_OP_SMOOTH_UNION || nodes[i].op == ISO_OP_INTERSECTION ||
+        nodes[i].op == ISO_OP_DIFFERENCE)
+      c->prog_hermite = false;
+  }
   c->band_valid = false;  // a band extraction is judged against its own program
   return ISO_OK;
 }
@@ -756,6 +766,127 @@ int iso_extract_mc_directed(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out*
     return ISO_OK;
   };
   return extract_lattice(c, L, nullptr, nullptr, out, &signs, nullptr, vals);
+}
+
+int iso_extract_emc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  if (desc->source != ISO_SOURCE_PROGRAM)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT,
+                "extended marching cubes samples normals at the crossings: program sources only");
+  if (!(epsilon >= 0.0f) || !std::isfinite(epsilon))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "epsilon must be finite and >= 0");
+  if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
+  if (!c->prog_directed)
+    return fail(c, ISO_ERR_PROGRAM, "the program has no directed form (smooth union)");
+  if (epsilon == 0.0f && !c->prog_hermite)
+    return fail(c, ISO_ERR_PROGRAM,
+                "analytic normals need shapes, unions and translations only; pass epsilon > 0 "
+                "for central differences");
+  if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
+  c->kept = false;
+  c->band_valid = false;
+  ISO_HIP(c, hipSetDevice(c->device));
+  const iso::McDims D = iso::mc_dims(L);
+  if (D.n_segs >= (int64_t(1) << 26))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");
+  iso::McBuffers B{};
+  if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
+  ISO_HIP(c, ensure(&c->dvals, static_cast<size_t>(D.n_points) * 3 * sizeof(float)));
+  float* vals = static_cast<float*>(c->dvals.ptr);
+  const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
+  // Directed signs and the active (mixed) cells.
+  uint64_t n_act = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    ISO_HIP(c, timed(c, "mc_signs_directed", [&] {
+              return iso::launch_signs_directed(ops, c->n_prog, L, D, B, vals, c->stream);
+            }));
+    ISO_HIP(c, timed(c, "mc_count", [&] { return iso::launch_count(L, D, B, c->stream); }));
+    if ((st = read_totals(c, B)) != ISO_OK) return st;
+    n_act = c->h_totals[iso::kTotActive];
+    if (n_act <= B.active_capacity) break;
+    if ((st = prepare_mc(c, D, n_act, &B)) != ISO_OK) return st;
+  }
+  // Vertex and triangle totals share one 62-bit scan word (32 + 30 bits).
+  if (n_act * 13 >= (1ull << 32) || n_act * 12 >= (1ull << 30))
+    return fail(c, ISO_ERR_OVERFLOW, "too many mixed cells for extended marching cubes");
+  const uint64_t na = n_act ? n_act : 1;
+  ISO_HIP(c, ensure(&c->emc_cells, na * sizeof(iso::EmcCell)));
+  ISO_HIP(c, ensure(&c->emc_first, 2 * na * sizeof(uint32_t)));
+  ISO_HIP(c, ensure(&c->emc_ctr, iso::kEmcSlots * sizeof(unsigned long long)));
+  ISO_HIP(c, hipMemsetAsync(c->emc_ctr.ptr, 0, iso::kEmcSlots * sizeof(unsigned long long),
+                            c->stream));
+  iso::scan::State S;
+  if ((st = prepare_scan(c, n_act, &S)) != ISO_OK) return st;
+  ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(S.err), 2, 0u, c->stream));
+  iso::EmcArgs A{};
+  A.L = L;
+  A.segs_per_row = static_cast<uint32_t>(D.segs_per_row);
+  A.values = vals;
+  A.active = B.active;
+  A.n_act = n_act;
+  A.ops = ops;
+  A.n_ops = c->n_prog;
+  A.epsilon = epsilon;
+  A.cells = static_cast<iso::EmcCell*>(c->emc_cells.ptr);
+  A.vfirst = static_cast<uint32_t*>(c->emc_first.ptr);
+  A.tfirst = A.vfirst + na;
+  A.counters = static_cast<unsigned long long*>(c->emc_ctr.ptr);
+  ISO_HIP(c, timed(c, "emc_cells", [&] { return iso::launch_emc_cells(A, S, c->stream); }));
+  ISO_HIP(c, hipMemcpyAsync(c->h_stage, S.ctl, 4 * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (c->h_stage[2]) return fail(c, ISO_ERR_DEVICE, "the cell scan did not complete");
+  const unsigned long long total = n_act ? c->h_stage[1] : 0ull;
+  const uint64_t nv = total & 0xffffffffull, nt = total >> 32, ni = 3 * nt;
+  out->vertex_count = nv;
+  out->index_count = ni;
+  const bool device = out->memory == ISO_MEMORY_DEVICE;
+  if (device) {
+    if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+        (ni && (!out->indices || out->index_capacity < ni)))
+      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    A.vertices = out->vertices;
+    A.indices = out->indices;
+  } else {
+    ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
+    ISO_HIP(c, ensure(&c->iout, (ni ? ni : 1) * sizeof(uint32_t)));
+    A.vertices = static_cast<float*>(c->vout.ptr);
+    A.indices = static_cast<uint32_t*>(c->iout.ptr);
+  }
+  A.vcap = nv;
+  A.icap = ni;
+  ISO_HIP(c, ensure(&c->emc_flips, (nt ? nt : 1) * sizeof(uint4)));
+  A.flips = static_cast<uint4*>(c->emc_flips.ptr);
+  if (n_act) {
+    ISO_HIP(c, timed(c, "emc_emit", [&] { return iso::launch_emc_emit(A, nt, c->stream); }));
+    ISO_HIP(c, timed(c, "emc_flips", [&] { return iso::launch_emc_flips(A, nt, c->stream); }));
+  }
+  ISO_HIP(c, hipMemcpyAsync(c->h_stage, c->emc_ctr.ptr, sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (c->h_stage[0]) return fail(c, ISO_ERR_DEVICE, "a feature point's SVD did not converge");
+  if (device) return ISO_OK;
+  if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+      (ni && (!out->indices || out->index_capacity < ni))) {
+    c->kept = true;
+    c->kept_vertices = nv;
+    c->kept_indices = ni;
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small; mesh kept for iso_mesh_fetch");
+  }
+  if (nv)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, A.vertices, nv * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+  if (ni)
+    ISO_HIP(c, hipMemcpyAsync(out->indices, A.indices, ni * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
 }
 
 int iso_program_sample_vector(iso_ctx* c, const float* points, uint64_t n, float* values) {

@@ -15,7 +15,8 @@
 namespace iso {
 namespace qef {
 
-constexpr int kRows = 8;
+constexpr int kRows = 8;     // dual contouring: the 8 cell corners
+constexpr int kMaxRows = 12;  // extended marching cubes: up to 12 edge crossings
 
 struct Vec3f {
   float x, y, z;
@@ -36,10 +37,14 @@ __device__ __forceinline__ Vec3f normalised_or_zero(Vec3f v) {
   return {v.x / l, v.y / l, v.z / l};
 }
 
-struct Svd {
-  double u[kRows][3];
+// The SVD of a rows x 3 matrix (rows <= R): every loop over rows runs to
+// `rows`, so the arithmetic is that of the reference's rows-row matrix.
+template <int R>
+struct SvdT {
+  double u[R][3];
   double v[3][3];
   double d[3];
+  int rows;
 };
 
 // Householder reflection of the n entries x[0], x[stride], ... (svd.rs:251-293).
@@ -78,25 +83,26 @@ __device__ __forceinline__ double householder(double* x, int n, int stride) {
 }
 
 // Bidiagonal factorisation (svd.rs:149-249).
-__device__ __forceinline__ void bidiagonalise(double m[kRows][3], double tau_u[3],
-                                              double tau_v[2]) {
+template <int R>
+__device__ __forceinline__ void bidiagonalise(double m[R][3], double tau_u[3], double tau_v[2],
+                                              int rows) {
   for (int i = 0; i < 3; ++i) {
-    double tau = householder(&m[i][i], kRows - i, 3);
+    double tau = householder(&m[i][i], rows - i, 3);
     tau_u[i] = tau;
     if (i + 1 < 3) {
       if (tau != 0.0) {
         for (int x = i + 1; x < 3; ++x) {
           double wx = m[i][x];
-          for (int y = i + 1; y < kRows; ++y) wx += m[y][x] * m[y][i];
+          for (int y = i + 1; y < rows; ++y) wx += m[y][x] * m[y][i];
           const double tw = tau * wx;
           m[i][x] -= tw;
-          for (int y = i + 1; y < kRows; ++y) m[y][x] -= tw * m[y][i];
+          for (int y = i + 1; y < rows; ++y) m[y][x] -= tw * m[y][i];
         }
       }
       tau = householder(&m[i][i + 1], i == 0 ? 2 : 1, 1);
       tau_v[i] = tau;
       if (tau != 0.0) {
-        for (int y = i + 1; y < kRows; ++y) {
+        for (int y = i + 1; y < rows; ++y) {
           double wy = m[y][i + 1];
           if (i == 0) wy += m[y][i + 2] * m[i][i + 2];
           const double tw = tau * wy;
@@ -109,8 +115,9 @@ __device__ __forceinline__ void bidiagonalise(double m[kRows][3], double tau_u[3
 }
 
 // Accumulates the reflections into U and V (svd.rs:295-410).
-__device__ __forceinline__ void unpack(double u[kRows][3], double v[3][3], double tau_u[3],
-                                       double tau_v[2]) {
+template <int R>
+__device__ __forceinline__ void unpack(double u[R][3], double v[3][3], double tau_u[3],
+                                       double tau_v[2], int rows) {
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) v[i][j] = i == j ? 1.0 : 0.0;
   for (int i = 1; i >= 0; --i) {
@@ -135,16 +142,16 @@ __device__ __forceinline__ void unpack(double u[kRows][3], double v[3][3], doubl
         u[i][2] = 0.0;
         if (i < 1) u[i][1] = 0.0;
       }
-      for (int y = i + 1; y < kRows; ++y) u[y][i] = 0.0;
+      for (int y = i + 1; y < rows; ++y) u[y][i] = 0.0;
     } else {
       for (int x = i + 1; x < 3; ++x) {
         double wx = 0.0;
-        for (int y = i + 1; y < kRows; ++y) wx += u[y][x] * u[y][i];
+        for (int y = i + 1; y < rows; ++y) wx += u[y][x] * u[y][i];
         const double tw = tau * wx;
         u[i][x] = -tw;
-        for (int y = i + 1; y < kRows; ++y) u[y][x] -= tw * u[y][i];
+        for (int y = i + 1; y < rows; ++y) u[y][x] -= tw * u[y][i];
       }
-      for (int y = i + 1; y < kRows; ++y) u[y][i] = u[y][i] * -tau;
+      for (int y = i + 1; y < rows; ++y) u[y][i] = u[y][i] * -tau;
       u[i][i] = 1.0 - tau;
     }
   }
@@ -193,8 +200,9 @@ __device__ __forceinline__ void chop(double* a, double* b, int n) {  // svd.rs:4
 
 // Rotates columns p and q (absolute) of the first `rows` rows by (c, s).
 template <int R>
-__device__ __forceinline__ void rotate(double m[R][3], int p, int q, double c, double s) {
-  for (int i = 0; i < R; ++i) {
+__device__ __forceinline__ void rotate(double m[R][3], int rows, int p, int q, double c,
+                                       double s) {
+  for (int i = 0; i < rows; ++i) {
     const double mp = m[i][p], mq = m[i][q];
     m[i][p] = mp * c - mq * s;
     m[i][q] = mp * s + mq * c;
@@ -210,27 +218,28 @@ __device__ __forceinline__ void swap_cols01(double v[3][3], int a) {
 }
 
 // QR step on the 2-column window [a, a+1] (svd.rs:689-823).
-__device__ __forceinline__ void qr_cols2(Svd& S, double* tu, double* tv, int a) {
+template <int R>
+__device__ __forceinline__ void qr_cols2(SvdT<R>& S, double* tu, double* tv, int a) {
   double c, s;
   if (tu[0] == 0.0) {
     givens(tv[0], tu[1], &c, &s);
     tu[0] = tv[0] * c - tu[1] * s;
     tv[0] = tv[0] * s + tu[1] * c;
     tu[1] = 0.0;
-    rotate<kRows>(S.u, a, a + 1, c, s);
+    rotate<R>(S.u, S.rows, a, a + 1, c, s);
     swap_cols01(S.v, a);
   } else if (tu[1] == 0.0) {
     givens(tu[0], tv[0], &c, &s);
     tu[0] = tu[0] * c - tv[0] * s;
     tv[0] = 0.0;
-    rotate<3>(S.v, a, a + 1, c, s);
+    rotate<3>(S.v, 3, a, a + 1, c, s);
   } else {
     schur(tu[0], tv[0], tu[1], &c, &s);
     double a11 = tu[0] * c - tv[0] * s;
     double a21 = -tu[1] * s;
     double a12 = tu[0] * s + tv[0] * c;
     double a22 = tu[1] * c;
-    rotate<3>(S.v, a, a + 1, c, s);
+    rotate<3>(S.v, 3, a, a + 1, c, s);
     if (a11 * a11 + a21 * a21 < a12 * a12 + a22 * a22) {
       double t = a11;
       a11 = a12;
@@ -244,17 +253,19 @@ __device__ __forceinline__ void qr_cols2(Svd& S, double* tu, double* tv, int a) 
     tu[0] = a11 * c - a21 * s;
     tv[0] = a12 * c - a22 * s;
     tu[1] = a12 * s + a22 * c;
-    rotate<kRows>(S.u, a, a + 1, c, s);
+    rotate<R>(S.u, S.rows, a, a + 1, c, s);
   }
 }
 
 // A zero on the diagonal inside the 3-column window (svd.rs:604-641).
-__device__ __forceinline__ void qr_middle(Svd& S, double* tu, double* tv, int cols, int col) {
+template <int R>
+__device__ __forceinline__ void qr_middle(SvdT<R>& S, double* tu, double* tv, int cols,
+                                          int col) {
   double x = tv[col], y = tu[col + 1];
   for (int j = col; j < cols - 1; ++j) {
     double c, s;
     givens(y, -x, &c, &s);
-    for (int i = 0; i < kRows; ++i) {
+    for (int i = 0; i < S.rows; ++i) {
       const double up = S.u[i][col], uq = S.u[i][j + 1];
       S.u[i][col] = up * c - uq * s;
       S.u[i][j + 1] = up * s + uq * c;
@@ -271,7 +282,8 @@ __device__ __forceinline__ void qr_middle(Svd& S, double* tu, double* tv, int co
 }
 
 // A zero at the end of the diagonal (svd.rs:643-671).
-__device__ __forceinline__ void qr_end(Svd& S, double* tu, double* tv) {
+template <int R>
+__device__ __forceinline__ void qr_end(SvdT<R>& S, double* tu, double* tv) {
   double x = tu[1], y = tv[1];
   for (int k = 1; k >= 0; --k) {
     double c, s;
@@ -302,7 +314,8 @@ __device__ __forceinline__ double qr_shift(const double* tu, const double* tv) {
 }
 
 // One implicit-shift QR sweep on the window [a, a+cols) (svd.rs:495-602).
-__device__ __forceinline__ void qr_step(Svd& S, double* tu, double* tv, int a, int cols) {
+template <int R>
+__device__ __forceinline__ void qr_step(SvdT<R>& S, double* tu, double* tv, int a, int cols) {
   if (cols == 2) {
     qr_cols2(S, tu, tv, a);
     return;
@@ -326,7 +339,7 @@ __device__ __forceinline__ void qr_step(Svd& S, double* tu, double* tv, int a, i
   for (int k = 0; k < cols - 1; ++k) {
     double c, s;
     givens(y, z, &c, &s);
-    rotate<3>(S.v, k, k + 1, c, s);
+    rotate<3>(S.v, 3, k, k + 1, c, s);
     double bk1 = bk * c - z * s;
     double ap1 = ap * c - bp * s;
     double bp1 = ap * s + bp * c;
@@ -341,7 +354,7 @@ __device__ __forceinline__ void qr_step(Svd& S, double* tu, double* tv, int a, i
     y = ak;
     z = zk;
     givens(y, z, &c, &s);
-    rotate<kRows>(S.u, k, k + 1, c, s);
+    rotate<R>(S.u, S.rows, k, k + 1, c, s);
     const double ak1 = ak * c - zk * s;
     bk1 = bk * c - ap * s;
     const double zk1 = bp * -s;
@@ -364,7 +377,8 @@ __device__ __forceinline__ void qr_step(Svd& S, double* tu, double* tv, int a, i
 // Diagonalisation (svd.rs:413-479).  The reference has no iteration cap;
 // `max_sweeps` bounds the loop on the device so that every thread finishes
 // (a cell that hits it is reported through *stalled).
-__device__ __forceinline__ void diagonalise(Svd& S, double tu[3], double tv[2], int max_sweeps,
+template <int R>
+__device__ __forceinline__ void diagonalise(SvdT<R>& S, double tu[3], double tv[2], int max_sweeps,
                                             bool* stalled) {
   chop(tu, tv, 3);
   int b = 2;
@@ -386,7 +400,8 @@ __device__ __forceinline__ void diagonalise(Svd& S, double tu[3], double tv[2], 
   }
 }
 
-__device__ __forceinline__ void sort_singular(Svd& S) {  // svd.rs:867-926
+template <int R>
+__device__ __forceinline__ void sort_singular(SvdT<R>& S) {  // svd.rs:867-926
   for (int j = 0; j < 3; ++j) {
     if (S.d[j] < 0.0) {
       for (int i = 0; i < 3; ++i) S.v[i][j] = -S.v[i][j];
@@ -406,7 +421,7 @@ __device__ __forceinline__ void sort_singular(Svd& S) {  // svd.rs:867-926
       double t = S.d[i];
       S.d[i] = S.d[imax];
       S.d[imax] = t;
-      for (int y = 0; y < kRows; ++y) {
+      for (int y = 0; y < S.rows; ++y) {
         t = S.u[y][i];
         S.u[y][i] = S.u[y][imax];
         S.u[y][imax] = t;
@@ -420,15 +435,19 @@ __device__ __forceinline__ void sort_singular(Svd& S) {  // svd.rs:867-926
   }
 }
 
-// TangentPlanes::new + MinimiseQEF::place_feature_with_tangents.
-__device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec3f normals[8],
-                                               bool* stalled) {
+// TangentPlanes::new (feature/mod.rs:91-148) over n <= R points and normals
+// + MinimiseQEF::place_feature_with_tangents (qef.rs:36-68); *feature
+// receives whether the planes were classified Edge or Corner.
+template <int R>
+__device__ __forceinline__ Vec3f place_feature_n(const Vec3f* corners, const Vec3f* normals,
+                                                 int n, bool* stalled, bool* feature) {
   const float kFeatureAngle = 0.866025388f;  // f32(0.8660254037844387), mod.rs:22
   Vec3f com{0.0f, 0.0f, 0.0f}, axis{0.0f, 0.0f, 0.0f};
   float min_angle = 3.40282347e+38f;
   float count = 0.0f;
-  for (int i = 0; i < 8; ++i) {
-    for (int j = 0; j < 8; ++j) {
+  *feature = false;
+  for (int i = 0; i < n; ++i) {
+    for (int j = 0; j < n; ++j) {
       const float angle = dot3(normals[i], normals[j]);
       if (angle < min_angle) {
         axis = cross3(normals[i], normals[j]);
@@ -444,9 +463,10 @@ __device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec
   com.y /= count;
   com.z /= count;
   if (min_angle > kFeatureAngle) return com;  // planar
+  *feature = true;
   axis = normalised_or_zero(axis);
   float min_c = 1.0f, max_c = -1.0f;
-  for (int i = 0; i < 8; ++i) {
+  for (int i = 0; i < n; ++i) {
     const float c = dot3(axis, normals[i]);
     min_c = fminf(min_c, c);
     max_c = fmaxf(max_c, c);
@@ -455,9 +475,10 @@ __device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec
   c = __builtin_sqrtf(1.0f - c * c);
   const bool edge = c > kFeatureAngle;
 
-  Svd S;
-  double b[kRows];
-  for (int i = 0; i < kRows; ++i) {
+  SvdT<R> S;
+  S.rows = n;
+  double b[R];
+  for (int i = 0; i < n; ++i) {
     S.u[i][0] = normals[i].x;
     S.u[i][1] = normals[i].y;
     S.u[i][2] = normals[i].z;
@@ -465,8 +486,8 @@ __device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec
     b[i] = static_cast<double>(dot3(r, normals[i]));
   }
   double tau_v[2] = {0.0, 0.0};
-  bidiagonalise(S.u, S.d, tau_v);
-  unpack(S.u, S.v, S.d, tau_v);
+  bidiagonalise<R>(S.u, S.d, tau_v, n);
+  unpack<R>(S.u, S.v, S.d, tau_v, n);
   diagonalise(S, S.d, tau_v, 1000, stalled);
   sort_singular(S);
   if (edge) {  // qef.rs:51-63: drop the smallest singular value
@@ -485,7 +506,7 @@ __device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec
   if (S.d[1] < 0.1) S.d[1] = 0.0;
   if (S.d[0] < 0.1) S.d[0] = 0.0;
   double w[3] = {0.0, 0.0, 0.0};
-  for (int i = 0; i < kRows; ++i)
+  for (int i = 0; i < n; ++i)
     if (b[i] != 0.0)
       for (int j = 0; j < 3; ++j) w[j] += b[i] * S.u[i][j];
   for (int i = 0; i < 3; ++i)
@@ -497,6 +518,13 @@ __device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec
     x[i] = static_cast<float>(t);
   }
   return {com.x + x[0], com.y + x[1], com.z + x[2]};
+}
+
+// The dual-contouring form: the 8 cell corners.
+__device__ __forceinline__ Vec3f place_feature(const Vec3f corners[8], const Vec3f normals[8],
+                                               bool* stalled) {
+  bool feature;
+  return place_feature_n<kRows>(corners, normals, kRows, stalled, &feature);
 }
 
 }  // namespace qef

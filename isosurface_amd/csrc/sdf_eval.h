@@ -262,4 +262,132 @@ __device__ __forceinline__ void sdf_eval_vector(const DevOp* __restrict__ ops, i
   out[2] = top[2];
 }
 
+// ---------------------------------------------------------------------------
+// HermiteSource::sample_normal (source.rs:41-48): the shapes' analytic
+// normals, not normalised, for programs of shapes, unions and translations
+// (Intersection and Difference have no HermiteSource; the host rejects
+// them).  Union takes the componentwise minimum of its operands' normals
+// (csg.rs:47-51).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void prim_normal(uint32_t code, const float* prm, float x, float y,
+                                            float z, float* out) {
+  if (code == kSphere) {  // sphere.rs:58-62
+    out[0] = x;
+    out[1] = y;
+    out[2] = z;
+  } else if (code == kTorus) {  // torus.rs:98-107
+    // (x, y, 0).normalised().unwrap_or((1, 0, 0)) * radius
+    const float l = __builtin_sqrtf((x * x + y * y) + 0.0f * 0.0f);
+    float rx = 1.0f, ry = 0.0f, rz = 0.0f;
+    if (!(__builtin_fabsf(l) < 1.1920929e-07f)) {
+      rx = x / l;
+      ry = y / l;
+      rz = 0.0f / l;
+    }
+    out[0] = x - rx * prm[0];
+    out[1] = y - ry * prm[0];
+    out[2] = z - rz * prm[0];
+  } else if (code == kBox) {  // rectangular_prism.rs:76-79, vector.rs:278-287
+    const float ax = __builtin_fabsf(x), ay = __builtin_fabsf(y), az = __builtin_fabsf(z);
+    out[0] = 0.0f;
+    out[1] = 0.0f;
+    out[2] = 0.0f;
+    if (ax > ay && ax > az)
+      out[0] = x;
+    else if (ay > az)
+      out[1] = y;
+    else
+      out[2] = z;
+  } else {  // kCylinder, cylinder.rs:74-84
+    const float zz = __builtin_fabsf(z) / prm[1];
+    const float r = __builtin_sqrtf(x * x + y * y) / prm[0];
+    if (zz > r) {
+      out[0] = 0.0f;
+      out[1] = 0.0f;
+      out[2] = z;
+    } else {
+      out[0] = x;
+      out[1] = y;
+      out[2] = 0.0f;
+    }
+  }
+}
+
+__device__ __forceinline__ void sdf_eval_hermite(const DevOp* __restrict__ ops, int n, float px,
+                                                 float py, float pz, float* out) {
+  float top[3] = {0.0f, 0.0f, 0.0f};
+  float st[kMaxValueStack][3];
+  int sp = 0;
+  float cx = px, cy = py, cz = pz;
+  float sx[kMaxPointStack], sy[kMaxPointStack], sz[kMaxPointStack];
+  int psp = 0;
+  bool have_top = false;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t code = ops[i].code;
+    if (code <= kCylinder) {
+      float qx = cx, qy = cy, qz = cz;
+      if (ops[i].translated) {
+        qx = cx - ops[i].off[0];
+        qy = cy - ops[i].off[1];
+        qz = cz - ops[i].off[2];
+      }
+      float v[3];
+      prim_normal(code, ops[i].prm, qx, qy, qz, v);
+      if (ops[i].combine == kPush) {
+        if (have_top) {
+          st[sp][0] = top[0];
+          st[sp][1] = top[1];
+          st[sp][2] = top[2];
+          ++sp;
+        }
+        top[0] = v[0];
+        top[1] = v[1];
+        top[2] = v[2];
+        have_top = true;
+      } else {  // kMin: Union(top, v)
+        top[0] = fminf(top[0], v[0]);
+        top[1] = fminf(top[1], v[1]);
+        top[2] = fminf(top[2], v[2]);
+      }
+    } else if (code == kPushTranslate) {
+      sx[psp] = cx;
+      sy[psp] = cy;
+      sz[psp] = cz;
+      ++psp;
+      cx = cx - ops[i].off[0];
+      cy = cy - ops[i].off[1];
+      cz = cz - ops[i].off[2];
+    } else if (code == kPopTranslate) {
+      --psp;
+      cx = sx[psp];
+      cy = sy[psp];
+      cz = sz[psp];
+    } else {  // kUnion of (below, top)
+      --sp;
+      top[0] = fminf(st[sp][0], top[0]);
+      top[1] = fminf(st[sp][1], top[1]);
+      top[2] = fminf(st[sp][2], top[2]);
+    }
+  }
+  out[0] = top[0];
+  out[1] = top[1];
+  out[2] = top[2];
+}
+
+// CentralDifference::sample_normal (source.rs:82-94): not normalised.
+__device__ __forceinline__ void sdf_central_difference(const DevOp* ops, int n_ops, float px,
+                                                       float py, float pz, float eps,
+                                                       float* out) {
+  const float vx = sdf_eval(ops, n_ops, px + eps, py + 0.0f, pz + 0.0f) -
+                   sdf_eval(ops, n_ops, px - eps, py - 0.0f, pz - 0.0f);
+  const float vy = sdf_eval(ops, n_ops, px + 0.0f, py + eps, pz + 0.0f) -
+                   sdf_eval(ops, n_ops, px - 0.0f, py - eps, pz - 0.0f);
+  const float vz = sdf_eval(ops, n_ops, px + 0.0f, py + 0.0f, pz + eps) -
+                   sdf_eval(ops, n_ops, px - 0.0f, py - 0.0f, pz - eps);
+  const float e2 = 2.0f * eps;
+  out[0] = vx / e2;
+  out[1] = vy / e2;
+  out[2] = vz / e2;
+}
+
 }  // namespace iso

@@ -287,6 +287,28 @@ class Context:
         nv, ni = int(out.vertex_count), int(out.index_count)
         return Mesh(verts[:nv].copy(), idx[:ni].copy())
 
+    def extract_emc(self, desc: A.IsoGridDesc, epsilon: float, vertex_capacity: int = 0,
+                    index_capacity: int = 0) -> Mesh:
+        """iso_extract_emc into host arrays (grown on ISO_ERR_CAPACITY)."""
+        vcap = max(int(vertex_capacity), 1)
+        icap = max(int(index_capacity), 3)
+        verts = np.empty((vcap, 3), dtype=np.float32)
+        idx = np.empty(icap, dtype=np.uint32)
+        out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
+                           A.ISO_MEMORY_HOST, 0)
+        st = self.lib.iso_extract_emc(self.handle, C.byref(desc), C.c_float(epsilon),
+                                      C.byref(out))
+        if st == A.ISO_ERR_CAPACITY:
+            nv, ni = int(out.vertex_count), int(out.index_count)
+            verts = np.empty((max(nv, 1), 3), dtype=np.float32)
+            idx = np.empty(max(ni, 3), dtype=np.uint32)
+            out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, max(nv, 1), max(ni, 3), 0, 0,
+                               A.ISO_MEMORY_HOST, 0)
+            st = self.lib.iso_mesh_fetch(self.handle, C.byref(out))
+        self.check(st, "iso_extract_emc")
+        nv, ni = int(out.vertex_count), int(out.index_count)
+        return Mesh(verts[:nv].copy(), idx[:ni].copy())
+
     def extract_lhmc(self, max_depth: int, vertex_capacity: int = 0,
                      index_capacity: int = 0) -> Mesh:
         """iso_extract_lhmc into host arrays (grown on ISO_ERR_CAPACITY)."""
@@ -518,6 +540,38 @@ class DualContouring:
 
     def extract(self, source: CentralDifference, extractor: Extractor) -> None:
         """DualContouring::extract (dual_contouring.rs:65-105)."""
+        _replay(self.extract_mesh(source), extractor)
+
+
+class ExtendedMarchingCubes:
+    """ExtendedMarchingCubes::new(size) (extended_marching_cubes.rs:43-57) on
+    the GPU.  The source is a program Source (its HermiteSource normals:
+    shapes, unions and translations) or CentralDifference(Source) (normals by
+    central differences); its values are the Directed distances."""
+
+    def __init__(self, size: int, context: Optional[Context] = None):
+        if size < 2:
+            raise ValueError("size must be at least 2")
+        self.size = int(size)
+        self.ctx = context or default_context()
+        self._last_counts = (0, 0)
+
+    def extract_mesh(self, source) -> Mesh:
+        if isinstance(source, CentralDifference):
+            epsilon = float(source.epsilon)
+        elif isinstance(source, Source):
+            epsilon = 0.0
+        else:
+            raise TypeError("ExtendedMarchingCubes needs a program Source (analytic normals) "
+                            "or CentralDifference(Source)")
+        self.ctx.upload_program(source.program())
+        desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_PROGRAM)
+        mesh = self.ctx.extract_emc(desc, epsilon, *self._last_counts)
+        self._last_counts = (len(mesh.vertices), len(mesh.indices))
+        return mesh
+
+    def extract(self, source, extractor: Extractor) -> None:
+        """ExtendedMarchingCubes::extract (extended_marching_cubes.rs:59-90)."""
         _replay(self.extract_mesh(source), extractor)
 
 

@@ -100,6 +100,25 @@ int main(int argc, char** argv) {
     dump(out + ".directed_host.u32", dhi);
     std::printf("directed: %zu vertices, %zu triangles\n", dv.size() / 3, di.size() / 3);
 
+    // ExtendedMarchingCubes::new(33) of a cube (its own normals) and of the
+    // cube behind CentralDifference.
+    const auto cube = isosurface::DemoSource(isosurface::Translate<isosurface::RectangularPrism>{
+        {0.013f, 0.021f, -0.017f}, isosurface::RectangularPrism{{0.2f, 0.2f, 0.2f}}});
+    isosurface::ExtendedMarchingCubes emc(33, ctx);
+    std::vector<float> ev;
+    std::vector<uint32_t> ei;
+    isosurface::IndexedVertices esink(ev, ei);
+    emc.extract(cube, esink);
+    dump(out + ".emc.f32", ev);
+    dump(out + ".emc.u32", ei);
+    std::vector<float> ecv;
+    std::vector<uint32_t> eci;
+    isosurface::IndexedVertices ecsink(ecv, eci);
+    emc.extract(isosurface::CentralDifference<std::remove_cv_t<decltype(cube)>>{cube, 1e-4f}, ecsink);
+    dump(out + ".emc_cd.f32", ecv);
+    dump(out + ".emc_cd.u32", eci);
+    std::printf("extended: %zu vertices, %zu triangles\n", ev.size() / 3, ei.size() / 3);
+
     // Error channel: a malformed program surfaces as isosurface::Error.
     try {
       isosurface::Program bad{isosurface::detail::node(ISO_OP_UNION)};

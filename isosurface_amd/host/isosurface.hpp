@@ -382,6 +382,62 @@ struct CentralDifference {
   void lower(Program& p) const { source.lower(p); }
 };
 
+template <class T>
+struct is_central_difference : std::false_type {};
+template <class S>
+struct is_central_difference<CentralDifference<S>> : std::true_type {};
+
+// ---- ExtendedMarchingCubes (extended_marching_cubes.rs:43-186) ----------------
+
+class ExtendedMarchingCubes {
+ public:
+  ExtendedMarchingCubes(size_t size, Context& ctx) : size_(size), ctx_(ctx) {
+    if (size < 2) throw Error(ISO_ERR_INVALID_ARGUMENT, "size must be at least 2");
+  }
+
+  // ExtendedMarchingCubes::extract: the source's Directed values, and its
+  // HermiteSource normals — the shapes' own (a lowerable source of shapes,
+  // unions and translations) or CentralDifference<S>'s.
+  template <class S>
+  void extract(const S& source, Extractor& extractor) {
+    static_assert(is_lowerable<S>::value, "ExtendedMarchingCubes::extract takes a lowerable source");
+    float epsilon = 0.0f;
+    if constexpr (is_central_difference<S>::value) epsilon = source.epsilon;
+    Program prog;
+    source.lower(prog);
+    ctx_.check(iso_program_upload(ctx_.get(), prog.data(), static_cast<uint32_t>(prog.size())),
+               "iso_program_upload");
+    iso_grid_desc desc;
+    ctx_.check(iso_grid_desc_reference(static_cast<int64_t>(size_), ISO_SOURCE_PROGRAM, &desc),
+               "iso_grid_desc_reference");
+    iso_mesh_out out{};
+    out.memory = ISO_MEMORY_HOST;
+    out.vertices = vertices_.data();
+    out.indices = indices_.data();
+    out.vertex_capacity = vertices_.size() / 3;
+    out.index_capacity = indices_.size();
+    int st = iso_extract_emc(ctx_.get(), &desc, epsilon, &out);
+    if (st == ISO_ERR_CAPACITY) {
+      vertices_.resize(3 * out.vertex_count);
+      indices_.resize(out.index_count);
+      out.vertices = vertices_.data();
+      out.indices = indices_.data();
+      out.vertex_capacity = out.vertex_count;
+      out.index_capacity = out.index_count;
+      st = iso_mesh_fetch(ctx_.get(), &out);
+    }
+    ctx_.check(st, "iso_extract_emc");
+    extractor.extract_vertices(vertices_.data(), out.vertex_count);
+    for (uint64_t i = 0; i < out.index_count; ++i) extractor.extract_index(indices_[i]);
+  }
+
+ private:
+  size_t size_;
+  Context& ctx_;
+  std::vector<float> vertices_;
+  std::vector<uint32_t> indices_;
+};
+
 // IndexedInterleavedNormals (extractor.rs:97-127): xyz then the source's
 // normal per vertex, and the indices.  OnlyInterleavedNormals
 // (extractor.rs:46-69) is the same without indices (keep_indices = false).

@@ -49,3 +49,12 @@ def test_cpp_mirror_matches_oracle(tmp_path):
         di = np.fromfile(f"{out}.{kind}.u32", dtype="<u4")
         assert dv.tobytes() == ref.vertices.tobytes(), kind
         assert np.array_equal(di, ref.indices), kind
+    # ExtendedMarchingCubes: a cube with its own normals, and behind CentralDifference
+    from isosurface_amd.sources import DemoSource, RectangularPrism, Translate
+    cube = DemoSource(Translate((0.013, 0.021, -0.017), RectangularPrism((0.2, 0.2, 0.2))))
+    for kind, eps in (("emc", 0.0), ("emc_cd", 1e-4)):
+        ref = O.extract_emc(cube.program(), O.reference_desc(33, 0), epsilon=eps)
+        ev = np.fromfile(f"{out}.{kind}.f32", dtype="<f4")
+        ei = np.fromfile(f"{out}.{kind}.u32", dtype="<u4")
+        assert ev.tobytes() == ref.vertices.tobytes(), kind
+        assert np.array_equal(ei, ref.indices), kind
