@@ -110,6 +110,9 @@ struct iso_ctx {
   MeshGraphKey m_key{};
   uint64_t active_capacity = 0;
   unsigned long long* h_totals = nullptr;  // pinned
+  // host-coherent totals that emit writes (host and device addresses)
+  unsigned long long* h_coh = nullptr;
+  unsigned long long* d_coh = nullptr;
 
   // mesh retained after a host-memory extraction whose buffers were too small
   uint64_t kept_vertices = 0, kept_indices = 0;
@@ -272,6 +275,16 @@ int iso_ctx_create(int device, void* stream, iso_ctx** out) {
     delete c;
     return ISO_ERR_OUT_OF_MEMORY;
   }
+  // Host-coherent totals for device-output extractions (optional: without
+  // them the totals are copied back after emit).
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_coh), iso::kTotSlots * sizeof(unsigned long long),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_coh), c->h_coh, 0) != hipSuccess) {
+    if (c->h_coh) (void)hipHostFree(c->h_coh);
+    c->h_coh = nullptr;
+    c->d_coh = nullptr;
+    (void)hipGetLastError();
+  }
   *out = c;
   return ISO_OK;
 }
@@ -301,6 +314,7 @@ int iso_ctx_destroy(iso_ctx* c) {
   if (c->m_graph) (void)hipGraphExecDestroy(c->m_graph);
   for (DevBuf* b : bufs) release(b);
   if (c->h_totals) (void)hipHostFree(c->h_totals);
+  if (c->h_coh) (void)hipHostFree(c->h_coh);
   if (c->h_dc) (void)hipHostFree(c->h_dc);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -543,6 +557,17 @@ int classify_and_count(iso_ctx* c, const float* grid, const iso::Lattice& L, con
   return ISO_OK;
 }
 
+// After an emit launched with B.host_totals set: the totals it handed over.
+int read_emitted_totals(iso_ctx* c) {
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  const volatile unsigned long long* h = c->h_coh;
+  for (int i = 0; i < iso::kTotSlots; ++i) c->h_totals[i] = h[i];
+  if (c->h_totals[iso::kTotErrors] & 2ull)
+    return fail(c, ISO_ERR_DEVICE, "count scan look-back did not complete");
+  return ISO_OK;
+}
+
 int read_totals(iso_ctx* c, const iso::McBuffers& B) {
   ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.totals, iso::kTotSlots * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, c->stream));
@@ -598,6 +623,11 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
     B.indices = out->indices;
     B.vertex_capacity = out->vertices ? out->vertex_capacity : 0;
     B.index_capacity = out->indices ? out->index_capacity : 0;
+    // Emit hands the totals to host-coherent memory: no copy after it.
+    B.host_totals = c->d_coh;
+    auto totals_after_emit = [&]() {
+      return B.host_totals ? read_emitted_totals(c) : read_totals(c, B);
+    };
     // Graph replay: the launches below depend only on the key's fields
     // (sample and program memory is read when the graph runs), so a capture
     // replays for any later call with an equal key.
@@ -620,7 +650,7 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
       const MeshGraphKey k = key_of(B);
       if (std::memcmp(&k, &c->m_key, sizeof(k)) == 0) {
         ISO_HIP(c, hipGraphLaunch(c->m_graph, c->stream));
-        if ((st = read_totals(c, B)) != ISO_OK) return st;
+        if ((st = totals_after_emit()) != ISO_OK) return st;
         const uint64_t n_act = c->h_totals[iso::kTotActive];
         replayed = n_act <= B.active_capacity;
         // A longer active-cell list changes the key: extract and capture anew.
@@ -630,7 +660,7 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
     for (int attempt = 0; attempt < 2 && !replayed; ++attempt) {
       if ((st = classify_and_count(c, grid, L, D, B, signs, values)) != ISO_OK) return st;
       ISO_HIP(c, timed(c, "mc_emit", emit));
-      if ((st = read_totals(c, B)) != ISO_OK) return st;
+      if ((st = totals_after_emit()) != ISO_OK) return st;
       const uint64_t n_act = c->h_totals[iso::kTotActive];
       if (n_act <= B.active_capacity) break;
       // The active-cell list was too short: grow it and extract again.

@@ -89,6 +89,9 @@ struct McBuffers {
   uint32_t* indices;
   uint64_t vertex_capacity;
   uint64_t index_capacity;
+  // When set (host-coherent memory, device address): emit copies the
+  // extraction's totals there, so the host reads them without a copy.
+  unsigned long long* host_totals;
 };
 
 // Each returns hipSuccess or the launch error.  The signs launchers also

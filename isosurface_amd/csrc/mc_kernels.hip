@@ -358,31 +358,59 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
         slot_store(&B.agg[blk].t, kSlotAggregate | total.t);
         slot_store(&B.agg[blk].a, kSlotAggregate | total.a);
       }
-      int64_t j = static_cast<int64_t>(blk) - 1;  // nearest predecessor of the window
+      // Each step inspects kWindows * 64 predecessors at once.  One window:
+      // wider steps (4 windows measured) made the 256^3 count slower, as a
+      // wider window more often holds a predecessor that has not published.
+      constexpr int kWindows = 1;
+      int64_t j = static_cast<int64_t>(blk) - 1;  // nearest predecessor of the step
       uint32_t spins = 0;
       while (true) {
-        const int64_t idx = j - lane;
-        // A predecessor is inclusive when all three inclusive words carry
-        // the inclusive status, aggregate when all three aggregate words do.
-        // Blocks before the first one act as an inclusive zero.
-        unsigned long long xv = 0ull, xt = 0ull, xa = 0ull;
-        int state = 2;
-        if (idx >= 0) {
-          xv = slot_load(&B.incl[idx].v);
-          xt = slot_load(&B.incl[idx].t);
-          xa = slot_load(&B.incl[idx].a);
-          if (((xv & xt & xa) >> 62) != 2ull) {
-            xv = slot_load(&B.agg[idx].v);
-            xt = slot_load(&B.agg[idx].t);
-            xa = slot_load(&B.agg[idx].a);
-            state = ((xv >> 62) == 1ull && (xt >> 62) == 1ull && (xa >> 62) == 1ull) ? 1 : 0;
+        unsigned long long xv[kWindows], xt[kWindows], xa[kWindows];
+        int state[kWindows];
+#pragma unroll
+        for (int k = 0; k < kWindows; ++k) {
+          const int64_t idx = j - lane - 64 * k;
+          // A predecessor is inclusive when all three inclusive words carry
+          // the inclusive status, aggregate when all three aggregate words
+          // do.  Blocks before the first one act as an inclusive zero.
+          xv[k] = 0ull;
+          xt[k] = 0ull;
+          xa[k] = 0ull;
+          state[k] = 2;
+          if (idx >= 0) {
+            xv[k] = slot_load(&B.incl[idx].v);
+            xt[k] = slot_load(&B.incl[idx].t);
+            xa[k] = slot_load(&B.incl[idx].a);
+            if (((xv[k] & xt[k] & xa[k]) >> 62) != 2ull) {
+              xv[k] = slot_load(&B.agg[idx].v);
+              xt[k] = slot_load(&B.agg[idx].t);
+              xa[k] = slot_load(&B.agg[idx].a);
+              state[k] = ((xv[k] >> 62) == 1ull && (xt[k] >> 62) == 1ull &&
+                          (xa[k] >> 62) == 1ull) ? 1 : 0;
+            }
           }
         }
-        const uint64_t incl = __ballot(state == 2);
-        const uint64_t zero = __ballot(state == 0);
-        const int stop = incl ? __builtin_ctzll(incl) : 64;
-        const uint64_t upto = stop < 64 ? (2ull << stop) - 1ull : ~0ull;
-        if (zero & upto) {
+        // The nearest inclusive predecessor (window sk, lane stop); every
+        // nearer one must have published its aggregate.
+        int sk = kWindows, stop = 64;
+        bool waiting = false;
+#pragma unroll
+        for (int k = 0; k < kWindows; ++k) {
+          const uint64_t incl = __ballot(state[k] == 2);
+          const uint64_t zero = __ballot(state[k] == 0);
+          const int st = incl ? __builtin_ctzll(incl) : 64;
+          const uint64_t upto = st < 64 ? (2ull << st) - 1ull : ~0ull;
+          if (zero & upto) {
+            waiting = true;
+            break;
+          }
+          if (st < 64) {
+            sk = k;
+            stop = st;
+            break;
+          }
+        }
+        if (waiting) {
           if (++spins > kLookBackSpinLimit) {
             if (lane == 0) atomicOr(&B.totals[kTotErrors], 2ull);
             break;
@@ -391,7 +419,14 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
           continue;
         }
         ScanTriple x{0ull, 0ull, 0ull};
-        if (lane <= stop) x = ScanTriple{xv & kSlotValue, xt & kSlotValue, xa & kSlotValue};
+#pragma unroll
+        for (int k = 0; k < kWindows; ++k) {
+          if (k < sk || (k == sk && lane <= stop)) {
+            x.v += xv[k] & kSlotValue;
+            x.t += xt[k] & kSlotValue;
+            x.a += xa[k] & kSlotValue;
+          }
+        }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
           x.v += __shfl_xor(x.v, d, 64);
@@ -401,8 +436,8 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
         prefix.v += x.v;
         prefix.t += x.t;
         prefix.a += x.a;
-        if (stop < 64) break;
-        j -= 64;
+        if (sk < kWindows) break;
+        j -= 64 * kWindows;
       }
       if (lane == 0) {
         slot_store(&B.incl[blk].v, kSlotInclusive | (prefix.v + total.v));
@@ -452,6 +487,7 @@ struct EmitArgs {
   uint32_t* indices;
   uint64_t vcap, icap;
   const unsigned long long* totals;
+  unsigned long long* host_totals;  // or null (McBuffers::host_totals)
   // kEmitBand: brick flags (bit 0: exact samples), bricks per x and y row
   const uint8_t* brick_flags;
   int64_t nbx, nby;
@@ -515,6 +551,9 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
   __shared__ uint8_t s_ocase[7][kEmitThreads];
 
   const int tid = threadIdx.x;
+  // The count kernel's totals are final here (the previous launch): hand
+  // them to the host with the mesh.
+  if (A.host_totals && blockIdx.x == 0 && tid < kTotSlots) A.host_totals[tid] = A.totals[tid];
   uint64_t n_act = A.totals[kTotActive];
   if (n_act > A.active_capacity) n_act = A.active_capacity;
 
@@ -602,10 +641,11 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
       for (int dy = 0; dy < 3; ++dy) {
         if (cy - 1 + dy >= 0 && cz - 1 + dz >= 0) {
           const int64_t q = fc - 1 + (dy - 1) * nx + (dz - 1) * nxy;  // sample x-1 (>= -1)
-          const int64_t wq = q >> 6;
-          const int sq = static_cast<int>(q & 63);
-          const uint64_t lo = S[wq], hi = S[wq + 1];
-          const uint64_t v = sq ? ((lo >> sq) | (hi << (64 - sq))) : lo;
+          // The three bits q..q+2 lie in the 64 bits that start at q's
+          // 32-bit word: one 8-byte load (4-byte aligned) per row.
+          uint64_t v;
+          __builtin_memcpy(&v, reinterpret_cast<const uint32_t*>(S) + (q >> 5), sizeof(v));
+          v >>= static_cast<int>(q & 31);
           cube |= static_cast<uint32_t>(v & 7ull) << (3 * dy + 9 * dz);
         }
       }
@@ -957,6 +997,7 @@ EmitArgs emit_args(const McBuffers& B) {
   A.vcap = B.vertex_capacity;
   A.icap = B.index_capacity;
   A.totals = B.totals;
+  A.host_totals = B.host_totals;
   A.brick_flags = nullptr;
   A.nbx = A.nby = 0;
   return A;
