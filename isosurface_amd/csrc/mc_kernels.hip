@@ -3,8 +3,9 @@
 // One extraction is three kernels:
 //   1. signs  one bit per lattice sample, "inside" = !(v > 0) exactly as
 //             classify_corners (marching_cubes_impl.rs:26-37, distance.rs:52-54).
-//             From a dense FP32 grid this is a pure HBM stream (16-byte
-//             loads, wave64 ballots); from a program it is the SDF
+//             From a dense FP32 grid this is a pure HBM stream (each load
+//             instruction reads 256 contiguous bytes per wavefront and its
+//             ballot is one sign word); from a program it is the SDF
 //             evaluation.  The launch also resets the extraction's state.
 //   2. count  one thread per 64-cell segment of a cell row: the segment's
 //             eight corner bit-planes come from four row words, vertex
