@@ -28,7 +28,7 @@ Printed JSON line:
                 spheres: the Fourier-feature MLP is trained on the GPU on 2M
                 samples the plain path generates, its margin calibrated on
                 held-out samples; band extraction is timed against the plain
-                dense GPU extraction of the same program, which judges it
+                dense GPU extraction of the same program, which checks it
                 (identical mesh, missed surface cells)
   dual_contouring, octree_mc
                 the sharp-feature workload (union of 32 seeded boxes):

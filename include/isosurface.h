@@ -291,7 +291,7 @@ typedef struct iso_batch_out {
 int iso_extract_batch(iso_ctx* ctx, iso_batch_out* out);
 
 /* ---- learned narrow band ----------------------------------------------------
- * No reference counterpart: this build's own acceleration of dense program
+ * No reference counterpart: the library's own acceleration of dense program
  * extraction.  A Fourier-feature MLP (99 features: p, sin and cos of
  * pi 2^(k/2) p for k < 16; three hidden layers of 64 ReLU units; bf16 MFMA
  * with f32 accumulation) is trained on (point, SDF) samples to predict the

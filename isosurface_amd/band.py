@@ -6,7 +6,7 @@
     band.train(pts, sdf, steps=..., batch=16384)
     margin = band.calibrate(pts2, sdf2, half_diag)     # held-out cull condition
     mesh, stats = band.extract_mesh(desc, margin)        # exact SDF only in the band
-    band.missed_cells(desc)                              # judged by the dense path
+    band.missed_cells(desc)                              # checked against the dense path
 """
 from __future__ import annotations
 

@@ -69,7 +69,7 @@ struct iso_ctx {
   // scratch
   DevBuf signs, seg, tplanes, vplanes, agg, incl, active, totals, vout, iout, pts, vals;
   DevBuf pvals;  // program samples of the current extraction (emit reads them)
-  DevBuf judge;  // dense sign words that judge a band extraction
+  DevBuf dense_check;  // dense sign words a band extraction is checked against
   DevBuf nrm;    // normals of iso_sample_normals (host-memory calls)
   // directed values: the uploaded grid, and a program's values of the
   // current extraction (3 floats per lattice point)
@@ -307,7 +307,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_v,       &c->nn_grad,    &c->nn_frags,  &c->nn_loss,  &c->nn_x,
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
-                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->judge,  &c->nrm,
+                    &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->dense_check,  &c->nrm,
                     &c->dgrid,      &c->dvals,      &c->emc_cells, &c->emc_first,
                     &c->emc_flips,  &c->emc_ctr};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
@@ -359,7 +359,7 @@ int iso_program_upload(iso_ctx* c, const iso_node* nodes, uint32_t n_nodes) {
         nodes[i].op == ISO_OP_DIFFERENCE)
       c->prog_hermite = false;
   }
-  c->band_valid = false;  // a band extraction is judged against its own program
+  c->band_valid = false;  // a band extraction is checked against its own program
   return ISO_OK;
 }
 
@@ -1949,9 +1949,9 @@ int iso_band_missed_cells(iso_ctx* c, const iso_grid_desc* desc, uint64_t* misse
   // extraction clears band_valid); the dense ones go to a buffer of their own.
   iso::McBuffers B{};
   if ((st = prepare_mc(c, D, 0, &B)) != ISO_OK) return st;
-  ISO_HIP(c, ensure(&c->judge, (D.n_sign_words + 3) * sizeof(uint64_t)));
+  ISO_HIP(c, ensure(&c->dense_check, (D.n_sign_words + 3) * sizeof(uint64_t)));
   iso::McBuffers J = B;
-  J.signs = static_cast<uint64_t*>(c->judge.ptr) + 1;
+  J.signs = static_cast<uint64_t*>(c->dense_check.ptr) + 1;
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   ISO_HIP(c, iso::launch_signs_program(ops, c->n_prog, L, D, J, nullptr, c->stream));
   unsigned long long* counters = static_cast<unsigned long long*>(c->br_count.ptr);

@@ -1,7 +1,7 @@
 """CPU restatement of the narrow-band network — TEST INFRASTRUCTURE ONLY.
 
 The learned narrow band has no counterpart in the reference crate: it is
-this build's own method (a Fourier-feature MLP that predicts the truncated
+the library's own method (a Fourier-feature MLP that predicts the truncated
 SDF at 8^3-brick centres).  This module restates the network the GPU runs,
 with the same rounding points, so the GPU forward pass and gradients can be
 checked against it; its own backward pass is pinned by finite differences

@@ -1,5 +1,5 @@
 """The learned narrow band on the GPU: MFMA forward and gradients against
-the CPU restatement (oracle/mlp.py), training, and band extraction judged
+the CPU restatement (oracle/mlp.py), training, and band extraction checked
 by the dense extraction (the mesh must be identical when no surface cell is
 culled)."""
 import numpy as np
@@ -102,7 +102,7 @@ def test_band_mesh_equals_dense(ctx, size):
     ctx.upload_program(src.program())
     desc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
     mesh, stats = band.extract_mesh(desc, margin)
-    missed = band.missed_cells(desc)  # judges the last extraction: before the dense one
+    missed = band.missed_cells(desc)  # checks the last extraction: before the dense one
     dense = MarchingCubes(size, ctx).extract_mesh(src)
     print(f"size {size} margin {margin:.4f} band {stats.band_bricks}/{stats.bricks} "
           f"missed {missed}", flush=True)
@@ -124,14 +124,14 @@ def test_band_without_culling_is_dense(ctx):
     dense = MarchingCubes(40, ctx).extract_mesh(src)
     assert mesh.vertices.tobytes() == dense.vertices.tobytes()
     assert np.array_equal(mesh.indices, dense.indices)
-    # any other extraction replaces the sign words the judge compares
+    # any other extraction replaces the sign words the dense check compares
     with pytest.raises(A.IsoError):
         band.missed_cells(desc)
 
 
 def test_untrained_band_reports_its_misses(ctx):
     # An untrained network with no margin culls surface bricks: the missed
-    # count is what the dense judge finds, and only a mesh without misses
+    # count is what the dense check finds, and only a mesh without misses
     # equals the dense one.
     src = W.sphere_source()
     band = NarrowBand(ctx)
@@ -143,7 +143,7 @@ def test_untrained_band_reports_its_misses(ctx):
     dense = MarchingCubes(48, ctx).extract_mesh(src)
     same = (len(mesh.indices) == len(dense.indices) and np.array_equal(mesh.indices, dense.indices)
             and mesh.vertices.tobytes() == dense.vertices.tobytes())
-    # the judge counts cells whose case differs: zero exactly when the meshes agree
+    # the dense check counts cells whose case differs: zero exactly when the meshes agree
     assert (missed == 0) == same
 
 
