@@ -289,13 +289,15 @@ def batched_chunks(ctx: Context, steps: int) -> dict:
 
 
 def brick_centres(size: int) -> np.ndarray:
-    """Centres of the 8^3-cell bricks of MarchingCubes::new(size) (x fastest),
+    """Centres of the band bricks of MarchingCubes::new(size) (x fastest),
     as the band kernels place them."""
+    from isosurface_amd.band import brick_size
+    bs = brick_size()
     sp = np.float32(1.0) / np.float32(size - 1)
     axes = []
     for n in (size, size, size + 1):
-        lo = np.arange(0, n - 1, 8)
-        hi = np.minimum(lo + 8, n - 1)
+        lo = np.arange(0, n - 1, bs)
+        hi = np.minimum(lo + bs, n - 1)
         axes.append(((0.5 * (lo + hi)).astype(np.float32) * sp).astype(np.float32))
     z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
     return np.ascontiguousarray(np.stack([x, y, z], -1).reshape(-1, 3), np.float32)

@@ -296,7 +296,7 @@ int iso_extract_batch(iso_ctx* ctx, iso_batch_out* out);
  * pi 2^(k/2) p for k < 16; three hidden layers of 64 ReLU units; bf16 MFMA
  * with f32 accumulation) is trained on (point, SDF) samples to predict the
  * truncated SDF.  iso_extract_mc_band predicts it at the centre of every
- * 8x8x8-cell brick, evaluates the program exactly only in bricks whose
+ * 4x4x4-cell brick, evaluates the program exactly only in bricks whose
  * prediction lies within their half-diagonal plus `margin` of zero, gives
  * every other sample its brick's predicted sign, and extracts as
  * iso_extract_mc does.  The result equals iso_extract_mc's whenever no
@@ -331,6 +331,9 @@ typedef struct iso_band_stats {
  * certified (the method without the network, for comparison). */
 #define ISO_BAND_CERTIFY 1u
 #define ISO_BAND_NO_NETWORK 2u
+
+/* Cells per edge of a narrow-band brick (the unit the band culls). */
+int iso_band_brick_size(void);
 
 /* Deterministic He-uniform initialisation of the context's network. */
 int iso_band_init(iso_ctx* ctx, uint32_t seed);

@@ -147,6 +147,7 @@ SIGNATURES = [
                                  C.POINTER(C.c_float)]),
     ("iso_band_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                     C.c_float, C.c_float, C.c_void_p, C.POINTER(C.c_float)]),
+    ("iso_band_brick_size", C.c_int, []),
     ("iso_band_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32]),
     ("iso_extract_mc_band", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float, C.c_uint32,
                                       C.POINTER(IsoMeshOut), C.POINTER(IsoBandStats)]),

@@ -136,9 +136,14 @@ class NarrowBand:
         return int(n.value)
 
 
+def brick_size() -> int:
+    """Cells per edge of a narrow-band brick (iso_band_brick_size)."""
+    return int(A.load().iso_band_brick_size())
+
+
 def brick_half_diagonal(size: int) -> float:
-    """Half-diagonal of an 8x8x8-cell brick of MarchingCubes::new(size)."""
-    return 0.5 * 8.0 * float(np.sqrt(3.0)) * float(np.float32(1.0) / np.float32(size - 1))
+    """Half-diagonal of a band brick of MarchingCubes::new(size)."""
+    return 0.5 * brick_size() * float(np.sqrt(3.0)) * float(np.float32(1.0) / np.float32(size - 1))
 
 
 def training_samples(ctx: Context, source: Source, n_uniform: int, n_surface: int,

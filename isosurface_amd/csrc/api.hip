@@ -1684,6 +1684,8 @@ int stage_in(iso_ctx* c, DevBuf* buf, const void* src, size_t bytes, int32_t mem
 
 extern "C" {
 
+int iso_band_brick_size(void) { return iso::kBrick; }
+
 int iso_band_init(iso_ctx* c, uint32_t seed) {
   if (!c) return ISO_ERR_INVALID_ARGUMENT;
   ISO_HIP(c, hipSetDevice(c->device));

@@ -1,6 +1,6 @@
 // Launch interface of the learned narrow-band extraction (host side).
 //
-// The lattice's cells are grouped in 8x8x8 bricks.  The network predicts
+// The lattice's cells are grouped in 4x4x4 bricks.  The network predicts
 // the (truncated) SDF at every brick centre; a brick whose prediction is
 // within its half-diagonal plus a calibrated margin of zero is in the band.
 // Band bricks get exact program samples; every other sample takes the
@@ -19,7 +19,7 @@
 
 namespace iso {
 
-constexpr int kBrick = 1 << kBandBrickShift;  // cells per brick edge (8)
+constexpr int kBrick = 1 << kBandBrickShift;  // cells per brick edge (4)
 
 // Brick flags
 enum : uint8_t {

@@ -28,7 +28,7 @@ struct McDims {
 constexpr int kCountBlock = 1024;  // threads per count workgroup (16 waves)
 constexpr int kCountRounds = 1;    // segments per count thread
 constexpr int kCountSegs = kCountBlock * kCountRounds;  // segments per count workgroup
-constexpr int kBandBrickShift = 3;  // narrow-band bricks: 8 x 8 x 8 cells
+constexpr int kBandBrickShift = 2;  // narrow-band bricks: 4 x 4 x 4 cells
 
 // Running totals of the decoupled look-back scan.
 struct ScanTriple {

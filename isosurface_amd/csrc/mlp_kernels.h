@@ -1,7 +1,7 @@
 // Launch interface of the narrow-band network kernels (host side).
 //
 // A Fourier-feature MLP (99 features padded to 112, three hidden layers of
-// 64 ReLU units, one output) predicts the truncated SDF at 8^3-brick
+// 64 ReLU units, one output) predicts the truncated SDF at 4^3-brick
 // centres.  Matrix products run on MFMA (v_mfma_f32_16x16x16_bf16): bf16
 // weights and activations, f32 accumulation; master weights and the Adam
 // moments are f32.  The parameter vector layout:

@@ -2,7 +2,7 @@
 
 The learned narrow band has no counterpart in the reference crate: it is
 the library's own method (a Fourier-feature MLP that predicts the truncated
-SDF at 8^3-brick centres).  This module restates the network the GPU runs,
+SDF at 4^3-brick centres).  This module restates the network the GPU runs,
 with the same rounding points, so the GPU forward pass and gradients can be
 checked against it; its own backward pass is pinned by finite differences
 in float64 (tests/test_oracle_mlp.py).  Only the tests, the smoke check and
