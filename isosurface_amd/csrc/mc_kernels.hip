@@ -285,10 +285,15 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
       uint64_t vp[4];
       owned_planes(sw, valid, cy == 0, cz == 0, w == 0, vp);
       nv = planes_count(vp, ~0ull);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) B.vplanes[4 * g + k] = vp[k];
       act[r] = mixed_cells(sw) & valid;
       na = popc64(act[r]);
+      // Only segments with a mixed cell are ever read back: emit reads the
+      // planes of its own segment and of owner cells, which hold a crossed
+      // edge and so are mixed.
+      if (na) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) B.vplanes[4 * g + k] = vp[k];
+      }
       uint64_t p0 = 0, p1 = 0, p2 = 0;  // per-cell triangle counts, bit-sliced
       for (uint64_t m = act[r]; m; m &= m - 1) {
         const int b = __builtin_ctzll(m);
@@ -298,9 +303,11 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
         p1 |= static_cast<uint64_t>((c >> 1) & 1u) << b;
         p2 |= static_cast<uint64_t>((c >> 2) & 1u) << b;
       }
-      B.tplanes[3 * g + 0] = p0;
-      B.tplanes[3 * g + 1] = p1;
-      B.tplanes[3 * g + 2] = p2;
+      if (na) {
+        B.tplanes[3 * g + 0] = p0;
+        B.tplanes[3 * g + 1] = p1;
+        B.tplanes[3 * g + 2] = p2;
+      }
     }
     const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt), ia = wave_incl_scan(na);
     if (__lane_id() == 63) {
