@@ -20,6 +20,10 @@ Printed JSON line:
   kernels       each kernel's own algorithmic bytes, duration and fraction
   program_path  the same extraction sampling the CSG program on the GPU
                 instead of reading the grid (VALU-bound, reported aside)
+  point_cloud, vertex_normals
+                PointCloud over the same resident grid, and the
+                CentralDifference(1e-4) normals of the mesh's vertices (the
+                normal sinks' pass), checked against the oracle's digests
   batched_chunks
                 4096 chunks of 16^3..64^3 cells, each its own seeded 1-8
                 primitive CSG program, extracted by one captured HIP graph
@@ -33,8 +37,9 @@ Printed JSON line:
   dual_contouring, octree_mc
                 the sharp-feature workload (union of 32 seeded boxes):
                 DualContouring at 128^3 with central-difference normals and
-                the f64 QEF, and LinearHashedMarchingCubes at max_depth 7;
-                each checked against the oracle's golden digests
+                the f64 QEF, LinearHashedMarchingCubes at max_depth 7,
+                MarchingCubes::<Directed> at 128^3 and ExtendedMarchingCubes
+                at 128^3; each checked against the oracle's golden digests
   cpu_baseline  the oracle (a single-threaded restatement of the reference
                 CPU path) on the same lattice, median of a few runs
 """
@@ -235,6 +240,87 @@ def side_workloads(ctx: Context, steps: int, with_cpu: bool) -> dict:
         lambda: from_oracle.extract_lhmc(src.program(), 7, 0))
     res["octree_mc"]["workload"] = ("LinearHashedMarchingCubes::new(7) on the union of 32 "
                                     "seeded boxes (cells/s counted against the 128^3 grid)")
+    res["directed_mc"] = run(
+        "iso_extract_mc_directed",
+        lambda out: ctx.lib.iso_extract_mc_directed(ctx.handle, C.byref(dc_desc), C.byref(out)),
+        "directed_boxes32_size128", 127 * 127 * 128,
+        lambda: from_oracle.extract_mc(src.program(), desc=from_oracle.reference_desc(128),
+                                       directed=True))
+    res["directed_mc"]["workload"] = ("MarchingCubes::<Directed>::new(128) on the union of 32 "
+                                      "seeded boxes (sample_vector evaluated on the GPU)")
+    res["extended_mc"] = run(
+        "iso_extract_emc",
+        lambda out: ctx.lib.iso_extract_emc(ctx.handle, C.byref(dc_desc), C.c_float(0.0),
+                                            C.byref(out)),
+        "emc_boxes32_size128", 127 * 127 * 128,
+        lambda: from_oracle.extract_emc(src.program(), from_oracle.reference_desc(128)))
+    res["extended_mc"]["workload"] = ("ExtendedMarchingCubes::new(128) on the union of 32 seeded "
+                                      "boxes with their own (HermiteSource) normals")
+    return res
+
+
+def points_and_normals(ctx: Context, desc, mesh: "DeviceMesh", nv: int, steps: int) -> dict:
+    """PointCloud over the resident 257^3 grid and the CentralDifference(1e-4)
+    normals of the MarchingCubes::new(257) vertices (the normal sinks'
+    per-vertex pass), both into device memory, checked against the oracle's
+    digests."""
+    golden = json.loads((ROOT / "tests" / "golden" / "mesh_digests.json").read_text())
+    gp = golden["pointcloud_spheres64_minus_torus_size257"]
+    gn = golden["normals_spheres64_minus_torus_size257_eps1e-4"]
+    res = {}
+    cap = gp["points"]
+    pbuf = C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 12 * cap, C.byref(pbuf)), "alloc")
+
+    def cloud():
+        out = A.IsoMeshOut(pbuf, None, cap, 0, 0, 0, A.ISO_MEMORY_DEVICE, 0)
+        ctx.check(ctx.lib.iso_extract_point_cloud(ctx.handle, C.byref(desc), C.byref(out)),
+                  "iso_extract_point_cloud")
+        return int(out.vertex_count)
+
+    n = cloud()
+    pts = np.empty((n, 3), np.float32)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, pts.ctypes.data, pbuf, pts.nbytes)
+    parity = n == gp["points"] and hashlib.sha256(pts.astype("<f4").tobytes()).hexdigest() == \
+        gp["points_sha256"]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cloud()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.lib.iso_device_free(ctx.handle, pbuf)
+    cells = (desc.nx - 1) * (desc.ny - 1) * (desc.nz - 1)
+    res["point_cloud"] = {
+        "workload": "PointCloud::new(257) over the resident 257x257x258 FP32 grid (device output)",
+        "ms_per_extraction": 1e3 * dt, "cells_per_s": cells / dt, "points": n,
+        "parity_vs_oracle": parity}
+    log(f"iso_extract_point_cloud: {1e3 * dt:.3f} ms, parity {parity}")
+
+    nbuf = C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 12 * nv, C.byref(nbuf)), "alloc")
+
+    def normals():
+        ctx.check(ctx.lib.iso_sample_normals(ctx.handle, mesh.v, nv, C.c_float(1e-4), nbuf,
+                                             A.ISO_MEMORY_DEVICE), "iso_sample_normals")
+
+    normals()
+    nrm = np.empty((nv, 3), np.float32)
+    ctx.lib.iso_memcpy_to_host(ctx.handle, nrm.ctypes.data, nbuf, nrm.nbytes)
+    parity = nv == gn["points"] and hashlib.sha256(nrm.astype("<f4").tobytes()).hexdigest() == \
+        gn["normals_sha256"]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        normals()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.lib.iso_device_free(ctx.handle, nbuf)
+    res["vertex_normals"] = {
+        "workload": ("CentralDifference::new_with_epsilon(CSG, 1e-4).sample_normal at the "
+                     f"{nv} vertices of MarchingCubes::new(257) (the normal sinks' pass, "
+                     "device memory)"),
+        "ms": 1e3 * dt, "normals_per_s": nv / dt, "parity_vs_oracle": parity}
+    log(f"iso_sample_normals: {1e3 * dt:.3f} ms for {nv} vertices, parity {parity}")
     return res
 
 
@@ -538,6 +624,9 @@ def main() -> int:
         "note": "64-sphere-minus-torus SDF evaluated on the GPU (VALU-bound, not an HBM stream)",
     }
     log(f"program path: {program_path['ms_per_step']:.3f} ms/step")
+    # The PointCloud and normal-sink rows on the same lattice and mesh.
+    out = extract(ctx, desc, mesh)
+    widened = points_and_normals(ctx, desc, mesh, nv, max(1, args.program_steps))
 
     result = {
         "metric": METRIC,
@@ -578,6 +667,7 @@ def main() -> int:
         "kernels": per_kernel,
         "traffic_per_kernel": traffic.get("kernels") if traffic else None,
         "program_path": program_path,
+        **widened,
     }
 
     if not args.no_side:

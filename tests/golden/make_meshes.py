@@ -1,5 +1,7 @@
 """Writes mesh_digests.json: counts and SHA-256 digests of the oracle's
-marching-cubes and dual-contouring meshes for the benchmark workloads.
+meshes (marching cubes Signed and Directed, dual contouring, octree MC,
+extended MC, batched chunks), point cloud and vertex normals for the
+benchmark workloads.
 
 The oracle (oracle/, a restatement of the reference's CPU extraction) is run
 here once; the GPU tests compare their output digests with these values, so
@@ -47,6 +49,10 @@ def main() -> None:
     }
     meshes += [(name, depth, O.extract_lhmc(src.program(), depth, 0))
                for name, (src, depth) in lhmc_cases.items()]
+    boxes = W.boxes_union(32, seed=4)
+    meshes.append(("directed_boxes32_size128", 128,
+                   O.extract_mc(boxes.program(), desc=O.reference_desc(128), directed=True)))
+    meshes.append(("emc_boxes32_size128", 128, O.extract_emc(boxes.program(), O.reference_desc(128))))
     for name, size, m in meshes:
         out[name] = {
             "size": size,
@@ -75,6 +81,16 @@ def main() -> None:
     }
     print("batch4096_seed6", out["batch4096_seed6"]["vertices"],
           out["batch4096_seed6"]["triangles"], flush=True)
+    # PointCloud and CentralDifference(1e-4) vertex normals of the CSG at 257.
+    csg = W.spheres_minus_torus()[0]
+    pts = O.extract_point_cloud(csg.program(), desc=O.reference_desc(257))
+    out["pointcloud_spheres64_minus_torus_size257"] = {
+        "size": 257, "points": int(len(pts)), "points_sha256": digest(pts.astype("<f4"))}
+    mv = O.extract_mc(csg.program(), desc=O.reference_desc(257)).vertices
+    nrm = O.sample_normals(csg.program(), mv, 1e-4)
+    out["normals_spheres64_minus_torus_size257_eps1e-4"] = {
+        "size": 257, "points": int(len(nrm)), "normals_sha256": digest(nrm.astype("<f4"))}
+    print("pointcloud", len(pts), "normals", len(nrm), flush=True)
     path = Path(__file__).with_name("mesh_digests.json")
     path.write_text(json.dumps(out, indent=1) + "\n")
 
