@@ -119,6 +119,25 @@ int main(int argc, char** argv) {
     dump(out + ".emc_cd.u32", eci);
     std::printf("extended: %zu vertices, %zu triangles\n", ev.size() / 3, ei.size() / 3);
 
+    // DualContouring::new(33, MinimiseQEF) over CentralDifference(cube, 1e-6),
+    // and LinearHashedMarchingCubes::new(6) of the cube.
+    isosurface::DualContouring dc(33, isosurface::MinimiseQEF{}, ctx);
+    std::vector<float> dcv;
+    std::vector<uint32_t> dci;
+    isosurface::IndexedVertices dcsink(dcv, dci);
+    dc.extract(isosurface::CentralDifference<std::remove_cv_t<decltype(cube)>>{cube, 1e-6f}, dcsink);
+    dump(out + ".dc.f32", dcv);
+    dump(out + ".dc.u32", dci);
+    isosurface::LinearHashedMarchingCubes lh(6, ctx);
+    std::vector<float> lhv;
+    std::vector<uint32_t> lhi;
+    isosurface::IndexedVertices lhsink(lhv, lhi);
+    lh.extract(cube, lhsink);
+    dump(out + ".lhmc.f32", lhv);
+    dump(out + ".lhmc.u32", lhi);
+    std::printf("dual contouring: %zu vertices; octree: %zu vertices\n", dcv.size() / 3,
+                lhv.size() / 3);
+
     // Error channel: a malformed program surfaces as isosurface::Error.
     try {
       isosurface::Program bad{isosurface::detail::node(ISO_OP_UNION)};

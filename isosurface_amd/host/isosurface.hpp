@@ -20,6 +20,10 @@
 // host over the reference lattice and uploaded.  Extractors receive
 // extract_vertex for every vertex in first-encounter order, then
 // extract_index for every index (extractor.rs:17-20, mesh.rs:91-100).
+// The other algorithms read the same way: MarchingCubes<Directed> (sources
+// with `Directed sample_vector(Vec3) const`), PointCloud, DualContouring
+// (over CentralDifference<S>), LinearHashedMarchingCubes and
+// ExtendedMarchingCubes; the normal sinks are IndexedInterleavedNormals<S>.
 // Failures throw isosurface::Error carrying the ABI status.
 #pragma once
 
@@ -433,6 +437,100 @@ class ExtendedMarchingCubes {
 
  private:
   size_t size_;
+  Context& ctx_;
+  std::vector<float> vertices_;
+  std::vector<uint32_t> indices_;
+};
+
+namespace detail {
+// Runs an extraction into host vectors reused across calls (grown through
+// iso_mesh_fetch on ISO_ERR_CAPACITY) and replays it into the extractor.
+template <class Call>
+void extract_host(Context& ctx, Call call, std::vector<float>& v, std::vector<uint32_t>& i,
+                  Extractor& extractor, const char* what) {
+  iso_mesh_out out{};
+  out.memory = ISO_MEMORY_HOST;
+  out.vertices = v.data();
+  out.indices = i.data();
+  out.vertex_capacity = v.size() / 3;
+  out.index_capacity = i.size();
+  int st = call(&out);
+  if (st == ISO_ERR_CAPACITY) {
+    v.resize(3 * out.vertex_count);
+    i.resize(out.index_count);
+    out.vertices = v.data();
+    out.indices = i.data();
+    out.vertex_capacity = out.vertex_count;
+    out.index_capacity = out.index_count;
+    st = iso_mesh_fetch(ctx.get(), &out);
+  }
+  ctx.check(st, what);
+  extractor.extract_vertices(v.data(), out.vertex_count);
+  for (uint64_t k = 0; k < out.index_count; ++k) extractor.extract_index(i[k]);
+}
+
+inline void upload(Context& ctx, const Program& prog) {
+  ctx.check(iso_program_upload(ctx.get(), prog.data(), static_cast<uint32_t>(prog.size())),
+            "iso_program_upload");
+}
+}  // namespace detail
+
+// ---- DualContouring (dual_contouring.rs:38-105) --------------------------------
+
+// MinimiseQEF (feature/qef.rs:24-68): the feature placement the GPU runs.
+struct MinimiseQEF {};
+
+class DualContouring {
+ public:
+  // DualContouring::new(size, MinimiseQEF {}).
+  DualContouring(size_t size, MinimiseQEF, Context& ctx) : size_(size), ctx_(ctx) {
+    if (size < 2) throw Error(ISO_ERR_INVALID_ARGUMENT, "size must be at least 2");
+  }
+
+  // DualContouring::extract over CentralDifference::new_with_epsilon(source, e).
+  template <class S>
+  void extract(const CentralDifference<S>& source, Extractor& extractor) {
+    Program prog;
+    source.lower(prog);
+    detail::upload(ctx_, prog);
+    iso_grid_desc desc;
+    ctx_.check(iso_grid_desc_reference(static_cast<int64_t>(size_), ISO_SOURCE_PROGRAM, &desc),
+               "iso_grid_desc_reference");
+    detail::extract_host(
+        ctx_, [&](iso_mesh_out* out) { return iso_extract_dc(ctx_.get(), &desc, source.epsilon, out); },
+        vertices_, indices_, extractor, "iso_extract_dc");
+  }
+
+ private:
+  size_t size_;
+  Context& ctx_;
+  std::vector<float> vertices_;
+  std::vector<uint32_t> indices_;
+};
+
+// ---- LinearHashedMarchingCubes (linear_hashed_marching_cubes.rs:38-84) --------
+
+class LinearHashedMarchingCubes {
+ public:
+  // LinearHashedMarchingCubes::new(max_depth), max_depth in [0, 9].
+  LinearHashedMarchingCubes(int max_depth, Context& ctx) : depth_(max_depth), ctx_(ctx) {
+    if (max_depth < 0 || max_depth > 9)
+      throw Error(ISO_ERR_INVALID_ARGUMENT, "max_depth must be in [0, 9]");
+  }
+
+  template <class S>
+  void extract(const S& source, Extractor& extractor) {
+    static_assert(is_lowerable<S>::value, "LinearHashedMarchingCubes::extract takes a lowerable source");
+    Program prog;
+    source.lower(prog);
+    detail::upload(ctx_, prog);
+    detail::extract_host(
+        ctx_, [&](iso_mesh_out* out) { return iso_extract_lhmc(ctx_.get(), depth_, out); },
+        vertices_, indices_, extractor, "iso_extract_lhmc");
+  }
+
+ private:
+  int depth_;
   Context& ctx_;
   std::vector<float> vertices_;
   std::vector<uint32_t> indices_;
