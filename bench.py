@@ -14,8 +14,9 @@ Printed JSON line:
   value         cells/s of the timed steps (wall clock, synchronised)
   roofline      the classify + interpolate + emit stage (its three kernels):
                 algorithmic bytes (4 B per sample + 12 B per vertex + 12 B
-                per triangle) / the sum of the kernels' mean HIP-event
-                durations over the timed steps, against MI355X's 8 TB/s HBM
+                per triangle) / the sum of the kernels' mean durations over
+                the timed steps (HIP events attached to each kernel's
+                dispatch, so they span the kernel alone), against MI355X's 8 TB/s HBM
                 peak; traffic from the committed counter summary
   kernels       each kernel's own algorithmic bytes, duration and fraction
   program_path  the same extraction sampling the CSG program on the GPU
@@ -33,7 +34,10 @@ Printed JSON line:
                 samples the plain path generates, its margin calibrated on
                 held-out samples; band extraction is timed against the plain
                 dense GPU extraction of the same program, which checks it
-                (identical mesh, missed surface cells)
+                (identical mesh, missed surface cells).  The network's culls
+                are statistical (a held-out calibration, no proof), so its
+                missed_cells can be nonzero; the certificate's culls are
+                proven by the field's 1-Lipschitz bound
   dual_contouring, octree_mc
                 the sharp-feature workload (union of 32 seeded boxes):
                 DualContouring at 128^3 with central-difference normals and
@@ -105,7 +109,7 @@ def timed_steps(ctx, desc, mesh, steps, graph=False):
 
 
 def kernel_pass(ctx, desc, mesh, steps):
-    """Per-kernel mean durations (HIP events on the context's stream)."""
+    """Per-kernel mean durations (HIP events taken by each kernel's dispatch)."""
     ctx.reset_kernel_stats()
     ctx.set_timing(True)
     for _ in range(steps):

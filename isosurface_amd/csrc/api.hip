@@ -175,15 +175,22 @@ hipEvent_t take_event(iso_ctx* c) {
   return e;
 }
 
-// Brackets one kernel launch with events when timing is on.
+// Times one launch when timing is on.  The MarchingCubes signs, count and
+// emit launchers take the event pair into the dispatch itself, so the pair
+// spans exactly the kernel's execution; any other launch is bracketed by
+// stream records, which also count the gaps the stream leaves around it.
 template <typename F>
 hipError_t timed(iso_ctx* c, const char* name, F&& launch) {
   if (!c->timing) return launch();
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) return launch();
   (void)hipEventRecord(a, c->stream);
+  iso::set_launch_events(a, b);
   hipError_t e = launch();
-  (void)hipEventRecord(b, c->stream);
+  if (iso::launch_events_pending()) {
+    iso::clear_launch_events();
+    (void)hipEventRecord(b, c->stream);
+  }
   c->pending.push_back({name, a, b});
   return e;
 }

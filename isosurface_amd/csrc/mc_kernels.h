@@ -94,6 +94,14 @@ struct McBuffers {
   unsigned long long* host_totals;
 };
 
+// Kernel timing: the next launch by one of the single-kernel launchers below
+// (signs, count, emit) records `start` and `stop` at the kernel's own start
+// and end (hipExtLaunchKernelGGL) and takes the pair; a launch that does not
+// take it leaves it pending (launch_events_pending) for the caller to clear.
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
+bool launch_events_pending();
+void clear_launch_events();
+
 // Each returns hipSuccess or the launch error.  The signs launchers also
 // reset the per-extraction state (guard words, totals, look-back words).
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,

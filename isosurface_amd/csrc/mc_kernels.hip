@@ -27,6 +27,7 @@
 // vertices are numbered by (owner cell, first appearance in the owner's
 // triangle list).  The owner of every edge follows from the edge's position
 // alone, so no hash table is needed.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "mc_device.h"
@@ -944,6 +945,34 @@ McDims mc_dims(const Lattice& L) {
   return D;
 }
 
+namespace {
+thread_local hipEvent_t t_ev_start = nullptr;
+thread_local hipEvent_t t_ev_stop = nullptr;
+
+// A launch that takes the pending timing events, if any.
+template <typename K, typename... Args>
+void launch_k(K kernel, dim3 grid, dim3 block, hipStream_t s, Args... args) {
+  if (t_ev_start) {
+    hipEvent_t a = t_ev_start, z = t_ev_stop;
+    t_ev_start = nullptr;
+    t_ev_stop = nullptr;
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, a, z, 0, args...);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+  }
+}
+}  // namespace
+
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+  t_ev_start = start;
+  t_ev_stop = stop;
+}
+bool launch_events_pending() { return t_ev_start != nullptr; }
+void clear_launch_events() {
+  t_ev_start = nullptr;
+  t_ev_stop = nullptr;
+}
+
 hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& D,
                              const McBuffers& B, hipStream_t s) {
   (void)L;
@@ -953,8 +982,8 @@ hipError_t launch_signs_grid(const float* grid, const Lattice& L, const McDims& 
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
   int64_t blocks = grid_for((D.n_sign_words + kSignWords - 1) / kSignWords);
   if (blocks < min_blocks) blocks = min_blocks;
-  hipLaunchKernelGGL(k_signs_grid, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, grid,
-                     D.n_points, D.n_sign_words, B.signs, R);
+  launch_k(k_signs_grid, dim3(static_cast<unsigned>(blocks)), dim3(256), s, grid,
+           D.n_points, D.n_sign_words, B.signs, R);
   return hipGetLastError();
 }
 
@@ -964,8 +993,8 @@ hipError_t launch_signs_program(const DevOp* ops, int n_ops, const Lattice& L, c
   int64_t blocks = grid_for(D.n_sign_words);
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
   if (blocks < min_blocks) blocks = min_blocks;
-  hipLaunchKernelGGL(k_signs_program, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
-                     n_ops, L, D.n_points, D.n_sign_words, B.signs, values, reset_args(D, B));
+  launch_k(k_signs_program, dim3(static_cast<unsigned>(blocks)), dim3(256), s, ops,
+           n_ops, L, D.n_points, D.n_sign_words, B.signs, values, reset_args(D, B));
   return hipGetLastError();
 }
 
@@ -1034,8 +1063,8 @@ unsigned emit_blocks(const McBuffers& B, int per_cu, K kernel) {
 hipError_t launch_count(const Lattice& L, const McDims& D, const McBuffers& B, hipStream_t s) {
   if (D.n_blocks > 0x7fffffff) return hipErrorInvalidValue;
   const SingleSpace sp = single_space(L, D, B.signs, nullptr, 0, nullptr, nullptr);
-  hipLaunchKernelGGL((k_count<SingleSpace, kCountRounds>), dim3(static_cast<unsigned>(D.n_blocks)),
-                     dim3(kCountBlock), 0, s, sp, D.n_segs, D.n_blocks, B);
+  launch_k(k_count<SingleSpace, kCountRounds>, dim3(static_cast<unsigned>(D.n_blocks)),
+           dim3(kCountBlock), s, sp, D.n_segs, D.n_blocks, B);
   return hipGetLastError();
 }
 
@@ -1046,13 +1075,13 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const flo
   const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, grid, positions);
   if (grid && positions) {
     auto k = k_emit<kEmitPositions, SingleSpace>;
-    hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
+    launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
   } else if (grid) {
     auto k = k_emit<kEmitGrid, SingleSpace>;
-    hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
+    launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
   } else {
     auto k = k_emit<kEmitProgram, SingleSpace>;
-    hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
+    launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
   }
   return hipGetLastError();
 }
@@ -1063,8 +1092,8 @@ hipError_t launch_signs_directed(const DevOp* ops, int n_ops, const Lattice& L, 
   int64_t blocks = grid_for(D.n_sign_words);
   const int64_t min_blocks = (D.n_blocks + 255) / 256;
   if (blocks < min_blocks) blocks = min_blocks;
-  hipLaunchKernelGGL(k_signs_directed, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
-                     n_ops, L, D.n_points, D.n_sign_words, B.signs, values, reset_args(D, B));
+  launch_k(k_signs_directed, dim3(static_cast<unsigned>(blocks)), dim3(256), s, ops,
+           n_ops, L, D.n_points, D.n_sign_words, B.signs, values, reset_args(D, B));
   return hipGetLastError();
 }
 
@@ -1074,7 +1103,7 @@ hipError_t launch_emit_directed(const float* values, const Lattice& L, const McD
   const EmitArgs A = emit_args(B);
   const SingleSpace sp = single_space(L, D, B.signs, nullptr, 0, values, nullptr);
   auto k = k_emit<kEmitDirected, SingleSpace>;
-  hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
+  launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
   return hipGetLastError();
 }
 
@@ -1099,7 +1128,7 @@ hipError_t launch_emit_band(const float* values, const uint8_t* brick_flags, int
   A.nby = nby;
   const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, values, nullptr);
   auto k = k_emit<kEmitBand, SingleSpace>;
-  hipLaunchKernelGGL(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), 0, s, A, sp);
+  launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
   return hipGetLastError();
 }
 
