@@ -34,10 +34,12 @@ Printed JSON line:
                 samples the plain path generates, its margin calibrated on
                 held-out samples; band extraction is timed against the plain
                 dense GPU extraction of the same program, which checks it
-                (identical mesh, missed surface cells).  The network's culls
-                are statistical (a held-out calibration, no proof), so its
-                missed_cells can be nonzero; the certificate's culls are
-                proven by the field's 1-Lipschitz bound
+                (identical mesh, missed surface cells).  The network's own
+                culls are statistical (a held-out calibration, no proof), so
+                its missed_cells can be nonzero; with the certificate a brick
+                leaves the band only when a super-brick or brick centre
+                sample proves it (the field is 1-Lipschitz), and the network
+                culls that no proof allows are counted (overturned_culls)
   dual_contouring, octree_mc
                 the sharp-feature workload (union of 32 seeded boxes):
                 DualContouring at 128^3 with central-difference normals and
@@ -464,6 +466,7 @@ def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
              "band_fraction": int(stats.band_bricks) / max(int(stats.bricks), 1),
              "exact_samples": int(stats.exact_samples),
              "centre_samples": int(stats.centre_samples),
+             "overturned_culls": int(stats.overturned_culls),
              "missed_cells": int(missed.value), "identical_to_dense": same}
         log(f"band [{mode}]: {1e3 * t_band:.1f} ms, x{t_dense / t_band:.2f}, band "
             f"{r['band_fraction']:.3f}, missed {missed.value}, identical {same}")

@@ -300,7 +300,8 @@ int iso_extract_batch(iso_ctx* ctx, iso_batch_out* out);
  * prediction lies within their half-diagonal plus `margin` of zero, gives
  * every other sample its brick's predicted sign, and extracts as
  * iso_extract_mc does.  The result equals iso_extract_mc's whenever no
- * surface cell lies in a culled brick (iso_band_missed_cells counts them). */
+ * surface cell lies in a culled brick (iso_band_missed_cells counts them),
+ * which the ISO_BAND_CERTIFY and ISO_BAND_NO_NETWORK proofs guarantee. */
 #define ISO_BAND_PARAMS 15617 /* length of the parameter vector */
 
 typedef struct iso_band_train_params {
@@ -317,18 +318,25 @@ typedef struct iso_band_train_params {
 } iso_band_train_params;
 
 typedef struct iso_band_stats {
-  uint64_t bricks;          /* bricks of the lattice */
-  uint64_t band_bricks;     /* bricks evaluated exactly */
-  uint64_t exact_samples;   /* program evaluations of the band (without emit's) */
-  uint64_t centre_samples;  /* program evaluations of the certificate */
+  uint64_t bricks;            /* bricks of the lattice */
+  uint64_t band_bricks;       /* bricks evaluated exactly */
+  uint64_t exact_samples;     /* program evaluations of the band (without emit's) */
+  uint64_t centre_samples;    /* program evaluations of the certificate
+                                 (super-brick and brick centres) */
+  uint64_t overturned_culls;  /* ISO_BAND_CERTIFY: bricks the network culled
+                                 that no proof allowed (kept in the band) */
 } iso_band_stats;
 
-/* iso_extract_mc_band flags.  ISO_BAND_CERTIFY: each brick the network
- * keeps in the band is evaluated at its centre, and leaves the band (with
- * the exact sign) when |f| exceeds its half-diagonal — exact for every
- * program, all of which are 1-Lipschitz; the network's culls stay
- * unverified.  ISO_BAND_NO_NETWORK: no network and no margin, every brick is
- * certified (the method without the network, for comparison). */
+/* iso_extract_mc_band flags.  Without flags the network's culls are final
+ * (a calibrated margin, not a proof: cells can be missed, which
+ * iso_band_missed_cells reports).  ISO_BAND_CERTIFY: a brick leaves the band
+ * only by a proof, whatever the network predicted: the program is evaluated
+ * at the centre of each super-brick (4^3 bricks), then at the centre of each
+ * brick no super-brick proves, and |f| above the box's half-diagonal proves
+ * the box's sign (every program is 1-Lipschitz); network culls the proofs
+ * overturn are counted.  No surface cell is missed.  ISO_BAND_NO_NETWORK:
+ * the same proofs without a network (the method without it, for
+ * comparison). */
 #define ISO_BAND_CERTIFY 1u
 #define ISO_BAND_NO_NETWORK 2u
 

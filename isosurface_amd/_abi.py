@@ -98,7 +98,8 @@ class IsoBandTrainParams(C.Structure):
 
 class IsoBandStats(C.Structure):
     _fields_ = [("bricks", C.c_uint64), ("band_bricks", C.c_uint64),
-                ("exact_samples", C.c_uint64), ("centre_samples", C.c_uint64)]
+                ("exact_samples", C.c_uint64), ("centre_samples", C.c_uint64),
+                ("overturned_culls", C.c_uint64)]
 
 
 class IsoKernelStat(C.Structure):

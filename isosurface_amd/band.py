@@ -106,9 +106,11 @@ class NarrowBand:
 
     def extract_mesh(self, desc: A.IsoGridDesc, margin: float, flags: int = 0,
                      vertex_capacity: int = 0, index_capacity: int = 0):
-        """Band extraction; flags: A.ISO_BAND_CERTIFY (the network's band
-        candidates are certified by an exact centre sample) or
-        A.ISO_BAND_NO_NETWORK (every brick certified, no network)."""
+        """Band extraction; flags: A.ISO_BAND_CERTIFY (a brick leaves the
+        band only when an exact super-brick or brick centre sample proves
+        it, whatever the network predicted; stats.overturned_culls counts the
+        network culls no proof allowed) or A.ISO_BAND_NO_NETWORK (the same
+        proofs, no network)."""
         vcap = max(int(vertex_capacity), 1)
         icap = max(int(index_capacity), 3)
         verts = np.empty((vcap, 3), np.float32)

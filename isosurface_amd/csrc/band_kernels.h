@@ -25,11 +25,18 @@ constexpr int kBrick = 1 << kBandBrickShift;  // cells per brick edge (4)
 enum : uint8_t {
   kBrickBand = 1,    // exact samples
   kBrickInside = 2,  // predicted sign (culled bricks): !(f > 0)
+  kBrickProven = 4,  // super-brick flags: the centre sample proves the sign
 };
 
+// Super-bricks: kCoarse^3 bricks, certified by one centre sample before
+// their bricks are.
+constexpr int kCoarse = 4;
+
 struct BrickGrid {
-  int64_t nbx, nby, nbz;  // bricks per axis: ceil(cells / 8)
+  int64_t nbx, nby, nbz;  // bricks per axis: ceil(cells / kBrick)
   int64_t n;              // nbx * nby * nbz
+  int64_t ncx, ncy, ncz;  // super-bricks per axis: ceil(bricks / kCoarse)
+  int64_t nc;             // ncx * ncy * ncz
 };
 
 BrickGrid brick_grid(const Lattice& L);
@@ -41,17 +48,27 @@ hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* cen
 constexpr float kCertifyRelSlack = 1.0e-4f;
 constexpr float kCertifyAbsSlack = 2.0e-5f;
 
-// Band / inside flags.  With predictions, a brick is a band candidate if
-// |pred| <= half_diag + margin, else it takes the predicted sign; without
-// them (pred null) every brick is a candidate.  With certify_ops, each
-// candidate's centre is evaluated exactly (count added to *certified) and a
-// candidate with |f| > half_diag (plus the f32 allowance) leaves the band
-// with the exact sign.  The list of band bricks (ascending) and its length
-// go to st.ctl[1].
+// Super-brick certificate: coarse[s] = kBrickProven (| kBrickInside) when
+// the program's value at super-brick s's centre is farther from zero than
+// the super-brick's half-diagonal (plus the f32 allowance), else 0.  G.nc
+// evaluations are added to *certified.
+hipError_t launch_coarse_certify(const DevOp* ops, int n_ops, const Lattice& L,
+                                 const BrickGrid& G, uint8_t* coarse,
+                                 unsigned long long* certified, hipStream_t s);
+
+// Band / inside flags.  With predictions alone, a brick is in the band if
+// |pred| <= half_diag + margin, else it takes the predicted sign.  With
+// certify_ops (and the super-brick flags `coarse`), a brick leaves the band
+// only when a proof allows it: its super-brick's flag, else its own centre
+// sample (|f| > half_diag plus the f32 allowance; count added to
+// *certified), whatever the network predicted.  A brick the network culled
+// that neither proof allows is counted in *overturned (when pred is set).
+// The list of band bricks (ascending) and its length go to st.ctl[1].
 hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
                               const float* centres, const DevOp* certify_ops, int n_ops,
-                              float margin, uint8_t* flags, uint32_t* band_list,
-                              unsigned long long* certified, const scan::State& st,
+                              const uint8_t* coarse, float margin, uint8_t* flags,
+                              uint32_t* band_list, unsigned long long* certified,
+                              unsigned long long* overturned, const scan::State& st,
                               hipStream_t s);
 // Sign bits of the whole lattice from the bricks' predicted signs (and the
 // extraction's state reset, like the other signs launchers).

@@ -50,15 +50,62 @@ __global__ __launch_bounds__(256) void k_brick_centres(Lattice L, BrickGrid G,
   half_diag[b] = 0.5f * sqrtf(d2);
 }
 
+// Adds the number of lanes with `on` set to *counter (one atomic per wave).
+__device__ __forceinline__ void count_lanes(bool on, unsigned long long* counter) {
+  const uint64_t m = __ballot(on);
+  if (on && __lane_id() == __builtin_ctzll(m))
+    atomicAdd(counter, static_cast<unsigned long long>(__popcll(m)));
+}
+
+// Every program is 1-Lipschitz (exact primitive distances, min, max,
+// negation, translation and the polynomial smooth minimum, whose gradient is
+// a convex combination of its operands'), so a centre value farther than the
+// half-diagonal from zero (plus a bound on the f32 evaluation error) proves
+// that every sample of the box has the centre's sign.
+__device__ __forceinline__ bool proves(float v, float half_diag) {
+  return fabsf(v) > half_diag * (1.0f + kCertifyRelSlack) + kCertifyAbsSlack;
+}
+
+__global__ __launch_bounds__(256) void k_coarse_certify(const DevOp* __restrict__ ops, int n_ops,
+                                                        Lattice L, BrickGrid G,
+                                                        uint8_t* __restrict__ coarse,
+                                                        unsigned long long* certified) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  count_lanes(s < G.nc, certified);
+  if (s >= G.nc) return;
+  const int64_t sx = s % G.ncx;
+  const int64_t r = s / G.ncx;
+  const int64_t c[3] = {sx, r % G.ncy, r / G.ncy};
+  const int64_t n[3] = {L.nx, L.ny, L.nz};
+  constexpr int64_t kSpan = static_cast<int64_t>(kBrick) * kCoarse;  // cells per edge
+  float p[3];
+  float d2 = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    // The closed sample box [lo, hi] holds every brick box of the super-brick.
+    const int64_t lo = kSpan * c[a];
+    const int64_t hi = lo + kSpan < n[a] - 1 ? lo + kSpan : n[a] - 1;
+    p[a] = L.origin[a] + 0.5f * static_cast<float>(lo + hi) * L.spacing[a];
+    const float ext = static_cast<float>(kSpan) * L.spacing[a];
+    d2 += ext * ext;
+  }
+  const float v = sdf_eval(ops, n_ops, p[0], p[1], p[2]);
+  coarse[s] = proves(v, 0.5f * sqrtf(d2))
+                  ? static_cast<uint8_t>(kBrickProven | (!(v > 0.0f) ? kBrickInside : 0))
+                  : static_cast<uint8_t>(0);
+}
+
 struct FlagLoad {
-  const float* pred;       // network predictions; null: every brick is a candidate
+  const float* pred;       // network predictions; null: no network
   const float* half_diag;
   const float* centres;
-  const DevOp* ops;        // non-null: certify the candidates
+  const DevOp* ops;        // non-null: a brick leaves the band only by a proof
   int n_ops;
+  const uint8_t* coarse;   // super-brick proofs (with ops)
+  int64_t nbx, nby, ncx, ncy;
   float margin;
   uint8_t* flags;
-  unsigned long long* certified;  // centre evaluations
+  unsigned long long* certified;   // brick centre evaluations
+  unsigned long long* overturned;  // network culls no proof allows
   __device__ unsigned long long operator()(uint64_t i) const {
     const float hd = half_diag[i];
     bool band = true, inside = false;
@@ -68,24 +115,25 @@ struct FlagLoad {
       band = !(fabsf(p) > hd + margin);
       inside = !(p > 0.0f);
     }
-    const bool evaluate = band && ops != nullptr;
     if (ops) {
-      const uint64_t m = __ballot(evaluate);
-      if (evaluate && __lane_id() == __builtin_ctzll(m))
-        atomicAdd(certified, static_cast<unsigned long long>(__popcll(m)));
-    }
-    if (evaluate) {
-      // Every program is 1-Lipschitz (exact primitive distances, min, max,
-      // negation, translation and the polynomial smooth minimum, whose
-      // gradient is a convex combination of its operands'), so a centre
-      // value farther than the half-diagonal from zero (plus a bound on the
-      // f32 evaluation error) proves the brick's closed sample box holds
-      // that sign everywhere.
-      const float v = sdf_eval(ops, n_ops, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]);
-      if (fabsf(v) > hd * (1.0f + kCertifyRelSlack) + kCertifyAbsSlack) {
-        band = false;
-        inside = !(v > 0.0f);
+      const bool net_cull = pred != nullptr && !band;
+      const int64_t bx = static_cast<int64_t>(i) % nbx;
+      const int64_t r = static_cast<int64_t>(i) / nbx;
+      const int64_t by = r % nby;
+      const int64_t bz = r / nby;
+      const uint8_t cf = coarse[((bz / kCoarse) * ncy + by / kCoarse) * ncx + bx / kCoarse];
+      bool proven = (cf & kBrickProven) != 0;
+      bool in = (cf & kBrickInside) != 0;
+      count_lanes(!proven, certified);
+      if (!proven) {
+        const float v =
+            sdf_eval(ops, n_ops, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]);
+        proven = proves(v, hd);
+        in = !(v > 0.0f);
       }
+      count_lanes(net_cull && !proven, overturned);
+      band = !proven;
+      inside = proven && in;
     }
     flags[i] = static_cast<uint8_t>((band ? kBrickBand : 0) | (inside ? kBrickInside : 0));
     return band ? 1ull : 0ull;
@@ -305,7 +353,20 @@ BrickGrid brick_grid(const Lattice& L) {
   G.nby = (L.ny - 1 + kBrick - 1) / kBrick;
   G.nbz = (L.nz - 1 + kBrick - 1) / kBrick;
   G.n = G.nbx * G.nby * G.nbz;
+  G.ncx = (G.nbx + kCoarse - 1) / kCoarse;
+  G.ncy = (G.nby + kCoarse - 1) / kCoarse;
+  G.ncz = (G.nbz + kCoarse - 1) / kCoarse;
+  G.nc = G.ncx * G.ncy * G.ncz;
   return G;
+}
+
+hipError_t launch_coarse_certify(const DevOp* ops, int n_ops, const Lattice& L,
+                                 const BrickGrid& G, uint8_t* coarse,
+                                 unsigned long long* certified, hipStream_t s) {
+  if (!ops || !coarse || !certified) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_coarse_certify, dim3(grid_of(G.nc, 1u << 30)), dim3(256), 0, s, ops, n_ops,
+                     L, G, coarse, certified);
+  return hipGetLastError();
 }
 
 hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* centres,
@@ -317,12 +378,15 @@ hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* cen
 
 hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
                               const float* centres, const DevOp* certify_ops, int n_ops,
-                              float margin, uint8_t* flags, uint32_t* band_list,
-                              unsigned long long* certified, const scan::State& st,
+                              const uint8_t* coarse, float margin, uint8_t* flags,
+                              uint32_t* band_list, unsigned long long* certified,
+                              unsigned long long* overturned, const scan::State& st,
                               hipStream_t s) {
   if (!pred && !certify_ops) return hipErrorInvalidValue;
-  if (certify_ops && (!centres || !certified)) return hipErrorInvalidValue;
-  FlagLoad load{pred, half_diag, centres, certify_ops, n_ops, margin, flags, certified};
+  if (certify_ops && (!centres || !certified || !coarse || !overturned))
+    return hipErrorInvalidValue;
+  FlagLoad load{pred,  half_diag, centres, certify_ops, n_ops,  coarse,     G.nbx,
+                G.nby, G.ncx,     G.ncy,   margin,      flags,  certified,  overturned};
   FlagSink sink{band_list};
   // Certificates evaluate the program per brick: one brick per thread.
   if (certify_ops) return scan::launch<1>(static_cast<uint64_t>(G.n), load, sink, st, s);

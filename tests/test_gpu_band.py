@@ -7,8 +7,10 @@ import pytest
 
 from isosurface_amd import MarchingCubes, _abi as A
 from isosurface_amd import workloads as W
-from isosurface_amd.band import N_PARAMS, NarrowBand, brick_half_diagonal, training_samples
+from isosurface_amd.band import (N_PARAMS, NarrowBand, brick_half_diagonal, brick_size,
+                                  training_samples)
 from oracle import mlp as M
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -147,26 +149,72 @@ def test_untrained_band_reports_its_misses(ctx):
     assert (missed == 0) == same
 
 
-@pytest.mark.parametrize("flags,seed", [(A.ISO_BAND_NO_NETWORK, 0), (A.ISO_BAND_CERTIFY, 9)])
-def test_certified_band_is_dense(ctx, flags, seed):
-    # The certificate alone (no network), or certifying what an untrained
-    # network keeps: no culled brick holds surface, the mesh is the dense
-    # one, and far fewer samples than the lattice's are evaluated.
+def certificate_centre_samples(program, size: int, brick: int, coarse: int = 4) -> int:
+    """Program evaluations of the two-level certificate over
+    MarchingCubes::new(size), restated on the CPU: one per super-brick
+    (coarse^3 bricks), plus one per brick of every super-brick whose centre
+    value is within its half-diagonal (plus the f32 allowance) of zero."""
+    d = O.reference_desc(size, 0)
+    n = (d.nx, d.ny, d.nz)
+    nb = [(k - 1 + brick - 1) // brick for k in n]
+    nc = [(b + coarse - 1) // coarse for b in nb]
+    span = brick * coarse
+    f = np.float32
+    count = 0
+    for sz in range(nc[2]):
+        for sy in range(nc[1]):
+            for sx in range(nc[0]):
+                c, p, d2 = (sx, sy, sz), [], f(0.0)
+                for a in range(3):
+                    lo = span * c[a]
+                    hi = min(lo + span, n[a] - 1)
+                    p.append(f(d.origin[a]) + f(0.5) * f(lo + hi) * f(d.spacing[a]))
+                    e = f(span) * f(d.spacing[a])
+                    d2 = f(d2 + e * e)
+                count += 1
+                hd = f(0.5) * np.sqrt(d2)
+                if abs(O.sample(program, *p)) > hd * f(1.0 + 1e-4) + f(2e-5):
+                    continue
+                count += int(np.prod([min(coarse * c[a] + coarse, nb[a]) - coarse * c[a]
+                                      for a in range(3)]))
+    return count
+
+
+@pytest.mark.parametrize("flags", [A.ISO_BAND_NO_NETWORK, A.ISO_BAND_CERTIFY])
+def test_certified_band_is_dense(ctx, flags):
+    # The proofs alone (no network), or checking a network that predicts 1.0
+    # everywhere, so that with no margin it culls every brick, surface ones
+    # included: a brick leaves the band only by a proof, so none holds
+    # surface, the mesh is the dense one, and far fewer samples than the
+    # lattice's are evaluated.  Both give the same band.
     src = W.spheres_minus_torus()[0]
     band = NarrowBand(ctx)
-    band.init(seed)
+    P = M.unpack(np.zeros(N_PARAMS, np.float32))
+    P["b4"][:] = 1.0
+    band.set_params(M.pack(P))
     ctx.upload_program(src.program())
     size = 130
     desc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
-    margin = 1e30 if flags == A.ISO_BAND_CERTIFY else 0.0  # the network keeps every brick
-    mesh, stats = band.extract_mesh(desc, margin, flags)
+    if flags == A.ISO_BAND_CERTIFY:
+        _, unchecked = band.extract_mesh(desc, 0.0)
+        assert unchecked.band_bricks == 0
+        assert band.missed_cells(desc) > 0  # what the proofs must catch
+    mesh, stats = band.extract_mesh(desc, 0.0, flags)
     missed = band.missed_cells(desc)
     dense = MarchingCubes(size, ctx).extract_mesh(src)
     print(f"band {stats.band_bricks}/{stats.bricks} exact {stats.exact_samples} "
-          f"centres {stats.centre_samples} missed {missed}", flush=True)
-    assert stats.centre_samples == stats.bricks
+          f"centres {stats.centre_samples} overturned {stats.overturned_culls} "
+          f"missed {missed}", flush=True)
+    assert stats.centre_samples == certificate_centre_samples(src.program(), size, brick_size())
+    assert stats.centre_samples < stats.bricks
     assert 0 < stats.band_bricks < stats.bricks // 2
     assert stats.exact_samples < 0.7 * size * size * (size + 1)
+    if flags == A.ISO_BAND_CERTIFY:
+        assert stats.overturned_culls == stats.band_bricks  # every band brick was a network cull
+        _, plain = band.extract_mesh(desc, 0.0, A.ISO_BAND_NO_NETWORK)
+        assert (stats.band_bricks, stats.exact_samples) == (plain.band_bricks, plain.exact_samples)
+    else:
+        assert stats.overturned_culls == 0
     assert missed == 0
     assert mesh.vertices.tobytes() == dense.vertices.tobytes()
     assert np.array_equal(mesh.indices, dense.indices)
@@ -181,12 +229,16 @@ def test_certificate_refines_a_trained_band(ctx):
     _, plain = band.extract_mesh(desc, margin)
     mesh, cert = band.extract_mesh(desc, margin, A.ISO_BAND_CERTIFY)
     missed = band.missed_cells(desc)
+    _, proofs = band.extract_mesh(desc, margin, A.ISO_BAND_NO_NETWORK)
     dense = MarchingCubes(size, ctx).extract_mesh(src)
     print(f"learned {plain.band_bricks}, certified {cert.band_bricks} of {cert.bricks}, "
-          f"centres {cert.centre_samples}", flush=True)
-    # the certificate only evaluates what the network kept, and only removes bricks
-    assert cert.centre_samples == plain.band_bricks
-    assert cert.band_bricks <= plain.band_bricks
+          f"centres {cert.centre_samples}, overturned {cert.overturned_culls}", flush=True)
+    # the proofs decide every brick: the band is the network-free one, and
+    # the trained network's culls are, up to the few the proofs overturn, proven
+    assert (cert.band_bricks, cert.centre_samples) == (proofs.band_bricks, proofs.centre_samples)
+    # every band brick is one the network kept or a cull the proofs overturned
+    assert cert.band_bricks <= plain.band_bricks + cert.overturned_culls
+    assert cert.overturned_culls <= cert.band_bricks
     assert missed == 0
     assert mesh.vertices.tobytes() == dense.vertices.tobytes()
     assert np.array_equal(mesh.indices, dense.indices)
