@@ -95,7 +95,7 @@ struct iso_ctx {
   bool b_tuned = false;
   // learned narrow band: network, training batch, brick buffers, counters
   DevBuf nn_theta, nn_m, nn_v, nn_grad, nn_frags, nn_loss, nn_x, nn_h, nn_dz, nn_g, nn_pts, nn_t,
-      nn_w, nn_out, br_centres, br_hd, br_pred, br_flags, br_list, br_count, br_coarse;
+      nn_w, nn_out, br_centres, br_hd, br_pred, br_flags, br_list, br_count, br_coarse, br_proof;
   bool nn_ready = false;
   uint32_t nn_step = 0;  // Adam steps taken
   int64_t nn_batch = 0;  // batch the activation buffers are sized for
@@ -316,7 +316,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
                     &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->dense_check,  &c->nrm,
                     &c->dgrid,      &c->dvals,      &c->emc_cells, &c->emc_first,
-                    &c->emc_flips,  &c->emc_ctr,    &c->br_coarse};
+                    &c->emc_flips,  &c->emc_ctr,    &c->br_coarse, &c->br_proof};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   if (c->m_graph) (void)hipGraphExecDestroy(c->m_graph);
   for (DevBuf* b : bufs) release(b);
@@ -1894,12 +1894,14 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
   ISO_HIP(c, ensure(&c->br_list, nb * sizeof(uint32_t)));
   ISO_HIP(c, ensure(&c->br_count, 4 * sizeof(unsigned long long)));
   ISO_HIP(c, ensure(&c->br_coarse, static_cast<size_t>(G.nc)));
+  ISO_HIP(c, ensure(&c->br_proof, nb));
   float* centres = static_cast<float*>(c->br_centres.ptr);
   float* hd = static_cast<float*>(c->br_hd.ptr);
   float* pred = static_cast<float*>(c->br_pred.ptr);
   uint8_t* bflags = static_cast<uint8_t*>(c->br_flags.ptr);
   uint32_t* list = static_cast<uint32_t*>(c->br_list.ptr);
   uint8_t* coarse = static_cast<uint8_t*>(c->br_coarse.ptr);
+  uint8_t* proof = static_cast<uint8_t*>(c->br_proof.ptr);
   // counters: exact samples, (the dense check's), centre samples, overturned culls
   unsigned long long* counters = static_cast<unsigned long long*>(c->br_count.ptr);
   iso::scan::State S{};
@@ -1914,10 +1916,13 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
             if (e == hipSuccess && certify)
               e = iso::launch_coarse_certify(ops, c->n_prog, L, G, coarse, counters + 2,
                                              c->stream);
+            if (e == hipSuccess && certify)
+              e = iso::launch_brick_certify(ops, c->n_prog, G, coarse, centres, hd, proof,
+                                            counters + 2, c->stream);
             if (e == hipSuccess)
-              e = iso::launch_brick_flags(G, use_network ? pred : nullptr, hd, centres,
-                                          certify ? ops : nullptr, c->n_prog, coarse, margin,
-                                          bflags, list, counters + 2, counters + 3, S, c->stream);
+              e = iso::launch_brick_flags(G, use_network ? pred : nullptr, hd,
+                                          certify ? proof : nullptr, margin, bflags, list,
+                                          counters + 3, S, c->stream);
             return e;
           }));
   if ((st = read_ctl(c, S)) != ISO_OK) return st;

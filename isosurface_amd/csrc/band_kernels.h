@@ -56,20 +56,24 @@ hipError_t launch_coarse_certify(const DevOp* ops, int n_ops, const Lattice& L,
                                  const BrickGrid& G, uint8_t* coarse,
                                  unsigned long long* certified, hipStream_t s);
 
+// Brick proofs: proof[b] = its super-brick's flag when that proves the
+// sign, else the same test at brick b's own centre (|f| > half_diag[b] plus
+// the f32 allowance; evaluations added to *certified).
+hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const BrickGrid& G,
+                                const uint8_t* coarse, const float* centres,
+                                const float* half_diag, uint8_t* proof,
+                                unsigned long long* certified, hipStream_t s);
+
 // Band / inside flags.  With predictions alone, a brick is in the band if
 // |pred| <= half_diag + margin, else it takes the predicted sign.  With
-// certify_ops (and the super-brick flags `coarse`), a brick leaves the band
-// only when a proof allows it: its super-brick's flag, else its own centre
-// sample (|f| > half_diag plus the f32 allowance; count added to
-// *certified), whatever the network predicted.  A brick the network culled
-// that neither proof allows is counted in *overturned (when pred is set).
+// `proof` (launch_brick_certify's flags), a brick leaves the band only when
+// its proof allows it, whatever the network predicted, and a brick the
+// network culled that its proof does not allow is counted in *overturned.
 // The list of band bricks (ascending) and its length go to st.ctl[1].
 hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
-                              const float* centres, const DevOp* certify_ops, int n_ops,
-                              const uint8_t* coarse, float margin, uint8_t* flags,
-                              uint32_t* band_list, unsigned long long* certified,
-                              unsigned long long* overturned, const scan::State& st,
-                              hipStream_t s);
+                              const uint8_t* proof, float margin, uint8_t* flags,
+                              uint32_t* band_list, unsigned long long* overturned,
+                              const scan::State& st, hipStream_t s);
 // Sign bits of the whole lattice from the bricks' predicted signs (and the
 // extraction's state reset, like the other signs launchers).
 hipError_t launch_band_background(const Lattice& L, const McDims& D, const BrickGrid& G,

@@ -94,46 +94,61 @@ __global__ __launch_bounds__(256) void k_coarse_certify(const DevOp* __restrict_
                   : static_cast<uint8_t>(0);
 }
 
+// Brick proofs: one wavefront per super-brick, one lane per brick (4 x 4 x 4
+// bricks, x fastest), so a wavefront either copies its super-brick's proof
+// or evaluates all of its bricks' centres together.
+__global__ __launch_bounds__(256) void k_brick_certify(const DevOp* __restrict__ ops, int n_ops,
+                                                       BrickGrid G,
+                                                       const uint8_t* __restrict__ coarse,
+                                                       const float* __restrict__ centres,
+                                                       const float* __restrict__ half_diag,
+                                                       uint8_t* __restrict__ proof,
+                                                       unsigned long long* certified) {
+  static_assert(kCoarse * kCoarse * kCoarse == 64, "one lane per brick of a super-brick");
+  const int lane = __lane_id();
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (s >= G.nc) return;  // uniform per wave
+  const int64_t sx = s % G.ncx;
+  const int64_t r = s / G.ncx;
+  const int64_t bx = kCoarse * sx + lane % kCoarse;
+  const int64_t by = kCoarse * (r % G.ncy) + (lane / kCoarse) % kCoarse;
+  const int64_t bz = kCoarse * (r / G.ncy) + lane / (kCoarse * kCoarse);
+  const bool live = bx < G.nbx && by < G.nby && bz < G.nbz;
+  const int64_t b = (bz * G.nby + by) * G.nbx + bx;
+  const uint8_t cf = coarse[s];
+  const bool evaluate = live && !(cf & kBrickProven);
+  count_lanes(evaluate, certified);
+  uint8_t out = cf;
+  if (evaluate) {
+    const float v = sdf_eval(ops, n_ops, centres[3 * b], centres[3 * b + 1], centres[3 * b + 2]);
+    out = proves(v, half_diag[b])
+              ? static_cast<uint8_t>(kBrickProven | (!(v > 0.0f) ? kBrickInside : 0))
+              : static_cast<uint8_t>(0);
+  }
+  if (live) proof[b] = out;
+}
+
 struct FlagLoad {
   const float* pred;       // network predictions; null: no network
   const float* half_diag;
-  const float* centres;
-  const DevOp* ops;        // non-null: a brick leaves the band only by a proof
-  int n_ops;
-  const uint8_t* coarse;   // super-brick proofs (with ops)
-  int64_t nbx, nby, ncx, ncy;
+  const uint8_t* proof;    // non-null: a brick leaves the band only by its proof
   float margin;
   uint8_t* flags;
-  unsigned long long* certified;   // brick centre evaluations
   unsigned long long* overturned;  // network culls no proof allows
   __device__ unsigned long long operator()(uint64_t i) const {
-    const float hd = half_diag[i];
     bool band = true, inside = false;
     if (pred) {
       const float p = pred[i];
       // A NaN prediction is never trusted: the brick stays in the band.
-      band = !(fabsf(p) > hd + margin);
+      band = !(fabsf(p) > half_diag[i] + margin);
       inside = !(p > 0.0f);
     }
-    if (ops) {
-      const bool net_cull = pred != nullptr && !band;
-      const int64_t bx = static_cast<int64_t>(i) % nbx;
-      const int64_t r = static_cast<int64_t>(i) / nbx;
-      const int64_t by = r % nby;
-      const int64_t bz = r / nby;
-      const uint8_t cf = coarse[((bz / kCoarse) * ncy + by / kCoarse) * ncx + bx / kCoarse];
-      bool proven = (cf & kBrickProven) != 0;
-      bool in = (cf & kBrickInside) != 0;
-      count_lanes(!proven, certified);
-      if (!proven) {
-        const float v =
-            sdf_eval(ops, n_ops, centres[3 * i], centres[3 * i + 1], centres[3 * i + 2]);
-        proven = proves(v, hd);
-        in = !(v > 0.0f);
-      }
-      count_lanes(net_cull && !proven, overturned);
+    if (proof) {
+      const uint8_t pf = proof[i];
+      const bool proven = (pf & kBrickProven) != 0;
+      count_lanes(pred != nullptr && !band && !proven, overturned);
       band = !proven;
-      inside = proven && in;
+      inside = proven && (pf & kBrickInside) != 0;
     }
     flags[i] = static_cast<uint8_t>((band ? kBrickBand : 0) | (inside ? kBrickInside : 0));
     return band ? 1ull : 0ull;
@@ -198,101 +213,160 @@ __device__ __forceinline__ uint32_t later_holders(int cx, int cy, int cz) {
   return m;
 }
 
-constexpr int kExactWaves = 4;                      // waves per workgroup, one brick each
-constexpr int kBoxRows = (kBrick + 1) * (kBrick + 1);  // x rows of a brick's sample box
+constexpr int kSide = kBrick + 1;                        // samples per edge of a brick box
+constexpr int kBoxRows = kSide * kSide;                  // x rows of a brick's sample box
+constexpr int kBoxSamples = kBoxRows * kSide;
+constexpr int kExactThreads = 256;
+constexpr int kGroupBricks = 32;                         // band bricks per workgroup pass
+constexpr int kBricksPerWave = kGroupBricks / (kExactThreads / 64);
+static_assert(kBoxSamples <= 256, "box positions are stored as bytes");
+static_assert(kSide <= 32, "a box row is one 32-bit mask");
 
-// One wavefront per band brick: its lanes walk the brick's closed sample
-// box flat (x fastest), evaluate the samples the brick owns, store their
-// values, and gather the sign bits per x row in LDS; then one lane per row
-// merges the row into the sign words (at most two words, OR for inside,
-// AND-NOT for outside, over the background the bricks' predictions wrote).
-__global__ __launch_bounds__(64 * kExactWaves) void k_band_exact(
+// Thirty-two band bricks per workgroup pass.  Each wavefront takes eight: it
+// reads a brick's and its 26 neighbours' band flags, walks the brick's
+// closed sample box (x fastest) and appends the samples the brick owns to
+// the brick's list in LDS.  Then all 256 lanes evaluate the 32 lists
+// as one queue (a lane's brick by binary search of the lists' offsets), so
+// lanes stay busy however few samples each brick owns, store the values,
+// and gather sign bits per x row in LDS; last, one lane per row merges the
+// row into the sign words (at most two words, OR for inside, AND-NOT for
+// outside, over the background the bricks' flags wrote).
+__global__ __launch_bounds__(kExactThreads) void k_band_exact(
     const DevOp* __restrict__ ops, int n_ops, Lattice L, BrickGrid G,
     const uint8_t* __restrict__ flags, const uint32_t* __restrict__ band_list, uint64_t n_band,
     uint64_t* __restrict__ S, float* __restrict__ values, unsigned long long* evaluated) {
-  __shared__ uint32_t s_in[kExactWaves][kBoxRows];
-  __shared__ uint32_t s_own[kExactWaves][kBoxRows];
+  __shared__ uint32_t s_in[kGroupBricks][kBoxRows];
+  __shared__ uint32_t s_own[kGroupBricks][kBoxRows];
+  __shared__ uint8_t s_pos[kGroupBricks][kBoxSamples];  // lx + kSide (ly + kSide lz)
+  __shared__ uint32_t s_lo[kGroupBricks][3];
+  __shared__ uint32_t s_n[kGroupBricks];
+  __shared__ uint32_t s_pre[kGroupBricks + 1];
   const int lane = __lane_id();
   const int wv = threadIdx.x >> 6;
   uint32_t count = 0;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kExactWaves; base < n_band;
-       base += static_cast<uint64_t>(gridDim.x) * kExactWaves) {
-    const uint64_t item = base + wv;
-    const bool live = item < n_band;  // uniform per wave
-    for (int r = lane; r < kBoxRows; r += 64) {
-      s_in[wv][r] = 0;
-      s_own[wv][r] = 0;
-    }
-    int64_t lo[3] = {0, 0, 0};
-    int ext[3] = {0, 0, 0};
-    uint32_t nb = 0;
-    if (live) {
-      const int64_t b = band_list[item];
-      const int64_t bc[3] = {b % G.nbx, (b / G.nbx) % G.nby, b / (G.nbx * G.nby)};
-      const int64_t nbk[3] = {G.nbx, G.nby, G.nbz};
-      const int64_t n[3] = {L.nx, L.ny, L.nz};
-      // Band flags of the 26 neighbours (and the brick itself, bit 13).
-      bool band = false;
-      if (lane < 27) {
-        const int64_t q[3] = {bc[0] + lane % 3 - 1, bc[1] + (lane / 3) % 3 - 1,
-                              bc[2] + lane / 9 - 1};
-        if (q[0] >= 0 && q[1] >= 0 && q[2] >= 0 && q[0] < nbk[0] && q[1] < nbk[1] &&
-            q[2] < nbk[2])
-          band = (flags[(q[2] * G.nby + q[1]) * G.nbx + q[0]] & kBrickBand) != 0;
-      }
-      nb = static_cast<uint32_t>(__ballot(band));
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = kBrick * bc[a];
-        int64_t hi = lo[a] + kBrick < n[a] - 1 ? lo[a] + kBrick : n[a] - 1;
-        // the high face goes to the next brick along the axis when it is in the band
-        const int up = a == 0 ? 14 : a == 1 ? 16 : 22;
-        if (hi == lo[a] + kBrick && (nb >> up) & 1u) hi -= 1;
-        ext[a] = static_cast<int>(hi - lo[a] + 1);
-      }
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kGroupBricks; base < n_band;
+       base += static_cast<uint64_t>(gridDim.x) * kGroupBricks) {
+    for (int r = threadIdx.x; r < kGroupBricks * kBoxRows; r += kExactThreads) {
+      (&s_in[0][0])[r] = 0;
+      (&s_own[0][0])[r] = 0;
     }
     __syncthreads();
-    if (live) {
-      const int exy = ext[0] * ext[1];
-      const int total = exy * ext[2];
-      for (int t = lane; t < total; t += 64) {
-        const int lz = t / exy;
-        const int rem = t - lz * exy;
-        const int ly = rem / ext[0];
-        const int lx = rem - ly * ext[0];
-        const int cx = lx == 0 ? 0 : lx == kBrick ? 2 : 1;
-        const int cy = ly == 0 ? 0 : ly == kBrick ? 2 : 1;
-        const int cz = lz == 0 ? 0 : lz == kBrick ? 2 : 1;
-        if (later_holders(cx, cy, cz) & nb) continue;
-        const int64_t x = lo[0] + lx, y = lo[1] + ly, z = lo[2] + lz;
-        const float v = sdf_eval(ops, n_ops, L.origin[0] + static_cast<float>(x) * L.spacing[0],
-                                 L.origin[1] + static_cast<float>(y) * L.spacing[1],
-                                 L.origin[2] + static_cast<float>(z) * L.spacing[2]);
-        if (values) values[(z * L.ny + y) * L.nx + x] = v;
-        const int r = ly + (kBrick + 1) * lz;
-        if (!(v > 0.0f)) atomicOr(&s_in[wv][r], 1u << lx);  // classify_corners, distance.rs:52-54
-        atomicOr(&s_own[wv][r], 1u << lx);
-        ++count;
-      }
-    }
-    __syncthreads();
-    if (live) {
-      for (int r = lane; r < kBoxRows; r += 64) {
-        const uint64_t own = s_own[wv][r];
-        if (!own) continue;
-        const uint64_t in = s_in[wv][r] & own;
-        const uint64_t out = own & ~in;
-        const int ly = r % (kBrick + 1), lz = r / (kBrick + 1);
-        const int64_t f0 = ((lo[2] + lz) * L.ny + lo[1] + ly) * L.nx + lo[0];
-        const int64_t w0 = f0 >> 6;
-        const int sh = static_cast<int>(f0 & 63);
-        unsigned long long* W = reinterpret_cast<unsigned long long*>(S);
-        if (in << sh) atomicOr(&W[w0], in << sh);
-        if (out << sh) atomicAnd(&W[w0], ~(out << sh));
-        if (sh) {
-          const uint64_t in1 = in >> (64 - sh), out1 = out >> (64 - sh);
-          if (in1) atomicOr(&W[w0 + 1], in1);
-          if (out1) atomicAnd(&W[w0 + 1], ~out1);
+#pragma unroll 1
+    for (int q = 0; q < kBricksPerWave; ++q) {
+      const int j = wv * kBricksPerWave + q;
+      const uint64_t item = base + j;
+      uint32_t n_own = 0;
+      if (item < n_band) {  // uniform per wave
+        const int64_t b = band_list[item];
+        const int64_t bc[3] = {b % G.nbx, (b / G.nbx) % G.nby, b / (G.nbx * G.nby)};
+        const int64_t nbk[3] = {G.nbx, G.nby, G.nbz};
+        const int64_t n[3] = {L.nx, L.ny, L.nz};
+        // Band flags of the 26 neighbours (and the brick itself, bit 13).
+        bool band = false;
+        if (lane < 27) {
+          const int64_t nq[3] = {bc[0] + lane % 3 - 1, bc[1] + (lane / 3) % 3 - 1,
+                                 bc[2] + lane / 9 - 1};
+          if (nq[0] >= 0 && nq[1] >= 0 && nq[2] >= 0 && nq[0] < nbk[0] && nq[1] < nbk[1] &&
+              nq[2] < nbk[2])
+            band = (flags[(nq[2] * G.nby + nq[1]) * G.nbx + nq[0]] & kBrickBand) != 0;
         }
+        const uint32_t nb = static_cast<uint32_t>(__ballot(band));
+        int64_t lo[3];
+        int ext[3];
+        for (int a = 0; a < 3; ++a) {
+          lo[a] = kBrick * bc[a];
+          int64_t hi = lo[a] + kBrick < n[a] - 1 ? lo[a] + kBrick : n[a] - 1;
+          // the high face goes to the next brick along the axis when it is in the band
+          const int up = a == 0 ? 14 : a == 1 ? 16 : 22;
+          if (hi == lo[a] + kBrick && (nb >> up) & 1u) hi -= 1;
+          ext[a] = static_cast<int>(hi - lo[a] + 1);
+        }
+        const int exy = ext[0] * ext[1];
+        const int total = exy * ext[2];
+#pragma unroll 1
+        for (int t0 = 0; t0 < total; t0 += 64) {  // uniform trip count
+          const int t = t0 + lane;
+          bool own = false;
+          int lx = 0, ly = 0, lz = 0;
+          if (t < total) {
+            lz = t / exy;
+            const int rem = t - lz * exy;
+            ly = rem / ext[0];
+            lx = rem - ly * ext[0];
+            const int cx = lx == 0 ? 0 : lx == kBrick ? 2 : 1;
+            const int cy = ly == 0 ? 0 : ly == kBrick ? 2 : 1;
+            const int cz = lz == 0 ? 0 : lz == kBrick ? 2 : 1;
+            own = !(later_holders(cx, cy, cz) & nb);
+          }
+          const uint64_t m = __ballot(own);
+          if (own) {
+            const uint32_t k = n_own + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1)));
+            s_pos[j][k] = static_cast<uint8_t>(lx + kSide * (ly + kSide * lz));
+            atomicOr(&s_own[j][ly + kSide * lz], 1u << lx);
+          }
+          n_own += static_cast<uint32_t>(__popcll(m));
+        }
+        if (lane == 0) {
+          s_lo[j][0] = static_cast<uint32_t>(lo[0]);
+          s_lo[j][1] = static_cast<uint32_t>(lo[1]);
+          s_lo[j][2] = static_cast<uint32_t>(lo[2]);
+        }
+      }
+      if (lane == 0) s_n[j] = n_own;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t a = 0;
+      for (int j = 0; j < kGroupBricks; ++j) {
+        s_pre[j] = a;
+        a += s_n[j];
+      }
+      s_pre[kGroupBricks] = a;
+    }
+    __syncthreads();
+    const uint32_t T = s_pre[kGroupBricks];
+#pragma unroll 1
+    for (uint32_t t = threadIdx.x; t < T; t += kExactThreads) {
+      int j = 0;  // the brick whose list holds queue entry t: s_pre[j] <= t < s_pre[j + 1]
+#pragma unroll
+      for (int step = kGroupBricks / 2; step >= 1; step >>= 1)
+        if (s_pre[j + step] <= t) j += step;
+      const int pos = s_pos[j][t - s_pre[j]];
+      const int lz = pos / kBoxRows;
+      const int ly = (pos - lz * kBoxRows) / kSide;
+      const int lx = pos - lz * kBoxRows - ly * kSide;
+      const int64_t x = static_cast<int64_t>(s_lo[j][0]) + lx;
+      const int64_t y = static_cast<int64_t>(s_lo[j][1]) + ly;
+      const int64_t z = static_cast<int64_t>(s_lo[j][2]) + lz;
+      const float v = sdf_eval(ops, n_ops, L.origin[0] + static_cast<float>(x) * L.spacing[0],
+                               L.origin[1] + static_cast<float>(y) * L.spacing[1],
+                               L.origin[2] + static_cast<float>(z) * L.spacing[2]);
+      if (values) values[(z * L.ny + y) * L.nx + x] = v;
+      // classify_corners, distance.rs:52-54
+      if (!(v > 0.0f)) atomicOr(&s_in[j][ly + kSide * lz], 1u << lx);
+      ++count;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < kGroupBricks * kBoxRows; r += kExactThreads) {
+      const int j = r / kBoxRows;
+      const int row = r - j * kBoxRows;
+      const uint64_t own = s_own[j][row];
+      if (!own) continue;
+      const uint64_t in = s_in[j][row] & own;
+      const uint64_t out = own & ~in;
+      const int ly = row % kSide, lz = row / kSide;
+      const int64_t f0 = ((static_cast<int64_t>(s_lo[j][2]) + lz) * L.ny +
+                          static_cast<int64_t>(s_lo[j][1]) + ly) * L.nx +
+                         static_cast<int64_t>(s_lo[j][0]);
+      const int64_t w0 = f0 >> 6;
+      const int sh = static_cast<int>(f0 & 63);
+      unsigned long long* W = reinterpret_cast<unsigned long long*>(S);
+      if (in << sh) atomicOr(&W[w0], in << sh);
+      if (out << sh) atomicAnd(&W[w0], ~(out << sh));
+      if (sh) {
+        const uint64_t in1 = in >> (64 - sh), out1 = out >> (64 - sh);
+        if (in1) atomicOr(&W[w0 + 1], in1);
+        if (out1) atomicAnd(&W[w0 + 1], ~out1);
       }
     }
     __syncthreads();
@@ -376,20 +450,27 @@ hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* cen
   return hipGetLastError();
 }
 
-hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
-                              const float* centres, const DevOp* certify_ops, int n_ops,
-                              const uint8_t* coarse, float margin, uint8_t* flags,
-                              uint32_t* band_list, unsigned long long* certified,
-                              unsigned long long* overturned, const scan::State& st,
-                              hipStream_t s) {
-  if (!pred && !certify_ops) return hipErrorInvalidValue;
-  if (certify_ops && (!centres || !certified || !coarse || !overturned))
+hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const BrickGrid& G,
+                                const uint8_t* coarse, const float* centres,
+                                const float* half_diag, uint8_t* proof,
+                                unsigned long long* certified, hipStream_t s) {
+  if (!ops || !coarse || !centres || !half_diag || !proof || !certified)
     return hipErrorInvalidValue;
-  FlagLoad load{pred,  half_diag, centres, certify_ops, n_ops,  coarse,     G.nbx,
-                G.nby, G.ncx,     G.ncy,   margin,      flags,  certified,  overturned};
+  const uint64_t blocks = (static_cast<uint64_t>(G.nc) + 3) / 4;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_brick_certify, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
+                     n_ops, G, coarse, centres, half_diag, proof, certified);
+  return hipGetLastError();
+}
+
+hipError_t launch_brick_flags(const BrickGrid& G, const float* pred, const float* half_diag,
+                              const uint8_t* proof, float margin, uint8_t* flags,
+                              uint32_t* band_list, unsigned long long* overturned,
+                              const scan::State& st, hipStream_t s) {
+  if (!pred && !proof) return hipErrorInvalidValue;
+  if (proof && !overturned) return hipErrorInvalidValue;
+  FlagLoad load{pred, half_diag, proof, margin, flags, overturned};
   FlagSink sink{band_list};
-  // Certificates evaluate the program per brick: one brick per thread.
-  if (certify_ops) return scan::launch<1>(static_cast<uint64_t>(G.n), load, sink, st, s);
   return scan::launch<scan::kItems>(static_cast<uint64_t>(G.n), load, sink, st, s);
 }
 
@@ -418,10 +499,10 @@ hipError_t launch_band_exact(const DevOp* ops, int n_ops, const Lattice& L, cons
                              hipStream_t s) {
   if (n_band == 0) return hipSuccess;
   if (!ops || !flags || !band_list || !B.signs || !evaluated) return hipErrorInvalidValue;
-  uint64_t blocks = (n_band + kExactWaves - 1) / kExactWaves;
+  uint64_t blocks = (n_band + kGroupBricks - 1) / kGroupBricks;
   if (blocks > (1u << 20)) blocks = 1u << 20;
-  hipLaunchKernelGGL(k_band_exact, dim3(static_cast<unsigned>(blocks)), dim3(64 * kExactWaves), 0,
-                     s, ops, n_ops, L, G, flags, band_list, n_band, B.signs, values, evaluated);
+  hipLaunchKernelGGL(k_band_exact, dim3(static_cast<unsigned>(blocks)), dim3(kExactThreads), 0, s,
+                     ops, n_ops, L, G, flags, band_list, n_band, B.signs, values, evaluated);
   return hipGetLastError();
 }
 
