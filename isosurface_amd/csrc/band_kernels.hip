@@ -231,7 +231,11 @@ static_assert(kSide <= 32, "a box row is one 32-bit mask");
 // and gather sign bits per x row in LDS; last, one lane per row merges the
 // row into the sign words (at most two words, OR for inside, AND-NOT for
 // outside, over the background the bricks' flags wrote).
-__global__ __launch_bounds__(kExactThreads) void k_band_exact(
+// Eight wavefronts per SIMD (64 VGPRs, five of them spilled to scratch)
+// hide the latency of the program evaluation better than the seven
+// that 72 unconstrained VGPRs allow: 117 ms instead of 138 ms at 1024^3.
+__global__ __launch_bounds__(kExactThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+k_band_exact(
     const DevOp* __restrict__ ops, int n_ops, Lattice L, BrickGrid G,
     const uint8_t* __restrict__ flags, const uint32_t* __restrict__ band_list, uint64_t n_band,
     uint64_t* __restrict__ S, float* __restrict__ values, unsigned long long* evaluated) {
