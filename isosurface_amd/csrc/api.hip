@@ -1115,8 +1115,9 @@ int prepare_scan(iso_ctx* c, uint64_t n, iso::scan::State* st) {
   return ISO_OK;
 }
 
-// Reads ctl[0..4) of the scan state (total in [1], status bits in [2]).
-int read_ctl(iso_ctx* c, const iso::scan::State& st) {
+// Reads ctl[0..4) of the scan state (total in [1], status bits in [2]);
+// `stage` names the failing pass in the error message.
+int read_ctl(iso_ctx* c, const iso::scan::State& st, const char* stage = "octree extraction") {
   ISO_HIP(c, hipMemcpyAsync(c->h_stage, st.ctl, 4 * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
@@ -1124,7 +1125,7 @@ int read_ctl(iso_ctx* c, const iso::scan::State& st) {
   if (c->h_stage[2]) {
     const unsigned long long e = c->h_stage[2];
     return fail(c, ISO_ERR_DEVICE,
-                std::string("octree extraction failed:") + ((e & iso::kLhmcErrScan) ? " scan" : "") +
+                std::string(stage) + " failed:" + ((e & iso::kLhmcErrScan) ? " scan" : "") +
                     ((e & iso::kLhmcErrWalk) ? " dual-corner walk" : "") +
                     ((e & iso::kLhmcErrTable) ? " edge table" : ""));
   }
@@ -1917,7 +1918,7 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
               e = iso::launch_coarse_certify(ops, c->n_prog, L, G, coarse, counters + 2,
                                              c->stream);
             if (e == hipSuccess && certify)
-              e = iso::launch_brick_certify(ops, c->n_prog, G, coarse, centres, hd, proof,
+              e = iso::launch_brick_certify(ops, c->n_prog, L, G, coarse, centres, hd, proof,
                                             counters + 2, c->stream);
             if (e == hipSuccess)
               e = iso::launch_brick_flags(G, use_network ? pred : nullptr, hd,
@@ -1925,13 +1926,16 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
                                           counters + 3, S, c->stream);
             return e;
           }));
-  if ((st = read_ctl(c, S)) != ISO_OK) return st;
+  if ((st = read_ctl(c, S, "band brick selection")) != ISO_OK) return st;
   const uint64_t n_band = c->h_stage[1];
   const iso::McDims D = iso::mc_dims(L);
   SignsStage signs = [&](const iso::McBuffers& B, float* values) -> int {
     ISO_HIP(c, timed(c, "band_background", [&] {
               return iso::launch_band_background(L, D, G, bflags, B, c->stream);
             }));
+    // The stage reruns when the active-cell list grows: count the exact
+    // samples of this run only.
+    ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(counters), 2, 0u, c->stream));
     ISO_HIP(c, timed(c, "band_exact", [&] {
               return iso::launch_band_exact(ops, c->n_prog, L, G, bflags, list, n_band, B, values,
                                             counters, c->stream);

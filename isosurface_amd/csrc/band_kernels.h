@@ -44,22 +44,29 @@ BrickGrid brick_grid(const Lattice& L);
 // Brick centres (3 floats each, brick order x fastest) and half-diagonals.
 hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* centres,
                                 float* half_diag, hipStream_t s);
-// f32 evaluation-error allowance of the Lipschitz certificate.
+// Allowances of the Lipschitz certificate beyond its computed bounds: the
+// half-diagonal's own f32 rounding (relative) and a fixed absolute margin.
 constexpr float kCertifyRelSlack = 1.0e-4f;
 constexpr float kCertifyAbsSlack = 2.0e-5f;
+// Bound on how far the f32 rounding of two lattice positions (a centre and a
+// sample) can move their Euclidean distance, from the lattice's largest
+// coordinate magnitude.
+float certify_position_slack(const Lattice& L);
 
 // Super-brick certificate: coarse[s] = kBrickProven (| kBrickInside) when
 // the program's value at super-brick s's centre is farther from zero than
-// the super-brick's half-diagonal (plus the f32 allowance), else 0.  G.nc
-// evaluations are added to *certified.
+// the super-brick's half-diagonal plus the bounds on the positions' and the
+// evaluations' f32 rounding (sdf_eval_bound), else 0.  G.nc evaluations are
+// added to *certified.
 hipError_t launch_coarse_certify(const DevOp* ops, int n_ops, const Lattice& L,
                                  const BrickGrid& G, uint8_t* coarse,
                                  unsigned long long* certified, hipStream_t s);
 
 // Brick proofs: proof[b] = its super-brick's flag when that proves the
-// sign, else the same test at brick b's own centre (|f| > half_diag[b] plus
-// the f32 allowance; evaluations added to *certified).
-hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const BrickGrid& G,
+// sign, else the same test at brick b's own centre (against half_diag[b];
+// evaluations added to *certified).
+hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const Lattice& L,
+                                const BrickGrid& G,
                                 const uint8_t* coarse, const float* centres,
                                 const float* half_diag, uint8_t* proof,
                                 unsigned long long* certified, hipStream_t s);

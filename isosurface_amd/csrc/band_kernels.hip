@@ -4,6 +4,8 @@
 
 #include "band_kernels.h"
 #include "mc_device.h"
+#include <cmath>
+
 #include "sdf_eval.h"
 
 namespace iso {
@@ -59,17 +61,30 @@ __device__ __forceinline__ void count_lanes(bool on, unsigned long long* counter
 
 // Every program is 1-Lipschitz (exact primitive distances, min, max,
 // negation, translation and the polynomial smooth minimum, whose gradient is
-// a convex combination of its operands'), so a centre value farther than the
-// half-diagonal from zero (plus a bound on the f32 evaluation error) proves
-// that every sample of the box has the centre's sign.
-__device__ __forceinline__ bool proves(float v, float half_diag) {
-  return fabsf(v) > half_diag * (1.0f + kCertifyRelSlack) + kCertifyAbsSlack;
+// a convex combination of its operands'), so a centre value farther from
+// zero than the distance to any sample of the box, plus the f32 evaluation
+// error at both, proves that every sample of the box has the centre's sign.
+// `widen` bounds the distance between the computed centre and any computed
+// sample position (the half-diagonal, with the f32 rounding of both
+// positions, pos_slack); the evaluation's own bound E holds at every point
+// within `widen` of the centre, so the sample's f32 value lies within
+// v +- (widen + 2E) and the sign is proven when |v| exceeds that.
+__device__ __forceinline__ uint8_t certify_at(const DevOp* __restrict__ ops, int n_ops, float x,
+                                              float y, float z, float half_diag,
+                                              float pos_slack) {
+  const float widen = half_diag * (1.0f + kCertifyRelSlack) + pos_slack;
+  float e;
+  const float v = sdf_eval_bound(ops, n_ops, x, y, z, widen, &e);
+  return fabsf(v) > widen + 2.0f * e + kCertifyAbsSlack
+             ? static_cast<uint8_t>(kBrickProven | (!(v > 0.0f) ? kBrickInside : 0))
+             : static_cast<uint8_t>(0);
 }
 
 __global__ __launch_bounds__(256) void k_coarse_certify(const DevOp* __restrict__ ops, int n_ops,
                                                         Lattice L, BrickGrid G,
                                                         uint8_t* __restrict__ coarse,
-                                                        unsigned long long* certified) {
+                                                        unsigned long long* certified,
+                                                        float pos_slack) {
   const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   count_lanes(s < G.nc, certified);
   if (s >= G.nc) return;
@@ -88,10 +103,7 @@ __global__ __launch_bounds__(256) void k_coarse_certify(const DevOp* __restrict_
     const float ext = static_cast<float>(kSpan) * L.spacing[a];
     d2 += ext * ext;
   }
-  const float v = sdf_eval(ops, n_ops, p[0], p[1], p[2]);
-  coarse[s] = proves(v, 0.5f * sqrtf(d2))
-                  ? static_cast<uint8_t>(kBrickProven | (!(v > 0.0f) ? kBrickInside : 0))
-                  : static_cast<uint8_t>(0);
+  coarse[s] = certify_at(ops, n_ops, p[0], p[1], p[2], 0.5f * sqrtf(d2), pos_slack);
 }
 
 // Brick proofs: one wavefront per super-brick, one lane per brick (4 x 4 x 4
@@ -103,7 +115,8 @@ __global__ __launch_bounds__(256) void k_brick_certify(const DevOp* __restrict__
                                                        const float* __restrict__ centres,
                                                        const float* __restrict__ half_diag,
                                                        uint8_t* __restrict__ proof,
-                                                       unsigned long long* certified) {
+                                                       unsigned long long* certified,
+                                                       float pos_slack) {
   static_assert(kCoarse * kCoarse * kCoarse == 64, "one lane per brick of a super-brick");
   const int lane = __lane_id();
   const int64_t s = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -120,10 +133,8 @@ __global__ __launch_bounds__(256) void k_brick_certify(const DevOp* __restrict__
   count_lanes(evaluate, certified);
   uint8_t out = cf;
   if (evaluate) {
-    const float v = sdf_eval(ops, n_ops, centres[3 * b], centres[3 * b + 1], centres[3 * b + 2]);
-    out = proves(v, half_diag[b])
-              ? static_cast<uint8_t>(kBrickProven | (!(v > 0.0f) ? kBrickInside : 0))
-              : static_cast<uint8_t>(0);
+    out = certify_at(ops, n_ops, centres[3 * b], centres[3 * b + 1], centres[3 * b + 2],
+                     half_diag[b], pos_slack);
   }
   if (live) proof[b] = out;
 }
@@ -443,7 +454,7 @@ hipError_t launch_coarse_certify(const DevOp* ops, int n_ops, const Lattice& L,
                                  unsigned long long* certified, hipStream_t s) {
   if (!ops || !coarse || !certified) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_coarse_certify, dim3(grid_of(G.nc, 1u << 30)), dim3(256), 0, s, ops, n_ops,
-                     L, G, coarse, certified);
+                     L, G, coarse, certified, certify_position_slack(L));
   return hipGetLastError();
 }
 
@@ -454,8 +465,26 @@ hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* cen
   return hipGetLastError();
 }
 
-hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const BrickGrid& G,
-                                const uint8_t* coarse, const float* centres,
+float certify_position_slack(const Lattice& L) {
+  // A lattice position origin + (float)i * spacing rounds twice (the
+  // product and the sum), each by at most 2^-24 of the magnitude, so a
+  // computed sample or centre coordinate is within 2^-23 M of the exact
+  // lattice position, M bounding |origin| + i |spacing| over the lattice.
+  // Two such points (centre and sample) add at most 2 sqrt(3) 2^-23 M to
+  // their Euclidean distance.
+  double m = 0.0;
+  const int64_t n[3] = {L.nx, L.ny, L.nz};
+  for (int a = 0; a < 3; ++a) {
+    const double ma = std::fabs(static_cast<double>(L.origin[a])) +
+                      static_cast<double>(n[a] > 0 ? n[a] - 1 : 0) *
+                          std::fabs(static_cast<double>(L.spacing[a]));
+    if (ma > m) m = ma;
+  }
+  return static_cast<float>(2.0 * std::sqrt(3.0) * std::ldexp(m, -23) * 1.01);
+}
+
+hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const Lattice& L,
+                                const BrickGrid& G, const uint8_t* coarse, const float* centres,
                                 const float* half_diag, uint8_t* proof,
                                 unsigned long long* certified, hipStream_t s) {
   if (!ops || !coarse || !centres || !half_diag || !proof || !certified)
@@ -463,7 +492,8 @@ hipError_t launch_brick_certify(const DevOp* ops, int n_ops, const BrickGrid& G,
   const uint64_t blocks = (static_cast<uint64_t>(G.nc) + 3) / 4;
   if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_brick_certify, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
-                     n_ops, G, coarse, centres, half_diag, proof, certified);
+                     n_ops, G, coarse, centres, half_diag, proof, certified,
+                     certify_position_slack(L));
   return hipGetLastError();
 }
 

@@ -121,6 +121,136 @@ __device__ __forceinline__ float sdf_eval(const DevOp* __restrict__ ops, int n, 
 }
 
 // ---------------------------------------------------------------------------
+// Evaluation with an error bound, for proofs.  Returns sdf_eval's value at
+// (px, py, pz) and sets *err to a bound E such that, for every point x within
+// Euclidean distance `widen` of (px, py, pz), the f32 evaluation at x differs
+// from the program's exact (real-arithmetic) value at x by at most E.  The
+// bound follows the evaluation: with u = 2^-24, a subtraction or sum rounds
+// by at most u times its magnitude, a primitive by at most 16u times its
+// operands' magnitude (squares, sums, square roots and the final subtraction
+// together, with room to spare), min, max and negation are exact, and the
+// smooth minimum (whose exact form is 1-Lipschitz in the max-norm of its
+// operands) adds its own roundings wherever the blend can be active in the
+// ball.  Every magnitude is taken over the ball (the value at the centre plus
+// `widen` plus its error), so E holds at every point of it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float bound_max3(float a, float b, float c) {
+  return fmaxf(a, fmaxf(b, c));
+}
+
+__device__ __forceinline__ float sdf_eval_bound(const DevOp* __restrict__ ops, int n, float px,
+                                                float py, float pz, float widen, float* err) {
+  constexpr float u = 5.9604645e-08f;  // 2^-24
+  float top = 0.0f, etop = 0.0f;
+  float st[kMaxValueStack], est[kMaxValueStack];
+  int sp = 0;
+  float cx = px, cy = py, cz = pz;
+  float pe = 0.0f;  // bound on the rounding of the current (translated) point, per coordinate
+  float sx[kMaxPointStack], sy[kMaxPointStack], sz[kMaxPointStack], spe[kMaxPointStack];
+  int psp = 0;
+  bool have_top = false;
+  auto translate_err = [&](float qx, float qy, float qz, float e) {
+    return e + u * (bound_max3(__builtin_fabsf(qx), __builtin_fabsf(qy), __builtin_fabsf(qz)) +
+                    widen + e);
+  };
+  auto blend = [&](float a, float ea, float b, float eb, float k) {
+    // exact form's sensitivity: max(ea, eb); roundings only where the blend
+    // can be active somewhere in the ball
+    float e = fmaxf(ea, eb);
+    if (__builtin_fabsf(a - b) < k + 2.0f * widen + 2.0f * (ea + eb) + u * (__builtin_fabsf(a) +
+                                                                            __builtin_fabsf(b))) {
+      const float mag = fminf(__builtin_fabsf(a), __builtin_fabsf(b)) + widen + e;
+      e += 8.0f * u * k + 2.0f * u * (mag + k);
+    }
+    return e;
+  };
+  for (int i = 0; i < n; ++i) {
+    const uint32_t code = ops[i].code;
+    if (code <= kCylinder) {
+      float qx = cx, qy = cy, qz = cz, qe = pe;
+      if (ops[i].translated) {
+        qx = cx - ops[i].off[0];
+        qy = cy - ops[i].off[1];
+        qz = cz - ops[i].off[2];
+        qe = translate_err(qx, qy, qz, pe);
+      }
+      const float v = prim_value(code, ops[i].prm, qx, qy, qz);
+      const float qmax =
+          bound_max3(__builtin_fabsf(qx), __builtin_fabsf(qy), __builtin_fabsf(qz)) + widen + qe;
+      const float pm = __builtin_fabsf(ops[i].prm[0]) + __builtin_fabsf(ops[i].prm[1]) +
+                       __builtin_fabsf(ops[i].prm[2]) + __builtin_fabsf(ops[i].prm[3]);
+      // point rounding (1-Lipschitz in the Euclidean norm) + the primitive's own
+      const float ev = 1.7320509f * qe + 16.0f * u * (1.7320509f * qmax + pm);
+      const uint32_t mode = ops[i].combine;
+      if (mode == kPush) {
+        if (have_top) {
+          st[sp] = top;
+          est[sp] = etop;
+          ++sp;
+        }
+        top = v;
+        etop = ev;
+        have_top = true;
+      } else if (mode == kMin) {
+        top = fminf(top, v);
+        etop = fmaxf(etop, ev);
+      } else if (mode == kMax) {
+        top = fmaxf(top, v);
+        etop = fmaxf(etop, ev);
+      } else if (mode == kDiff) {
+        top = fmaxf(v, -top);
+        etop = fmaxf(etop, ev);
+      } else if (mode == kDiffRev) {
+        top = fmaxf(top, -v);
+        etop = fmaxf(etop, ev);
+      } else {
+        etop = blend(top, etop, v, ev, ops[i].k);
+        top = smooth_min(top, v, ops[i].k);
+      }
+    } else if (code == kPushTranslate) {
+      sx[psp] = cx;
+      sy[psp] = cy;
+      sz[psp] = cz;
+      spe[psp] = pe;
+      ++psp;
+      cx = cx - ops[i].off[0];
+      cy = cy - ops[i].off[1];
+      cz = cz - ops[i].off[2];
+      pe = translate_err(cx, cy, cz, pe);
+    } else if (code == kPopTranslate) {
+      --psp;
+      cx = sx[psp];
+      cy = sy[psp];
+      cz = sz[psp];
+      pe = spe[psp];
+    } else {
+      const float t = top, et = etop;
+      --sp;
+      const float w = st[sp], ew = est[sp];
+      if (code == kUnion) {
+        top = fminf(w, t);
+        etop = fmaxf(ew, et);
+      } else if (code == kIntersection) {
+        top = fmaxf(w, t);
+        etop = fmaxf(ew, et);
+      } else if (code == kDifference) {
+        top = fmaxf(t, -w);
+        etop = fmaxf(ew, et);
+      } else if (code == kDifferenceRev) {
+        top = fmaxf(w, -t);
+        etop = fmaxf(ew, et);
+      } else {
+        etop = blend(w, ew, t, et, ops[i].k);
+        top = smooth_min(w, t, ops[i].k);
+      }
+    }
+  }
+  // the bound's own f32 arithmetic rounds too
+  *err = etop * 1.001f;
+  return top;
+}
+
+// ---------------------------------------------------------------------------
 // Directed distances: VectorSource::sample_vector (source.rs:32-38), the
 // signed distance to the surface along each cardinal axis, or f32::MAX where
 // the axis line through the point misses the shape.  Each restates the

@@ -37,14 +37,20 @@ double or_sample_f64(const iso_node* prog, uint32_t n, double x, double y, doubl
 }
 
 int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* desc, float* out) {
+  return or_sample_lattice_layers(prog, n, desc, 0, desc->nz, out);
+}
+
+int or_sample_lattice_layers(const iso_node* prog, uint32_t n, const iso_grid_desc* desc,
+                             int64_t z0, int64_t z1, float* out) {
   orc::Program t;
   if (!orc::load_program(prog, n, &t)) return ISO_ERR_PROGRAM;
   const iso_grid_desc& d = *desc;
-  for (int64_t z = 0; z < d.nz; ++z)
+  if (z0 < 0 || z1 > d.nz || z0 > z1) return ISO_ERR_INVALID_ARGUMENT;
+  for (int64_t z = z0; z < z1; ++z)
     for (int64_t y = 0; y < d.ny; ++y)
       for (int64_t x = 0; x < d.nx; ++x) {
         const F3 p{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
-        out[(z * d.ny + y) * d.nx + x] = orc::sample(t, p);
+        out[((z - z0) * d.ny + y) * d.nx + x] = orc::sample(t, p);
       }
   return ISO_OK;
 }

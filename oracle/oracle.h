@@ -40,6 +40,11 @@ int or_program_check(const iso_node* prog, uint32_t n);
 
 /* Dense samples of a program over a lattice (x fastest). */
 int or_sample_lattice(const iso_node* prog, uint32_t n, const iso_grid_desc* desc, float* out);
+/* Layers [z0, z1) of the same lattice into out ((z1 - z0) * ny * nx floats):
+ * the same values as or_sample_lattice, so callers may sample disjoint
+ * layer ranges on several threads. */
+int or_sample_lattice_layers(const iso_node* prog, uint32_t n, const iso_grid_desc* desc,
+                             int64_t z0, int64_t z1, float* out);
 
 /* MarchingCubes::extract with IndexedVertices over a lattice whose samples
  * come from `prog` (if grid == NULL) or from the dense `grid`.

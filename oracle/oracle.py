@@ -63,6 +63,10 @@ def load():
         lib.or_sample_lattice.restype = C.c_int
         lib.or_sample_lattice.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.POINTER(IsoGridDesc),
                                           C.c_void_p]
+        lib.or_sample_lattice_layers.restype = C.c_int
+        lib.or_sample_lattice_layers.argtypes = [C.POINTER(IsoNode), C.c_uint32,
+                                                 C.POINTER(IsoGridDesc), C.c_int64, C.c_int64,
+                                                 C.c_void_p]
         lib.or_extract_mc.restype = C.c_int
         lib.or_extract_mc.argtypes = [C.POINTER(IsoNode), C.c_uint32, C.c_void_p,
                                       C.POINTER(IsoGridDesc), C.c_int, C.POINTER(OrMesh)]
@@ -151,6 +155,29 @@ def sample_lattice(program, desc: IsoGridDesc) -> np.ndarray:
     out = np.empty((desc.nz, desc.ny, desc.nx), dtype=np.float32)
     st = lib.or_sample_lattice(arr, len(program), C.byref(desc), out.ctypes.data)
     assert st == 0, st
+    return out
+
+
+def sample_lattice_threads(program, desc: IsoGridDesc, threads: int = 8) -> np.ndarray:
+    """sample_lattice on several threads (disjoint layer ranges; ctypes
+    releases the interpreter lock during each call): the same values."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    lib = load()
+    arr = nodes(program)
+    desc = as_desc(desc)
+    out = np.empty((desc.nz, desc.ny, desc.nx), dtype=np.float32)
+    bounds = np.linspace(0, desc.nz, max(1, threads) * 4 + 1).astype(np.int64)
+
+    def run(k):
+        z0, z1 = int(bounds[k]), int(bounds[k + 1])
+        if z1 > z0:
+            st = lib.or_sample_lattice_layers(arr, len(program), C.byref(desc), z0, z1,
+                                              out[z0:z1].ctypes.data)
+            assert st == 0, st
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        list(ex.map(run, range(len(bounds) - 1)))
     return out
 
 

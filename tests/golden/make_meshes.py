@@ -7,7 +7,8 @@ The oracle (oracle/, a restatement of the reference's CPU extraction) is run
 here once; the GPU tests compare their output digests with these values, so
 full-size parity needs no CPU re-run on the GPU host.
 
-usage: python tests/golden/make_meshes.py
+usage: python tests/golden/make_meshes.py [--only NAME[,NAME...]]
+(--only recomputes the named entries and keeps the others as they are)
 """
 import hashlib
 import json
@@ -25,7 +26,51 @@ def digest(a) -> str:
     return hashlib.sha256(a.tobytes()).hexdigest()
 
 
+SMOOTH_SIZES = (129, 257)
+
+
+def smooth_name(size: int) -> str:
+    return f"smooth_spheres1024_k0.02_size{size}"
+
+
+def smooth_cases(names=None):
+    """Marching cubes of the smooth-min (k=0.02) blend of 1024 seeded spheres
+    (the learned-band field), sampled by the oracle on 8 threads."""
+    prog = W.smooth_spheres(1024, 0.02, 5).program()
+    out = {}
+    for size in SMOOTH_SIZES:
+        if names is not None and smooth_name(size) not in names:
+            continue
+        d = O.reference_desc(size)
+        grid = O.sample_lattice_threads(prog, d, threads=8)
+        out[smooth_name(size)] = (size, O.extract_mc(grid=grid, desc=d))
+    return out
+
+
+def mesh_entry(size, m) -> dict:
+    return {
+        "size": size,
+        "vertices": int(len(m.vertices)),
+        "triangles": int(len(m.indices) // 3),
+        "vertices_sha256": digest(m.vertices.astype("<f4")),
+        "indices_sha256": digest(m.indices.astype("<u4")),
+    }
+
+
 def main() -> None:
+    path = Path(__file__).with_name("mesh_digests.json")
+    if len(sys.argv) > 2 and sys.argv[1] == "--only":
+        names = set(sys.argv[2].split(","))
+        known = {smooth_name(size) for size in SMOOTH_SIZES}
+        if names - known:
+            raise SystemExit(f"--only supports {sorted(known)}; unknown: {sorted(names - known)}")
+        out = json.loads(path.read_text())
+        fresh = {k: mesh_entry(*v) for k, v in smooth_cases(names).items()}
+        out.update(fresh)
+        for k, v in fresh.items():
+            print(k, v["vertices"], v["triangles"], flush=True)
+        path.write_text(json.dumps(out, indent=1) + "\n")
+        return
     cases = {
         "sphere_r0.35_size65": (W.sphere_source(), 65),
         "spheres64_minus_torus_size65": (W.spheres_minus_torus()[0], 65),
@@ -53,14 +98,9 @@ def main() -> None:
     meshes.append(("directed_boxes32_size128", 128,
                    O.extract_mc(boxes.program(), desc=O.reference_desc(128), directed=True)))
     meshes.append(("emc_boxes32_size128", 128, O.extract_emc(boxes.program(), O.reference_desc(128))))
+    meshes += [(name, size, m) for name, (size, m) in smooth_cases().items()]
     for name, size, m in meshes:
-        out[name] = {
-            "size": size,
-            "vertices": int(len(m.vertices)),
-            "triangles": int(len(m.indices) // 3),
-            "vertices_sha256": digest(m.vertices.astype("<f4")),
-            "indices_sha256": digest(m.indices.astype("<u4")),
-        }
+        out[name] = mesh_entry(size, m)
         print(name, out[name]["vertices"], out[name]["triangles"], flush=True)
     # The batched-chunk workload: per-chunk oracle meshes concatenated in chunk order.
     from isosurface_amd.engine import chunk_desc  # noqa: E402
@@ -91,7 +131,6 @@ def main() -> None:
     out["normals_spheres64_minus_torus_size257_eps1e-4"] = {
         "size": 257, "points": int(len(nrm)), "normals_sha256": digest(nrm.astype("<f4"))}
     print("pointcloud", len(pts), "normals", len(nrm), flush=True)
-    path = Path(__file__).with_name("mesh_digests.json")
     path.write_text(json.dumps(out, indent=1) + "\n")
 
 

@@ -60,6 +60,44 @@ def test_smooth_union_closed_forms():
     assert O.sample(far, 0.0, 0.0, 0.0) == -1.0
 
 
+@pytest.mark.parametrize("gap,h2", [(0.25, 0.25), (0.125, 0.5625), (0.375, 0.0625)])
+def test_smooth_union_mid_range_closed_forms(gap, h2):
+    # Concentric spheres of radii 1 and 1 + gap give a = -1, b = -(1 + gap) at
+    # the centre, every value exact in f32.  With k = 0.5, h = (k - gap) / k
+    # and the blend is min(a, b) - h^2 k / 4: at gap = k/2 that is min - k/16.
+    # Writing h for h^2, or dropping the 1/4, gives another value at each gap.
+    k = 0.5
+    prog = SmoothUnion(Sphere(1.0), Sphere(1.0 + gap), k).program()
+    want = -(1.0 + gap) - h2 * k / 4.0
+    assert O.sample(prog, 0.0, 0.0, 0.0) == want
+    # symmetric in its operands
+    assert O.sample(SmoothUnion(Sphere(1.0 + gap), Sphere(1.0), k).program(), 0.0, 0.0, 0.0) == want
+    if gap == 0.25:
+        assert want == -1.25 - k / 16.0
+
+
+def test_smooth_union_is_continuous_and_one_lipschitz():
+    # The band's proof needs a 1-Lipschitz field.  Along a line through two
+    # blended spheres the smooth union lies between min(a, b) - k/4 and
+    # min(a, b), has no jump where |a - b| crosses k, and changes by at most
+    # the step length between neighbouring samples (f32 rounding aside).
+    k = 0.2
+    prog = SmoothUnion(Sphere(0.3), Translate((0.45, 0.0, 0.0), Sphere(0.25)), k).program()
+    a_prog = Sphere(0.3).program()
+    b_prog = Translate((0.45, 0.0, 0.0), Sphere(0.25)).program()
+    xs = np.linspace(-0.6, 1.0, 4001, dtype=np.float64)
+    f = np.array([O.sample(prog, float(x), 0.05, 0.0) for x in xs])
+    a = np.array([O.sample(a_prog, float(x), 0.05, 0.0) for x in xs])
+    b = np.array([O.sample(b_prog, float(x), 0.05, 0.0) for x in xs])
+    lo = np.minimum(a, b)
+    assert np.all(f <= lo + 1e-6) and np.all(f >= lo - k / 4 - 1e-6)
+    step = xs[1] - xs[0]
+    assert np.max(np.abs(np.diff(f))) <= step * (1 + 1e-3) + 1e-6
+    # inside the blend region the value is strictly below the minimum
+    mid = np.abs(a - b) < 0.5 * k
+    assert mid.any() and np.all(f[mid] < lo[mid])
+
+
 def test_f64_twin_agrees():
     src, _ = W.spheres_minus_torus()
     prog = src.program()
