@@ -168,7 +168,7 @@ int or_extract_point_cloud(const iso_node* prog, uint32_t n, const float* grid,
   out->n_vertices = pts.size() / 3;
   out->vertices = static_cast<float*>(std::malloc((pts.size() ? pts.size() : 1) * sizeof(float)));
   if (!out->vertices) return ISO_ERR_DEVICE;
-  std::memcpy(out->vertices, pts.data(), pts.size() * sizeof(float));
+  if (!pts.empty()) std::memcpy(out->vertices, pts.data(), pts.size() * sizeof(float));
   return ISO_OK;
 }
 

@@ -6,13 +6,16 @@ import this module, and only as the checker.  The product never does.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle.so"
+# ISO_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build, oracle/_san/liboracle.so).
+LIB_PATH = Path(os.environ.get("ISO_ORACLE_LIB", str(HERE / "liboracle.so")))
 
 
 class IsoNode(C.Structure):
