@@ -12,7 +12,8 @@ digests of the oracle's mesh).
 
 Printed JSON line:
   value         cells/s of the timed steps (wall clock, synchronised)
-  roofline      the classify + interpolate + emit stage (its three kernels):
+  roofline      the classify + interpolate + emit stage of the headline path
+                (the single-pass kernel, or the three-pass path's kernels):
                 algorithmic bytes (4 B per sample + 12 B per vertex + 12 B
                 per triangle) / the sum of the kernels' mean durations over
                 the timed steps (HIP events attached to each kernel's
@@ -95,27 +96,28 @@ class DeviceMesh:
         self.ctx.lib.iso_device_free(self.ctx.handle, self.i)
 
 
-def extract(ctx: Context, desc, mesh: DeviceMesh, graph: bool = False) -> A.IsoMeshOut:
-    out = mesh.out(A.ISO_MESH_GRAPH if graph else 0)
+def extract(ctx: Context, desc, mesh: DeviceMesh, graph: bool = False,
+            flags: int = 0) -> A.IsoMeshOut:
+    out = mesh.out(flags | (A.ISO_MESH_GRAPH if graph else 0))
     ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out)), "iso_extract_mc")
     return out
 
 
-def timed_steps(ctx, desc, mesh, steps, graph=False):
+def timed_steps(ctx, desc, mesh, steps, graph=False, flags=0):
     ctx.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        extract(ctx, desc, mesh, graph)
+        extract(ctx, desc, mesh, graph, flags)
     ctx.synchronize()
     return time.perf_counter() - t0
 
 
-def kernel_pass(ctx, desc, mesh, steps):
+def kernel_pass(ctx, desc, mesh, steps, flags=0):
     """Per-kernel mean durations (HIP events taken by each kernel's dispatch)."""
     ctx.reset_kernel_stats()
     ctx.set_timing(True)
     for _ in range(steps):
-        extract(ctx, desc, mesh)
+        extract(ctx, desc, mesh, flags=flags)
     ctx.synchronize()
     ctx.set_timing(False)
     stats = ctx.kernel_stats()
@@ -139,7 +141,7 @@ def measured_traffic():
                 out["stage"] = int(parts[1])
             else:
                 out["kernels"][name] = int(parts[1])
-    return out if "stage" in out else None
+    return out if out["kernels"] else None
 
 
 def digest(ctx, mesh: DeviceMesh, out) -> tuple:
@@ -150,6 +152,17 @@ def digest(ctx, mesh: DeviceMesh, out) -> tuple:
     ctx.lib.iso_memcpy_to_host(ctx.handle, i.ctypes.data, mesh.i, i.nbytes)
     return (hashlib.sha256(v.astype("<f4").tobytes()).hexdigest(),
             hashlib.sha256(i.astype("<u4").tobytes()).hexdigest())
+
+
+def host_cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or the platform's name."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def cpu_baseline(grid: np.ndarray, size: int, budget_s: float) -> dict:
@@ -177,8 +190,8 @@ def cpu_baseline(grid: np.ndarray, size: int, budget_s: float) -> dict:
         "kind": "port",
         "sample": (f"oracle MarchingCubes over the same resident {size}x{size}x{size + 1} grid, "
                    f"{len(times)} runs, median {med:.3f} s, pinned={pinned}, "
-                   f"host {platform.processor() or platform.machine()}, "
-                   f"{os.cpu_count()} logical cpus"),
+                   f"host CPU {host_cpu_model()}, {os.cpu_count()} logical cpus"),
+        "host_cpu": host_cpu_model(),
     }
 
 
@@ -558,10 +571,12 @@ def main() -> int:
         log(f"parity vs oracle digests: {parity}")
 
     if args.traffic_probe:
-        # Counter passes read per-dispatch FETCH_SIZE / WRITE_SIZE: the grid
-        # kernels of a few extractions, then copies of exactly 64 MiB.
+        # Counter passes read per-dispatch FETCH_SIZE / WRITE_SIZE: a few
+        # single-pass and three-pass extractions, then copies of exactly 64 MiB.
         for _ in range(3):
             extract(ctx, desc, mesh)
+        for _ in range(3):
+            extract(ctx, desc, mesh, flags=A.ISO_MESH_SINGLE_PASS)
         nb = 64 << 20
         a, b = C.c_void_p(), C.c_void_p()
         ctx.check(ctx.lib.iso_device_alloc(ctx.handle, nb, C.byref(a)), "alloc")
@@ -573,50 +588,75 @@ def main() -> int:
         print(json.dumps({"traffic_probe": True, "copy_bytes": nb, "samples": n}), flush=True)
         return 0
 
-    # Each step is one iso_extract_mc call (three launches) that returns the
-    # mesh counts to the host.  The same call with ISO_MESH_GRAPH (the three
-    # launches replayed as one captured graph) is timed beside it.
+    # Each step is one iso_extract_mc call that returns the mesh counts to
+    # the host, by the three-pass path (classify, count and emit launches,
+    # the library's default) and by the single pass (one launch,
+    # ISO_MESH_SINGLE_PASS); the headline is the faster of the two, the
+    # other and the three-pass graph replay are reported beside it.
+    single = A.ISO_MESH_SINGLE_PASS
     for _ in range(args.warmup):
         extract(ctx, desc, mesh, graph=True)
     graph_elapsed = timed_steps(ctx, desc, mesh, args.steps, graph=True)
     for _ in range(args.warmup):
+        extract(ctx, desc, mesh, flags=single)
+    single_elapsed = timed_steps(ctx, desc, mesh, args.steps, flags=single)
+    for _ in range(args.warmup):
         extract(ctx, desc, mesh)
-    elapsed = timed_steps(ctx, desc, mesh, args.steps)
-    # Clear the mesh so that the check below reads what a replay wrote.
+    three_elapsed = timed_steps(ctx, desc, mesh, args.steps)
+    # Clear the mesh so that the checks below read what each path wrote.
     zeros = np.zeros(max(3 * mesh.nv, mesh.ni), np.uint32)
-    ctx.lib.iso_memcpy_to_device(ctx.handle, mesh.v, zeros.ctypes.data, 12 * mesh.nv)
-    ctx.lib.iso_memcpy_to_device(ctx.handle, mesh.i, zeros.ctypes.data, 4 * mesh.ni)
-    gout = extract(ctx, desc, mesh, graph=True)
-    if parity is not None:
-        gv, gi = digest(ctx, mesh, gout)
-        parity = parity and gv == golden[key]["vertices_sha256"] and gi == golden[key]["indices_sha256"]
-        log(f"graph replay parity vs oracle digests: {parity}")
+    for flags, graph in ((single, False), (0, True)):
+        ctx.lib.iso_memcpy_to_device(ctx.handle, mesh.v, zeros.ctypes.data, 12 * mesh.nv)
+        ctx.lib.iso_memcpy_to_device(ctx.handle, mesh.i, zeros.ctypes.data, 4 * mesh.ni)
+        o = extract(ctx, desc, mesh, graph=graph, flags=flags)
+        if parity is not None:
+            dv, di = digest(ctx, mesh, o)
+            parity = parity and dv == golden[key]["vertices_sha256"] and \
+                di == golden[key]["indices_sha256"]
+    log(f"single-pass and graph replay parity vs oracle digests: {parity}")
+    use_single = single_elapsed < three_elapsed
+    elapsed = single_elapsed if use_single else three_elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     value = cells * args.steps / elapsed
-    log(f"grid path: {ms_per_step:.4f} ms/step ({1e3 * graph_elapsed / args.steps:.4f} with "
-        f"graph replay), {value / 1e9:.2f} G cells/s")
+    log(f"grid path: {1e3 * three_elapsed / args.steps:.4f} ms/step three-pass, "
+        f"{1e3 * single_elapsed / args.steps:.4f} single pass, "
+        f"{1e3 * graph_elapsed / args.steps:.4f} three-pass graph replay; "
+        f"headline {value / 1e9:.2f} G cells/s")
 
-    kernels = kernel_pass(ctx, desc, mesh, args.steps)
+    three_kernels = kernel_pass(ctx, desc, mesh, args.steps)
+    single_kernels = kernel_pass(ctx, desc, mesh, args.steps, flags=single)
     # The classify + interpolate + emit stage is the unit of the roofline:
     # algorithmic bytes 4 per sample read + 12 per vertex + 12 per triangle
-    # written, over the summed mean durations of its three kernels.
-    stage = ["mc_signs_grid", "mc_count", "mc_emit"]
-    stage_us = sum(kernels[k]["mean_us"] for k in stage)
+    # written, over the headline path's kernel time (the single kernel's mean
+    # duration, or the three-pass path's three kernels summed).
     stage_bytes = 4 * n + 12 * nv + 4 * ni
+    three_stage = ["mc_signs_grid", "mc_count", "mc_emit"]
+    three_us = sum(three_kernels[k]["mean_us"] for k in three_stage)
+    single_us = single_kernels["mc_fused"]["mean_us"]
+    stage_us = single_us if use_single else three_us
     achieved = stage_bytes / (stage_us * 1e-6) / 1e9
     segs = (desc.nx - 1 + 63) // 64 * (desc.ny - 1) * (desc.nz - 1)
     kernel_bytes = {
+        "mc_fused": stage_bytes,                   # every sample once, mesh out
         "mc_signs_grid": 4 * n + n // 8,           # every sample once, one bit out
         "mc_count": n // 8 + 8 * segs,              # sign bits in, first ids out
         "mc_emit": 8 * nv + 12 * nv + 4 * ni,       # endpoint samples in, mesh out
     }
+    all_kernels = {**three_kernels, **single_kernels}
     per_kernel = {
-        k: {"mean_us": round(kernels[k]["mean_us"], 3), "algorithmic_bytes": b,
-            "achieved_GBps": b / (kernels[k]["mean_us"] * 1e-6) / 1e9,
-            "frac": b / (kernels[k]["mean_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS}
+        k: {"mean_us": round(all_kernels[k]["mean_us"], 3), "algorithmic_bytes": b,
+            "achieved_GBps": b / (all_kernels[k]["mean_us"] * 1e-6) / 1e9,
+            "frac": b / (all_kernels[k]["mean_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS}
         for k, b in kernel_bytes.items()}
     traffic = measured_traffic()
-    log(f"kernels: {json.dumps(kernels)}")
+    stage_traffic = None
+    if traffic:
+        if use_single:
+            stage_traffic = traffic["kernels"].get("mc_fused")
+        elif all(k in traffic["kernels"] for k in three_stage):
+            stage_traffic = sum(traffic["kernels"][k] for k in three_stage)
+    log(f"kernels: three-pass {json.dumps(three_kernels)} single pass "
+        f"{json.dumps(single_kernels)}")
 
     # Program path: the CSG sampled by the extraction itself (no grid).
     pdesc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
@@ -643,7 +683,10 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
-        "graph_replay_ms_per_step": 1e3 * graph_elapsed / args.steps,
+        "path": "single pass (one launch)" if use_single else "three-pass (classify, count, emit)",
+        "three_pass_ms_per_step": 1e3 * three_elapsed / args.steps,
+        "single_pass_ms_per_step": 1e3 * single_elapsed / args.steps,
+        "three_pass_graph_replay_ms_per_step": 1e3 * graph_elapsed / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -661,15 +704,19 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_signs_grid + k_count + k_emit (classify, count and scan, "
-                      "interpolate and emit: one extraction)",
+            "kernel": ("k_mc_fused (classify, count, look-back scan, interpolate and emit: "
+                       "one extraction, one launch)") if use_single else
+                      ("k_signs_grid + k_count + k_emit (classify, count and scan, "
+                       "interpolate and emit: one extraction)"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic.get("stage") if traffic else None,
+            "traffic": stage_traffic,
             "algorithmic_bytes": stage_bytes,
             "mean_us": stage_us,
+            "three_pass_mean_us": three_us,
+            "single_pass_mean_us": single_us,
         },
         "kernels": per_kernel,
         "traffic_per_kernel": traffic.get("kernels") if traffic else None,

@@ -191,6 +191,11 @@ typedef struct iso_mesh_out {
  * host memory, while kernel timing is on, and for a context on the null
  * stream (which cannot be captured).  Other entry points ignore the field. */
 #define ISO_MESH_GRAPH 1
+/* ISO_MESH_SINGLE_PASS (iso_extract_mc of a grid source): extract in one
+ * launch that reads every sample once (per-tile classify, count, look-back
+ * scan and emit) instead of the separate classify, count and emit launches;
+ * the same output.  Ignored with ISO_MESH_GRAPH and for program sources. */
+#define ISO_MESH_SINGLE_PASS 2
 
 /* Marching cubes over the lattice: replaces MarchingCubes::extract with an
  * IndexedVertices extractor (marching_cubes.rs:59-82, extractor.rs:72-94).

@@ -85,6 +85,7 @@ class IsoBatchOut(C.Structure):
 
 
 ISO_MESH_GRAPH = 1
+ISO_MESH_SINGLE_PASS = 2
 ISO_BAND_PARAMS = 15617
 ISO_BAND_CERTIFY = 1
 ISO_BAND_NO_NETWORK = 2

@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["api.hip", "mc_kernels.hip", "dc_kernels.hip", "lhmc_kernels.hip", "mlp_kernels.hip",
+SOURCES = ["api.hip", "mc_kernels.hip", "mc_fused.hip", "dc_kernels.hip", "lhmc_kernels.hip", "mlp_kernels.hip",
            "band_kernels.hip", "emc_kernels.hip", "program.cpp"]
 
 

@@ -102,6 +102,11 @@ struct iso_ctx {
   iso_grid_desc band_desc{};  // lattice of the last band extraction
   bool band_valid = false;
 
+  // single-pass grid extraction: row-aligned signs, tile flags, look-back
+  // words, row first ids and control words (all epoch-tagged), and the epoch
+  DevBuf f_signs, f_flags, f_slots, f_rowbase, f_ctl;
+  uint32_t f_epoch = 0;
+
   // captured extraction and the buffers it was captured with
   hipGraphExec_t b_graph = nullptr;
   iso::McBuffers g_buffers{};
@@ -316,7 +321,8 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
                     &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->dense_check,  &c->nrm,
                     &c->dgrid,      &c->dvals,      &c->emc_cells, &c->emc_first,
-                    &c->emc_flips,  &c->emc_ctr,    &c->br_coarse, &c->br_proof};
+                    &c->emc_flips,  &c->emc_ctr,    &c->br_coarse, &c->br_proof,
+                    &c->f_signs,    &c->f_flags,    &c->f_slots,   &c->f_rowbase, &c->f_ctl};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
   if (c->m_graph) (void)hipGraphExecDestroy(c->m_graph);
   for (DevBuf* b : bufs) release(b);
@@ -596,11 +602,165 @@ struct BandEmit {
   int64_t nbx, nby;
 };
 
+// Grows b to `bytes`; a new allocation is zero-filled (the single-pass
+// kernel's epoch-tagged words must start below every epoch).
+int ensure_zeroed(iso_ctx* c, DevBuf* b, size_t bytes) {
+  const void* before = b->ptr;
+  ISO_HIP(c, ensure(b, bytes));
+  if (b->ptr != before) ISO_HIP(c, hipMemsetAsync(b->ptr, 0, b->bytes, c->stream));
+  return ISO_OK;
+}
+
+int fused_launch_and_read(iso_ctx* c, const iso::Lattice& L, const float* grid,
+                          const iso::FusedPlan& P, iso::FusedBuffers B, uint32_t epoch,
+                          bool timing, uint64_t* nv, uint64_t* nt);
+
+// One single-pass extraction (mc_fused.hip) of a grid source into `vout` /
+// `iout` (device) with the given capacities; sets *nv, *nt.
+int run_fused(iso_ctx* c, const iso::Lattice& L, const float* grid, const iso::FusedPlan& P,
+              float* vout, uint32_t* iout, uint64_t vcap, uint64_t icap, uint64_t* nv,
+              uint64_t* nt) {
+  int st;
+  if ((st = ensure_zeroed(c, &c->f_signs, P.sign_words * sizeof(uint64_t))) != ISO_OK) return st;
+  if ((st = ensure_zeroed(c, &c->f_flags, P.n_tickets * sizeof(uint64_t))) != ISO_OK) return st;
+  if ((st = ensure_zeroed(c, &c->f_slots, 4 * P.n_tickets * sizeof(uint64_t))) != ISO_OK)
+    return st;
+  if ((st = ensure_zeroed(c, &c->f_rowbase, P.n_rows * sizeof(uint64_t))) != ISO_OK) return st;
+  if ((st = ensure_zeroed(c, &c->f_ctl, iso::kFusedCtlSlots * sizeof(uint64_t))) != ISO_OK)
+    return st;
+  // Tags keep 22 bits of the epoch: clear the words before it wraps.
+  c->f_epoch += 1;
+  if ((c->f_epoch & 0x3FFFFFu) == 0u) {
+    for (DevBuf* b : {&c->f_flags, &c->f_slots, &c->f_rowbase, &c->f_ctl})
+      ISO_HIP(c, hipMemsetAsync(b->ptr, 0, b->bytes, c->stream));
+    c->f_epoch = 1;
+  }
+  iso::FusedBuffers B{};
+  B.signs = static_cast<uint64_t*>(c->f_signs.ptr);
+  B.flags = static_cast<unsigned long long*>(c->f_flags.ptr);
+  B.slots = static_cast<unsigned long long*>(c->f_slots.ptr);
+  B.rowbase = static_cast<unsigned long long*>(c->f_rowbase.ptr);
+  B.ctl = static_cast<unsigned long long*>(c->f_ctl.ptr);
+  B.vertices = vout;
+  B.indices = iout;
+  B.vcap = vout ? vcap : 0;
+  B.icap = iout ? icap : 0;
+  return fused_launch_and_read(c, L, grid, P, B, c->f_epoch, true, nv, nt);
+}
+
+// Launches the single pass over buffers B with `epoch` and reads back its
+// totals; `timing`: the launch may take kernel-timing events.
+int fused_launch_and_read(iso_ctx* c, const iso::Lattice& L, const float* grid,
+                          const iso::FusedPlan& P, iso::FusedBuffers B, uint32_t epoch,
+                          bool timing, uint64_t* nv, uint64_t* nt) {
+  B.host_totals = c->d_coh;
+  // The kernel's last tile writes the totals, and any tile its error flags,
+  // into host-coherent memory (without it: copied from the control words).
+  if (c->h_coh) {
+    volatile unsigned long long* h = c->h_coh;
+    h[iso::kTotVertices] = ~0ull;
+    h[iso::kTotTriangles] = ~0ull;
+    h[iso::kTotErrors] = 0ull;
+    h[iso::kTotErrors + 1] = 0ull;
+  }
+  if (timing) {
+    ISO_HIP(c, timed(c, "mc_fused", [&] {
+              return iso::launch_mc_fused(grid, L, P, B, epoch, c->stream);
+            }));
+  } else {
+    ISO_HIP(c, iso::launch_mc_fused(grid, L, P, B, epoch, c->stream));
+  }
+  bool overflow = false, gave_up = false;
+  if (c->h_coh) {
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    const volatile unsigned long long* h = c->h_coh;
+    *nv = h[iso::kTotVertices];
+    *nt = h[iso::kTotTriangles];
+    overflow = h[iso::kTotErrors] != 0ull;
+    gave_up = h[iso::kTotErrors + 1] != 0ull;
+  } else {
+    ISO_HIP(c, hipMemcpyAsync(c->h_totals, B.ctl + iso::kFusedVertices,
+                              3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    ISO_HIP(c, hipStreamSynchronize(c->stream));
+    *nv = c->h_totals[0];
+    *nt = c->h_totals[1];
+    const unsigned long long e = c->h_totals[2];
+    const bool current = (e >> 32) == epoch;
+    overflow = current && (e & 1ull);
+    gave_up = current && (e & 2ull);
+  }
+  if (timing) collect_timing(c);
+  if (gave_up) return fail(c, ISO_ERR_DEVICE, "single-pass extraction: a wait between tiles gave up");
+  if (*nv == ~0ull || *nt == ~0ull)
+    return fail(c, ISO_ERR_DEVICE, "single-pass extraction: totals not written");
+  if (overflow) return fail(c, ISO_ERR_OVERFLOW, "mesh exceeds u32 indices");
+  return ISO_OK;
+}
+
+// Marching cubes of a grid source in one launch (see mc_fused.hip).
+int extract_fused(iso_ctx* c, const iso::Lattice& L, const float* grid, const iso::FusedPlan& P,
+                  iso_mesh_out* out) {
+  int st;
+  uint64_t nv = 0, nt = 0;
+  if (out->memory == ISO_MEMORY_DEVICE) {
+    if ((st = run_fused(c, L, grid, P, out->vertices, out->indices,
+                        out->vertices ? out->vertex_capacity : 0,
+                        out->indices ? out->index_capacity : 0, &nv, &nt)) != ISO_OK)
+      return st;
+    out->vertex_count = nv;
+    out->index_count = 3 * nt;
+    if (nv > (out->vertices ? out->vertex_capacity : 0) ||
+        3 * nt > (out->indices ? out->index_capacity : 0))
+      return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+    return ISO_OK;
+  }
+  // Host memory: extract into the context's device buffers (sized from the
+  // last extraction), again with room if they were too small, then copy.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const uint64_t vcap = c->vout.bytes / (3 * sizeof(float));
+    const uint64_t icap = c->iout.bytes / sizeof(uint32_t);
+    if ((st = run_fused(c, L, grid, P, static_cast<float*>(c->vout.ptr),
+                        static_cast<uint32_t*>(c->iout.ptr), c->vout.ptr ? vcap : 0,
+                        c->iout.ptr ? icap : 0, &nv, &nt)) != ISO_OK)
+      return st;
+    if (nv <= vcap && 3 * nt <= icap && (nv == 0 || c->vout.ptr) && (nt == 0 || c->iout.ptr))
+      break;
+    ISO_HIP(c, ensure(&c->vout, (nv ? nv : 1) * 3 * sizeof(float)));
+    ISO_HIP(c, ensure(&c->iout, (nt ? 3 * nt : 1) * sizeof(uint32_t)));
+  }
+  const uint64_t ni = 3 * nt;
+  out->vertex_count = nv;
+  out->index_count = ni;
+  if ((nv && (!out->vertices || out->vertex_capacity < nv)) ||
+      (ni && (!out->indices || out->index_capacity < ni))) {
+    c->kept = true;
+    c->kept_vertices = nv;
+    c->kept_indices = ni;
+    return fail(c, ISO_ERR_CAPACITY, "output buffers too small; mesh kept for iso_mesh_fetch");
+  }
+  if (nv)
+    ISO_HIP(c, hipMemcpyAsync(out->vertices, c->vout.ptr, nv * 3 * sizeof(float),
+                              hipMemcpyDeviceToHost, c->stream));
+  if (ni)
+    ISO_HIP(c, hipMemcpyAsync(out->indices, c->iout.ptr, ni * sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, c->stream));
+  ISO_HIP(c, hipStreamSynchronize(c->stream));
+  return ISO_OK;
+}
+
 int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const float* positions,
                     iso_mesh_out* out, const SignsStage* signs = nullptr,
                     const BandEmit* band = nullptr, const float* directed = nullptr) {
   int st = ISO_OK;
   c->band_valid = false;  // the context's sign words are about to change
+  // The single pass over a grid source, when asked for (not with a captured
+  // graph, whose launches the single pass, keyed on a per-extraction epoch,
+  // cannot replay).
+  if (grid && !positions && !signs && !band && !directed &&
+      (out->flags & ISO_MESH_SINGLE_PASS) && !(out->flags & ISO_MESH_GRAPH)) {
+    iso::FusedPlan P;
+    if (iso::fused_plan(L, &P)) return extract_fused(c, L, grid, P, out);
+  }
   const iso::McDims D = iso::mc_dims(L);
   if (D.n_segs >= (int64_t(1) << 26))
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice has 2^26 or more 64-cell segments");

@@ -101,6 +101,8 @@ struct McBuffers {
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
 bool launch_events_pending();
 void clear_launch_events();
+// Takes the pending pair (false if none), for launchers in other files.
+bool take_launch_events(hipEvent_t* start, hipEvent_t* stop);
 
 // Each returns hipSuccess or the launch error.  The signs launchers also
 // reset the per-extraction state (guard words, totals, look-back words).
@@ -167,5 +169,57 @@ hipError_t launch_batch_offsets(const BatchPlan& P, const McBuffers& B, uint64_t
                                 uint64_t* ioff, hipStream_t s);
 hipError_t launch_sample_points(const DevOp* ops, int n_ops, const float* pts, uint64_t n,
                                 float* out, hipStream_t s);
+
+// Single-pass marching cubes over a dense grid (mc_fused.hip): tiles of R
+// consecutive cell rows of one cell layer, TL tiles per layer, taken in
+// order by persistent 512-thread workgroups; TL more tickets produce the
+// sign rows of sample layer 0.
+struct FusedPlan {
+  int32_t R;             // cell rows per tile
+  int32_t TL;            // tiles per cell layer
+  int64_t n_tiles;       // TL * (nz - 1)
+  int64_t n_tickets;     // n_tiles + TL
+  int32_t wpr;           // sign words per sample row: ceil(nx / 64)
+  int32_t spr;           // 64-cell segments per cell row: ceil((nx - 1) / 64)
+  uint64_t lds_bytes;    // dynamic LDS per workgroup
+  uint64_t sign_words;   // nz * ny * wpr
+  uint64_t n_rows;       // cell rows: (nz - 1) * (ny - 1)
+};
+// Dynamic LDS bound: with the static tables and owner slots (29 KB) it
+// keeps a workgroup within 64 KB (the 257^3 lattice's tiles need 31 KB).
+constexpr uint64_t kFusedMaxLds = 34 * 1024;
+
+// Slots of FusedBuffers::ctl.
+enum : int {
+  kFusedTicket = 0,     // two ticket counters: extraction e draws from kFusedTicket + (e & 1)
+  kFusedVertices = 2,
+  kFusedTriangles = 3,
+  kFusedErrors = 4,     // epoch << 32 | bit 0: u32 id overflow, bit 1: a wait gave up
+  kFusedCtlSlots = 8,
+};
+
+struct FusedBuffers {
+  uint64_t* signs;               // sign_words: row-aligned sign bits (x fastest)
+  unsigned long long* flags;     // n_tickets: epoch once the ticket's sign rows are published
+  unsigned long long* slots;     // 4 * n_tickets: look-back aggregates, then inclusive prefixes
+  unsigned long long* rowbase;   // n_rows: tagged first vertex id of each cell row
+  unsigned long long* ctl;       // kFusedCtlSlots
+  float* vertices;
+  uint32_t* indices;
+  uint64_t vcap, icap;
+  // When set (host-coherent memory, kTotSlots words): the last tile writes
+  // the vertex and triangle totals there (kTotVertices, kTotTriangles); 1
+  // goes to kTotErrors on an id overflow and to kTotErrors + 1 when a wait
+  // between tiles gave up (the caller zeroes both before the launch).
+  unsigned long long* host_totals;
+};
+
+// False when the lattice does not fit the tiling (no cells, a row wider
+// than 65,536 cells, or more LDS than kFusedMaxLds).
+bool fused_plan(const Lattice& L, FusedPlan* P);
+// One extraction; `epoch` must differ from the previous extraction's on
+// the same buffers (scratch starts zeroed, so the first epoch is nonzero).
+hipError_t launch_mc_fused(const float* grid, const Lattice& L, const FusedPlan& P,
+                           const FusedBuffers& B, uint32_t epoch, hipStream_t s);
 
 }  // namespace iso

@@ -971,6 +971,14 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop) {
   t_ev_stop = stop;
 }
 bool launch_events_pending() { return t_ev_start != nullptr; }
+bool take_launch_events(hipEvent_t* start, hipEvent_t* stop) {
+  if (!t_ev_start) return false;
+  *start = t_ev_start;
+  *stop = t_ev_stop;
+  t_ev_start = nullptr;
+  t_ev_stop = nullptr;
+  return true;
+}
 void clear_launch_events() {
   t_ev_start = nullptr;
   t_ev_stop = nullptr;

@@ -243,15 +243,17 @@ class Context:
         return d
 
     def extract_mc(self, desc: A.IsoGridDesc, vertex_capacity: int = 0,
-                   index_capacity: int = 0, directed: bool = False) -> Mesh:
+                   index_capacity: int = 0, directed: bool = False, flags: int = 0) -> Mesh:
         """iso_extract_mc (iso_extract_mc_directed with directed=True) into
-        host arrays (grown on ISO_ERR_CAPACITY)."""
+        host arrays (grown on ISO_ERR_CAPACITY); `flags` are ISO_MESH_*
+        (A.ISO_MESH_SINGLE_PASS: one launch over a grid source instead of
+        the classify / count / emit launches)."""
         vcap = max(int(vertex_capacity), 1)
         icap = max(int(index_capacity), 3)
         verts = np.empty((vcap, 3), dtype=np.float32)
         idx = np.empty(icap, dtype=np.uint32)
         out = A.IsoMeshOut(verts.ctypes.data, idx.ctypes.data, vcap, icap, 0, 0,
-                           A.ISO_MEMORY_HOST, 0)
+                           A.ISO_MEMORY_HOST, flags)
         fn = self.lib.iso_extract_mc_directed if directed else self.lib.iso_extract_mc
         st = fn(self.handle, C.byref(desc), C.byref(out))
         if st == A.ISO_ERR_CAPACITY:
