@@ -202,6 +202,20 @@ typedef struct iso_mesh_out {
  * Vertex order, positions and index lists are the reference's. */
 int iso_extract_mc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
 
+/* iso_extract_mc of a bound grid (ISO_SOURCE_GRID) into device memory,
+ * by the single pass, that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Its scratch is one static device array
+ * per GPU (32.5 MiB, held until the process ends), so the lattice is
+ * limited to 28 MiB of sign words (about 513^3 samples), 65,536 tiles and
+ * 262,144 cell rows; larger ones give ISO_ERR_INVALID_ARGUMENT.  Calls on
+ * one GPU share the array: consecutive calls are independent, calls that
+ * overlap in time on one GPU (other threads, other streams) race.  Same
+ * output, counts and ISO_ERR_CAPACITY rule as iso_extract_mc with device
+ * outputs; out->flags is ignored. */
+int iso_extract_mc_noalloc(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_out* out);
+
 /* MarchingCubes::<Directed>::extract with IndexedVertices
  * (marching_cubes.rs:59-82 over distance.rs:72-103): a lattice point is
  * inside when no component of its directed distance is positive, and each

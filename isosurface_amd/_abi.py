@@ -128,6 +128,7 @@ SIGNATURES = [
     ("iso_grid_bind_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64]),
     ("iso_grid_desc_reference", C.c_int, [C.c_int64, C.c_int32, C.POINTER(IsoGridDesc)]),
     ("iso_extract_mc", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.POINTER(IsoMeshOut)]),
+    ("iso_extract_mc_noalloc", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.POINTER(IsoMeshOut)]),
     ("iso_extract_mc_directed", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc),
                                           C.POINTER(IsoMeshOut)]),
     ("iso_extract_emc", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float,

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -237,6 +238,53 @@ int check_lattice(iso_ctx* c, const iso_grid_desc* d, iso::Lattice* L) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of the extraction that allocates nothing (iso_extract_mc_noalloc).
+// The single pass's scratch words live in one file-scope device array in
+// fixed regions, each only ever holding its own kind of word, so a word of
+// one kind is never read as another; every word but the sign rows carries
+// the extraction's epoch, and the epochs of one device only grow, so the
+// array is never cleared between calls (only when the epoch's 22 tag bits
+// wrap).  Each GPU holds its own copy of the array; its device address is
+// looked up once per device, at context creation, and used from then on.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNaSignBytes = size_t(28) << 20;  // row-aligned sign words
+constexpr uint64_t kNaMaxTickets = uint64_t(1) << 16;
+constexpr uint64_t kNaMaxRows = uint64_t(1) << 18;
+constexpr size_t kNaFlagsOff = kNaSignBytes;
+constexpr size_t kNaSlotsOff = kNaFlagsOff + kNaMaxTickets * 8;
+constexpr size_t kNaRowbaseOff = kNaSlotsOff + 4 * kNaMaxTickets * 8;
+constexpr size_t kNaCtlOff = kNaRowbaseOff + kNaMaxRows * 8;
+constexpr size_t kNaBytes = kNaCtlOff + 256;
+constexpr int kNaMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+struct NoAllocDevice {
+  unsigned char* ptr = nullptr;  // this device's copy of `scratch`
+  uint32_t epoch = 0;
+  bool looked_up = false;
+  hipError_t status = hipSuccess;
+};
+std::mutex g_na_mutex;
+NoAllocDevice g_na[kNaMaxDevices];
+
+// With `device` current: the address of its copy of the array, once.
+void noalloc_setup(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return;
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  NoAllocDevice& d = g_na[device];
+  if (d.looked_up) return;
+  void* p = nullptr;
+  d.status = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  d.ptr = d.status == hipSuccess ? static_cast<unsigned char*>(p) : nullptr;
+  d.looked_up = true;
+}
+}  // namespace
+
 extern "C" {
 
 int iso_abi_version(void) { return ISO_ABI_VERSION; }
@@ -266,6 +314,7 @@ int iso_ctx_create(int device, void* stream, iso_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return ISO_ERR_DEVICE;
   iso_ctx* c = new iso_ctx();
   c->device = device;
+  noalloc_setup(device);
   if (stream) {
     c->stream = static_cast<hipStream_t>(stream);
   } else {
@@ -928,6 +977,70 @@ int iso_extract_mc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
   c->kept = false;
   ISO_HIP(c, hipSetDevice(c->device));
   return extract_lattice(c, L, from_grid ? c->grid : nullptr, nullptr, out);
+}
+
+int iso_extract_mc_noalloc(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
+  if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
+  iso::Lattice L;
+  int st = check_lattice(c, desc, &L);
+  if (st != ISO_OK) return st;
+  if (desc->source != ISO_SOURCE_GRID)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "the no-allocation extraction takes a grid source");
+  if (!c->grid) return fail(c, ISO_ERR_NO_SOURCE, "no grid bound");
+  if (c->gnx != L.nx || c->gny != L.ny || c->gnz != L.nz)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice dims differ from the bound grid");
+  if (out->memory != ISO_MEMORY_DEVICE)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "the no-allocation extraction writes device memory");
+  iso::FusedPlan P;
+  if (!iso::fused_plan(L, &P))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "lattice rows are too wide for the single pass");
+  if (P.sign_words * sizeof(uint64_t) > kNaSignBytes ||
+      static_cast<uint64_t>(P.n_tickets) > kNaMaxTickets || P.n_rows > kNaMaxRows)
+    return fail(c, ISO_ERR_INVALID_ARGUMENT,
+                "lattice exceeds the no-allocation scratch (28 MiB of sign words, about 513^3 "
+                "samples; 65,536 tiles; 262,144 cell rows)");
+  if (c->device < 0 || c->device >= kNaMaxDevices)
+    return fail(c, ISO_ERR_DEVICE, "device index beyond the no-allocation table");
+  ISO_HIP(c, hipSetDevice(c->device));
+  unsigned char* base = nullptr;
+  uint32_t epoch = 0;
+  bool wrapped = false;
+  {
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    NoAllocDevice& d = g_na[c->device];
+    if (!d.looked_up || !d.ptr)
+      return fail(c, ISO_ERR_DEVICE,
+                  std::string("no-allocation scratch address: ") + hipGetErrorString(d.status));
+    base = d.ptr;
+    d.epoch += 1;
+    if ((d.epoch & 0x3FFFFFu) == 0u) {  // tags keep 22 bits of the epoch
+      d.epoch = 1;
+      wrapped = true;
+    }
+    epoch = d.epoch;
+  }
+  if (wrapped)  // clear every tagged word (a fill kernel: no memset node, no allocation)
+    ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(base + kNaFlagsOff),
+                                    (kNaBytes - kNaFlagsOff) / 4, 0u, c->stream));
+  iso::FusedBuffers B{};
+  B.signs = reinterpret_cast<uint64_t*>(base);
+  B.flags = reinterpret_cast<unsigned long long*>(base + kNaFlagsOff);
+  B.slots = reinterpret_cast<unsigned long long*>(base + kNaSlotsOff);
+  B.rowbase = reinterpret_cast<unsigned long long*>(base + kNaRowbaseOff);
+  B.ctl = reinterpret_cast<unsigned long long*>(base + kNaCtlOff);
+  B.vertices = out->vertices;
+  B.indices = out->indices;
+  B.vcap = out->vertices ? out->vertex_capacity : 0;
+  B.icap = out->indices ? out->index_capacity : 0;
+  c->kept = false;
+  c->band_valid = false;
+  uint64_t nv = 0, nt = 0;
+  if ((st = fused_launch_and_read(c, L, c->grid, P, B, epoch, false, &nv, &nt)) != ISO_OK)
+    return st;
+  out->vertex_count = nv;
+  out->index_count = 3 * nt;
+  if (nv > B.vcap || 3 * nt > B.icap) return fail(c, ISO_ERR_CAPACITY, "output buffers too small");
+  return ISO_OK;
 }
 
 int iso_extract_mc_directed(iso_ctx* c, const iso_grid_desc* desc, iso_mesh_out* out) {
