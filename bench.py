@@ -20,6 +20,10 @@ Printed JSON line:
                 dispatch, so they span the kernel alone), against MI355X's 8 TB/s HBM
                 peak; traffic from the committed counter summary
   kernels       each kernel's own algorithmic bytes, duration and fraction
+  device_copy   a 1 GiB device-to-device copy: the measured bandwidth the
+                stage's fraction is also quoted against
+  large_grid    MarchingCubes::new(513) over a 541 MB resident grid (above the
+                256 MB Infinity Cache) by both paths, meshes compared
   program_path  the same extraction sampling the CSG program on the GPU
                 instead of reading the grid (VALU-bound, reported aside)
   point_cloud, vertex_normals
@@ -408,6 +412,84 @@ def brick_centres(size: int) -> np.ndarray:
     return np.ascontiguousarray(np.stack([x, y, z], -1).reshape(-1, 3), np.float32)
 
 
+def copy_peak(ctx: Context, mib: int = 1024, reps: int = 10) -> dict:
+    """Device-to-device copy bandwidth on this GPU (read + write bytes per
+    second), the measured ceiling the stage's fraction is also quoted
+    against."""
+    nb = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, nb, C.byref(a)), "alloc")
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, nb, C.byref(b)), "alloc")
+    ctx.check(ctx.lib.iso_memcpy_device(ctx.handle, b, a, nb), "copy")
+    times = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.check(ctx.lib.iso_memcpy_device(ctx.handle, b, a, nb), "copy")
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    ctx.lib.iso_device_free(ctx.handle, a)
+    ctx.lib.iso_device_free(ctx.handle, b)
+    best = min(times)
+    gbs = 2 * nb / best / 1e9
+    log(f"device copy: {mib} MiB in {1e3 * best:.3f} ms, {gbs:.0f} GB/s (read + write)")
+    return {"bytes": nb, "ms": 1e3 * best, "GBps": gbs,
+            "note": "hipMemcpy device to device, best of runs, read + write bytes"}
+
+
+def large_grid(ctx: Context, size: int, steps: int, copy_gbs: float) -> dict:
+    """MarchingCubes::new(size) over a resident grid larger than the 256 MB
+    Infinity Cache (size 513: 513 x 513 x 514 FP32 samples, 541 MB), by both
+    paths; the two meshes must be identical."""
+    src, _ = W.spheres_minus_torus()
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(size, A.ISO_SOURCE_GRID)
+    n = desc.nx * desc.ny * desc.nz
+    cells = (desc.nx - 1) * (desc.ny - 1) * (desc.nz - 1)
+    dgrid = C.c_void_p()
+    ctx.check(ctx.lib.iso_device_alloc(ctx.handle, 4 * n, C.byref(dgrid)), "alloc grid")
+    ctx.check(ctx.lib.iso_program_sample_lattice(ctx.handle, C.byref(desc), dgrid,
+                                                 A.ISO_MEMORY_DEVICE), "sample_lattice")
+    ctx.check(ctx.lib.iso_grid_bind_device(ctx.handle, dgrid, desc.nx, desc.ny, desc.nz), "bind")
+    probe = DeviceMesh(ctx, 1, 3)
+    out = probe.out()
+    st = ctx.lib.iso_extract_mc(ctx.handle, C.byref(desc), C.byref(out))
+    if st not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+        ctx.check(st, "iso_extract_mc")
+    nv, ni = int(out.vertex_count), int(out.index_count)
+    probe.free()
+    mesh = DeviceMesh(ctx, max(nv, 1), max(ni, 3))
+    single = A.ISO_MESH_SINGLE_PASS
+    res = {"workload": (f"MarchingCubes::new({size}) over a resident FP32 "
+                        f"{desc.nx}x{desc.ny}x{desc.nz} grid ({4 * n / 1e6:.0f} MB, above the "
+                        "256 MB Infinity Cache) of the union of 64 seeded spheres minus a torus"),
+           "cells": cells, "samples": n, "vertices": nv, "triangles": ni // 3}
+    digests = {}
+    stage_bytes = 4 * n + 12 * nv + 4 * ni
+    for name, flags, stage in (("three_pass", 0, ["mc_signs_grid", "mc_count", "mc_emit"]),
+                               ("single_pass", single, ["mc_fused"])):
+        o = extract(ctx, desc, mesh, flags=flags)
+        digests[name] = digest(ctx, mesh, o)
+        for _ in range(2):
+            extract(ctx, desc, mesh, flags=flags)
+        el = timed_steps(ctx, desc, mesh, steps, flags=flags)
+        ks = kernel_pass(ctx, desc, mesh, steps, flags=flags)
+        us = sum(ks[k]["mean_us"] for k in stage)
+        achieved = stage_bytes / (us * 1e-6) / 1e9
+        res[name] = {"ms_per_step": 1e3 * el / steps, "cells_per_s": cells * steps / el,
+                     "kernels_us": {k: round(v["mean_us"], 2) for k, v in ks.items()},
+                     "stage_us": us, "achieved_GBps": achieved,
+                     "frac_of_8TBps": achieved / HBM_PEAK_GBS,
+                     "frac_of_measured_copy": achieved / copy_gbs if copy_gbs else None}
+        log(f"{size}^3 {name}: {1e3 * el / steps:.3f} ms/step, stage {us:.1f} us, "
+            f"{achieved:.0f} GB/s = {achieved / HBM_PEAK_GBS:.3f} of 8 TB/s")
+    res["algorithmic_bytes"] = stage_bytes
+    res["paths_identical"] = digests["three_pass"] == digests["single_pass"]
+    mesh.free()
+    ctx.lib.iso_device_free(ctx.handle, dgrid)
+    return res
+
+
 def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
     """The learned narrow band against the dense path at size^3 cells."""
     from isosurface_amd.band import NarrowBand, brick_half_diagonal, training_samples
@@ -525,6 +607,8 @@ def main() -> int:
                     help="training steps of the learned-band network")
     ap.add_argument("--band-size", type=int, default=1024,
                     help="cells per axis of the learned-band workload (0: skip it)")
+    ap.add_argument("--large-size", type=int, default=513,
+                    help="size of the above-Infinity-Cache grid workload (0: skip it)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a few grid extractions and three 64 MiB device copies (the "
                          "scale reference), for hardware-counter passes")
@@ -658,6 +742,7 @@ def main() -> int:
     log(f"kernels: three-pass {json.dumps(three_kernels)} single pass "
         f"{json.dumps(single_kernels)}")
 
+    copy = copy_peak(ctx)
     # Program path: the CSG sampled by the extraction itself (no grid).
     pdesc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
     extract(ctx, pdesc, mesh)
@@ -717,12 +802,20 @@ def main() -> int:
             "mean_us": stage_us,
             "three_pass_mean_us": three_us,
             "single_pass_mean_us": single_us,
+            "measured_copy_GBps": copy["GBps"],
+            "frac_of_measured_copy": achieved / copy["GBps"],
         },
+        "device_copy": copy,
         "kernels": per_kernel,
         "traffic_per_kernel": traffic.get("kernels") if traffic else None,
         "program_path": program_path,
         **widened,
     }
+
+    if args.large_size > 0:
+        result["large_grid"] = large_grid(ctx, args.large_size, max(5, args.steps // 5),
+                                          copy["GBps"])
+        ctx.upload_program(src.program())
 
     if not args.no_side:
         if args.band_size > 0:
