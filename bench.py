@@ -148,6 +148,82 @@ def measured_traffic():
     return out if out["kernels"] else None
 
 
+# VALU issue peak: 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 FP32 VALU
+# instruction per SIMD every 2 clocks (32 lanes a clock: the rate behind the
+# 157.3 TFLOP/s FP32 vector figure).
+VALU_PEAK_GINSTR = 256 * 4 * 2.4e9 / 2 / 1e9
+FP32_PEAK_TFLOPS = 157.3
+
+
+def valu_counters():
+    """VALU wave-instructions and FP32 FLOPs per launch of the VALU-bound
+    kernels, from the committed SQ counter passes over `bench.py
+    --valu-probe`: {kernel: {"instr": n, "flop": n}}, or None."""
+    path = ROOT / "profiles" / "valu_counters.md"
+    if not path.exists():
+        return None
+    out = {}
+    for line in path.read_text().splitlines():
+        parts = line.strip().split()
+        if len(parts) != 2:
+            continue
+        for key, field in (("valu_instr.", "instr"), ("valu_f32_flop.", "flop")):
+            if parts[0].startswith(key):
+                out.setdefault(parts[0][len(key):].rstrip(":"), {})[field] = int(parts[1])
+    return out or None
+
+
+def valu_roofline(durations_us: dict) -> dict:
+    """Achieved VALU issue rate of each kernel: counted instructions per
+    launch over its live mean duration."""
+    counts = valu_counters()
+    if not counts:
+        return None
+    res = {}
+    for k, us in durations_us.items():
+        if k in counts and us and "instr" in counts[k]:
+            rate = counts[k]["instr"] / (us * 1e-6) / 1e9
+            flops = counts[k].get("flop", 0) / (us * 1e-6) / 1e12
+            res[k] = {"bound": "valu", "valu_wave_instr": counts[k]["instr"], "mean_us": us,
+                      "achieved": rate, "peak": VALU_PEAK_GINSTR, "unit": "G wave-instr/s",
+                      "frac": rate / VALU_PEAK_GINSTR,
+                      "f32_tflops": flops, "f32_frac": flops / FP32_PEAK_TFLOPS}
+    return res
+
+
+def valu_probe(ctx: Context, size: int) -> None:
+    """A few launches of each VALU-bound kernel, for SQ counter passes: the
+    program-path signs at size^3, DualContouring::new(128) over
+    CentralDifference on the 32-box union, and the certificate-only narrow
+    band at 1025^3 on the smooth spheres (the bench's configurations)."""
+    src, _ = W.spheres_minus_torus()
+    ctx.upload_program(src.program())
+    pdesc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
+    mesh = DeviceMesh(ctx, 1 << 20, 3 << 21)
+    for _ in range(3):
+        ctx.check(ctx.lib.iso_extract_mc(ctx.handle, C.byref(pdesc), C.byref(mesh.out())), "p")
+    mesh.free()
+    boxes = W.boxes_union(32, seed=4)
+    ctx.upload_program(boxes.program())
+    ddesc = ctx.reference_desc(128, A.ISO_SOURCE_PROGRAM)
+    mesh = DeviceMesh(ctx, 1 << 20, 3 << 21)
+    for _ in range(3):
+        ctx.check(ctx.lib.iso_extract_dc(ctx.handle, C.byref(ddesc), C.c_float(1e-6),
+                                         C.byref(mesh.out())), "dc")
+    mesh.free()
+    smooth = W.smooth_spheres(1024, k=0.02, seed=5)
+    ctx.upload_program(smooth.program())
+    bdesc = ctx.reference_desc(1025, A.ISO_SOURCE_PROGRAM)
+    mesh = DeviceMesh(ctx, 1 << 23, 3 << 24)
+    for _ in range(2):
+        st = ctx.lib.iso_extract_mc_band(ctx.handle, C.byref(bdesc), C.c_float(0.0),
+                                         A.ISO_BAND_NO_NETWORK, C.byref(mesh.out()), None)
+        if st not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+            ctx.check(st, "band")
+    mesh.free()
+    ctx.synchronize()
+
+
 def digest(ctx, mesh: DeviceMesh, out) -> tuple:
     nv, ni = int(out.vertex_count), int(out.index_count)
     v = np.empty((nv, 3), np.float32)
@@ -256,6 +332,15 @@ def side_workloads(ctx: Context, steps: int, with_cpu: bool) -> dict:
     res["dual_contouring"]["workload"] = (
         "DualContouring::new(128, MinimiseQEF) on CentralDifference(union of 32 seeded boxes), "
         "128x128x129 primal samples, f64 QEF per mixed cell")
+    res["dual_contouring_hermite"] = run(
+        "iso_extract_dc (analytic normals)",
+        lambda out: ctx.lib.iso_extract_dc(ctx.handle, C.byref(dc_desc), C.c_float(0.0),
+                                           C.byref(out)),
+        "dc_hermite_boxes32_size128", 127 * 127 * 128,
+        lambda: from_oracle.extract_dc(src.program(), from_oracle.reference_desc(128), 0.0))
+    res["dual_contouring_hermite"]["workload"] = (
+        "DualContouring::new(128, MinimiseQEF) on the union of 32 seeded boxes with their own "
+        "(HermiteSource) normals, 128x128x129 primal samples")
     res["octree_mc"] = run(
         "iso_extract_lhmc",
         lambda out: ctx.lib.iso_extract_lhmc(ctx.handle, 7, C.byref(out)),
@@ -490,12 +575,14 @@ def large_grid(ctx: Context, size: int, steps: int, copy_gbs: float) -> dict:
     return res
 
 
-def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
-    """The learned narrow band against the dense path at size^3 cells."""
+def learned_band(ctx: Context, size: int, reps: int, steps: int, seeds=(13, 14, 15)) -> dict:
+    """The learned narrow band against the dense path at size^3 cells, with
+    4x4x4 and 8x8x8-cell bricks; the network is trained once per seed."""
     from isosurface_amd.band import NarrowBand, brick_half_diagonal, training_samples
     src = W.smooth_spheres(1024, k=0.02, seed=5)
     upper = (1.0, 1.0, (size + 1) / size)  # the lattice's extra z layer
-    hd = brick_half_diagonal(size + 1)
+    bricks = {"4": 0, "8": A.ISO_BAND_BRICK8}
+    hd = {b: brick_half_diagonal(size + 1, f) for b, f in bricks.items()}
     t0 = time.perf_counter()
     pts, sdf = training_samples(ctx, src, 1_000_000, 1_000_000, seed=11, coarse=257, upper=upper)
     cal_p, cal_s = training_samples(ctx, src, 500_000, 500_000, seed=12, coarse=257, upper=upper)
@@ -504,15 +591,8 @@ def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
     centres = brick_centres(size + 1)
     # The same network before training: its calibrated margin keeps every brick.
     band.init(7)
-    m0 = band.calibrate(cal_p, cal_s, hd)
-    untrained_fraction = float(np.mean(~(np.abs(band.predict(centres)) > hd + m0)))
-    overshoot, lr = 8.0, 2e-3
-    t0 = time.perf_counter()
-    weight = (1.0 + 9.0 * np.exp(-np.abs(sdf) / 0.01)).astype(np.float32)
-    loss = band.train(pts, sdf, weight, steps=steps, batch=16384, lr=lr, lr_min=5e-5,
-                      clamp=0.05, seed=13, overshoot=overshoot)
-    t_train = time.perf_counter() - t0
-    margin = band.calibrate(cal_p, cal_s, hd)
+    m0 = band.calibrate(cal_p, cal_s, hd["4"])
+    untrained_fraction = float(np.mean(~(np.abs(band.predict(centres)) > hd["4"] + m0)))
     ctx.upload_program(src.program())
     desc = ctx.reference_desc(size + 1, A.ISO_SOURCE_PROGRAM)
     cells = size * size * (size + 1)
@@ -536,7 +616,7 @@ def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
     t_dense = min(times)
     dd = digest(ctx, mesh, out)
 
-    def run(flags: int, mode: str) -> dict:
+    def run(flags: int, mode: str, margin: float) -> dict:
         stats = A.IsoBandStats()
         times = []
         for _ in range(max(1, reps)):
@@ -556,39 +636,77 @@ def learned_band(ctx: Context, size: int, reps: int, steps: int) -> dict:
         missed = C.c_uint64(0)
         ctx.check(ctx.lib.iso_band_missed_cells(ctx.handle, C.byref(desc), C.byref(missed)),
                   "missed")
-        r = {"band_ms": 1e3 * t_band, "speedup": t_dense / t_band,
-             "band_cells_per_s": cells / t_band, "band_bricks": int(stats.band_bricks),
+        # one more run with per-kernel events
+        ctx.reset_kernel_stats()
+        ctx.set_timing(True)
+        st2 = ctx.lib.iso_extract_mc_band(ctx.handle, C.byref(desc), margin, flags,
+                                          C.byref(mesh.out()), None)
+        ctx.synchronize()
+        ctx.set_timing(False)
+        if st2 not in (A.ISO_OK, A.ISO_ERR_CAPACITY):
+            ctx.check(st2, "band")
+        kus = {k: round(1e3 * ms / max(n, 1), 2) for k, (n, ms) in ctx.kernel_stats().items()}
+        ctx.reset_kernel_stats()
+        r = {"band_ms": 1e3 * t_band, "kernels_us": kus, "speedup": t_dense / t_band,
+             "band_cells_per_s": cells / t_band, "bricks": int(stats.bricks),
+             "band_bricks": int(stats.band_bricks),
              "band_fraction": int(stats.band_bricks) / max(int(stats.bricks), 1),
              "exact_samples": int(stats.exact_samples),
              "centre_samples": int(stats.centre_samples),
              "overturned_culls": int(stats.overturned_culls),
-             "missed_cells": int(missed.value), "identical_to_dense": same}
+             "missed_cells": int(missed.value), "identical_to_dense": same, "margin": margin}
         log(f"band [{mode}]: {1e3 * t_band:.1f} ms, x{t_dense / t_band:.2f}, band "
             f"{r['band_fraction']:.3f}, missed {missed.value}, identical {same}")
         return r
 
-    learned = run(0, "learned")
-    certified = run(A.ISO_BAND_CERTIFY, "learned+certificate")
-    exact_only = run(A.ISO_BAND_NO_NETWORK, "certificate only")
+    overshoot, lr = 8.0, 2e-3
+    weight = (1.0 + 9.0 * np.exp(-np.abs(sdf) / 0.01)).astype(np.float32)
+    per_seed, training = [], []
+    first = {}
+    for i, seed in enumerate(seeds):
+        band.init(7)
+        t0 = time.perf_counter()
+        loss = band.train(pts, sdf, weight, steps=steps, batch=16384, lr=lr, lr_min=5e-5,
+                          clamp=0.05, seed=seed, overshoot=overshoot)
+        training.append({"seed": seed, "final_loss": loss,
+                         "train_ms": 1e3 * (time.perf_counter() - t0)})
+        margins = {b: band.calibrate(cal_p, cal_s, hd[b]) for b in bricks}
+        row = {"seed": seed}
+        for b, f in bricks.items():
+            row[f"brick{b}"] = run(f, f"learned, {b}^3 bricks, seed {seed}", margins[b])
+            if i == 0:
+                first[b] = {
+                    "learned": row[f"brick{b}"],
+                    "learned_with_certificate": run(
+                        f | A.ISO_BAND_CERTIFY, f"learned+certificate, {b}^3 bricks", margins[b]),
+                    "certificate_without_network": run(
+                        f | A.ISO_BAND_NO_NETWORK, f"certificate only, {b}^3 bricks", margins[b]),
+                }
+        per_seed.append(row)
     mesh.free()
     res = {
         "workload": (f"MarchingCubes::new({size + 1}) ({cells} cells) on the smooth-min (k=0.02) "
                      "of 1024 seeded spheres, radii U[0.01,0.05], program source"),
         "dense_ms": 1e3 * t_dense, "dense_cells_per_s": cells / t_dense,
-        "bricks": int(centres.shape[0]), "margin": margin, "half_diagonal": hd,
+        "bricks": int(centres.shape[0]), "half_diagonal": hd["4"],
         "vertices": nv, "triangles": ni // 3,
-        **learned,
+        **first["4"]["learned"],
         "untrained_network_band_fraction": untrained_fraction,
-        "learned_with_certificate": certified,
-        "certificate_without_network": exact_only,
+        "learned_with_certificate": first["4"]["learned_with_certificate"],
+        "certificate_without_network": first["4"]["certificate_without_network"],
+        "brick8": {"half_diagonal": hd["8"], **first["8"]},
+        "learned_only_over_seeds": per_seed,
         "training": {"samples": len(sdf), "steps": steps, "batch": 16384, "lr": lr,
-                     "overshoot": overshoot, "final_loss": loss, "train_ms": 1e3 * t_train,
+                     "overshoot": overshoot, "runs": training,
                      "sample_generation_ms": 1e3 * t_samples},
     }
-    log(f"learned band: dense {1e3 * t_dense:.1f} ms; learned x{learned['speedup']:.2f} "
-        f"(band {learned['band_fraction']:.3f}, untrained {untrained_fraction:.3f}); with "
-        f"certificate x{certified['speedup']:.2f}; certificate alone "
-        f"x{exact_only['speedup']:.2f}")
+    for b in bricks:
+        sp = [r[f"brick{b}"]["speedup"] for r in per_seed]
+        ms = [r[f"brick{b}"]["missed_cells"] for r in per_seed]
+        log(f"learned band, {b}^3 bricks: dense {1e3 * t_dense:.1f} ms; learned x"
+            f"{', x'.join(f'{v:.2f}' for v in sp)} over seeds (missed {ms}); with certificate x"
+            f"{first[b]['learned_with_certificate']['speedup']:.2f}; certificate alone x"
+            f"{first[b]['certificate_without_network']['speedup']:.2f}")
     return res
 
 
@@ -609,6 +727,9 @@ def main() -> int:
                     help="cells per axis of the learned-band workload (0: skip it)")
     ap.add_argument("--large-size", type=int, default=513,
                     help="size of the above-Infinity-Cache grid workload (0: skip it)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only a few launches of the VALU-bound kernels (program-path "
+                         "signs, DC features, band exact samples), for SQ counter passes")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a few grid extractions and three 64 MiB device copies (the "
                          "scale reference), for hardware-counter passes")
@@ -620,6 +741,11 @@ def main() -> int:
 
     size = args.size
     ctx = Context(0)
+    if args.valu_probe:
+        valu_probe(ctx, size)
+        print(json.dumps({"valu_probe": True, "size": size}), flush=True)
+        ctx.close()
+        return 0
     src, _ = W.spheres_minus_torus()
     ctx.upload_program(src.program())
     desc = ctx.reference_desc(size, A.ISO_SOURCE_GRID)
@@ -822,6 +948,16 @@ def main() -> int:
             result["learned_band"] = learned_band(ctx, args.band_size, 2, args.band_steps)
         result["batched_chunks"] = batched_chunks(ctx, max(2, args.program_steps))
         result.update(side_workloads(ctx, max(1, args.program_steps), not args.no_cpu_baseline))
+
+    # VALU-bound kernels against the VALU issue peak: counted instructions
+    # per launch (committed SQ counter pass) over this run's durations.
+    durations = {"mc_signs_program": program_path["kernels_us"].get("mc_signs_program")}
+    if "dual_contouring" in result:
+        durations["dc_feature"] = result["dual_contouring"]["kernels_us"].get("dc_feature")
+    if "learned_band" in result:
+        durations["band_exact"] = result["learned_band"]["certificate_without_network"][
+            "kernels_us"].get("band_exact")
+    result["valu_roofline"] = valu_roofline(durations)
 
     if not args.no_cpu_baseline:
         host_grid = np.empty((desc.nz, desc.ny, desc.nx), np.float32)

@@ -255,7 +255,11 @@ int iso_extract_point_cloud(iso_ctx* ctx, const iso_grid_desc* desc, iso_mesh_ou
  * primal lattice and must name ISO_SOURCE_PROGRAM; the mesh is the
  * reference's: vertex order, positions (feature points from the reference's
  * f64 SVD) and index lists.  ISO_ERR_DEVICE if a cell's SVD fails to
- * converge within a fixed bound of QR sweeps (the reference loops on). */
+ * converge within a fixed bound of QR sweeps (the reference loops on).
+ * epsilon 0: the program's own HermiteSource normals instead (the reference's
+ * DualContouring over a source that implements HermiteSource itself,
+ * dual_contouring.rs:65-69, source.rs:43-48): shapes, unions and
+ * translations only, else ISO_ERR_PROGRAM. */
 int iso_extract_dc(iso_ctx* ctx, const iso_grid_desc* desc, float epsilon, iso_mesh_out* out);
 
 /* Marching cubes over a linear-hashed octree of the current program:
@@ -315,7 +319,7 @@ int iso_extract_batch(iso_ctx* ctx, iso_batch_out* out);
  * pi 2^(k/2) p for k < 16; three hidden layers of 64 ReLU units; bf16 MFMA
  * with f32 accumulation) is trained on (point, SDF) samples to predict the
  * truncated SDF.  iso_extract_mc_band predicts it at the centre of every
- * 4x4x4-cell brick, evaluates the program exactly only in bricks whose
+ * brick (4x4x4 cells, or 8x8x8 with ISO_BAND_BRICK8), evaluates the program exactly only in bricks whose
  * prediction lies within their half-diagonal plus `margin` of zero, gives
  * every other sample its brick's predicted sign, and extracts as
  * iso_extract_mc does.  The result equals iso_extract_mc's whenever no
@@ -355,12 +359,15 @@ typedef struct iso_band_stats {
  * the box's sign (every program is 1-Lipschitz); network culls the proofs
  * overturn are counted.  No surface cell is missed.  ISO_BAND_NO_NETWORK:
  * the same proofs without a network (the method without it, for
- * comparison). */
+ * comparison).  ISO_BAND_BRICK8: bricks of 8x8x8 cells instead of 4x4x4
+ * (an eighth of the predictions and proofs, a coarser band). */
 #define ISO_BAND_CERTIFY 1u
 #define ISO_BAND_NO_NETWORK 2u
+#define ISO_BAND_BRICK8 4u
 
-/* Cells per edge of a narrow-band brick (the unit the band culls). */
-int iso_band_brick_size(void);
+/* Cells per edge of a narrow-band brick (the unit the band culls) under
+ * iso_extract_mc_band's `flags`: 4, or 8 with ISO_BAND_BRICK8. */
+int iso_band_brick_size(uint32_t flags);
 
 /* Deterministic He-uniform initialisation of the context's network. */
 int iso_band_init(iso_ctx* ctx, uint32_t seed);

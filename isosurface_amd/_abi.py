@@ -89,6 +89,7 @@ ISO_MESH_SINGLE_PASS = 2
 ISO_BAND_PARAMS = 15617
 ISO_BAND_CERTIFY = 1
 ISO_BAND_NO_NETWORK = 2
+ISO_BAND_BRICK8 = 4
 
 
 class IsoBandTrainParams(C.Structure):
@@ -150,7 +151,7 @@ SIGNATURES = [
                                  C.POINTER(C.c_float)]),
     ("iso_band_gradient", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                     C.c_float, C.c_float, C.c_void_p, C.POINTER(C.c_float)]),
-    ("iso_band_brick_size", C.c_int, []),
+    ("iso_band_brick_size", C.c_int, [C.c_uint32]),
     ("iso_band_predict", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32]),
     ("iso_extract_mc_band", C.c_int, [C.c_void_p, C.POINTER(IsoGridDesc), C.c_float, C.c_uint32,
                                       C.POINTER(IsoMeshOut), C.POINTER(IsoBandStats)]),

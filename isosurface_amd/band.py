@@ -110,7 +110,7 @@ class NarrowBand:
         band only when an exact super-brick or brick centre sample proves
         it, whatever the network predicted; stats.overturned_culls counts the
         network culls no proof allowed) or A.ISO_BAND_NO_NETWORK (the same
-        proofs, no network)."""
+        proofs, no network), with A.ISO_BAND_BRICK8 for 8x8x8-cell bricks."""
         vcap = max(int(vertex_capacity), 1)
         icap = max(int(index_capacity), 3)
         verts = np.empty((vcap, 3), np.float32)
@@ -138,14 +138,16 @@ class NarrowBand:
         return int(n.value)
 
 
-def brick_size() -> int:
-    """Cells per edge of a narrow-band brick (iso_band_brick_size)."""
-    return int(A.load().iso_band_brick_size())
+def brick_size(flags: int = 0) -> int:
+    """Cells per edge of a narrow-band brick under iso_extract_mc_band's
+    flags (iso_band_brick_size): 4, or 8 with A.ISO_BAND_BRICK8."""
+    return int(A.load().iso_band_brick_size(flags))
 
 
-def brick_half_diagonal(size: int) -> float:
+def brick_half_diagonal(size: int, flags: int = 0) -> float:
     """Half-diagonal of a band brick of MarchingCubes::new(size)."""
-    return 0.5 * brick_size() * float(np.sqrt(3.0)) * float(np.float32(1.0) / np.float32(size - 1))
+    return (0.5 * brick_size(flags) * float(np.sqrt(3.0)) *
+            float(np.float32(1.0) / np.float32(size - 1)))
 
 
 def training_samples(ctx: Context, source: Source, n_uniform: int, n_surface: int,

@@ -87,7 +87,7 @@ struct iso_ctx {
   unsigned long long* h_stage = nullptr;  // pinned, 8 words
 
   // batched chunks: lowered programs, chunk table, sign words, offsets
-  DevBuf b_ops, b_chunks, b_segstart, b_wordstart, b_signs, b_voff, b_ioff;
+  DevBuf b_ops, b_chunks, b_segstart, b_wordstart, b_signs, b_voff, b_ioff, b_values;
   iso::BatchPlan plan{};
   uint64_t b_cells = 0;
   bool b_ready = false;
@@ -364,7 +364,8 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
                     &c->oc_towner,  &c->oc_tid,     &c->sc_slot,   &c->sc_ctl,
                     &c->b_ops,      &c->b_chunks,   &c->b_segstart, &c->b_wordstart,
-                    &c->b_signs,    &c->b_voff,     &c->b_ioff,    &c->nn_theta, &c->nn_m,
+                    &c->b_signs,    &c->b_voff,     &c->b_ioff,    &c->b_values,
+                    &c->nn_theta,   &c->nn_m,
                     &c->nn_v,       &c->nn_grad,    &c->nn_frags,  &c->nn_loss,  &c->nn_x,
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
@@ -649,6 +650,7 @@ int prepare_scan(iso_ctx* c, uint64_t n, iso::scan::State* st);
 struct BandEmit {
   const uint8_t* flags;
   int64_t nbx, nby;
+  int shift;  // bricks of 2^shift cells per edge
 };
 
 // Grows b to `bytes`; a new allocation is zero-filled (the single-pass
@@ -828,8 +830,8 @@ int extract_lattice(iso_ctx* c, const iso::Lattice& L, const float* grid, const 
   auto emit = [&]() -> hipError_t {
     if (directed) return iso::launch_emit_directed(directed, L, D, B, c->stream);
     if (band && values)
-      return iso::launch_emit_band(values, band->flags, band->nbx, band->nby, ops, c->n_prog, L,
-                                   D, B, c->stream);
+      return iso::launch_emit_band(values, band->flags, band->nbx, band->nby, band->shift, ops,
+                                   c->n_prog, L, D, B, c->stream);
     const float* src = grid ? grid : (band ? nullptr : values);
     return iso::launch_emit(src, ops, c->n_prog, positions, L, D, B, c->stream);
   };
@@ -1296,8 +1298,12 @@ int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mes
   if (desc->source != ISO_SOURCE_PROGRAM)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "dual contouring needs a program source (normals)");
   if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
-  if (!(epsilon > 0.0f) || !(epsilon < 3.0e38f))
-    return fail(c, ISO_ERR_INVALID_ARGUMENT, "epsilon must be positive and finite");
+  if (!(epsilon >= 0.0f) || !(epsilon < 3.0e38f))
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "epsilon must be finite and >= 0");
+  if (epsilon == 0.0f && !c->prog_hermite)
+    return fail(c, ISO_ERR_PROGRAM,
+                "analytic normals need shapes, unions and translations only; pass epsilon > 0 "
+                "for central differences");
   if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
   c->kept = false;
@@ -1616,9 +1622,15 @@ int iso_batch_upload(iso_ctx* c, const iso_node* nodes, uint64_t n_nodes,
   ISO_HIP(c, ensure(&c->b_wordstart, (n_chunks + 1) * sizeof(uint64_t)));
   ISO_HIP(c, ensure(&c->b_voff, (n_chunks + 1) * sizeof(uint64_t)));
   ISO_HIP(c, ensure(&c->b_ioff, (n_chunks + 1) * sizeof(uint64_t)));
+  // The programs' samples, 64 per sign word, for emit to read (as the
+  // single-lattice program path stores them); without the memory, emit
+  // evaluates the programs at the edge endpoints.
+  const bool stored = ensure(&c->b_values, words * 64 * sizeof(float)) == hipSuccess;
+  if (!stored) (void)hipGetLastError();
   std::vector<iso::CellSpace> table(n_chunks);
   const iso::DevOp* dops = static_cast<const iso::DevOp*>(c->b_ops.ptr);
   const uint64_t* dsigns = static_cast<const uint64_t*>(c->b_signs.ptr);
+  const float* dvalues = static_cast<const float*>(c->b_values.ptr);
   for (uint32_t i = 0; i < n_chunks; ++i) {
     const iso_grid_desc& g = chunks[i].grid;
     iso::CellSpace& C = table[i];
@@ -1635,6 +1647,7 @@ int iso_batch_upload(iso_ctx* c, const iso_node* nodes, uint64_t n_nodes,
     C.seg_off = seg_start[i];
     C.ops = dops + op_off[i];
     C.n_ops = static_cast<int32_t>((i + 1 < n_chunks ? op_off[i + 1] : ops.size()) - op_off[i]);
+    C.grid = stored ? dvalues + 64 * word_start[i] : nullptr;
   }
   ISO_HIP(c, hipMemcpyAsync(c->b_ops.ptr, ops.data(), ops.size() * sizeof(iso::DevOp),
                             hipMemcpyHostToDevice, c->stream));
@@ -1651,6 +1664,7 @@ int iso_batch_upload(iso_ctx* c, const iso_node* nodes, uint64_t n_nodes,
   c->plan.n_chunks = n_chunks;
   c->plan.n_words = static_cast<int64_t>(words);
   c->plan.n_segs = static_cast<int64_t>(segs);
+  c->plan.stored_values = stored;
   c->b_cells = cells;
   c->b_ready = true;
   return ISO_OK;
@@ -1965,7 +1979,9 @@ int stage_in(iso_ctx* c, DevBuf* buf, const void* src, size_t bytes, int32_t mem
 
 extern "C" {
 
-int iso_band_brick_size(void) { return iso::kBrick; }
+int iso_band_brick_size(uint32_t flags) {
+  return 1 << ((flags & ISO_BAND_BRICK8) ? iso::kBandBrickShiftMax : iso::kBandBrickShiftDefault);
+}
 
 int iso_band_init(iso_ctx* c, uint32_t seed) {
   if (!c) return ISO_ERR_INVALID_ARGUMENT;
@@ -2146,7 +2162,7 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
   if (desc->source != ISO_SOURCE_PROGRAM)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "band extraction needs a program source");
   if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
-  if (flags & ~(ISO_BAND_CERTIFY | ISO_BAND_NO_NETWORK))
+  if (flags & ~(ISO_BAND_CERTIFY | ISO_BAND_NO_NETWORK | ISO_BAND_BRICK8))
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown band flags");
   const bool use_network = !(flags & ISO_BAND_NO_NETWORK);
   const bool certify = (flags & (ISO_BAND_CERTIFY | ISO_BAND_NO_NETWORK)) != 0;
@@ -2159,7 +2175,8 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
   ISO_HIP(c, hipSetDevice(c->device));
   iso::mlp::Model M{};
   if (use_network && (st = nn_model(c, &M)) != ISO_OK) return st;
-  const iso::BrickGrid G = iso::brick_grid(L);
+  const iso::BrickGrid G = iso::brick_grid(
+      L, (flags & ISO_BAND_BRICK8) ? iso::kBandBrickShiftMax : iso::kBandBrickShiftDefault);
   const uint64_t nb = static_cast<uint64_t>(G.n);
   ISO_HIP(c, ensure(&c->br_centres, 3 * nb * sizeof(float)));
   ISO_HIP(c, ensure(&c->br_hd, nb * sizeof(float)));
@@ -2216,7 +2233,7 @@ int iso_extract_mc_band(iso_ctx* c, const iso_grid_desc* desc, float margin, uin
     return ISO_OK;
   };
   c->band_valid = false;
-  const BandEmit band{bflags, G.nbx, G.nby};
+  const BandEmit band{bflags, G.nbx, G.nby, G.shift};
   st = extract_lattice(c, L, nullptr, nullptr, out, &signs, &band);
   if (st != ISO_OK && st != ISO_ERR_CAPACITY) return st;
   c->band_desc = *desc;

@@ -1,6 +1,6 @@
 // Launch interface of the learned narrow-band extraction (host side).
 //
-// The lattice's cells are grouped in 4x4x4 bricks.  The network predicts
+// The lattice's cells are grouped in bricks of 4x4x4 or 8x8x8 cells.  The network predicts
 // the (truncated) SDF at every brick centre; a brick whose prediction is
 // within its half-diagonal plus a calibrated margin of zero is in the band.
 // Band bricks get exact program samples; every other sample takes the
@@ -19,8 +19,6 @@
 
 namespace iso {
 
-constexpr int kBrick = 1 << kBandBrickShift;  // cells per brick edge (4)
-
 // Brick flags
 enum : uint8_t {
   kBrickBand = 1,    // exact samples
@@ -33,13 +31,16 @@ enum : uint8_t {
 constexpr int kCoarse = 4;
 
 struct BrickGrid {
-  int64_t nbx, nby, nbz;  // bricks per axis: ceil(cells / kBrick)
+  int32_t shift;          // bricks of 2^shift cells per edge
+  int32_t brick;          // 2^shift
+  int64_t nbx, nby, nbz;  // bricks per axis: ceil(cells / brick)
   int64_t n;              // nbx * nby * nbz
   int64_t ncx, ncy, ncz;  // super-bricks per axis: ceil(bricks / kCoarse)
   int64_t nc;             // ncx * ncy * ncz
 };
 
-BrickGrid brick_grid(const Lattice& L);
+// shift: kBandBrickShiftDefault or kBandBrickShiftMax.
+BrickGrid brick_grid(const Lattice& L, int shift);
 
 // Brick centres (3 floats each, brick order x fastest) and half-diagonals.
 hipError_t launch_brick_centres(const Lattice& L, const BrickGrid& G, float* centres,

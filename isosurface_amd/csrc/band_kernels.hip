@@ -5,6 +5,7 @@
 #include "band_kernels.h"
 #include "mc_device.h"
 #include <cmath>
+#include <type_traits>
 
 #include "sdf_eval.h"
 
@@ -27,8 +28,8 @@ __device__ __forceinline__ BrickBox brick_box(const Lattice& L, const BrickGrid&
   const int64_t c[3] = {bx, by, bz};
   BrickBox B;
   for (int a = 0; a < 3; ++a) {
-    B.lo[a] = kBrick * c[a];
-    B.hi[a] = B.lo[a] + kBrick < n[a] - 1 ? B.lo[a] + kBrick : n[a] - 1;
+    B.lo[a] = static_cast<int64_t>(G.brick) * c[a];
+    B.hi[a] = B.lo[a] + G.brick < n[a] - 1 ? B.lo[a] + G.brick : n[a] - 1;
   }
   return B;
 }
@@ -46,7 +47,7 @@ __global__ __launch_bounds__(256) void k_brick_centres(Lattice L, BrickGrid G,
   for (int a = 0; a < 3; ++a) {
     const float mid = 0.5f * static_cast<float>(B.lo[a] + B.hi[a]);
     centres[3 * b + a] = L.origin[a] + mid * L.spacing[a];
-    const float ext = static_cast<float>(kBrick) * L.spacing[a];
+    const float ext = static_cast<float>(G.brick) * L.spacing[a];
     d2 += ext * ext;
   }
   half_diag[b] = 0.5f * sqrtf(d2);
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void k_coarse_certify(const DevOp* __restrict_
   const int64_t r = s / G.ncx;
   const int64_t c[3] = {sx, r % G.ncy, r / G.ncy};
   const int64_t n[3] = {L.nx, L.ny, L.nz};
-  constexpr int64_t kSpan = static_cast<int64_t>(kBrick) * kCoarse;  // cells per edge
+  const int64_t kSpan = static_cast<int64_t>(G.brick) * kCoarse;  // cells per edge
   float p[3];
   float d2 = 0.0f;
   for (int a = 0; a < 3; ++a) {
@@ -173,8 +174,8 @@ struct FlagSink {
   }
 };
 
-__device__ __forceinline__ int64_t brick_of(int64_t c, int64_t nb) {
-  const int64_t b = c / kBrick;
+__device__ __forceinline__ int64_t brick_of(int64_t c, int shift, int64_t nb) {
+  const int64_t b = c >> shift;
   return b < nb ? b : nb - 1;
 }
 
@@ -200,8 +201,9 @@ __global__ __launch_bounds__(256) void k_band_background(Lattice L, BrickGrid G,
       const uint32_t r = ff - z * nxy;
       const uint32_t y = r / nx;
       const uint32_t x = r - y * nx;
-      const int64_t b = (brick_of(z, G.nbz) * G.nby + brick_of(y, G.nby)) * G.nbx +
-                        brick_of(x, G.nbx);
+      const int64_t b =
+          (brick_of(z, G.shift, G.nbz) * G.nby + brick_of(y, G.shift, G.nby)) * G.nbx +
+          brick_of(x, G.shift, G.nbx);
       inside = (flags[b] & kBrickInside) != 0;
     }
     const uint64_t m = __ballot(inside);
@@ -224,17 +226,26 @@ __device__ __forceinline__ uint32_t later_holders(int cx, int cy, int cz) {
   return m;
 }
 
-constexpr int kSide = kBrick + 1;                        // samples per edge of a brick box
-constexpr int kBoxRows = kSide * kSide;                  // x rows of a brick's sample box
-constexpr int kBoxSamples = kBoxRows * kSide;
 constexpr int kExactThreads = 256;
-constexpr int kGroupBricks = 32;                         // band bricks per workgroup pass
-constexpr int kBricksPerWave = kGroupBricks / (kExactThreads / 64);
-static_assert(kBoxSamples <= 256, "box positions are stored as bytes");
-static_assert(kSide <= 32, "a box row is one 32-bit mask");
 
-// Thirty-two band bricks per workgroup pass.  Each wavefront takes eight: it
-// reads a brick's and its 26 neighbours' band flags, walks the brick's
+// Sizes of the exact-sample kernel for bricks of 2^kShift cells per edge.
+// Eight-cell bricks take fewer bricks per workgroup pass, so a workgroup's
+// LDS stays near 17 KB and eight workgroups still fit a CU.
+template <int kShift>
+struct ExactShape {
+  static constexpr int kBrick = 1 << kShift;              // cells per brick edge
+  static constexpr int kSide = kBrick + 1;                 // samples per edge of a brick box
+  static constexpr int kBoxRows = kSide * kSide;           // x rows of a brick's sample box
+  static constexpr int kBoxSamples = kBoxRows * kSide;
+  static constexpr int kGroupBricks = kShift == 2 ? 32 : 8;  // band bricks per workgroup pass
+  static constexpr int kBricksPerWave = kGroupBricks / (kExactThreads / 64);
+  // box position lx + kSide (ly + kSide lz)
+  using Pos = typename std::conditional<(kBoxSamples <= 256), uint8_t, uint16_t>::type;
+  static_assert(kSide <= 32, "a box row is one 32-bit mask");
+};
+
+// Thirty-two band bricks (of 4^3 cells; eight of 8^3) per workgroup pass.
+// Each wavefront takes a quarter of them: it reads a brick's and its 26 neighbours' band flags, walks the brick's
 // closed sample box (x fastest) and appends the samples the brick owns to
 // the brick's list in LDS.  Then all 256 lanes evaluate the 32 lists
 // as one queue (a lane's brick by binary search of the lists' offsets), so
@@ -245,14 +256,18 @@ static_assert(kSide <= 32, "a box row is one 32-bit mask");
 // Eight wavefronts per SIMD (64 VGPRs, five of them spilled to scratch)
 // hide the latency of the program evaluation better than the seven
 // that 72 unconstrained VGPRs allow: 117 ms instead of 138 ms at 1024^3.
+template <int kShift>
 __global__ __launch_bounds__(kExactThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
 k_band_exact(
     const DevOp* __restrict__ ops, int n_ops, Lattice L, BrickGrid G,
     const uint8_t* __restrict__ flags, const uint32_t* __restrict__ band_list, uint64_t n_band,
     uint64_t* __restrict__ S, float* __restrict__ values, unsigned long long* evaluated) {
+  using X = ExactShape<kShift>;
+  constexpr int kBrick = X::kBrick, kSide = X::kSide, kBoxRows = X::kBoxRows;
+  constexpr int kGroupBricks = X::kGroupBricks, kBricksPerWave = X::kBricksPerWave;
   __shared__ uint32_t s_in[kGroupBricks][kBoxRows];
   __shared__ uint32_t s_own[kGroupBricks][kBoxRows];
-  __shared__ uint8_t s_pos[kGroupBricks][kBoxSamples];  // lx + kSide (ly + kSide lz)
+  __shared__ typename X::Pos s_pos[kGroupBricks][X::kBoxSamples];
   __shared__ uint32_t s_lo[kGroupBricks][3];
   __shared__ uint32_t s_n[kGroupBricks];
   __shared__ uint32_t s_pre[kGroupBricks + 1];
@@ -316,7 +331,7 @@ k_band_exact(
           const uint64_t m = __ballot(own);
           if (own) {
             const uint32_t k = n_own + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1)));
-            s_pos[j][k] = static_cast<uint8_t>(lx + kSide * (ly + kSide * lz));
+            s_pos[j][k] = static_cast<typename X::Pos>(lx + kSide * (ly + kSide * lz));
             atomicOr(&s_own[j][ly + kSide * lz], 1u << lx);
           }
           n_own += static_cast<uint32_t>(__popcll(m));
@@ -436,11 +451,13 @@ unsigned grid_of(uint64_t n, uint64_t cap) {
 
 }  // namespace
 
-BrickGrid brick_grid(const Lattice& L) {
+BrickGrid brick_grid(const Lattice& L, int shift) {
   BrickGrid G;
-  G.nbx = (L.nx - 1 + kBrick - 1) / kBrick;
-  G.nby = (L.ny - 1 + kBrick - 1) / kBrick;
-  G.nbz = (L.nz - 1 + kBrick - 1) / kBrick;
+  G.shift = shift;
+  G.brick = 1 << shift;
+  G.nbx = (L.nx - 1 + G.brick - 1) / G.brick;
+  G.nby = (L.ny - 1 + G.brick - 1) / G.brick;
+  G.nbz = (L.nz - 1 + G.brick - 1) / G.brick;
   G.n = G.nbx * G.nby * G.nbz;
   G.ncx = (G.nbx + kCoarse - 1) / kCoarse;
   G.ncy = (G.nby + kCoarse - 1) / kCoarse;
@@ -533,11 +550,16 @@ hipError_t launch_band_exact(const DevOp* ops, int n_ops, const Lattice& L, cons
                              hipStream_t s) {
   if (n_band == 0) return hipSuccess;
   if (!ops || !flags || !band_list || !B.signs || !evaluated) return hipErrorInvalidValue;
-  uint64_t blocks = (n_band + kGroupBricks - 1) / kGroupBricks;
-  if (blocks > (1u << 20)) blocks = 1u << 20;
-  hipLaunchKernelGGL(k_band_exact, dim3(static_cast<unsigned>(blocks)), dim3(kExactThreads), 0, s,
-                     ops, n_ops, L, G, flags, band_list, n_band, B.signs, values, evaluated);
-  return hipGetLastError();
+  auto launch = [&](auto kernel, uint64_t group) {
+    uint64_t blocks = (n_band + group - 1) / group;
+    if (blocks > (1u << 20)) blocks = 1u << 20;
+    hipLaunchKernelGGL(kernel, dim3(static_cast<unsigned>(blocks)), dim3(kExactThreads), 0, s, ops,
+                       n_ops, L, G, flags, band_list, n_band, B.signs, values, evaluated);
+    return hipGetLastError();
+  };
+  if (G.shift == 2) return launch(k_band_exact<2>, ExactShape<2>::kGroupBricks);
+  if (G.shift == 3) return launch(k_band_exact<3>, ExactShape<3>::kGroupBricks);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_band_diff(const Lattice& L, const McDims& D, const uint64_t* signs_a,

@@ -10,7 +10,9 @@
 //               entry in the mixed-cell list, in ascending cell order by a
 //               single-pass scan;
 //   3. feature  one thread per listed mixed cell: corner normals by central
-//               differences of the program (source.rs:82-94), normalised or
+//               differences of the program (source.rs:82-94), or with
+//               epsilon 0 the program's own HermiteSource normals
+//               (implicit/*.rs sample_normal, csg.rs:47-51), normalised or
 //               zero (marching_cubes_impl.rs:58-69), then TangentPlanes and
 //               the QEF solve (feature/mod.rs:91-148, feature/qef.rs:36-68)
 //               with the reference's f64 SVD.
@@ -114,10 +116,16 @@ struct DualSink {
   }
 };
 
-// CentralDifference::sample_normal (source.rs:82-94) at p, then
+// CentralDifference::sample_normal (source.rs:82-94) at p, or with eps 0
+// the program's HermiteSource::sample_normal, then
 // normalised().unwrap_or_default() (marching_cubes_impl.rs:63-67).
 __device__ __forceinline__ qef::Vec3f central_normal(const DevOp* ops, int n_ops, float px,
                                                      float py, float pz, float eps) {
+  if (eps == 0.0f) {
+    float raw[3];
+    sdf_eval_hermite(ops, n_ops, px, py, pz, raw);
+    return qef::normalised_or_zero(qef::Vec3f{raw[0], raw[1], raw[2]});
+  }
   const float vx = sdf_eval(ops, n_ops, px + eps, py + 0.0f, pz + 0.0f) -
                    sdf_eval(ops, n_ops, px - eps, py - 0.0f, pz - 0.0f);
   const float vy = sdf_eval(ops, n_ops, px + 0.0f, py + eps, pz + 0.0f) -

@@ -80,29 +80,34 @@ struct InteriorTables {
   uint8_t need[256];
 };
 
+// Edge e of an interior cell: its owner's slot | the edge's local id in the
+// owner << 3.
+constexpr uint8_t interior_code(int e) {
+  const EdgeGeom g = make_geom();
+  const int axis = g.axis[e];
+  const int r[3] = {axis != 0 && g.minp[e][0] == 0 ? 1 : 0,
+                    axis != 1 && g.minp[e][1] == 0 ? 1 : 0,
+                    axis != 2 && g.minp[e][2] == 0 ? 1 : 0};
+  const int slot = r[0] | (r[1] << 1) | (r[2] << 2);
+  const int owner_edge = g.of_axis_min[axis][(g.minp[e][0] | r[0]) |
+                                             ((g.minp[e][1] | r[1]) << 1) |
+                                             ((g.minp[e][2] | r[2]) << 2)];
+  return static_cast<uint8_t>(slot | (owner_edge << 3));
+}
+
 constexpr InteriorTables make_interior() {
   InteriorTables t{};
-  const EdgeGeom g = make_geom();
   const tables::McTables& T = tables::kTables;
-  uint8_t slot[12] = {}, owner_edge[12] = {};
-  for (int e = 0; e < 12; ++e) {
-    const int axis = g.axis[e];
-    const int r[3] = {axis != 0 && g.minp[e][0] == 0 ? 1 : 0,
-                      axis != 1 && g.minp[e][1] == 0 ? 1 : 0,
-                      axis != 2 && g.minp[e][2] == 0 ? 1 : 0};
-    slot[e] = static_cast<uint8_t>(r[0] | (r[1] << 1) | (r[2] << 2));
-    owner_edge[e] = static_cast<uint8_t>(
-        g.of_axis_min[axis][(g.minp[e][0] | r[0]) | ((g.minp[e][1] | r[1]) << 1) |
-                            ((g.minp[e][2] | r[2]) << 2)]);
-  }
+  uint8_t code[12] = {};
+  for (int e = 0; e < 12; ++e) code[e] = interior_code(e);
   for (int c = 0; c < 256; ++c) {
     uint32_t need = 0;
     for (int e = 0; e < 12; ++e)
-      if ((T.edge_mask[c] >> e) & 1u) need |= 1u << slot[e];
+      if ((T.edge_mask[c] >> e) & 1u) need |= 1u << (code[e] & 7);
     t.need[c] = static_cast<uint8_t>(need);
     for (int j = 0; j < 16; ++j) {
       const int e = T.tri[c][j];
-      t.ix[c][j] = e < 0 ? 0 : static_cast<uint8_t>(slot[e] | (owner_edge[e] << 3));
+      t.ix[c][j] = e < 0 ? 0 : code[e];
     }
   }
   return t;
@@ -329,6 +334,50 @@ __device__ __forceinline__ void owned_planes(const SegWords& s, uint64_t valid, 
     if (z0) sliced_add(p, 4, v0 & (c[0] ^ c[3]));
     if (y0) sliced_add(p, 4, v0 & (c[0] ^ c[4]));
   }
+}
+
+// Per-cell triangle counts of the cells in `m` (mixed cells only), bit-sliced
+// in three planes, with no table lookup.  For every case other than 0 and
+// 255 the triangle table's count equals
+//   V - 2 F - 4 H,
+// V = corners inside, F = cube faces whose four corners are all inside,
+// H = 1 when the two corners outside are antipodal.  The table triangulates
+// each surface patch as a fan over its crossed edges, (edges - 2)
+// triangles; a cell has 3 V - 2 I crossed edges (I: cube edges with both
+// corners inside) and, but for the antipodal case, one patch per connected
+// component of its inside corners, V - I + F components on the cube's
+// graph (whose cycles there are the full faces, and the hexagon around an
+// antipodal pair, for which the table keeps two separate triangles).  The
+// identity is checked against the table, case by case, in the tests.
+__device__ __forceinline__ void tri_planes(const SegWords& s, uint64_t m, uint64_t* t) {
+  const uint64_t* c = s.c;
+  auto full_add = [](uint64_t a, uint64_t b, uint64_t d, uint64_t* carry) {
+    const uint64_t x = a ^ b;
+    *carry = (a & b) | (x & d);
+    return x ^ d;
+  };
+  // V mod 8 (V is 1..7 in a mixed cell): a carry-save sum of the corner planes
+  uint64_t k1, k2, k4, k5;
+  const uint64_t s1 = full_add(c[0], c[1], c[2], &k1);
+  const uint64_t s2 = full_add(c[3], c[4], c[5], &k2);
+  const uint64_t v0 = full_add(s1, s2, c[6] ^ c[7], &k4);
+  const uint64_t twos = full_add(k1, k2, c[6] & c[7], &k5);
+  const uint64_t v1 = twos ^ k4, v2 = k5 ^ (twos & k4);
+  // F mod 4 (at most 3 in a mixed cell); faces x=0 {0,3,4,7}, x=1 {1,2,5,6},
+  // y=0 {0,1,4,5}, y=1 {2,3,6,7}, z=0 {0,1,2,3}, z=1 {4,5,6,7}
+  const uint64_t c04 = c[0] & c[4], c37 = c[3] & c[7], c15 = c[1] & c[5], c26 = c[2] & c[6];
+  uint64_t ka, kb;
+  const uint64_t sa = full_add(c04 & c37, c15 & c26, c04 & c15, &ka);
+  const uint64_t sb = full_add(c26 & c37, c[0] & c[1] & c[2] & c[3], c[4] & c[5] & c[6] & c[7], &kb);
+  const uint64_t f0 = sa ^ sb, f1 = ka ^ kb ^ (sa & sb);
+  // H: six corners inside, and an antipodal pair (0,6), (1,7), (2,4) or (3,5) outside
+  const uint64_t h = v2 & v1 & ~v0 &
+                     (~(c[0] | c[6]) | ~(c[1] | c[7]) | ~(c[2] | c[4]) | ~(c[3] | c[5]));
+  // bit 0 is V's; bits 1-2 are (V >> 1) - F - 2 H, which lies in 0..2
+  const uint64_t d0 = v1 ^ f0, d1 = v2 ^ f1 ^ (~v1 & f0);
+  t[0] = v0 & m;
+  t[1] = d0 & m;
+  t[2] = (d1 ^ h) & m;
 }
 
 __device__ __forceinline__ uint32_t planes_count(const uint64_t* p, uint64_t m) {

@@ -434,7 +434,11 @@ k_mc_fused(const float* __restrict__ g, Lattice L,
         nv = planes_count(vp, ~0ull);
         mix = mixed_cells(sw) & valid;
         na = popc64(mix);
-        for (uint64_t m = mix; m; m &= m - 1) nt += s_t.tri_count[case_of(sw, __builtin_ctzll(m))];
+        if (mix) {
+          uint64_t tp[3];
+          tri_planes(sw, mix, tp);
+          nt = popc64(tp[0]) + 2 * popc64(tp[1]) + 4 * popc64(tp[2]);
+        }
       }
       // One scan of the three counts packed in 64 bits (per tile: mixed
       // cells < 2^16, triangles < 2^18, vertices < 2^19).

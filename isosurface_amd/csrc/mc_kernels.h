@@ -28,7 +28,10 @@ struct McDims {
 constexpr int kCountBlock = 1024;  // threads per count workgroup (16 waves)
 constexpr int kCountRounds = 1;    // segments per count thread
 constexpr int kCountSegs = kCountBlock * kCountRounds;  // segments per count workgroup
-constexpr int kBandBrickShift = 2;  // narrow-band bricks: 4 x 4 x 4 cells
+// Narrow-band bricks: 2^shift cells per edge, shift 2 (4 x 4 x 4, the
+// default) or 3 (8 x 8 x 8).
+constexpr int kBandBrickShiftDefault = 2;
+constexpr int kBandBrickShiftMax = 3;
 
 // Running totals of the decoupled look-back scan.
 struct ScanTriple {
@@ -67,10 +70,10 @@ struct CellSpace {
   const uint64_t* signs;    // the lattice's flat sign bits
   uint32_t segs_per_row;
   uint32_t seg_off;         // global index of the lattice's first segment
-  const DevOp* ops;         // program source (values re-evaluated in emit)
+  const DevOp* ops;         // program source
   int32_t n_ops;
   int32_t reserved;
-  const float* grid;        // dense samples, or null for a program source
+  const float* grid;        // dense samples (a batch's stored program samples), or null
   const float* positions;   // per-point vertex positions, or null
 };
 
@@ -133,10 +136,10 @@ hipError_t launch_point_cloud(const Lattice& L, const McDims& D, const McBuffers
                               float* points, uint64_t capacity, hipStream_t s);
 // Emit of a narrow-band extraction: corner values from `values` (stored by
 // the band's exact sampling) for cells of bricks flagged exact (bit 0 of
-// brick_flags, bricks of 2^kBandBrickShift cells, nbx x nby per z layer),
-// the program elsewhere.
+// brick_flags, bricks of 2^brick_shift cells per edge, nbx x nby per z
+// layer), the program elsewhere.
 hipError_t launch_emit_band(const float* values, const uint8_t* brick_flags, int64_t nbx,
-                            int64_t nby, const DevOp* ops, int n_ops, const Lattice& L,
+                            int64_t nby, int brick_shift, const DevOp* ops, int n_ops, const Lattice& L,
                             const McDims& D, const McBuffers& B, hipStream_t s);
 hipError_t launch_sample_lattice(const DevOp* ops, int n_ops, const Lattice& L, float* out,
                                  hipStream_t s);
@@ -152,6 +155,11 @@ struct BatchPlan {
   uint32_t n_chunks;
   int64_t n_words;   // sign words of all chunks
   int64_t n_segs;    // segments of all chunks
+  // The chunks' `grid` pointers name a buffer of 64 samples per sign word
+  // (the chunk's word w holds samples 64 w ..): the signs stage stores the
+  // program's samples there and emit reads them.  False: no such buffer,
+  // emit evaluates the programs at the edge endpoints.
+  bool stored_values;
 };
 
 // Count workgroups of a batch at `rounds` segments per thread (1 or 2).

@@ -162,7 +162,8 @@ k_signs_directed(const DevOp* __restrict__ ops, int n_ops,
 // Batched chunks: one wavefront per sign word of the concatenated word
 // numbering; the word's chunk is found by binary search, so a wavefront
 // always evaluates a single chunk's program.  Thread i < n_chunks also
-// clears chunk i's guard words.
+// clears chunk i's guard words.  A chunk with a `grid` buffer gets its
+// samples stored there for emit.
 __global__ __launch_bounds__(256) void k_batch_signs(const CellSpace* __restrict__ chunks,
                                                      const uint64_t* __restrict__ word_start,
                                                      uint32_t n_chunks, ResetArgs R) {
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(256) void k_batch_signs(const CellSpace* __restrict
       const float v = sdf_eval(C.ops, C.n_ops, C.origin[0] + static_cast<float>(x) * C.spacing[0],
                                C.origin[1] + static_cast<float>(y) * C.spacing[1],
                                C.origin[2] + static_cast<float>(z) * C.spacing[2]);
+      if (C.grid) const_cast<float*>(C.grid)[f] = v;
       inside = !(v > 0.0f);
     }
     const uint64_t m = __ballot(inside);
@@ -244,12 +246,10 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
                                                        McBuffers B) {
   constexpr int kCountSegs = kCountBlock * kCountRounds;  // segments per workgroup
   constexpr int kWaves = kCountBlock / 64;
-  __shared__ uint8_t s_ntri[256];
   __shared__ uint32_t s_wv[kCountRounds][kWaves], s_wt[kCountRounds][kWaves],
       s_wa[kCountRounds][kWaves];
   __shared__ uint32_t s_block;
   __shared__ ScanTriple s_prefix;
-  for (int i = threadIdx.x; i < 256; i += blockDim.x) s_ntri[i] = c_tables.tri_count[i];
   if (threadIdx.x == 0)
     s_block = atomicAdd(reinterpret_cast<unsigned int*>(&B.totals[kTotTicket]), 1u);
   __syncthreads();
@@ -297,20 +297,12 @@ __global__ __launch_bounds__(kCountBlock) void k_count(Space sp, int64_t n_segs,
       if (na) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) B.vplanes[4 * g + k] = vp[k];
-      }
-      uint64_t p0 = 0, p1 = 0, p2 = 0;  // per-cell triangle counts, bit-sliced
-      for (uint64_t m = act[r]; m; m &= m - 1) {
-        const int b = __builtin_ctzll(m);
-        const uint32_t c = s_ntri[case_of(sw, b)];
-        nt += c;
-        p0 |= static_cast<uint64_t>(c & 1u) << b;
-        p1 |= static_cast<uint64_t>((c >> 1) & 1u) << b;
-        p2 |= static_cast<uint64_t>((c >> 2) & 1u) << b;
-      }
-      if (na) {
-        B.tplanes[3 * g + 0] = p0;
-        B.tplanes[3 * g + 1] = p1;
-        B.tplanes[3 * g + 2] = p2;
+        uint64_t tp[3];  // per-cell triangle counts, bit-sliced
+        tri_planes(sw, act[r], tp);
+        nt = popc64(tp[0]) + 2 * popc64(tp[1]) + 4 * popc64(tp[2]);
+        B.tplanes[3 * g + 0] = tp[0];
+        B.tplanes[3 * g + 1] = tp[1];
+        B.tplanes[3 * g + 2] = tp[2];
       }
     }
     const uint32_t iv = wave_incl_scan(nv), it = wave_incl_scan(nt), ia = wave_incl_scan(na);
@@ -503,6 +495,7 @@ struct EmitArgs {
   // kEmitBand: brick flags (bit 0: exact samples), bricks per x and y row
   const uint8_t* brick_flags;
   int64_t nbx, nby;
+  int brick_shift;  // bricks of 2^brick_shift cells per edge
 };
 
 // Where an extraction's sample values and vertex positions come from.
@@ -635,8 +628,8 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
     // only active when a predicted sign was wrong, and evaluate the program.
     bool stored = kSrc != kEmitProgram && kSrc != kEmitDirected;
     if (kSrc == kEmitBand)
-      stored = (A.brick_flags[((cz >> kBandBrickShift) * A.nby + (cy >> kBandBrickShift)) * A.nbx +
-                              (cx >> kBandBrickShift)] &
+      stored = (A.brick_flags[((cz >> A.brick_shift) * A.nby + (cy >> A.brick_shift)) * A.nbx +
+                              (cx >> A.brick_shift)] &
                 1u) != 0;
     const bool interior = cx >= 1 && cy >= 1 && cz >= 1;
     const bool deep = cx >= 2 && cy >= 2 && cz >= 2;  // so are all its owners
@@ -1048,6 +1041,7 @@ EmitArgs emit_args(const McBuffers& B) {
   A.host_totals = B.host_totals;
   A.brick_flags = nullptr;
   A.nbx = A.nby = 0;
+  A.brick_shift = 0;
   return A;
 }
 
@@ -1130,13 +1124,17 @@ hipError_t launch_point_cloud(const Lattice& L, const McDims& D, const McBuffers
 }
 
 hipError_t launch_emit_band(const float* values, const uint8_t* brick_flags, int64_t nbx,
-                            int64_t nby, const DevOp* ops, int n_ops, const Lattice& L,
-                            const McDims& D, const McBuffers& B, hipStream_t s) {
-  if (!values || !brick_flags || !ops || nbx < 1 || nby < 1) return hipErrorInvalidValue;
+                            int64_t nby, int brick_shift, const DevOp* ops, int n_ops,
+                            const Lattice& L, const McDims& D, const McBuffers& B,
+                            hipStream_t s) {
+  if (!values || !brick_flags || !ops || nbx < 1 || nby < 1 || brick_shift < 1 ||
+      brick_shift > kBandBrickShiftMax)
+    return hipErrorInvalidValue;
   EmitArgs A = emit_args(B);
   A.brick_flags = brick_flags;
   A.nbx = nbx;
   A.nby = nby;
+  A.brick_shift = brick_shift;
   const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, values, nullptr);
   auto k = k_emit<kEmitBand, SingleSpace>;
   launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
@@ -1200,9 +1198,15 @@ hipError_t launch_batch_emit(const BatchPlan& P, const McBuffers& B, int blocks_
   sp.chunks = P.chunks;
   sp.seg_start = P.seg_start;
   sp.n_chunks = P.n_chunks;
-  auto k = k_emit<kEmitProgram, BatchSpace>;
-  hipLaunchKernelGGL(k, dim3(emit_blocks(B, blocks_per_cu, k)), dim3(kEmitThreads), 0, s,
-                     emit_args(B), sp);
+  if (P.stored_values) {
+    auto k = k_emit<kEmitGrid, BatchSpace>;
+    hipLaunchKernelGGL(k, dim3(emit_blocks(B, blocks_per_cu, k)), dim3(kEmitThreads), 0, s,
+                       emit_args(B), sp);
+  } else {
+    auto k = k_emit<kEmitProgram, BatchSpace>;
+    hipLaunchKernelGGL(k, dim3(emit_blocks(B, blocks_per_cu, k)), dim3(kEmitThreads), 0, s,
+                       emit_args(B), sp);
+  }
   return hipGetLastError();
 }
 

@@ -5,6 +5,12 @@
 //   A operand: lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
 //   B operand: lane l holds B[4 (l / 16) + j][l % 16]
 //   D result:  lane l holds D[4 (l / 16) + j][l % 16]
+// The inference forward uses gfx950's double-K form
+// (v_mfma_f32_16x16x32_bf16: lane l holds K values 8 (l / 16) + j,
+// j = 0..7) on two adjacent 16-wide K blocks at once: lane l passes its
+// four values of block 2i then its four of block 2i+1, for A and B alike,
+// which the instruction reads as one 32-wide block with its K index
+// permuted the same way in both operands — the same sum of products.
 // A wavefront pushes 16 samples (the N dimension) through the network:
 // layer outputs land in exactly the layout the next layer's B operand
 // needs, so activations stay in registers from the features to the output.
@@ -23,12 +29,13 @@ namespace {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// w_k = pi * 2^(k / 2) rounded to f32
-__constant__ float c_freq[kFreq] = {
-    0x1.921fb6p+1f, 0x1.1c5832p+2f, 0x1.921fb6p+2f, 0x1.1c5832p+3f,
-    0x1.921fb6p+3f, 0x1.1c5832p+4f, 0x1.921fb6p+4f, 0x1.1c5832p+5f,
-    0x1.921fb6p+5f, 0x1.1c5832p+6f, 0x1.921fb6p+6f, 0x1.1c5832p+7f,
-    0x1.921fb6p+7f, 0x1.1c5832p+8f, 0x1.921fb6p+8f, 0x1.1c5832p+9f};
+// c_k = 2^(k / 2) / 2 revolutions per unit (2 pi c_k = pi 2^(k / 2)),
+// rounded to f32: powers of two for even k, sqrt(2) scaled for odd k.
+__constant__ float c_rev[kFreq] = {
+    0x1p-1f, 0x1.6a09e6p-1f, 0x1p+0f, 0x1.6a09e6p+0f,
+    0x1p+1f, 0x1.6a09e6p+1f, 0x1p+2f, 0x1.6a09e6p+2f,
+    0x1p+3f, 0x1.6a09e6p+3f, 0x1p+4f, 0x1.6a09e6p+4f,
+    0x1p+5f, 0x1.6a09e6p+5f, 0x1p+6f, 0x1.6a09e6p+6f};
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
   const uint32_t u = __float_as_uint(f);
@@ -44,6 +51,17 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+// Two adjacent K blocks in one double-K MFMA (see the tiling note above).
+__device__ __forceinline__ f32x4 mfma2(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c) {
+  const s16x8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+  const s16x8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                 __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
 __device__ __forceinline__ bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
   bf16x4 r;
   r[0] = static_cast<short>(a);
@@ -53,15 +71,19 @@ __device__ __forceinline__ bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint
   return r;
 }
 
-// Feature fi (0..111) of point p: p_d, sin(w_k p_d), cos(w_k p_d), zero padding.
+// Feature fi (0..111) of point p: p_d, sin(2 pi r), cos(2 pi r) with
+// r = c_k p_d in revolutions, zero padding.  v_sin_f32 / v_cos_f32 take
+// revolutions; the fractional part of an f32 r is exact, so reducing r to
+// it first changes nothing and keeps every argument in their input range.
 __device__ __forceinline__ float feature(int fi, float px, float py, float pz) {
   if (fi < 3) return fi == 0 ? px : (fi == 1 ? py : pz);
   if (fi >= kFeat) return 0.0f;
   const int q = fi - 3;
   const int k = q / 6, r = q - 6 * k;
   const int d = r < 3 ? r : r - 3;
-  const float a = c_freq[k] * (d == 0 ? px : (d == 1 ? py : pz));
-  return r < 3 ? sinf(a) : cosf(a);
+  // the fractional part (exact) keeps the argument in the instructions' range
+  const float rev = __builtin_amdgcn_fractf(c_rev[k] * (d == 0 ? px : (d == 1 ? py : pz)));
+  return r < 3 ? __builtin_amdgcn_sinf(rev) : __builtin_amdgcn_cosf(rev);
 }
 
 // Fragments of one 16-row block set, read from LDS: lane's 4 bf16 of block (mb, kb).
@@ -78,7 +100,9 @@ struct Tile {
 // Forward pass of 16 samples; lane l's sample is l % 16 at (px, py, pz).
 // sF: fragments F1 F2 F3 in LDS; sB: b1 b2 b3 w4 b4 in LDS.  Layer 1
 // consumes the features one K-block at a time (computed just in time, and
-// stored hidden-major to xdst[row * n + s] when xdst is given).
+// stored hidden-major to xdst[row * n + s] when xdst is given).  kDoubleK:
+// pairs of K blocks per double-K MFMA (no xdst).
+template <bool kDoubleK>
 __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB, float px,
                                              float py, float pz, int lane, Tile& T,
                                              uint16_t* xdst, int64_t n, int64_t s) {
@@ -94,20 +118,52 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
   f32x4 acc1[4];
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) acc1[mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 1
-  for (int kb = 0; kb < kKB1; ++kb) {
+  auto features = [&](int kb) {
     const int f0 = 16 * kb + r0;
     uint16_t xb[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) xb[j] = to_bf16(feature(f0 + j, px, py, pz));
-    if (xdst) {
+    if (!kDoubleK && xdst) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) xdst[(f0 + j) * n + s] = xb[j];
     }
-    const bf16x4 x = pack4(xb[0], xb[1], xb[2], xb[3]);
+    return pack4(xb[0], xb[1], xb[2], xb[3]);
+  };
+  if (kDoubleK) {
+#pragma unroll 1
+    for (int kb = 0; kb + 1 < kKB1; kb += 2) {
+      const bf16x4 x0 = features(kb), x1 = features(kb + 1);
 #pragma unroll
-    for (int mb = 0; mb < 4; ++mb) acc1[mb] = mfma(frag(F1, kKB1, mb, kb, lane), x, acc1[mb]);
+      for (int mb = 0; mb < 4; ++mb)
+        acc1[mb] = mfma2(frag(F1, kKB1, mb, kb, lane), frag(F1, kKB1, mb, kb + 1, lane), x0, x1,
+                         acc1[mb]);
+    }
+    if (kKB1 & 1) {
+      const bf16x4 x = features(kKB1 - 1);
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb)
+        acc1[mb] = mfma(frag(F1, kKB1, mb, kKB1 - 1, lane), x, acc1[mb]);
+    }
+  } else {
+#pragma unroll 1
+    for (int kb = 0; kb < kKB1; ++kb) {
+      const bf16x4 x = features(kb);
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) acc1[mb] = mfma(frag(F1, kKB1, mb, kb, lane), x, acc1[mb]);
+    }
   }
+  // A 64-wide K product: two double-K MFMAs, or four.
+  auto product64 = [&](const uint16_t* F, int mb, const bf16x4* h) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (kDoubleK) {
+      acc = mfma2(frag(F, 4, mb, 0, lane), frag(F, 4, mb, 1, lane), h[0], h[1], acc);
+      acc = mfma2(frag(F, 4, mb, 2, lane), frag(F, 4, mb, 3, lane), h[2], h[3], acc);
+    } else {
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F, 4, mb, kb, lane), h[kb], acc);
+    }
+    return acc;
+  };
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
     uint16_t h[4];
@@ -117,9 +173,7 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
   }
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F2, 4, mb, kb, lane), T.h1[kb], acc);
+    const f32x4 acc = product64(F2, mb, T.h1);
     uint16_t h[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) h[j] = to_bf16(fmaxf(acc[j] + b2[16 * mb + r0 + j], 0.0f));
@@ -128,9 +182,7 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
   float part = 0.0f;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F3, 4, mb, kb, lane), T.h2[kb], acc);
+    const f32x4 acc = product64(F3, mb, T.h2);
     uint16_t h[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -176,7 +228,7 @@ __global__ __launch_bounds__(kThreads) void k_forward(Model M, const float* __re
     const float px = ok ? points[3 * s] : 0.0f, py = ok ? points[3 * s + 1] : 0.0f,
                 pz = ok ? points[3 * s + 2] : 0.0f;
     Tile T;
-    forward_tile(sF, sB, px, py, pz, lane, T, nullptr, 0, 0);
+    forward_tile<true>(sF, sB, px, py, pz, lane, T, nullptr, 0, 0);
     if (ok && lane < 16) out[s] = T.f;
   }
 }
@@ -216,7 +268,7 @@ __global__ __launch_bounds__(kThreads) void k_train_fb(Model M, Batch B, Samples
     const float t = fminf(fmaxf(S.sdf[src], -S.clamp), S.clamp);
     const float wgt = S.weight ? S.weight[src] : 1.0f;
     Tile T;
-    forward_tile(sF, sB, px, py, pz, lane, T, B.x, B.n, s);
+    forward_tile<false>(sF, sB, px, py, pz, lane, T, B.x, B.n, s);
     // Least squares plus (overshoot - 1) e^2, e the part of the error that
     // lies outside the interval between 0 and the target.
     const float r = T.f - t;

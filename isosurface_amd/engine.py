@@ -530,17 +530,24 @@ class DualContouring:
         self.ctx = context or default_context()
         self._last_counts = (0, 0)
 
-    def extract_mesh(self, source: CentralDifference) -> Mesh:
-        if not isinstance(source, CentralDifference):
-            raise TypeError("DualContouring needs a HermiteSource: wrap the program source "
-                            "in CentralDifference")
+    def extract_mesh(self, source) -> Mesh:
+        """The source is CentralDifference(Source) (normals by central
+        differences) or a program Source whose HermiteSource normals are
+        analytic (shapes, unions and translations)."""
+        if isinstance(source, CentralDifference):
+            epsilon = float(source.epsilon)
+        elif isinstance(source, Source):
+            epsilon = 0.0
+        else:
+            raise TypeError("DualContouring needs a HermiteSource: a program Source (analytic "
+                            "normals) or CentralDifference(Source)")
         self.ctx.upload_program(source.program())
         desc = self.ctx.reference_desc(self.size, A.ISO_SOURCE_PROGRAM)
-        mesh = self.ctx.extract_dc(desc, source.epsilon, *self._last_counts)
+        mesh = self.ctx.extract_dc(desc, epsilon, *self._last_counts)
         self._last_counts = (len(mesh.vertices), len(mesh.indices))
         return mesh
 
-    def extract(self, source: CentralDifference, extractor: Extractor) -> None:
+    def extract(self, source, extractor: Extractor) -> None:
         """DualContouring::extract (dual_contouring.rs:65-105)."""
         _replay(self.extract_mesh(source), extractor)
 

@@ -128,6 +128,13 @@ int main(int argc, char** argv) {
     dc.extract(isosurface::CentralDifference<std::remove_cv_t<decltype(cube)>>{cube, 1e-6f}, dcsink);
     dump(out + ".dc.f32", dcv);
     dump(out + ".dc.u32", dci);
+    // ... and over the cube's own (HermiteSource) normals.
+    std::vector<float> dcav;
+    std::vector<uint32_t> dcai;
+    isosurface::IndexedVertices dcasink(dcav, dcai);
+    dc.extract(cube, dcasink);
+    dump(out + ".dc_hermite.f32", dcav);
+    dump(out + ".dc_hermite.u32", dcai);
     isosurface::LinearHashedMarchingCubes lh(6, ctx);
     std::vector<float> lhv;
     std::vector<uint32_t> lhi;

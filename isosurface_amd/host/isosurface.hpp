@@ -487,9 +487,14 @@ class DualContouring {
     if (size < 2) throw Error(ISO_ERR_INVALID_ARGUMENT, "size must be at least 2");
   }
 
-  // DualContouring::extract over CentralDifference::new_with_epsilon(source, e).
+  // DualContouring::extract over a HermiteSource: CentralDifference<S>
+  // (CentralDifference::new_with_epsilon(source, e)) or a lowerable source of
+  // shapes, unions and translations, whose own analytic normals are used.
   template <class S>
-  void extract(const CentralDifference<S>& source, Extractor& extractor) {
+  void extract(const S& source, Extractor& extractor) {
+    static_assert(is_lowerable<S>::value, "DualContouring::extract takes a lowerable source");
+    float epsilon = 0.0f;
+    if constexpr (is_central_difference<S>::value) epsilon = source.epsilon;
     Program prog;
     source.lower(prog);
     detail::upload(ctx_, prog);
@@ -497,7 +502,7 @@ class DualContouring {
     ctx_.check(iso_grid_desc_reference(static_cast<int64_t>(size_), ISO_SOURCE_PROGRAM, &desc),
                "iso_grid_desc_reference");
     detail::extract_host(
-        ctx_, [&](iso_mesh_out* out) { return iso_extract_dc(ctx_.get(), &desc, source.epsilon, out); },
+        ctx_, [&](iso_mesh_out* out) { return iso_extract_dc(ctx_.get(), &desc, epsilon, out); },
         vertices_, indices_, extractor, "iso_extract_dc");
   }
 

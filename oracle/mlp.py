@@ -9,9 +9,12 @@ in float64 (tests/test_oracle_mlp.py).  Only the tests, the smoke check and
 the benchmark's CPU-baseline leg use it.
 
 Network (parameter vector layout identical to the device's):
-  features  x = [p, sin(w_k p), cos(w_k p)], w_k = pi * 2^(k/2), k = 0..15,
+  features  x = [p, sin(2 pi r_k), cos(2 pi r_k)], r_k = c_k p rounded to
+            float32, c_k = 2^(k/2) / 2 (so 2 pi c_k = pi 2^(k/2)), k = 0..15,
             99 values in the order p_xyz, then per k: sin_xyz, cos_xyz;
-            padded with zeros to 112
+            padded with zeros to 112.  The angle is kept in revolutions,
+            r_k, whose fractional part is exact in float32: the device's
+            sine and cosine take revolutions.
   layers    h1 = relu(W1 x + b1), h2 = relu(W2 h1 + b2), h3 = relu(W3 h2 + b3)
             (64 units each), f = w4 . h3 + b4
   rounding  x, W1..W3, h1..h3 in bfloat16 (round to nearest even);
@@ -32,7 +35,8 @@ HIDDEN = 64
 N_FREQ = 16
 N_FEAT = 3 + 6 * N_FREQ  # 99
 FEAT_PAD = 112
-FREQS = (np.pi * 2.0 ** (np.arange(N_FREQ) / 2.0)).astype(np.float32)
+# revolutions per unit: 2^(k/2) / 2 in float32 (exact for even k)
+REVS = (2.0 ** (np.arange(N_FREQ) / 2.0) / 2.0).astype(np.float32)
 
 # parameter vector layout
 SIZES = [("W1", (HIDDEN, FEAT_PAD)), ("b1", (HIDDEN,)), ("W2", (HIDDEN, HIDDEN)),
@@ -70,9 +74,9 @@ def features(p: np.ndarray) -> np.ndarray:
     x = np.zeros((n, FEAT_PAD), np.float32)
     x[:, 0:3] = p
     for k in range(N_FREQ):
-        a = (FREQS[k] * p).astype(np.float32)
-        x[:, 3 + 6 * k:6 + 6 * k] = np.sin(a)
-        x[:, 6 + 6 * k:9 + 6 * k] = np.cos(a)
+        r = (REVS[k] * p).astype(np.float32).astype(np.float64)
+        x[:, 3 + 6 * k:6 + 6 * k] = np.sin(2.0 * np.pi * r)
+        x[:, 6 + 6 * k:9 + 6 * k] = np.cos(2.0 * np.pi * r)
     return x
 
 

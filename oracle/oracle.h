@@ -90,7 +90,9 @@ int or_edge_points_f64(const iso_node* prog, uint32_t n, const iso_grid_desc* de
                        const int64_t* keys, uint64_t n_keys, double* out_xyz);
 
 /* DualContouring::extract with MinimiseQEF and CentralDifference normals
- * (epsilon as in CentralDifference::new_with_epsilon) over a program lattice. */
+ * (epsilon as in CentralDifference::new_with_epsilon) over a program lattice;
+ * epsilon 0: the program's own HermiteSource normals (shapes, unions and
+ * translations only, else ISO_ERR_PROGRAM). */
 int or_extract_dc(const iso_node* prog, uint32_t n, const iso_grid_desc* desc, float epsilon,
                   or_mesh* out);
 /* The reference SVD solve of an n x 3 system (row-major a, n <= 12). */

@@ -349,7 +349,8 @@ def _dc_bind(lib):
 
 
 def extract_dc(program, desc: IsoGridDesc, epsilon: float = 1e-6) -> OracleMesh:
-    """DualContouring::extract (MinimiseQEF, CentralDifference), restated."""
+    """DualContouring::extract (MinimiseQEF), restated: CentralDifference
+    normals, or with epsilon 0 the program's own HermiteSource normals."""
     lib = load()
     _dc_bind(lib)
     arr = nodes(program)

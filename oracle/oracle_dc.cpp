@@ -1,7 +1,8 @@
 // CPU oracle, dual contouring — TEST INFRASTRUCTURE ONLY (see oracle.h).
 //
 // Restates DualContouring::extract with MinimiseQEF and CentralDifference
-// normals: dual_contouring.rs:65-105, traversal/dual_grid.rs:52-95,
+// normals (or, with epsilon 0, the program's own HermiteSource normals,
+// implicit/*.rs sample_normal and csg.rs:47-51): dual_contouring.rs:65-105, traversal/dual_grid.rs:52-95,
 // marching_cubes_impl.rs:58-69, source.rs:82-94, feature/mod.rs:91-148,
 // feature/qef.rs:36-68 and the SVD of math/svd.rs:25-84, 130-972 (f64).
 #include <cmath>
@@ -539,6 +540,8 @@ extern "C" int or_extract_dc(const iso_node* prog, uint32_t n, const iso_grid_de
   if (d.nx < 2 || d.ny < 2 || d.nz < 2) return ISO_ERR_INVALID_ARGUMENT;
   orc::Program p;
   if (!orc::load_program(prog, n, &p)) return ISO_ERR_PROGRAM;
+  if (!(epsilon >= 0.0f)) return ISO_ERR_INVALID_ARGUMENT;
+  if (epsilon == 0.0f && !orc::program_has_hermite(p)) return ISO_ERR_PROGRAM;
   const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
   const int64_t dnx = nx - 1;  // dual row width (size_minus_one, dual_grid.rs:62)
 
@@ -553,8 +556,11 @@ extern "C" int or_extract_dc(const iso_node* prog, uint32_t n, const iso_grid_de
     F3 q{orc::coord(d, 0, x), orc::coord(d, 1, y), orc::coord(d, 2, z)};
     return {q, orc::sample(p, q)};
   };
-  // CentralDifference::sample_normal (source.rs:82-94), normalised or zero.
+  // CentralDifference::sample_normal (source.rs:82-94), or with epsilon 0
+  // the program's HermiteSource::sample_normal, normalised or zero
+  // (marching_cubes_impl.rs:58-69).
   auto normal = [&](F3 q) -> F3 {
+    if (epsilon == 0.0f) return normalised_or_zero(orc::program_hermite_normal(p, q));
     const float vx = orc::sample(p, F3{q.x + epsilon, q.y + 0.0f, q.z + 0.0f}) -
                      orc::sample(p, F3{q.x - epsilon, q.y - 0.0f, q.z - 0.0f});
     const float vy = orc::sample(p, F3{q.x + 0.0f, q.y + epsilon, q.z + 0.0f}) -

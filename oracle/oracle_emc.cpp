@@ -221,6 +221,14 @@ extern "C" int or_extract_emc(const iso_node* prog, uint32_t n, float epsilon,
   return ISO_OK;
 }
 
+namespace orc {
+
+bool program_has_hermite(const Program& t) { return has_hermite(t); }
+
+F3 program_hermite_normal(const Program& t, F3 p) { return hermite_normal(t, t.root, p); }
+
+}  // namespace orc
+
 // HermiteSource::sample_normal of a program at one point (not normalised).
 extern "C" int or_sample_hermite_normal(const iso_node* prog, uint32_t n, float x, float y,
                                         float z, float* out) {

@@ -526,4 +526,9 @@ inline void finish(const MeshBuilder& mb, or_mesh* out) {
   }
 }
 
+// HermiteSource::sample_normal of the whole program at p (not normalised),
+// and whether the program has one (oracle_emc.cpp).
+bool program_has_hermite(const Program& t);
+F3 program_hermite_normal(const Program& t, F3 p);
+
 }  // namespace orc

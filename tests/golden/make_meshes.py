@@ -57,15 +57,28 @@ def mesh_entry(size, m) -> dict:
     }
 
 
+# DualContouring over the sources' own HermiteSource normals (epsilon 0).
+DC_HERMITE = {
+    "dc_hermite_sphere_r0.35_size33": (lambda: W.sphere_source(), 33),
+    "dc_hermite_boxes32_size128": (lambda: W.boxes_union(32, seed=4), 128),
+}
+
+
+def dc_hermite_cases(names=None):
+    return {name: (size, O.extract_dc(make().program(), O.reference_desc(size), 0.0))
+            for name, (make, size) in DC_HERMITE.items() if names is None or name in names}
+
+
 def main() -> None:
     path = Path(__file__).with_name("mesh_digests.json")
     if len(sys.argv) > 2 and sys.argv[1] == "--only":
         names = set(sys.argv[2].split(","))
-        known = {smooth_name(size) for size in SMOOTH_SIZES}
+        known = {smooth_name(size) for size in SMOOTH_SIZES} | set(DC_HERMITE)
         if names - known:
             raise SystemExit(f"--only supports {sorted(known)}; unknown: {sorted(names - known)}")
         out = json.loads(path.read_text())
         fresh = {k: mesh_entry(*v) for k, v in smooth_cases(names).items()}
+        fresh.update({k: mesh_entry(*v) for k, v in dc_hermite_cases(names).items()})
         out.update(fresh)
         for k, v in fresh.items():
             print(k, v["vertices"], v["triangles"], flush=True)
@@ -99,6 +112,7 @@ def main() -> None:
                    O.extract_mc(boxes.program(), desc=O.reference_desc(128), directed=True)))
     meshes.append(("emc_boxes32_size128", 128, O.extract_emc(boxes.program(), O.reference_desc(128))))
     meshes += [(name, size, m) for name, (size, m) in smooth_cases().items()]
+    meshes += [(name, size, m) for name, (size, m) in dc_hermite_cases().items()]
     for name, size, m in meshes:
         out[name] = mesh_entry(size, m)
         print(name, out[name]["vertices"], out[name]["triangles"], flush=True)

@@ -114,13 +114,16 @@ def test_band_mesh_equals_dense(ctx, size):
     assert np.array_equal(mesh.indices, dense.indices)
 
 
-def test_band_without_culling_is_dense(ctx):
+@pytest.mark.parametrize("brick", [0, A.ISO_BAND_BRICK8])
+def test_band_without_culling_is_dense(ctx, brick):
     src = W.sphere_source()
     band = NarrowBand(ctx)
     band.init(0)
     ctx.upload_program(src.program())
     desc = ctx.reference_desc(40, A.ISO_SOURCE_PROGRAM)
-    mesh, stats = band.extract_mesh(desc, 1e30)
+    mesh, stats = band.extract_mesh(desc, 1e30, brick)
+    per_axis = [-(-(n - 1) // brick_size(brick)) for n in (desc.nx, desc.ny, desc.nz)]
+    assert stats.bricks == int(np.prod(per_axis))
     assert stats.band_bricks == stats.bricks
     assert band.missed_cells(desc) == 0
     dense = MarchingCubes(40, ctx).extract_mesh(src)
@@ -184,8 +187,9 @@ def certificate_centre_samples(program, size: int, brick: int, coarse: int = 4,
     return count
 
 
+@pytest.mark.parametrize("brick", [0, A.ISO_BAND_BRICK8])
 @pytest.mark.parametrize("flags", [A.ISO_BAND_NO_NETWORK, A.ISO_BAND_CERTIFY])
-def test_certified_band_is_dense(ctx, flags):
+def test_certified_band_is_dense(ctx, flags, brick):
     # The proofs alone (no network), or checking a network that predicts 1.0
     # everywhere, so that with no margin it culls every brick, surface ones
     # included: a brick leaves the band only by a proof, so none holds
@@ -200,24 +204,24 @@ def test_certified_band_is_dense(ctx, flags):
     size = 130
     desc = ctx.reference_desc(size, A.ISO_SOURCE_PROGRAM)
     if flags == A.ISO_BAND_CERTIFY:
-        _, unchecked = band.extract_mesh(desc, 0.0)
+        _, unchecked = band.extract_mesh(desc, 0.0, brick)
         assert unchecked.band_bricks == 0
         assert band.missed_cells(desc) > 0  # what the proofs must catch
-    mesh, stats = band.extract_mesh(desc, 0.0, flags)
+    mesh, stats = band.extract_mesh(desc, 0.0, flags | brick)
     missed = band.missed_cells(desc)
     dense = MarchingCubes(size, ctx).extract_mesh(src)
     print(f"band {stats.band_bricks}/{stats.bricks} exact {stats.exact_samples} "
           f"centres {stats.centre_samples} overturned {stats.overturned_culls} "
           f"missed {missed}", flush=True)
-    lo = certificate_centre_samples(src.program(), size, brick_size())
-    hi = certificate_centre_samples(src.program(), size, brick_size(), extra=2e-5)
+    lo = certificate_centre_samples(src.program(), size, brick_size(brick))
+    hi = certificate_centre_samples(src.program(), size, brick_size(brick), extra=2e-5)
     assert lo <= stats.centre_samples <= hi
     assert stats.centre_samples < stats.bricks
     assert 0 < stats.band_bricks < stats.bricks // 2
     assert stats.exact_samples < 0.7 * size * size * (size + 1)
     if flags == A.ISO_BAND_CERTIFY:
         assert stats.overturned_culls == stats.band_bricks  # every band brick was a network cull
-        _, plain = band.extract_mesh(desc, 0.0, A.ISO_BAND_NO_NETWORK)
+        _, plain = band.extract_mesh(desc, 0.0, A.ISO_BAND_NO_NETWORK | brick)
         assert (stats.band_bricks, stats.exact_samples) == (plain.band_bricks, plain.exact_samples)
     else:
         assert stats.overturned_culls == 0
@@ -262,8 +266,9 @@ def test_band_error_paths(ctx):
     desc = ctx.reference_desc(17, A.ISO_SOURCE_PROGRAM)
     assert lib.iso_extract_mc_band(ctx.handle, C.byref(desc), -1.0, 0, C.byref(out), None) == \
         A.ISO_ERR_INVALID_ARGUMENT
-    assert lib.iso_extract_mc_band(ctx.handle, C.byref(desc), 0.01, 4, C.byref(out), None) == \
+    assert lib.iso_extract_mc_band(ctx.handle, C.byref(desc), 0.01, 8, C.byref(out), None) == \
         A.ISO_ERR_INVALID_ARGUMENT
+    assert lib.iso_band_brick_size(0) == 4 and lib.iso_band_brick_size(A.ISO_BAND_BRICK8) == 8
     bad = np.zeros(10, np.float32)
     assert lib.iso_band_set_params(ctx.handle, bad.ctypes.data, 10) == A.ISO_ERR_INVALID_ARGUMENT
     p = np.zeros((8, 3), np.float32)
@@ -297,7 +302,8 @@ def test_certificate_at_a_far_origin(ctx):
     P = M.unpack(np.zeros(N_PARAMS, np.float32))
     P["b4"][:] = 1.0
     band.set_params(M.pack(P))
-    for flags in (A.ISO_BAND_NO_NETWORK, A.ISO_BAND_CERTIFY):
+    for flags in (A.ISO_BAND_NO_NETWORK, A.ISO_BAND_CERTIFY,
+                  A.ISO_BAND_CERTIFY | A.ISO_BAND_BRICK8):
         mesh, stats = band.extract_mesh(desc, 0.0, flags)
         missed = band.missed_cells(desc)
         print(f"flags {flags}: band {stats.band_bricks}/{stats.bricks} missed {missed}", flush=True)

@@ -62,6 +62,9 @@ def test_cpp_mirror_matches_oracle(tmp_path):
     ref = O.extract_dc(cube.program(), O.reference_desc(33, 0), 1e-6)
     assert np.fromfile(f"{out}.dc.f32", dtype="<f4").tobytes() == ref.vertices.tobytes()
     assert np.array_equal(np.fromfile(f"{out}.dc.u32", dtype="<u4"), ref.indices)
+    ref = O.extract_dc(cube.program(), O.reference_desc(33, 0), 0.0)
+    assert np.fromfile(f"{out}.dc_hermite.f32", dtype="<f4").tobytes() == ref.vertices.tobytes()
+    assert np.array_equal(np.fromfile(f"{out}.dc_hermite.u32", dtype="<u4"), ref.indices)
     ref = O.extract_lhmc(cube.program(), 6, 0)
     assert np.fromfile(f"{out}.lhmc.f32", dtype="<f4").tobytes() == ref.vertices.tobytes()
     assert np.array_equal(np.fromfile(f"{out}.lhmc.u32", dtype="<u4"), ref.indices)

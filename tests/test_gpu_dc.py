@@ -151,7 +151,7 @@ def test_dc_error_paths(ctx):
     assert lib.iso_extract_dc(ctx.handle, C.byref(d), C.c_float(1e-6), C.byref(out)) == \
         A.ISO_ERR_INVALID_ARGUMENT
     d = ctx.reference_desc(9, A.ISO_SOURCE_PROGRAM)
-    for bad in (0.0, -1e-6, float("nan"), float("inf")):
+    for bad in (-1e-6, float("nan"), float("inf")):  # 0: the program's own normals
         assert lib.iso_extract_dc(ctx.handle, C.byref(d), C.c_float(bad), C.byref(out)) == \
             A.ISO_ERR_INVALID_ARGUMENT
     # Host output buffers too small: the mesh is kept for iso_mesh_fetch.
@@ -160,5 +160,44 @@ def test_dc_error_paths(ctx):
     nv, ni = int(out.vertex_count), int(out.index_count)
     ref = O.extract_dc(src.program(), O.reference_desc(9), 1e-6)
     assert (nv, ni) == (len(ref.vertices), len(ref.indices))
+    with pytest.raises(A.IsoError):  # Difference has no analytic normals
+        DualContouring(9, MinimiseQEF(), ctx).extract_mesh(SOURCES["box_minus_sphere"]())
+
+
+HERMITE = ["sphere", "box", "cylinder", "torus", "boxes32"]
+
+
+@pytest.mark.parametrize("name", HERMITE)
+@pytest.mark.parametrize("size", [9, 24, 33])
+def test_dc_hermite_normals_match_oracle(ctx, name, size):
+    # DualContouring over a source that is its own HermiteSource (epsilon 0:
+    # the program's analytic normals, dual_contouring.rs:65-69, source.rs:43-48).
+    src = SOURCES[name]()
+    gpu = DualContouring(size, MinimiseQEF(), ctx).extract_mesh(src)
+    ref = O.extract_dc(src.program(), O.reference_desc(size), 0.0)
+    print(f"{name} size={size} V={len(ref.vertices)} T={len(ref.indices) // 3}", flush=True)
+    assert len(ref.vertices) > 0
+    assert_same_mesh(gpu, ref)
+
+
+def test_dc_hermite_needs_analytic_normals(ctx):
+    src = SOURCES["box_minus_sphere"]()  # Difference has no HermiteSource form
+    ctx.upload_program(src.program())
+    desc = ctx.reference_desc(9, A.ISO_SOURCE_PROGRAM)
+    out = A.IsoMeshOut(None, None, 0, 0, 0, 0, A.ISO_MEMORY_HOST, 0)
+    assert ctx.lib.iso_extract_dc(ctx.handle, C.byref(desc), C.c_float(0.0), C.byref(out)) == \
+        A.ISO_ERR_PROGRAM
+    assert ctx.lib.iso_extract_dc(ctx.handle, C.byref(desc), C.c_float(-1.0), C.byref(out)) == \
+        A.ISO_ERR_INVALID_ARGUMENT
     with pytest.raises(TypeError):
-        DualContouring(9, MinimiseQEF(), ctx).extract_mesh(src)
+        DualContouring(9, MinimiseQEF(), ctx).extract_mesh(object())
+
+
+@pytest.mark.parametrize("name", ["dc_hermite_sphere_r0.35_size33", "dc_hermite_boxes32_size128"])
+def test_dc_hermite_matches_golden_digests(ctx, name):
+    g = DIGESTS[name]
+    src = W.boxes_union(32, seed=4) if "boxes32" in name else W.sphere_source()
+    mesh = DualContouring(g["size"], MinimiseQEF(), ctx).extract_mesh(src)
+    assert len(mesh.vertices) == g["vertices"] and len(mesh.indices) == 3 * g["triangles"]
+    assert hashlib.sha256(mesh.vertices.astype("<f4").tobytes()).hexdigest() == g["vertices_sha256"]
+    assert hashlib.sha256(mesh.indices.astype("<u4").tobytes()).hexdigest() == g["indices_sha256"]

@@ -5,7 +5,9 @@ import numpy as np
 import pytest
 
 from isosurface_amd import workloads as W
-from isosurface_amd.sources import DemoSource, RectangularPrism, Sphere, Translate
+from isosurface_amd import _abi as A
+from isosurface_amd.sources import (DemoSource, RectangularPrism, Sphere, Torus, Translate,
+                                    Union)
 from oracle import oracle as O
 
 
@@ -193,3 +195,65 @@ def test_dc_vertices_hug_the_box_surface():
     sdf = np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(q.max(axis=1), 0)
     cell = 1.0 / (size - 1)
     assert np.all(np.abs(sdf) <= cell * np.sqrt(3) + 1e-6)
+
+
+@pytest.mark.parametrize("name,src,size", [
+    ("sphere", DemoSource(Sphere(0.3)), 17),
+    ("sphere", DemoSource(Sphere(0.3)), 33),
+    ("torus", DemoSource(Torus(0.28, 0.09)), 21)])
+def test_dc_hermite_normals_are_the_central_difference_limit(name, src, size):
+    # DualContouring over the program's own HermiteSource normals (epsilon 0)
+    # against CentralDifference normals on smooth shapes, whose exact
+    # gradients the analytic normals are: the topology depends on the signs
+    # alone and is the same; the vertices converge to the epsilon-0 ones as
+    # the step shrinks (until f32 cancellation takes over below 1e-3).
+    # A wrong analytic normal (sign, axis, ring radius) moves feature points
+    # by a good part of a cell.  (Union normals are the reference's
+    # componentwise minimum, csg.rs:47-51, not a gradient: no limit there.)
+    prog = src.program()
+    exact = O.extract_dc(prog, O.reference_desc(size), 0.0)
+    assert len(exact.vertices) > 0
+    cell = 1.0 / (size - 1)
+    dev = {}
+    for eps in (1e-2, 1e-3):
+        approx = O.extract_dc(prog, O.reference_desc(size), eps)
+        assert np.array_equal(exact.indices, approx.indices)
+        dev[eps] = np.abs(exact.vertices.astype(np.float64) - approx.vertices).max() / cell
+    print(f"{name} {size}: max vertex deviation {dev} cells", flush=True)
+    assert dev[1e-3] <= 1e-3
+    assert dev[1e-3] <= dev[1e-2]
+
+
+def test_dc_hermite_box_keeps_topology_and_hugs_the_surface():
+    # Exact box normals are face normals (clamp_to_cardinal_axis), where
+    # central differences give radial normals near the box's edges: the
+    # feature points differ, the sign-only topology does not, and the mesh
+    # vertices stay within a cell diagonal of the box.
+    h = np.float32([0.213, 0.177, 0.261])
+    size = 21
+    m = O.extract_dc(DemoSource(RectangularPrism(tuple(h))).program(), O.reference_desc(size),
+                     0.0)
+    cell = 1.0 / (size - 1)
+    q = np.abs(m.vertices.astype(np.float64) - 0.5) - h
+    sdf = np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(q.max(axis=1), 0)
+    assert np.all(np.abs(sdf) <= cell * np.sqrt(3) + 1e-6)
+    cd = O.extract_dc(DemoSource(RectangularPrism(tuple(h))).program(), O.reference_desc(size),
+                      1e-6)
+    q2 = np.abs(cd.vertices.astype(np.float64) - 0.5) - h
+    sdf2 = np.linalg.norm(np.maximum(q2, 0), axis=1) + np.minimum(q2.max(axis=1), 0)
+    assert np.array_equal(m.indices, cd.indices)
+    print(f"box: mean |sdf| hermite {np.abs(sdf).mean() / cell:.3e}, central "
+          f"{np.abs(sdf2).mean() / cell:.3e} cells", flush=True)
+
+
+def test_dc_hermite_rejects_programs_without_normals():
+    import ctypes as C
+    from isosurface_amd.sources import Difference
+    lib = O.load()
+    O._dc_bind(lib)
+    prog = DemoSource(Difference(Sphere(0.3), RectangularPrism((0.2, 0.2, 0.2)))).program()
+    m = O.OrMesh()
+    d = O.reference_desc(9)
+    assert lib.or_extract_dc(O.nodes(prog), len(prog), C.byref(d), 0.0, C.byref(m)) == \
+        A.ISO_ERR_PROGRAM
+    lib.or_mesh_free(C.byref(m))

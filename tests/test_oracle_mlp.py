@@ -39,7 +39,13 @@ def test_feature_closed_forms():
     # k = 2: w = 2 pi -> cos(2 pi * 0.5) = -1
     assert abs(x[3 + 12 + 3 + 1] + 1.0) < 1e-6
     assert np.all(x[M.N_FEAT:] == 0.0)
-    assert M.FREQS[1] == np.float32(np.pi * np.sqrt(2.0))
+    # c_k = 2^(k/2) / 2 revolutions per unit: 1/2 (k = 0), sqrt(2)/2 (k = 1), 64 (k = 14)
+    assert M.REVS[0] == 0.5 and M.REVS[14] == 64.0
+    assert M.REVS[1] == np.float32(np.sqrt(2.0) / 2.0)
+    # k = 1: w = pi sqrt(2); at p = 1/sqrt(2) the angle is pi: sin = 0, cos = -1
+    q = np.array([[np.float32(1.0 / np.sqrt(2.0)), 0.0, 0.0]], np.float32)
+    y = M.features(q)[0]
+    assert abs(y[3 + 6]) < 1e-6 and abs(y[6 + 6] + 1.0) < 1e-6
 
 
 def test_parameter_layout():

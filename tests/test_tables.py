@@ -62,3 +62,55 @@ def test_triangles_are_nondegenerate():
         row = [int(v) for v in t[case] if v >= 0]
         for k in range(0, len(row), 3):
             assert len(set(row[k:k + 3])) == 3
+
+
+def _faces_and_antipodes():
+    faces = [sum(1 << k for k in range(8) if CORNER[k][a] == v) for a in range(3) for v in (0, 1)]
+    anti = {k: CORNER.index(tuple(1 - c for c in CORNER[k])) for k in range(8)}
+    return faces, anti
+
+
+def test_triangle_count_closed_form():
+    # The count and single-pass kernels take a cell's triangle count from
+    # its corners, V - 2F - 4H (V corners inside, F faces with all four
+    # corners inside, H = 1 when the two corners outside are antipodal),
+    # instead of the table: it must equal the table's count in every mixed case.
+    t = O.tri_table()
+    faces, anti = _faces_and_antipodes()
+    for case in range(1, 255):
+        n = int(np.count_nonzero(t[case] >= 0)) // 3
+        v = bin(case).count("1")
+        f = sum(1 for m in faces if case & m == m)
+        out = [k for k in range(8) if not (case >> k) & 1]
+        h = int(len(out) == 2 and anti[out[0]] == out[1])
+        assert n == v - 2 * f - 4 * h, case
+
+
+def test_triangle_count_gate_sequence():
+    # The kernels' bit-sliced evaluation of V - 2F - 4H (mcd::tri_planes),
+    # run here with one bit per case instead of one per cell.
+    full = (1 << 256) - 1
+    c = [sum(((case >> k) & 1) << case for case in range(256)) for k in range(8)]
+
+    def full_add(a, b, d):
+        x = a ^ b
+        return x ^ d, (a & b) | (x & d)
+
+    s1, k1 = full_add(c[0], c[1], c[2])
+    s2, k2 = full_add(c[3], c[4], c[5])
+    v0, k4 = full_add(s1, s2, c[6] ^ c[7])
+    twos, k5 = full_add(k1, k2, c[6] & c[7])
+    v1, v2 = twos ^ k4, k5 ^ (twos & k4)
+    c04, c37, c15, c26 = c[0] & c[4], c[3] & c[7], c[1] & c[5], c[2] & c[6]
+    sa, ka = full_add(c04 & c37, c15 & c26, c04 & c15)
+    sb, kb = full_add(c26 & c37, c[0] & c[1] & c[2] & c[3], c[4] & c[5] & c[6] & c[7])
+    f0, f1 = sa ^ sb, ka ^ kb ^ (sa & sb)
+    inv = lambda x: full & ~x  # noqa: E731
+    h = v2 & v1 & inv(v0) & (inv(c[0] | c[6]) | inv(c[1] | c[7]) | inv(c[2] | c[4]) |
+                             inv(c[3] | c[5]))
+    d0, d1 = v1 ^ f0, v2 ^ f1 ^ (inv(v1) & f0)
+    planes = [v0, d0, d1 ^ h]
+    t = O.tri_table()
+    for case in range(1, 255):
+        got = sum(((planes[i] >> case) & 1) << i for i in range(3))
+        assert got == int(np.count_nonzero(t[case] >= 0)) // 3, case
