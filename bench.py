@@ -471,15 +471,32 @@ def batched_chunks(ctx: Context, steps: int) -> dict:
         call()
     ctx.synchronize()
     dt = (time.perf_counter() - t0) / steps
+    # One eager pass (same tuned shapes) with per-kernel events; graph
+    # replays record none.
+    flags = A.ISO_BATCH_AUTOTUNE
+    ctx.reset_kernel_stats()
+    ctx.set_timing(True)
+    call()
+    ctx.synchronize()
+    ctx.set_timing(False)
+    kus = {k: round(1e3 * ms / max(c, 1), 2) for k, (c, ms) in ctx.kernel_stats().items()}
+    ctx.reset_kernel_stats()
     ctx.lib.iso_device_free(ctx.handle, v)
     ctx.lib.iso_device_free(ctx.handle, i)
-    log(f"batched chunks: {1e3 * dt:.3f} ms per batch, parity {parity}")
+    nvo, nio = int(out.vertex_count), int(out.index_count)
+    # emit's algorithmic bytes: the mesh written, and the two stored corner
+    # samples each vertex interpolates between
+    emit_bytes = 12 * nvo + 4 * nio + 8 * nvo
+    emit_us = kus.get("batch_emit")
+    log(f"batched chunks: {1e3 * dt:.3f} ms per batch, parity {parity}, kernels {kus}")
     return {"workload": ("4096 chunks, S^3 cells with S uniform in {16,32,48,64}, each a seeded "
                          "1-8 sphere/box/torus CSG program (seed 6); one HIP graph replay per "
                          "batch, launch shapes autotuned on the first extraction"),
             "cells": cells, "ms_per_batch": 1e3 * dt, "cells_per_s": cells / dt,
-            "first_call_ms": 1e3 * first, "vertices": int(out.vertex_count),
-            "triangles": int(out.index_count) // 3, "parity_vs_oracle": parity}
+            "first_call_ms": 1e3 * first, "vertices": nvo,
+            "triangles": nio // 3, "parity_vs_oracle": parity, "kernels_us": kus,
+            "emit_algorithmic_bytes": emit_bytes,
+            "emit_GBps": emit_bytes / (emit_us * 1e-6) / 1e9 if emit_us else None}
 
 
 def brick_centres(size: int) -> np.ndarray:
