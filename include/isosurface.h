@@ -269,7 +269,8 @@ int iso_extract_dc(iso_ctx* ctx, const iso_grid_desc* desc, float epsilon, iso_m
  * the reference's for the primal-vertex visiting order "ascending Morton
  * key" (the reference visits them in std HashMap order, which differs from
  * run to run); Morton::level()'s truncated-constant quirk is reproduced.
- * max_depth in [0, 9]. */
+ * max_depth in [0, 20] (nodes reach depth max_depth + 1, whose Morton keys
+ * use 64 bits at 20). */
 int iso_extract_lhmc(iso_ctx* ctx, int32_t max_depth, iso_mesh_out* out);
 
 /* ---- batched chunks ---------------------------------------------------------

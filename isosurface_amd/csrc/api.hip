@@ -81,9 +81,11 @@ struct iso_ctx {
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
-  // octree marching cubes: nodes, dense primal lattice, cubes, edge table, scan state
+  // octree marching cubes: nodes, dense primal lattice (or the sorted path's
+  // pairs and sort scratch), cubes, edge table, scan state
   DevBuf oc_keys, oc_values, oc_leaf, oc_dense, oc_primal, oc_plevel, oc_corner, oc_case,
-      oc_base, oc_tkeys, oc_towner, oc_tid, sc_slot, sc_ctl;
+      oc_base, oc_tkeys, oc_towner, oc_tid, sc_slot, sc_ctl, oc_pkeys, oc_plevels, oc_skeys,
+      oc_slevels, oc_sort;
   unsigned long long* h_stage = nullptr;  // pinned, 8 words
 
   // batched chunks: lowered programs, chunk table, sign words, offsets
@@ -363,6 +365,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->oc_keys,    &c->oc_values,  &c->oc_leaf,   &c->oc_dense, &c->oc_primal,
                     &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
                     &c->oc_towner,  &c->oc_tid,     &c->sc_slot,   &c->sc_ctl,
+                    &c->oc_pkeys,   &c->oc_plevels, &c->oc_skeys,  &c->oc_slevels, &c->oc_sort,
                     &c->b_ops,      &c->b_chunks,   &c->b_segstart, &c->b_wordstart,
                     &c->b_signs,    &c->b_voff,     &c->b_ioff,    &c->b_values,
                     &c->nn_theta,   &c->nn_m,
@@ -1419,7 +1422,7 @@ int iso_extract_lhmc(iso_ctx* c, int32_t max_depth, iso_mesh_out* out) {
   if (!c || !out) return fail(c, ISO_ERR_INVALID_ARGUMENT, "null argument");
   if (c->n_prog == 0) return fail(c, ISO_ERR_NO_SOURCE, "no program uploaded");
   if (max_depth < 0 || max_depth > iso::kLhmcMaxDepth)
-    return fail(c, ISO_ERR_INVALID_ARGUMENT, "max_depth must be in [0, 9]");
+    return fail(c, ISO_ERR_INVALID_ARGUMENT, "max_depth must be in [0, 20]");
   if (out->memory != ISO_MEMORY_HOST && out->memory != ISO_MEMORY_DEVICE)
     return fail(c, ISO_ERR_INVALID_ARGUMENT, "unknown output memory kind");
   c->kept = false;
@@ -1427,9 +1430,15 @@ int iso_extract_lhmc(iso_ctx* c, int32_t max_depth, iso_mesh_out* out) {
   int st = ISO_OK;
   const iso::DevOp* ops = static_cast<const iso::DevOp*>(c->prog.ptr);
   const int d1 = (max_depth > 2 ? max_depth : 2) + 1;  // deepest possible node
-  const uint64_t n_dense = 1ull << (3 * d1);
+  const bool dense_primal = d1 <= iso::kLhmcDenseDepth;
+  const uint64_t n_dense = dense_primal ? 1ull << (3 * d1) : 0;
+  // Morton::level's split points, found once with the host's libm log
+  static uint64_t splits[iso::kLhmcMaxLevels];
+  static std::once_flag splits_once;
+  std::call_once(splits_once, [] { iso::lhmc_level_splits(splits); });
+  ISO_HIP(c, iso::launch_lhmc_levels(splits, c->stream));
   iso::scan::State S{};
-  if ((st = prepare_scan(c, n_dense, &S)) != ISO_OK) return st;
+  if ((st = prepare_scan(c, 1u << 16, &S)) != ISO_OK) return st;
   // status word: cleared once, accumulated by every kernel below
   ISO_HIP(c, iso::launch_fill_u32(reinterpret_cast<uint32_t*>(S.err), 2, 0u, c->stream));
 
@@ -1455,6 +1464,7 @@ int iso_extract_lhmc(iso_ctx* c, int32_t max_depth, iso_mesh_out* out) {
         (st = grow_keep(c, &c->oc_values, need * 4, static_cast<size_t>(off[d]) * 4)) != ISO_OK ||
         (st = grow_keep(c, &c->oc_leaf, need, static_cast<size_t>(off[d]))) != ISO_OK)
       return st;
+    if ((st = prepare_scan(c, static_cast<uint64_t>(n), &S)) != ISO_OK) return st;
     ISO_HIP(c, timed(c, "lhmc_level", [&] {
               return iso::launch_lhmc_level(
                   ops, c->n_prog, max_depth, static_cast<uint64_t*>(c->oc_keys.ptr),
@@ -1477,17 +1487,37 @@ int iso_extract_lhmc(iso_ctx* c, int32_t max_depth, iso_mesh_out* out) {
 
   // 2. primal vertices in ascending Morton order
   iso::LhmcCubes C{};
-  uint64_t pv_cap = 8 * static_cast<uint64_t>(n_nodes);
-  if (pv_cap > n_dense) pv_cap = n_dense;
-  ISO_HIP(c, ensure(&c->oc_dense, n_dense * sizeof(uint32_t)));
+  const uint64_t n_pairs = 8 * static_cast<uint64_t>(n_nodes);
+  uint64_t pv_cap = n_pairs;
+  if (dense_primal && pv_cap > n_dense) pv_cap = n_dense;
   ISO_HIP(c, ensure(&c->oc_primal, pv_cap * sizeof(uint64_t)));
   ISO_HIP(c, ensure(&c->oc_plevel, pv_cap));
   C.primal = static_cast<uint64_t*>(c->oc_primal.ptr);
   C.primal_level = static_cast<uint8_t*>(c->oc_plevel.ptr);
-  ISO_HIP(c, timed(c, "lhmc_primal", [&] {
-            return iso::launch_lhmc_primal(N, d1, static_cast<uint32_t*>(c->oc_dense.ptr),
-                                           n_dense, C, pv_cap, S, c->stream);
-          }));
+  if ((st = prepare_scan(c, dense_primal ? n_dense : n_pairs, &S)) != ISO_OK) return st;
+  if (dense_primal) {
+    ISO_HIP(c, ensure(&c->oc_dense, n_dense * sizeof(uint32_t)));
+    ISO_HIP(c, timed(c, "lhmc_primal", [&] {
+              return iso::launch_lhmc_primal(N, d1, static_cast<uint32_t*>(c->oc_dense.ptr),
+                                             n_dense, C, pv_cap, S, c->stream);
+            }));
+  } else {
+    const size_t temp = iso::lhmc_sort_temp_bytes(n_pairs);
+    if (temp == 0) return fail(c, ISO_ERR_DEVICE, "radix sort size query failed");
+    ISO_HIP(c, ensure(&c->oc_pkeys, n_pairs * sizeof(uint64_t)));
+    ISO_HIP(c, ensure(&c->oc_skeys, n_pairs * sizeof(uint64_t)));
+    ISO_HIP(c, ensure(&c->oc_plevels, n_pairs));
+    ISO_HIP(c, ensure(&c->oc_slevels, n_pairs));
+    ISO_HIP(c, ensure(&c->oc_sort, temp));
+    ISO_HIP(c, timed(c, "lhmc_primal", [&] {
+              return iso::launch_lhmc_primal_sorted(
+                  N, static_cast<uint64_t*>(c->oc_pkeys.ptr),
+                  static_cast<uint8_t*>(c->oc_plevels.ptr),
+                  static_cast<uint64_t*>(c->oc_skeys.ptr),
+                  static_cast<uint8_t*>(c->oc_slevels.ptr), c->oc_sort.ptr, temp, C, pv_cap, S,
+                  c->stream);
+            }));
+  }
   if ((st = read_ctl(c, S)) != ISO_OK) return st;
   const uint64_t n_pv = c->h_stage[1];
   if (n_pv > pv_cap) return fail(c, ISO_ERR_DEVICE, "primal vertex list overflow");

@@ -19,12 +19,17 @@
 // The output is the reference's for the primal-vertex order "ascending
 // Morton key" (the reference's own order is a std HashMap's, random per
 // process).  Morton::level() keeps the reference's quirk: the truncated
-// constant 0.48089834696 (morton.rs:35, 54-59) puts every exact power of 8
-// one level low.  Below 8^11 that is exactly the integer rule used here,
-// which is why the depth is limited.  A node is refined while its level is
+// constant 0.48089834696 (morton.rs:35, 54-59) puts the first keys of each
+// depth one level low (exactly the power of 8 below 8^11, then more: a
+// table of split points found with the host's libm log).  Deep trees
+// (beyond kLhmcDenseDepth) list their primal vertices by a radix sort of
+// the leaves' (vertex, level) pairs instead of a dense lattice.  A node is refined while its level is
 // below M = max(2, max_depth); the origin-corner node of depth M has level
 // M - 1, so nodes reach depth M + 1, the "dense depth" of the primal lattice.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "lhmc_kernels.h"
 #include "mc_tables.h"
@@ -49,15 +54,17 @@ constexpr float kSqrtOf3 = 1.73205080757f;  // distance.rs:18
 constexpr unsigned long long kEmpty = ~0ull;
 
 __constant__ tables::McTables c_lt = tables::kTables;
+__constant__ uint64_t c_level_split[kLhmcMaxLevels];  // lhmc_level_splits
 __constant__ int c_remap[8] = {2, 3, 1, 0, 6, 7, 5, 4};  // REMAP_CUBE, marching_cubes_tables.rs:29
 
 __device__ __forceinline__ int depth_of(uint64_t k) { return (63 - __clzll(k)) / 3; }
 
-// Morton::level() for keys below 8^11.
+// Morton::level() (morton.rs:53-59): the depth, one less for the first
+// c_level_split[d] keys of depth d.
 __device__ __forceinline__ int level_of(uint64_t k) {
   if (k == 0) return 0;
   const int d = depth_of(k);
-  return (d >= 1 && k == (1ull << (3 * d))) ? d - 1 : d;
+  return (d >= 1 && k - (1ull << (3 * d)) < c_level_split[d]) ? d - 1 : d;
 }
 
 __device__ __forceinline__ float size_of(uint64_t k) {  // morton.rs:73-75
@@ -237,10 +244,60 @@ struct PrimalSink {
   uint64_t* primal;
   uint8_t* level;
   uint64_t capacity;
+  int d1;
   __device__ void operator()(uint64_t i, unsigned long long v, unsigned long long ex) const {
     if (!v || ex >= capacity) return;
-    primal[ex] = i;
+    // the lattice index back to the vertex's Morton key at MAX_LEVEL
+    primal[ex] = (i | (1ull << (3 * d1))) << (3 * (kMaxLevel - d1));
     level[ex] = static_cast<uint8_t>(dense[i] - 1);
+  }
+};
+
+// Sorted path: every node's eight primal vertices as (key, level) pairs;
+// a node that is not a leaf, or a vertex outside the domain, writes key 0
+// (every primal key has bit 63 set, so 0 sorts first and marks none).
+__global__ __launch_bounds__(256) void k_primal_pairs(LhmcNodes N, int64_t n_nodes,
+                                                      uint64_t* __restrict__ keys,
+                                                      uint8_t* __restrict__ levels) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n_nodes;
+       i += stride) {
+    const bool leaf = N.leaf[i] != 0;
+    const uint64_t key = N.keys[i];
+    const int lv = level_of(key);
+    for (unsigned w = 0; w < 8; ++w) {
+      const uint64_t v = leaf ? primal_vertex(key, lv, w) : 0ull;
+      keys[8 * i + w] = v;
+      levels[8 * i + w] = static_cast<uint8_t>(lv);
+    }
+  }
+}
+
+// The last pair of each run of equal nonzero keys (a vertex is shared by at
+// most eight leaves), listed with the run's deepest level.
+struct RunLoad {
+  const uint64_t* keys;
+  uint64_t n;
+  __device__ unsigned long long operator()(uint64_t i) const {
+    const uint64_t k = keys[i];
+    return (k != 0 && (i + 1 == n || keys[i + 1] != k)) ? 1ull : 0ull;
+  }
+};
+
+struct RunSink {
+  const uint64_t* keys;
+  const uint8_t* levels;
+  uint64_t* primal;
+  uint8_t* level;
+  uint64_t capacity;
+  __device__ void operator()(uint64_t i, unsigned long long v, unsigned long long ex) const {
+    if (!v || ex >= capacity) return;
+    const uint64_t k = keys[i];
+    uint8_t lv = levels[i];
+    for (uint64_t j = i; j > 0 && keys[j - 1] == k; --j)
+      if (levels[j - 1] > lv) lv = levels[j - 1];
+    primal[ex] = k;
+    level[ex] = lv;
   }
 };
 
@@ -251,7 +308,7 @@ struct CubeLoad {
   int d1;
   unsigned long long* err;
   __device__ unsigned long long operator()(uint64_t p) const {
-    const uint64_t key = (C.primal[p] | (1ull << (3 * d1))) << (3 * (kMaxLevel - d1));
+    const uint64_t key = C.primal[p];
     const int lv = C.primal_level[p];
     uint32_t cube = 0;
     for (int i = 0; i < 8; ++i) {
@@ -429,8 +486,65 @@ hipError_t launch_lhmc_primal(const LhmcNodes& N, int dense_depth, uint32_t* den
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   PrimalLoad load{dense};
-  PrimalSink sink{dense, C.primal, C.primal_level, capacity};
+  PrimalSink sink{dense, C.primal, C.primal_level, capacity, dense_depth};
   return scan::launch<scan::kItems>(n_dense, load, sink, st, s);
+}
+
+size_t lhmc_sort_temp_bytes(uint64_t n_pairs) {
+  size_t bytes = 0;
+  if (rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const uint64_t*>(nullptr),
+                                static_cast<uint64_t*>(nullptr),
+                                static_cast<const uint8_t*>(nullptr),
+                                static_cast<uint8_t*>(nullptr), static_cast<size_t>(n_pairs), 0,
+                                64) != hipSuccess)
+    return 0;
+  return bytes;
+}
+
+hipError_t launch_lhmc_primal_sorted(const LhmcNodes& N, uint64_t* pair_keys,
+                                     uint8_t* pair_levels, uint64_t* sorted_keys,
+                                     uint8_t* sorted_levels, void* temp, size_t temp_bytes,
+                                     const LhmcCubes& C, uint64_t capacity,
+                                     const scan::State& st, hipStream_t s) {
+  const int64_t n_nodes = N.level_off[N.n_levels];
+  const uint64_t n_pairs = 8 * static_cast<uint64_t>(n_nodes);
+  hipLaunchKernelGGL(k_primal_pairs, dim3(grid_of(static_cast<uint64_t>(n_nodes))), dim3(256), 0,
+                     s, N, n_nodes, pair_keys, pair_levels);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  size_t bytes = temp_bytes;
+  e = rocprim::radix_sort_pairs(temp, bytes, static_cast<const uint64_t*>(pair_keys), sorted_keys,
+                                static_cast<const uint8_t*>(pair_levels), sorted_levels,
+                                static_cast<size_t>(n_pairs), 0, 64, s);
+  if (e != hipSuccess) return e;
+  RunLoad load{sorted_keys, n_pairs};
+  RunSink sink{sorted_keys, sorted_levels, C.primal, C.primal_level, capacity};
+  return scan::launch<scan::kItems>(n_pairs, load, sink, st, s);
+}
+
+void lhmc_level_splits(uint64_t split[kLhmcMaxLevels]) {
+  // Morton::level, morton.rs:53-59 (the same libm log, f64)
+  auto level = [](uint64_t a) {
+    return a == 0 ? 0 : static_cast<int>(std::floor(std::log(static_cast<double>(a)) * 0.48089834696));
+  };
+  split[0] = 0;
+  for (int d = 1; d < kLhmcMaxLevels; ++d) {
+    const uint64_t base = 1ull << (3 * d);
+    // the first offset k at which level(base + k) reaches d (level is
+    // monotone in the key, and d at the depth's last key, offset hi)
+    uint64_t lo = 0, hi = (d == 21) ? ~0ull - base : 7 * base - 1;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (level(base + mid) >= d) hi = mid;
+      else lo = mid + 1;
+    }
+    split[d] = lo;
+  }
+}
+
+hipError_t launch_lhmc_levels(const uint64_t split[kLhmcMaxLevels], hipStream_t s) {
+  return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_level_split), split,
+                                kLhmcMaxLevels * sizeof(uint64_t), 0, hipMemcpyHostToDevice, s);
 }
 
 hipError_t launch_lhmc_cubes(const LhmcNodes& N, int dense_depth, const LhmcCubes& C,

@@ -587,11 +587,11 @@ class ExtendedMarchingCubes:
 class LinearHashedMarchingCubes:
     """LinearHashedMarchingCubes::new(max_depth)
     (linear_hashed_marching_cubes.rs:51-56) on the GPU.  The source is
-    sampled over the unit cube at octree node centres; max_depth <= 9."""
+    sampled over the unit cube at octree node centres; max_depth <= 20."""
 
     def __init__(self, max_depth: int, context: Optional[Context] = None):
-        if not 0 <= int(max_depth) <= 9:
-            raise ValueError("max_depth must be in [0, 9]")
+        if not 0 <= int(max_depth) <= 20:
+            raise ValueError("max_depth must be in [0, 20]")
         self.max_depth = int(max_depth)
         self.ctx = context or default_context()
         self._last_counts = (0, 0)

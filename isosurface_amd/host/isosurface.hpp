@@ -517,10 +517,10 @@ class DualContouring {
 
 class LinearHashedMarchingCubes {
  public:
-  // LinearHashedMarchingCubes::new(max_depth), max_depth in [0, 9].
+  // LinearHashedMarchingCubes::new(max_depth), max_depth in [0, 20].
   LinearHashedMarchingCubes(int max_depth, Context& ctx) : depth_(max_depth), ctx_(ctx) {
-    if (max_depth < 0 || max_depth > 9)
-      throw Error(ISO_ERR_INVALID_ARGUMENT, "max_depth must be in [0, 9]");
+    if (max_depth < 0 || max_depth > 20)
+      throw Error(ISO_ERR_INVALID_ARGUMENT, "max_depth must be in [0, 20]");
   }
 
   template <class S>

@@ -173,7 +173,7 @@ int or_lhmc_octree(const iso_node* prog, uint32_t n, int max_depth, uint64_t cap
                    uint64_t* keys, float* values, uint8_t* is_leaf, uint64_t* n_nodes) {
   orc::Program p;
   if (!orc::load_program(prog, n, &p)) return ISO_ERR_PROGRAM;
-  if (max_depth < 0 || max_depth > 10) return ISO_ERR_INVALID_ARGUMENT;
+  if (max_depth < 0 || max_depth > 20) return ISO_ERR_INVALID_ARGUMENT;  // keys fit 64 bits
   Octree t;
   build(p, max_depth, &t);
   *n_nodes = t.order.size();
@@ -195,7 +195,7 @@ int or_extract_lhmc(const iso_node* prog, uint32_t n, int max_depth, uint64_t or
   std::memset(out, 0, sizeof(*out));
   orc::Program p;
   if (!orc::load_program(prog, n, &p)) return ISO_ERR_PROGRAM;
-  if (max_depth < 0 || max_depth > 10) return ISO_ERR_INVALID_ARGUMENT;
+  if (max_depth < 0 || max_depth > 20) return ISO_ERR_INVALID_ARGUMENT;  // keys fit 64 bits
   Octree t;
   build(p, max_depth, &t);
 

@@ -109,7 +109,7 @@ def test_lhmc_error_paths(ctx):
     lib = ctx.lib
     out = A.IsoMeshOut(None, None, 0, 0, 0, 0, A.ISO_MEMORY_HOST, 0)
     ctx.upload_program(W.sphere_source().program())
-    for bad in (-1, 10, 64):
+    for bad in (-1, 21, 64):
         assert lib.iso_extract_lhmc(ctx.handle, bad, C.byref(out)) == A.ISO_ERR_INVALID_ARGUMENT
     assert lib.iso_extract_lhmc(ctx.handle, 4, C.byref(out)) == A.ISO_ERR_CAPACITY
     ref = O.extract_lhmc(W.sphere_source().program(), 4, 0)
@@ -120,4 +120,43 @@ def test_lhmc_error_paths(ctx):
     finally:
         fresh.close()
     with pytest.raises(ValueError):
-        LinearHashedMarchingCubes(10, ctx)
+        LinearHashedMarchingCubes(21, ctx)
+
+
+SQRT3 = float(np.sqrt(3.0))
+DEEP = {
+    # a sphere through the domain's origin corner: the octree refines down
+    # the corner nodes, whose keys lie at each depth's level split (the first
+    # keys of depth d have level d - 1: 1 key below depth 11, then 2, 11, 93,
+    # ... 58185 at depth 16)
+    "origin_sphere": [(0.05, 9), (0.02, 10), (0.01, 11), (0.004, 12), (0.002, 13),
+                      (0.0005, 16)],
+    # a small sphere away from the corner, refined to the deepest depth
+    "small_sphere": [(1e-3, 14), (1e-4, 17), (2e-5, 20)],
+}
+
+
+@pytest.mark.parametrize("name,scale,depth", [(n, s, d) for n, v in DEEP.items() for s, d in v])
+def test_lhmc_deep_trees_match_oracle(ctx, name, scale, depth):
+    # max_depth 9 and beyond: primal vertices by the sorted path, levels by
+    # the split table; bit for bit against the oracle (libm log level()).
+    if name == "origin_sphere":
+        src = Translate((scale, scale, scale), Sphere(scale * SQRT3))
+    else:
+        src = Translate((0.3, 0.4, 0.6), Sphere(scale))
+    gpu = LinearHashedMarchingCubes(depth, ctx).extract_mesh(src)
+    ref = O.extract_lhmc(src.program(), depth, 0)
+    print(f"{name} r={scale} depth={depth} V={len(ref.vertices)} T={len(ref.indices) // 3}",
+          flush=True)
+    assert len(ref.vertices) > 0
+    assert_same_mesh(gpu, ref)
+
+
+def test_lhmc_dense_and_sorted_primal_paths_agree(ctx):
+    # max_depth 8 lists primal vertices on a dense lattice and 9 by sorting;
+    # the same source through both equals the oracle at each depth
+    src = SOURCES["cyl_union"]()
+    for depth in (8, 9):
+        gpu = LinearHashedMarchingCubes(depth, ctx).extract_mesh(src)
+        ref = O.extract_lhmc(src.program(), depth, 0)
+        assert_same_mesh(gpu, ref)

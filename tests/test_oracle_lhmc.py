@@ -274,3 +274,43 @@ def test_lhmc_sphere_is_closed():
     assert len(m.vertices) - len(counts) + len(t) == 2
     r = np.linalg.norm(m.vertices.astype(np.float64) - 0.5, axis=1)
     assert np.all(np.abs(r - 0.35) < 1.0 / 64)
+
+
+def oracle_split(lib, d):
+    """The first offset k of depth d's keys [8^d, 8^(d+1)) at which the
+    oracle's level() reaches d (bisection; level() is monotone)."""
+    base = 8 ** d
+    lo, hi = 0, (2 ** 64 - 1 - base) if d == 21 else 7 * base - 1
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if lib.or_morton_level(base + mid) >= d:
+            hi = mid
+        else:
+            lo = mid + 1
+    return lo
+
+
+def test_level_split_points_follow_the_truncated_constant():
+    # level(a) = floor(ln(a) c) with c = 0.48089834696 < 1/(3 ln 2): the keys of
+    # depth d below exp(d / c) = 8^d exp(d (1/c - 3 ln 2)) sit one level low.
+    # The oracle's split points (with libm's f64 log) must sit at that bound:
+    # exactly up to depth 16, within the f64 rounding of ln(a) c beyond.
+    from decimal import ROUND_CEILING, Decimal, getcontext
+    getcontext().prec = 60
+    c = Decimal("0.48089834696")
+    lib = O.morton()
+    splits = []
+    for d in range(1, 22):
+        base = 8 ** d
+        bound = int(((Decimal(d) / c).exp() - base).to_integral_value(rounding=ROUND_CEILING))
+        got = oracle_split(lib, d)
+        splits.append(got)
+        assert lib.or_morton_level(base + got) == d
+        assert lib.or_morton_level(base + got - 1) == d - 1
+        assert lib.or_morton_level(8 * base - 1 if d < 21 else 2 ** 64 - 1) == d
+        if d <= 16:
+            assert got == bound, d
+        else:
+            assert abs(got - bound) <= base >> 40, d
+    # one key (the power of 8) per depth below 8^11, then growing runs
+    assert splits[:10] == [1] * 10 and splits[10:13] == [2, 11, 93]
