@@ -165,7 +165,8 @@ struct BatchPlan {
 // Count workgroups of a batch at `rounds` segments per thread (1 or 2).
 int64_t batch_count_blocks(const BatchPlan& P, int rounds);
 // Program sign bits of every chunk (and the extraction's state reset, sized
-// for the finest count tile); `words_per_wave` sign words per wavefront.
+// for the finest count tile); the grid gives each wavefront about
+// words_per_wave runs of consecutive sign words.
 hipError_t launch_batch_signs(const BatchPlan& P, const McBuffers& B, int words_per_wave,
                               hipStream_t s);
 hipError_t launch_batch_count(const BatchPlan& P, const McBuffers& B, int rounds,

@@ -159,11 +159,15 @@ k_signs_directed(const DevOp* __restrict__ ops, int n_ops,
   }
 }
 
-// Batched chunks: one wavefront per sign word of the concatenated word
-// numbering; the word's chunk is found by binary search, so a wavefront
-// always evaluates a single chunk's program.  Thread i < n_chunks also
+// Batched chunks: a wavefront takes runs of kBatchRun consecutive sign words
+// of the concatenated word numbering; it finds the first word's chunk by
+// binary search and steps to the next chunk as the run crosses into it, so
+// each word is evaluated with a single chunk's program and the search (a
+// chain of dependent loads) is paid once per run.  Thread i < n_chunks also
 // clears chunk i's guard words.  A chunk with a `grid` buffer gets its
 // samples stored there for emit.
+constexpr int kBatchRun = 8;
+
 __global__ __launch_bounds__(256) void k_batch_signs(const CellSpace* __restrict__ chunks,
                                                      const uint64_t* __restrict__ word_start,
                                                      uint32_t n_chunks, ResetArgs R) {
@@ -180,34 +184,39 @@ __global__ __launch_bounds__(256) void k_batch_signs(const CellSpace* __restrict
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
   const int64_t n_words = static_cast<int64_t>(word_start[n_chunks]);
-  for (int64_t w = wave; w < n_words; w += n_waves) {
-    uint32_t lo = 0, hi = n_chunks;  // last chunk whose first word <= w
+  for (int64_t w0 = wave * kBatchRun; w0 < n_words; w0 += n_waves * kBatchRun) {
+    uint32_t lo = 0, hi = n_chunks;  // last chunk whose first word <= w0
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (static_cast<int64_t>(word_start[mid]) <= w) lo = mid;
+      if (static_cast<int64_t>(word_start[mid]) <= w0) lo = mid;
       else hi = mid;
     }
-    const CellSpace& C = chunks[lo];
-    const int64_t lw = w - static_cast<int64_t>(word_start[lo]);
-    const int64_t f = lw * 64 + lane;
-    const int64_t n_points = C.nx * C.ny * C.nz;
-    bool inside = false;
-    if (f < n_points) {
-      const uint32_t nx = static_cast<uint32_t>(C.nx);
-      const uint32_t nxy = static_cast<uint32_t>(C.nx * C.ny);
-      const uint32_t ff = static_cast<uint32_t>(f);
-      const uint32_t z = ff / nxy;
-      const uint32_t r = ff - z * nxy;
-      const uint32_t y = r / nx;
-      const uint32_t x = r - y * nx;
-      const float v = sdf_eval(C.ops, C.n_ops, C.origin[0] + static_cast<float>(x) * C.spacing[0],
-                               C.origin[1] + static_cast<float>(y) * C.spacing[1],
-                               C.origin[2] + static_cast<float>(z) * C.spacing[2]);
-      if (C.grid) const_cast<float*>(C.grid)[f] = v;
-      inside = !(v > 0.0f);
+    const int64_t w1 = w0 + kBatchRun < n_words ? w0 + kBatchRun : n_words;
+    for (int64_t w = w0; w < w1; ++w) {
+      while (lo + 1 < n_chunks && static_cast<int64_t>(word_start[lo + 1]) <= w) ++lo;
+      const CellSpace& C = chunks[lo];
+      const int64_t lw = w - static_cast<int64_t>(word_start[lo]);
+      const int64_t f = lw * 64 + lane;
+      const int64_t n_points = C.nx * C.ny * C.nz;
+      bool inside = false;
+      if (f < n_points) {
+        const uint32_t nx = static_cast<uint32_t>(C.nx);
+        const uint32_t nxy = static_cast<uint32_t>(C.nx * C.ny);
+        const uint32_t ff = static_cast<uint32_t>(f);
+        const uint32_t z = ff / nxy;
+        const uint32_t r = ff - z * nxy;
+        const uint32_t y = r / nx;
+        const uint32_t x = r - y * nx;
+        const float v =
+            sdf_eval(C.ops, C.n_ops, C.origin[0] + static_cast<float>(x) * C.spacing[0],
+                     C.origin[1] + static_cast<float>(y) * C.spacing[1],
+                     C.origin[2] + static_cast<float>(z) * C.spacing[2]);
+        if (C.grid) const_cast<float*>(C.grid)[f] = v;
+        inside = !(v > 0.0f);
+      }
+      const uint64_t m = __ballot(inside);
+      if (lane == 0) const_cast<uint64_t*>(C.signs)[lw] = m;
     }
-    const uint64_t m = __ballot(inside);
-    if (lane == 0) const_cast<uint64_t*>(C.signs)[lw] = m;
   }
 }
 
@@ -1153,7 +1162,8 @@ hipError_t launch_batch_signs(const BatchPlan& P, const McBuffers& B, int words_
   R.n_blocks = n_blocks;
   R.totals = B.totals;
   const int64_t wpw = words_per_wave > 0 ? words_per_wave : 1;
-  int64_t blocks = (P.n_words + 4 * wpw - 1) / (4 * wpw);
+  const int64_t per_block = 4 * kBatchRun * wpw;  // sign words per workgroup
+  int64_t blocks = (P.n_words + per_block - 1) / per_block;
   if (blocks < 1) blocks = 1;
   if (blocks > 65536) blocks = 65536;
   const int64_t min_blocks = (n_blocks + 255) / 256;
