@@ -387,10 +387,10 @@ __device__ __forceinline__ uint32_t planes_count(const uint64_t* p, uint64_t m) 
 // Cube index of the cell whose low corner sits at (dx, dy, dz) of the 3x3x3
 // sign neighbourhood `cube` (bit dx + 3 dy + 9 dz).
 __device__ __forceinline__ uint32_t case_in_cube(uint32_t cube, int dx, int dy, int dz) {
+  // corners 0, 1, 3 sit at bits 0, 1, 3 of s; 2 at 4; 4, 5, 7 at 9, 10, 12;
+  // 6 at 13
   const uint32_t s = cube >> (dx + 3 * dy + 9 * dz);
-  return (s & 1u) | ((s >> 1) & 1u) << 1 | ((s >> 4) & 1u) << 2 | ((s >> 3) & 1u) << 3 |
-         ((s >> 9) & 1u) << 4 | ((s >> 10) & 1u) << 5 | ((s >> 13) & 1u) << 6 |
-         ((s >> 12) & 1u) << 7;
+  return (s & 0xBu) | ((s >> 2) & 0x4u) | ((s >> 5) & 0xB0u) | ((s >> 7) & 0x40u);
 }
 
 // A predecessor publishes its aggregate right after its own counting, so the
