@@ -587,11 +587,14 @@ __global__ __launch_bounds__(kEmitThreads) void k_emit(EmitArgs A, Space sp) {
   const uint64_t first = vblock * chunks_per * kEmitThreads;
   uint64_t last = first + chunks_per * kEmitThreads;
   if (last > n_act) last = n_act;
+  // The first active entry is in flight while the tables are staged.
+  const uint64_t i0 = first + tid;
+  const uint32_t rec0 = i0 < last ? A.active[i0] : 0u;
   load_table16(&s_t, &c_emit);
   __syncthreads();
 
-  for (uint64_t i = first + tid; i < last; i += kEmitThreads) {
-    const uint32_t rec = A.active[i];
+  for (uint64_t i = i0; i < last; i += kEmitThreads) {
+    const uint32_t rec = i == i0 ? rec0 : A.active[i];
     const uint32_t g = rec >> 6;  // global segment
     const int b = static_cast<int>(rec & 63u);
     uint32_t lg;                  // segment within its lattice
