@@ -37,7 +37,14 @@ constexpr int kFragF2 = kFragF1 + kHidden * kFeatPad;
 constexpr int kFragF3 = kFragF2 + kHidden * kHidden;
 constexpr int kFragT2 = kFragF3 + kHidden * kHidden;
 constexpr int kFragT3 = kFragT2 + kHidden * kHidden;
-constexpr int kFrags = kFragT3 + kHidden * kHidden;
+constexpr int kTrainFrags = kFragT3 + kHidden * kHidden;  // F1 F2 F3 T2 T3
+// F1I: W1 with its columns in the inference forward's feature order (row
+// 16 kb + 4 g + j of the features holds, for kb < 6, sin (kb < 3) or cos of
+// coordinate kb % 3 at frequency 4 j + g; rows 96-98 the coordinates; the
+// rest padding), so each K block's features share one function per
+// wavefront.
+constexpr int kFragF1I = kTrainFrags;
+constexpr int kFrags = kFragF1I + kHidden * kFeatPad;
 
 // Device state of one model.
 struct Model {

@@ -11,6 +11,11 @@
 // four values of block 2i then its four of block 2i+1, for A and B alike,
 // which the instruction reads as one 32-wide block with its K index
 // permuted the same way in both operands — the same sum of products.
+// Its features come in another row order than training's (F1I, see
+// mlp_kernels.h): in each K block every lane evaluates the same function
+// (sin or cos of one coordinate) at its own frequencies, c_k = 2^(k / 2) / 2
+// for k = 4 j + g, g = l / 16, i.e. c_g 4^j with c_g a power-of-two multiple
+// of 1/2 or of sqrt(2)/2 — the same f32 multipliers, so the same features.
 // A wavefront pushes 16 samples (the N dimension) through the network:
 // layer outputs land in exactly the layout the next layer's B operand
 // needs, so activations stay in registers from the features to the output.
@@ -37,10 +42,13 @@ __constant__ float c_rev[kFreq] = {
     0x1p+3f, 0x1.6a09e6p+3f, 0x1p+4f, 0x1.6a09e6p+4f,
     0x1p+5f, 0x1.6a09e6p+5f, 0x1p+6f, 0x1.6a09e6p+6f};
 
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+
+// f32 -> bf16, nearest with ties to even (gfx950's v_cvt_pk_bf16_f32; a NaN
+// stays a quiet NaN).
 __device__ __forceinline__ uint16_t to_bf16(float f) {
-  const uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);  // NaN
-  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);  // nearest, ties to even
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 
 __device__ __forceinline__ float from_bf16(uint16_t b) {
@@ -60,6 +68,11 @@ __device__ __forceinline__ f32x4 mfma2(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b
   const s16x8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
                                                  __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// Four f32 to bf16 (two v_cvt_pk_bf16_f32).
+__device__ __forceinline__ bf16x4 pack4f(float a, float b, float c, float d) {
+  return __builtin_bit_cast(bf16x4, __builtin_convertvector(f32x4v{a, b, c, d}, bf16x4v));
 }
 
 __device__ __forceinline__ bf16x4 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
@@ -129,17 +142,40 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
     }
     return pack4(xb[0], xb[1], xb[2], xb[3]);
   };
+  // Inference feature order (F1I): block kb < 6 holds function kb at the
+  // lane's four frequencies, block 6 the coordinates (rows 0-2 of lanes
+  // 0-15) and padding.
+  const int g = lane >> 4;
+  const float cg = __builtin_amdgcn_ldexpf((g & 1) ? 0x1.6a09e6p-1f : 0x1p-1f, g >> 1);
+  auto features_i = [&](int kb) {
+    float xf[4];
+    if (kb < 6) {
+      const int d = kb % 3;
+      const float m = cg * (d == 0 ? px : (d == 1 ? py : pz));  // c_k p for j = 0
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        // c_g 4^j p: a power-of-two scaling of m, exact
+        const float rev = __builtin_amdgcn_fractf(m * static_cast<float>(1 << (2 * j)));
+        xf[j] = kb < 3 ? __builtin_amdgcn_sinf(rev) : __builtin_amdgcn_cosf(rev);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        xf[j] = g == 0 && j < 3 ? (j == 0 ? px : (j == 1 ? py : pz)) : 0.0f;
+    }
+    return pack4f(xf[0], xf[1], xf[2], xf[3]);
+  };
   if (kDoubleK) {
 #pragma unroll 1
     for (int kb = 0; kb + 1 < kKB1; kb += 2) {
-      const bf16x4 x0 = features(kb), x1 = features(kb + 1);
+      const bf16x4 x0 = features_i(kb), x1 = features_i(kb + 1);
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
         acc1[mb] = mfma2(frag(F1, kKB1, mb, kb, lane), frag(F1, kKB1, mb, kb + 1, lane), x0, x1,
                          acc1[mb]);
     }
     if (kKB1 & 1) {
-      const bf16x4 x = features(kKB1 - 1);
+      const bf16x4 x = features_i(kKB1 - 1);
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb)
         acc1[mb] = mfma(frag(F1, kKB1, mb, kKB1 - 1, lane), x, acc1[mb]);
@@ -164,32 +200,22 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
     }
     return acc;
   };
+  // Bias, ReLU and the bf16 rounding of a layer's 16 x 16 output block.
+  auto activate = [&](const f32x4& acc, const float* b) {
+    return pack4f(fmaxf(acc[0] + b[r0 + 0], 0.0f), fmaxf(acc[1] + b[r0 + 1], 0.0f),
+                  fmaxf(acc[2] + b[r0 + 2], 0.0f), fmaxf(acc[3] + b[r0 + 3], 0.0f));
+  };
 #pragma unroll
-  for (int mb = 0; mb < 4; ++mb) {
-    uint16_t h[4];
+  for (int mb = 0; mb < 4; ++mb) T.h1[mb] = activate(acc1[mb], b1 + 16 * mb);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) h[j] = to_bf16(fmaxf(acc1[mb][j] + b1[16 * mb + r0 + j], 0.0f));
-    T.h1[mb] = pack4(h[0], h[1], h[2], h[3]);
-  }
-#pragma unroll
-  for (int mb = 0; mb < 4; ++mb) {
-    const f32x4 acc = product64(F2, mb, T.h1);
-    uint16_t h[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) h[j] = to_bf16(fmaxf(acc[j] + b2[16 * mb + r0 + j], 0.0f));
-    T.h2[mb] = pack4(h[0], h[1], h[2], h[3]);
-  }
+  for (int mb = 0; mb < 4; ++mb) T.h2[mb] = activate(product64(F2, mb, T.h1), b2 + 16 * mb);
   float part = 0.0f;
 #pragma unroll
   for (int mb = 0; mb < 4; ++mb) {
-    const f32x4 acc = product64(F3, mb, T.h2);
-    uint16_t h[4];
+    T.h3[mb] = activate(product64(F3, mb, T.h2), b3 + 16 * mb);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      h[j] = to_bf16(fmaxf(acc[j] + b3[16 * mb + r0 + j], 0.0f));
-      part += w4[16 * mb + r0 + j] * from_bf16(h[j]);
-    }
-    T.h3[mb] = pack4(h[0], h[1], h[2], h[3]);
+    for (int j = 0; j < 4; ++j)
+      part += w4[16 * mb + r0 + j] * from_bf16(static_cast<uint16_t>(T.h3[mb][j]));
   }
   // The four lanes of a sample (l, l^16, l^32, l^48) hold its 64 terms.
   part += __shfl_xor(part, 16, 64);
@@ -198,7 +224,7 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
 }
 
 constexpr int kThreads = 256;
-constexpr int kLdsFrags = kFragT3;  // forward fragments only (F1 F2 F3)
+constexpr int kLdsFrags = kFragT3;  // forward fragments only (F1I F2 F3)
 
 // Loads fragments [0, n_frag) and the bias block into LDS.
 __device__ __forceinline__ void load_model(const Model& M, uint16_t* sF, int n_frag, float* sB) {
@@ -213,16 +239,31 @@ __device__ __forceinline__ void load_model(const Model& M, uint16_t* sF, int n_f
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_forward(Model M, const float* __restrict__ points,
-                                                      int64_t n, float* __restrict__ out) {
+// 512 threads: the fragments' 40 KB of LDS then allow as many waves per SIMD
+// as the registers do.
+constexpr int kFwdThreads = 512;
+
+__global__ __launch_bounds__(kFwdThreads) void k_forward(Model M, const float* __restrict__ points,
+                                                         int64_t n, float* __restrict__ out) {
   __shared__ __align__(16) uint16_t sF[kLdsFrags];
   __shared__ float sB[4 * kHidden + 1];
-  load_model(M, sF, kLdsFrags, sB);
+  // F1I in F1's place, then F2 F3, and the bias block
+  {
+    const uint32_t* f1 = reinterpret_cast<const uint32_t*>(M.frags + kFragF1I);
+    const uint32_t* f23 = reinterpret_cast<const uint32_t*>(M.frags + kFragF2);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(sF);
+    constexpr int n1 = (kFragF2 - kFragF1) / 2, n = kLdsFrags / 2;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = i < n1 ? f1[i] : f23[i - n1];
+  }
+  load_model(M, sF, 0, sB);
   __syncthreads();
   const int lane = __lane_id();
-  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
-  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (kThreads / 64);
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kFwdThreads / 64) + (threadIdx.x >> 6);
+  const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (kFwdThreads / 64);
   for (int64_t t0 = wave * 16; t0 < n; t0 += n_waves * 16) {
+    // The fragments are re-read from LDS for every tile rather than held in
+    // registers across tiles: fewer registers, more waves per SIMD.
+    asm volatile("" ::: "memory");
     const int64_t s = t0 + (lane & 15);
     const bool ok = s < n;
     const float px = ok ? points[3 * s] : 0.0f, py = ok ? points[3 * s + 1] : 0.0f,
@@ -247,9 +288,9 @@ __device__ __forceinline__ uint32_t hash32(uint32_t a, uint32_t b, uint32_t c) {
 // per-sample dL/df, accumulates the loss.
 __global__ __launch_bounds__(kThreads) void k_train_fb(Model M, Batch B, Samples S,
                                                        uint32_t step) {
-  __shared__ __align__(16) uint16_t sF[kFrags];
+  __shared__ __align__(16) uint16_t sF[kTrainFrags];
   __shared__ float sB[4 * kHidden + 1];
-  load_model(M, sF, kFrags, sB);
+  load_model(M, sF, kTrainFrags, sB);
   __syncthreads();
   const int lane = __lane_id();
   const int r0 = 4 * (lane >> 4);
@@ -414,7 +455,16 @@ __global__ __launch_bounds__(kThreads) void k_bgrad(Model M, Batch B) {
   if (lane == 0) atomicAdd(&M.grad[dst], acc);
 }
 
-// Fragment slot of master weight index p (layers 1-3), and of its transpose.
+// Inference feature row of training feature column c (see kFragF1I).
+__device__ __forceinline__ int inference_row(int c) {
+  if (c < 3) return 96 + c;       // coordinates
+  if (c >= kFeat) return c;       // padding
+  const int q = c - 3, k = q / 6, t = q - 6 * k;  // frequency k, function t
+  return 16 * t + 4 * (k & 3) + (k >> 2);
+}
+
+// Fragment slot of master weight index p (layers 1-3), and of its transpose
+// (layers 2-3) or its inference-order copy (layer 1).
 __device__ __forceinline__ void frag_slots(int p, int* f, int* ft) {
   *f = -1;
   *ft = -1;
@@ -432,6 +482,11 @@ __device__ __forceinline__ void frag_slots(int p, int* f, int* ft) {
   const int row = q / cols, c = q - row * cols;
   // W[row][c] sits in block (row / 16, c / 16) at lane (row % 16) + 16 ((c % 16) / 4), slot c % 4
   *f = fbase + (((row >> 4) * kbn + (c >> 4)) * 64 + (row & 15) + 16 * ((c & 15) >> 2)) * 4 + (c & 3);
+  if (base == kOffW1) {
+    const int ci = inference_row(c);
+    *ft = kFragF1I + (((row >> 4) * kbn + (ci >> 4)) * 64 + (row & 15) + 16 * ((ci & 15) >> 2)) * 4 +
+          (ci & 3);
+  }
   if (tbase >= 0) {
     // W^T[c][row]: block (c / 16, row / 16), lane (c % 16) + 16 ((row % 16) / 4), slot row % 4
     *ft = tbase + (((c >> 4) * 4 + (row >> 4)) * 64 + (c & 15) + 16 * ((row & 15) >> 2)) * 4 +
@@ -487,9 +542,10 @@ hipError_t launch_pack(const Model& M, hipStream_t s) {
 hipError_t launch_forward(const Model& M, const float* points, int64_t n, float* out,
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  int64_t blocks = (n + 16 * 4 - 1) / (16 * 4);  // one 16-sample tile per wavefront
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_forward, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, s, M,
+  constexpr int64_t per_block = 16 * (kFwdThreads / 64);  // a 16-sample tile per wavefront
+  int64_t blocks = (n + per_block - 1) / per_block;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_forward, dim3(static_cast<unsigned>(blocks)), dim3(kFwdThreads), 0, s, M,
                      points, n, out);
   return hipGetLastError();
 }
