@@ -44,9 +44,15 @@ hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, 
 // each, device), unnormalised, into normals (3 floats each, device).
 hipError_t launch_sample_normals(const DevOp* ops, int n_ops, const float* points, uint64_t n,
                                  float epsilon, float* normals, hipStream_t s);
-// Feature points of the listed mixed cells (normals by central differences
-// of step epsilon, or the program's analytic normals when epsilon is 0).
-hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
-                             const DcBuffers& B, hipStream_t s);
+// The eight corner normals of each of the first n_mixed listed mixed cells
+// (central differences of step epsilon, or the program's analytic normals
+// when epsilon is 0, normalised), 24 floats per cell, one thread per corner.
+hipError_t launch_dc_normals(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
+                             const DcBuffers& B, uint64_t n_mixed, float* normals,
+                             hipStream_t s);
+// Feature points of the first n_mixed listed mixed cells from their corner
+// normals (TangentPlanes and the QEF solve), one thread per cell.
+hipError_t launch_dc_feature(const Lattice& L, const DcBuffers& B, uint64_t n_mixed,
+                             const float* normals, hipStream_t s);
 
 }  // namespace iso

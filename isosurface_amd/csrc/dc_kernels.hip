@@ -156,30 +156,52 @@ __global__ __launch_bounds__(256) void k_sample_normals(const DevOp* __restrict_
   out[3 * i + 2] = vz / e2;
 }
 
+// Corner k of mixed cell c (z-major cell index over (nx-1) x (ny-1) cells).
+__device__ __forceinline__ qef::Vec3f cell_corner(const Lattice& L, uint32_t c, int k) {
+  const uint32_t cnx = static_cast<uint32_t>(L.nx - 1);
+  const uint32_t cnxy = static_cast<uint32_t>((L.nx - 1) * (L.ny - 1));
+  const uint32_t z = c / cnxy;
+  const uint32_t r = c - z * cnxy;
+  const uint32_t y = r / cnx;
+  const uint32_t x = r - y * cnx;
+  return {lattice_coord(L, 0, x + tables::kCorner[k][0]),
+          lattice_coord(L, 1, y + tables::kCorner[k][1]),
+          lattice_coord(L, 2, z + tables::kCorner[k][2])};
+}
+
+// The corner normals of the mixed cells, one thread per (cell, corner):
+// normals[24 i + 3 k + a] for corner k of mixed cell i.
+__global__ __launch_bounds__(256) void k_dc_normals(const DevOp* __restrict__ ops, int n_ops,
+                                                    Lattice L, float eps,
+                                                    const uint32_t* __restrict__ mixed,
+                                                    uint64_t n_mixed, float* __restrict__ normals) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= 8 * n_mixed) return;
+  const uint64_t i = t >> 3;
+  const int k = static_cast<int>(t & 7);
+  const qef::Vec3f p = cell_corner(L, mixed[i], k);
+  const qef::Vec3f n = central_normal(ops, n_ops, p.x, p.y, p.z, eps);
+  normals[3 * t + 0] = n.x;
+  normals[3 * t + 1] = n.y;
+  normals[3 * t + 2] = n.z;
+}
+
 constexpr int kFeatureThreads = 128;
 
 __global__ __launch_bounds__(kFeatureThreads) void k_dc_feature(
-    const DevOp* __restrict__ ops, int n_ops, Lattice L, float eps,
-    const uint32_t* __restrict__ mixed, float* __restrict__ dual_points,
+    Lattice L, const uint32_t* __restrict__ mixed, uint64_t n_mixed,
+    const float* __restrict__ cnormals, float* __restrict__ dual_points,
     unsigned long long* __restrict__ counters) {
-  const uint64_t n_mixed = counters[kDcMixed];
-  const uint32_t cnx = static_cast<uint32_t>(L.nx - 1);
-  const uint32_t cnxy = static_cast<uint32_t>((L.nx - 1) * (L.ny - 1));
   bool stalled = false;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kFeatureThreads + threadIdx.x; i < n_mixed;
-       i += static_cast<uint64_t>(gridDim.x) * kFeatureThreads) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kFeatureThreads + threadIdx.x;
+  if (i < n_mixed) {
     const uint32_t c = mixed[i];
-    const uint32_t z = c / cnxy;
-    const uint32_t r = c - z * cnxy;
-    const uint32_t y = r / cnx;
-    const uint32_t x = r - y * cnx;
     qef::Vec3f corners[8], normals[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      corners[k] = {lattice_coord(L, 0, x + tables::kCorner[k][0]),
-                    lattice_coord(L, 1, y + tables::kCorner[k][1]),
-                    lattice_coord(L, 2, z + tables::kCorner[k][2])};
-      normals[k] = central_normal(ops, n_ops, corners[k].x, corners[k].y, corners[k].z, eps);
+      corners[k] = cell_corner(L, c, k);
+      const float* q = cnormals + 24 * i + 3 * k;
+      normals[k] = {q[0], q[1], q[2]};
     }
     const qef::Vec3f p = qef::place_feature(corners, normals, &stalled);
     dual_points[3 * static_cast<uint64_t>(c) + 0] = p.x;
@@ -187,13 +209,6 @@ __global__ __launch_bounds__(kFeatureThreads) void k_dc_feature(
     dual_points[3 * static_cast<uint64_t>(c) + 2] = p.z;
   }
   if (stalled) atomicOr(&counters[kDcStalled], 1ull);
-}
-
-int device_cus_dc() {
-  int device = 0, cus = 256;
-  if (hipGetDevice(&device) == hipSuccess)
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  return cus > 0 ? cus : 256;
 }
 
 }  // namespace
@@ -236,17 +251,26 @@ hipError_t launch_sample_normals(const DevOp* ops, int n_ops, const float* point
   return hipGetLastError();
 }
 
-hipError_t launch_dc_feature(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
-                             const DcBuffers& B, hipStream_t s) {
-  if (!ops) return hipErrorInvalidValue;
-  // Grid-stride over the mixed list, whose length is only known on the device.
-  const int64_t n_cells = (L.nx - 1) * (L.ny - 1) * (L.nz - 1);
-  int64_t blocks = (n_cells + kFeatureThreads - 1) / kFeatureThreads;
-  const int64_t cap = 16LL * device_cus_dc();
-  if (blocks > cap) blocks = cap;
-  if (blocks < 1) blocks = 1;
+hipError_t launch_dc_normals(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
+                             const DcBuffers& B, uint64_t n_mixed, float* normals,
+                             hipStream_t s) {
+  if (!ops || !normals) return hipErrorInvalidValue;
+  if (n_mixed == 0) return hipSuccess;
+  const uint64_t blocks = (8 * n_mixed + 255) / 256;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_dc_normals, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
+                     n_ops, L, epsilon, B.mixed, n_mixed, normals);
+  return hipGetLastError();
+}
+
+hipError_t launch_dc_feature(const Lattice& L, const DcBuffers& B, uint64_t n_mixed,
+                             const float* normals, hipStream_t s) {
+  if (!normals) return hipErrorInvalidValue;
+  if (n_mixed == 0) return hipSuccess;
+  const uint64_t blocks = (n_mixed + kFeatureThreads - 1) / kFeatureThreads;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_dc_feature, dim3(static_cast<unsigned>(blocks)), dim3(kFeatureThreads), 0,
-                     s, ops, n_ops, L, epsilon, B.mixed, B.dual_points, B.counters);
+                     s, L, B.mixed, n_mixed, normals, B.dual_points, B.counters);
   return hipGetLastError();
 }
 
