@@ -184,18 +184,29 @@ __global__ __launch_bounds__(256) void k_batch_signs(const CellSpace* __restrict
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * 4;
   const int64_t n_words = static_cast<int64_t>(word_start[n_chunks]);
+  // A word's chunk is the same for the whole wavefront: reading the table
+  // entries through readfirstlane keeps the chunk index in a scalar
+  // register, so the chunk record and its program are scalar loads and the
+  // program's operands scalar operands.
+  auto first_word = [&](uint32_t i) -> int64_t {
+    const uint64_t v = word_start[i];
+    const uint32_t lo32 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi32 = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi32) << 32) | lo32);
+  };
   for (int64_t w0 = wave * kBatchRun; w0 < n_words; w0 += n_waves * kBatchRun) {
     uint32_t lo = 0, hi = n_chunks;  // last chunk whose first word <= w0
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (static_cast<int64_t>(word_start[mid]) <= w0) lo = mid;
+      if (first_word(mid) <= w0) lo = mid;
       else hi = mid;
     }
     const int64_t w1 = w0 + kBatchRun < n_words ? w0 + kBatchRun : n_words;
     for (int64_t w = w0; w < w1; ++w) {
-      while (lo + 1 < n_chunks && static_cast<int64_t>(word_start[lo + 1]) <= w) ++lo;
+      while (lo + 1 < n_chunks && first_word(lo + 1) <= w) ++lo;
+      lo = __builtin_amdgcn_readfirstlane(lo);
       const CellSpace& C = chunks[lo];
-      const int64_t lw = w - static_cast<int64_t>(word_start[lo]);
+      const int64_t lw = w - first_word(lo);
       const int64_t f = lw * 64 + lane;
       const int64_t n_points = C.nx * C.ny * C.nz;
       bool inside = false;
