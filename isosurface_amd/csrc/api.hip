@@ -77,7 +77,7 @@ struct iso_ctx {
   DevBuf dgrid, dvals;
   int64_t dgnx = 0, dgny = 0, dgnz = 0;
   // extended marching cubes: per active cell state, first ids, flip records
-  DevBuf emc_cells, emc_first, emc_flips, emc_ctr;
+  DevBuf emc_cells, emc_first, emc_flips, emc_ctr, emc_cross;
   // dual contouring: primal samples, dual values and points, mixed list, counters
   DevBuf dc_normals, dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
@@ -373,7 +373,7 @@ int iso_ctx_destroy(iso_ctx* c) {
                     &c->nn_h,       &c->nn_dz,      &c->nn_g,      &c->nn_pts,   &c->nn_t,
                     &c->nn_w,       &c->nn_out,     &c->br_centres, &c->br_hd,   &c->br_pred,
                     &c->br_flags,   &c->br_list,    &c->br_count,  &c->pvals,  &c->dense_check,  &c->nrm,
-                    &c->dgrid,      &c->dvals,      &c->emc_cells, &c->emc_first,
+                    &c->dgrid,      &c->dvals,      &c->emc_cells, &c->emc_first, &c->emc_cross,
                     &c->emc_flips,  &c->emc_ctr,    &c->br_coarse, &c->br_proof,
                     &c->f_signs,    &c->f_flags,    &c->f_slots,   &c->f_rowbase, &c->f_ctl};
   if (c->b_graph) (void)hipGraphExecDestroy(c->b_graph);
@@ -1150,6 +1150,9 @@ int iso_extract_emc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_me
   A.vfirst = static_cast<uint32_t*>(c->emc_first.ptr);
   A.tfirst = A.vfirst + na;
   A.counters = static_cast<unsigned long long*>(c->emc_ctr.ptr);
+  ISO_HIP(c, ensure(&c->emc_cross, na * 72 * sizeof(float)));
+  A.crossings = static_cast<float*>(c->emc_cross.ptr);
+  ISO_HIP(c, timed(c, "emc_crossings", [&] { return iso::launch_emc_crossings(A, c->stream); }));
   ISO_HIP(c, timed(c, "emc_cells", [&] { return iso::launch_emc_cells(A, S, c->stream); }));
   ISO_HIP(c, hipMemcpyAsync(c->h_stage, S.ctl, 4 * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, c->stream));

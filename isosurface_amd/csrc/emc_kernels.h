@@ -44,6 +44,9 @@ struct EmcArgs {
   const DevOp* ops;
   int32_t n_ops;
   float epsilon;           // > 0: CentralDifference normals; 0: HermiteSource
+  // 72 per active cell: per local edge e, the crossing point (floats 6e..6e+2)
+  // and its normalised normal (6e+3..6e+5), written for crossed edges only
+  float* crossings;
   EmcCell* cells;          // n_act
   uint32_t* vfirst;        // n_act: first vertex id of each cell
   uint32_t* tfirst;        // n_act: first triangle of each cell
@@ -54,6 +57,9 @@ struct EmcArgs {
   unsigned long long* counters;
 };
 
+// Stage 0: the crossing points and normals of every active cell's crossed
+// edges, one thread per (cell, edge).
+hipError_t launch_emc_crossings(const EmcArgs& A, hipStream_t s);
 // Stage 1: cells + scan (state for n_act items at one item per thread).
 // The totals land in scan state ctl[1]: triangles << 32 | vertices.
 hipError_t launch_emc_cells(const EmcArgs& A, const scan::State& st, hipStream_t s);

@@ -120,34 +120,62 @@ __device__ __forceinline__ uint32_t vertex_id(const EmcArgs& A, const CellAt& c,
 // ---------------------------------------------------------------------------
 // 1. per-cell tangent planes and feature points, fused with the id scan
 // ---------------------------------------------------------------------------
+// Cube index of an active cell from its corners' directed values
+// (Directed::is_positive, distance.rs:76-79).
+__device__ __forceinline__ uint32_t cell_cube(const EmcArgs& A, const CellAt& c) {
+  const int64_t nx = A.L.nx, nxy = A.L.nx * A.L.ny;
+  uint32_t cube = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int64_t f = c.f + tables::kCorner[k][2] * nxy + tables::kCorner[k][1] * nx +
+                      tables::kCorner[k][0];
+    const float* v = A.values + 3 * f;
+    if (!(v[0] > 0.0f || v[1] > 0.0f || v[2] > 0.0f)) cube |= 1u << k;
+  }
+  return cube;
+}
+
+// The crossing and normal of edge e of active cell i, one thread each
+// (marching_cubes_impl.rs:39-90, TangentPlanes::from_edge_crossings,
+// feature/mod.rs:77-89), so the program evaluations of a cell run side by
+// side instead of in its scan item's chain.
+__global__ __launch_bounds__(256) void k_emc_crossings(EmcArgs A) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= 12 * A.n_act) return;
+  const uint64_t i = t / 12;
+  const int e = static_cast<int>(t - 12 * i);
+  const CellAt c = decode(A, A.active[i]);
+  if (!((c_mct.edge_mask[cell_cube(A, c)] >> e) & 1u)) return;
+  float p[3], raw[3];
+  edge_crossing(A, c, e, p);
+  if (A.epsilon > 0.0f)
+    sdf_central_difference(A.ops, A.n_ops, p[0], p[1], p[2], A.epsilon, raw);
+  else
+    sdf_eval_hermite(A.ops, A.n_ops, p[0], p[1], p[2], raw);
+  const qef::Vec3f n = qef::normalised_or_zero(qef::Vec3f{raw[0], raw[1], raw[2]});
+  float* q = A.crossings + 6 * t;
+  q[0] = p[0];
+  q[1] = p[1];
+  q[2] = p[2];
+  q[3] = n.x;
+  q[4] = n.y;
+  q[5] = n.z;
+}
+
 struct CellLoad {
   EmcArgs A;
   __device__ unsigned long long operator()(uint64_t i) const {
     const CellAt c = decode(A, A.active[i]);
-    const int64_t nx = A.L.nx, nxy = A.L.nx * A.L.ny;
-    uint32_t cube = 0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {  // Directed::is_positive (distance.rs:76-79)
-      const int64_t f = c.f + tables::kCorner[k][2] * nxy + tables::kCorner[k][1] * nx +
-                        tables::kCorner[k][0];
-      const float* v = A.values + 3 * f;
-      if (!(v[0] > 0.0f || v[1] > 0.0f || v[2] > 0.0f)) cube |= 1u << k;
-    }
+    const uint32_t cube = cell_cube(A, c);
     const uint32_t mask = c_mct.edge_mask[cube];
-    // Crossings and their normals in edge order (marching_cubes_impl.rs:
-    // 39-90, TangentPlanes::from_edge_crossings, feature/mod.rs:77-89).
+    // Crossings and their normals in edge order.
     qef::Vec3f pts[12], nrm[12];
     int m = 0;
     for (int e = 0; e < 12; ++e) {
       if (!((mask >> e) & 1u)) continue;
-      float p[3], raw[3];
-      edge_crossing(A, c, e, p);
-      if (A.epsilon > 0.0f)
-        sdf_central_difference(A.ops, A.n_ops, p[0], p[1], p[2], A.epsilon, raw);
-      else
-        sdf_eval_hermite(A.ops, A.n_ops, p[0], p[1], p[2], raw);
-      pts[m] = qef::Vec3f{p[0], p[1], p[2]};
-      nrm[m] = qef::normalised_or_zero(qef::Vec3f{raw[0], raw[1], raw[2]});
+      const float* q = A.crossings + 6 * (12 * i + e);
+      pts[m] = qef::Vec3f{q[0], q[1], q[2]};
+      nrm[m] = qef::Vec3f{q[3], q[4], q[5]};
       ++m;
     }
     bool feature = false, stalled = false;
@@ -372,6 +400,15 @@ unsigned grid_of(uint64_t n) {
 }
 
 }  // namespace
+
+hipError_t launch_emc_crossings(const EmcArgs& A, hipStream_t s) {
+  if (A.n_act == 0) return hipSuccess;
+  if (!A.crossings) return hipErrorInvalidValue;
+  const uint64_t blocks = (12 * A.n_act + 255) / 256;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_emc_crossings, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, A);
+  return hipGetLastError();
+}
 
 hipError_t launch_emc_cells(const EmcArgs& A, const scan::State& st, hipStream_t s) {
   return scan::launch<1>(A.n_act, CellLoad{A}, CellSink{A.vfirst, A.tfirst}, st, s);
