@@ -194,7 +194,8 @@ def valu_roofline(durations_us: dict) -> dict:
 def valu_probe(ctx: Context, size: int) -> None:
     """A few launches of each VALU-bound kernel, for SQ counter passes: the
     program-path signs at size^3, DualContouring::new(128) over
-    CentralDifference on the 32-box union, and the certificate-only narrow
+    CentralDifference on the 32-box union (corner normals and feature
+    points), and the certificate-only narrow
     band at 1025^3 on the smooth spheres (the bench's configurations)."""
     src, _ = W.spheres_minus_torus()
     ctx.upload_program(src.program())
@@ -970,7 +971,8 @@ def main() -> int:
     # per launch (committed SQ counter pass) over this run's durations.
     durations = {"mc_signs_program": program_path["kernels_us"].get("mc_signs_program")}
     if "dual_contouring" in result:
-        durations["dc_feature"] = result["dual_contouring"]["kernels_us"].get("dc_feature")
+        for k in ("dc_normals", "dc_feature"):
+            durations[k] = result["dual_contouring"]["kernels_us"].get(k)
     if "learned_band" in result:
         durations["band_exact"] = result["learned_band"]["certificate_without_network"][
             "kernels_us"].get("band_exact")

@@ -79,7 +79,7 @@ struct iso_ctx {
   // extended marching cubes: per active cell state, first ids, flip records
   DevBuf emc_cells, emc_first, emc_flips, emc_ctr, emc_cross;
   // dual contouring: primal samples, dual values and points, mixed list, counters
-  DevBuf dc_normals, dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
+  DevBuf dc_normals, dc_plist, dc_pindex, dc_samples, dc_values, dc_points, dc_mixed, dc_flags, dc_counters;
   unsigned long long* h_dc = nullptr;  // pinned
   // octree marching cubes: nodes, dense primal lattice (or the sorted path's
   // pairs and sort scratch), cubes, edge table, scan state
@@ -360,7 +360,7 @@ int iso_ctx_destroy(iso_ctx* c) {
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
   DevBuf* bufs[] = {&c->prog,       &c->grid_owned, &c->signs,     &c->seg,      &c->tplanes, &c->vplanes,
                     &c->agg,        &c->incl,       &c->active,    &c->totals,
-                    &c->vout,       &c->iout,       &c->pts,       &c->vals,     &c->dc_normals, &c->dc_samples,
+                    &c->vout,       &c->iout,       &c->pts,       &c->vals,     &c->dc_normals, &c->dc_plist, &c->dc_pindex, &c->dc_samples,
                     &c->dc_values,  &c->dc_points,  &c->dc_mixed,  &c->dc_flags,  &c->dc_counters,
                     &c->oc_keys,    &c->oc_values,  &c->oc_leaf,   &c->oc_dense, &c->oc_primal,
                     &c->oc_plevel,  &c->oc_corner,  &c->oc_case,   &c->oc_base,  &c->oc_tkeys,
@@ -1347,21 +1347,40 @@ int iso_extract_dc(iso_ctx* c, const iso_grid_desc* desc, float epsilon, iso_mes
   iso::scan::State S{};
   if ((st = prepare_scan(c, n_cells, &S)) != ISO_OK) return st;
   ISO_HIP(c, timed(c, "dc_dual", [&] { return iso::launch_dc_dual(L, B, S, c->stream); }));
-  // The mixed-cell count sizes the normals pass: one thread per cell corner.
+  // Central differences evaluate the program six times per normal: the
+  // mixed cells' corners are listed once each and their normals shared.
+  // Analytic normals cost one evaluation: each cell corner takes its own.
+  const bool shared = epsilon > 0.0f;
+  uint32_t* plist = nullptr;
+  uint32_t* pindex = nullptr;
+  if (shared) {
+    ISO_HIP(c, ensure(&c->dc_plist, n_points * sizeof(uint32_t)));
+    ISO_HIP(c, ensure(&c->dc_pindex, n_points * sizeof(uint32_t)));
+    plist = static_cast<uint32_t*>(c->dc_plist.ptr);
+    pindex = static_cast<uint32_t*>(c->dc_pindex.ptr);
+    if ((st = prepare_scan(c, n_points, &S)) != ISO_OK) return st;
+    ISO_HIP(c, timed(c, "dc_points", [&] {
+              return iso::launch_dc_corner_points(L, B, plist, pindex, S, c->stream);
+            }));
+  }
+  // The counts size the normals and feature passes.
   ISO_HIP(c, hipMemcpyAsync(c->h_dc, B.counters, iso::kDcSlots * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, c->stream));
   ISO_HIP(c, hipStreamSynchronize(c->stream));
   if (c->h_dc[iso::kDcScanErr])
-    return fail(c, ISO_ERR_DEVICE, "the mixed-cell scan did not complete");
+    return fail(c, ISO_ERR_DEVICE, "a dual-contouring scan did not complete");
   const uint64_t n_mixed = c->h_dc[iso::kDcMixed];
-  ISO_HIP(c, ensure(&c->dc_normals, (n_mixed ? n_mixed : 1) * 24 * sizeof(float)));
-  float* cnormals = static_cast<float*>(c->dc_normals.ptr);
+  const uint64_t n_normals = shared ? c->h_dc[iso::kDcPoints] : 8 * n_mixed;
+  ISO_HIP(c, ensure(&c->dc_normals, (n_normals ? n_normals : 1) * 3 * sizeof(float)));
+  float* pnormals = static_cast<float*>(c->dc_normals.ptr);
   ISO_HIP(c, timed(c, "dc_normals", [&] {
-            return iso::launch_dc_normals(ops, c->n_prog, L, epsilon, B, n_mixed, cnormals,
-                                          c->stream);
+            return shared ? iso::launch_dc_normals(ops, c->n_prog, L, epsilon, plist, n_normals,
+                                                   pnormals, c->stream)
+                          : iso::launch_dc_cell_normals(ops, c->n_prog, L, epsilon, B, n_mixed,
+                                                        pnormals, c->stream);
           }));
   ISO_HIP(c, timed(c, "dc_feature", [&] {
-            return iso::launch_dc_feature(L, B, n_mixed, cnormals, c->stream);
+            return iso::launch_dc_feature(L, B, n_mixed, pindex, pnormals, c->stream);
           }));
   st = extract_lattice(c, Ld, B.dual_values, B.dual_points, out);
   // The stall flag is final once the stream has passed the feature kernel,

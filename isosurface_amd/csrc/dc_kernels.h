@@ -19,8 +19,9 @@ namespace iso {
 enum : int {
   kDcMixed = 0,    // number of mixed primal cells listed
   kDcStalled = 1,  // nonzero when a QEF's SVD hit the sweep bound
-  kDcScanErr = 2,  // nonzero when the mixed-cell scan's look-back gave up
-  kDcSlots = 3,
+  kDcScanErr = 2,  // nonzero when a scan's look-back gave up
+  kDcPoints = 3,   // number of lattice points that are a corner of a mixed cell
+  kDcSlots = 4,
 };
 
 struct DcBuffers {
@@ -44,15 +45,30 @@ hipError_t launch_dc_dual(const Lattice& L, const DcBuffers& B, scan::State st, 
 // each, device), unnormalised, into normals (3 floats each, device).
 hipError_t launch_sample_normals(const DevOp* ops, int n_ops, const float* points, uint64_t n,
                                  float epsilon, float* normals, hipStream_t s);
-// The eight corner normals of each of the first n_mixed listed mixed cells
-// (central differences of step epsilon, or the program's analytic normals
-// when epsilon is 0, normalised), 24 floats per cell, one thread per corner.
+// The lattice points that are a corner of a mixed cell (from the mixed
+// flags), ascending: plist (capacity n_points) gets their indices, pindex
+// (n_points) each one's position in plist, counters[kDcPoints] their count;
+// a single-pass scan whose state `st` holds scan::blocks_for(n_points,
+// scan::kItems) slots (its err word is redirected to counters[kDcScanErr]).
+hipError_t launch_dc_corner_points(const Lattice& L, const DcBuffers& B, uint32_t* plist,
+                                   uint32_t* pindex, scan::State st, hipStream_t s);
+// The normal at each of the n_points listed points (central differences of
+// step epsilon, or the program's analytic normals when epsilon is 0,
+// normalised), 3 floats each, one thread per point: each corner shared by
+// several mixed cells is evaluated once.
 hipError_t launch_dc_normals(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
-                             const DcBuffers& B, uint64_t n_mixed, float* normals,
+                             const uint32_t* plist, uint64_t n_points, float* normals,
                              hipStream_t s);
-// Feature points of the first n_mixed listed mixed cells from their corner
-// normals (TangentPlanes and the QEF solve), one thread per cell.
+// The eight corner normals of each of the first n_mixed listed mixed cells,
+// 24 floats per cell, one thread per (cell, corner).
+hipError_t launch_dc_cell_normals(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
+                                  const DcBuffers& B, uint64_t n_mixed, float* normals,
+                                  hipStream_t s);
+// Feature points of the first n_mixed listed mixed cells from their
+// corners' normals (per listed point through pindex, or with pindex null
+// per cell corner as launch_dc_cell_normals writes them), TangentPlanes and
+// the QEF solve, one thread per cell.
 hipError_t launch_dc_feature(const Lattice& L, const DcBuffers& B, uint64_t n_mixed,
-                             const float* normals, hipStream_t s);
+                             const uint32_t* pindex, const float* normals, hipStream_t s);
 
 }  // namespace iso

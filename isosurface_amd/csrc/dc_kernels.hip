@@ -169,29 +169,106 @@ __device__ __forceinline__ qef::Vec3f cell_corner(const Lattice& L, uint32_t c, 
           lattice_coord(L, 2, z + tables::kCorner[k][2])};
 }
 
+// Flat lattice index of corner k of cell c.
+__device__ __forceinline__ uint32_t corner_point(const Lattice& L, uint32_t c, int k) {
+  const uint32_t cnx = static_cast<uint32_t>(L.nx - 1);
+  const uint32_t cnxy = static_cast<uint32_t>((L.nx - 1) * (L.ny - 1));
+  const uint32_t z = c / cnxy;
+  const uint32_t r = c - z * cnxy;
+  const uint32_t y = r / cnx;
+  const uint32_t x = r - y * cnx;
+  return ((z + tables::kCorner[k][2]) * static_cast<uint32_t>(L.ny) + y + tables::kCorner[k][1]) *
+             static_cast<uint32_t>(L.nx) +
+         x + tables::kCorner[k][0];
+}
+
+// Scan load: whether lattice point p is a corner of a mixed cell (one of
+// the up to eight cells around it is flagged).
+struct PointLoad {
+  Lattice L;
+  const uint8_t* mixed_flag;
+  __device__ unsigned long long operator()(uint64_t p) const {
+    const uint32_t nx = static_cast<uint32_t>(L.nx), nxy = static_cast<uint32_t>(L.nx * L.ny);
+    const uint32_t pp = static_cast<uint32_t>(p);
+    const uint32_t z = pp / nxy;
+    const uint32_t r = pp - z * nxy;
+    const uint32_t y = r / nx;
+    const uint32_t x = r - y * nx;
+    const uint32_t cnx = nx - 1, cny = static_cast<uint32_t>(L.ny - 1),
+                   cnz = static_cast<uint32_t>(L.nz - 1);
+    unsigned long long any = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+      if ((dx && x == 0) || (dy && y == 0) || (dz && z == 0)) continue;
+      const uint32_t cx = x - dx, cy = y - dy, cz = z - dz;
+      if (cx >= cnx || cy >= cny || cz >= cnz) continue;
+      any |= mixed_flag[(cz * cny + cy) * cnx + cx];
+    }
+    return any ? 1ull : 0ull;
+  }
+};
+
+// Scan sink: the points' list slots and index map, and at the last point
+// the count.
+struct PointSink {
+  uint64_t n;
+  uint32_t* plist;
+  uint32_t* pindex;
+  unsigned long long* counters;
+  __device__ void operator()(uint64_t p, unsigned long long v, unsigned long long ex) const {
+    if (v) {
+      plist[ex] = static_cast<uint32_t>(p);
+      pindex[p] = static_cast<uint32_t>(ex);
+    }
+    if (p == n - 1) counters[kDcPoints] = ex + v;
+  }
+};
+
 // The corner normals of the mixed cells, one thread per (cell, corner):
-// normals[24 i + 3 k + a] for corner k of mixed cell i.
-__global__ __launch_bounds__(256) void k_dc_normals(const DevOp* __restrict__ ops, int n_ops,
-                                                    Lattice L, float eps,
-                                                    const uint32_t* __restrict__ mixed,
-                                                    uint64_t n_mixed, float* __restrict__ normals) {
+// normals[24 i + 3 k + a] for corner k of mixed cell i (no shared-corner
+// list: for normals that cost one evaluation, listing the corners costs
+// more than evaluating each cell's own).
+__global__ __launch_bounds__(256) void k_dc_cell_normals(const DevOp* __restrict__ ops,
+                                                         int n_ops, Lattice L, float eps,
+                                                         const uint32_t* __restrict__ mixed,
+                                                         uint64_t n_mixed,
+                                                         float* __restrict__ normals) {
   const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (t >= 8 * n_mixed) return;
-  const uint64_t i = t >> 3;
-  const int k = static_cast<int>(t & 7);
-  const qef::Vec3f p = cell_corner(L, mixed[i], k);
+  const qef::Vec3f p = cell_corner(L, mixed[t >> 3], static_cast<int>(t & 7));
   const qef::Vec3f n = central_normal(ops, n_ops, p.x, p.y, p.z, eps);
   normals[3 * t + 0] = n.x;
   normals[3 * t + 1] = n.y;
   normals[3 * t + 2] = n.z;
 }
 
+// The normal at each listed lattice point, one thread per point.
+__global__ __launch_bounds__(256) void k_dc_normals(const DevOp* __restrict__ ops, int n_ops,
+                                                    Lattice L, float eps,
+                                                    const uint32_t* __restrict__ plist,
+                                                    uint64_t n_points, float* __restrict__ normals) {
+  const uint64_t u = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (u >= n_points) return;
+  const uint32_t nx = static_cast<uint32_t>(L.nx), nxy = static_cast<uint32_t>(L.nx * L.ny);
+  const uint32_t p = plist[u];
+  const uint32_t z = p / nxy;
+  const uint32_t r = p - z * nxy;
+  const uint32_t y = r / nx;
+  const uint32_t x = r - y * nx;
+  const qef::Vec3f n = central_normal(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
+                                      lattice_coord(L, 2, z), eps);
+  normals[3 * u + 0] = n.x;
+  normals[3 * u + 1] = n.y;
+  normals[3 * u + 2] = n.z;
+}
+
 constexpr int kFeatureThreads = 128;
 
 __global__ __launch_bounds__(kFeatureThreads) void k_dc_feature(
     Lattice L, const uint32_t* __restrict__ mixed, uint64_t n_mixed,
-    const float* __restrict__ cnormals, float* __restrict__ dual_points,
-    unsigned long long* __restrict__ counters) {
+    const uint32_t* __restrict__ pindex, const float* __restrict__ pnormals,
+    float* __restrict__ dual_points, unsigned long long* __restrict__ counters) {
   bool stalled = false;
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kFeatureThreads + threadIdx.x;
   if (i < n_mixed) {
@@ -200,7 +277,8 @@ __global__ __launch_bounds__(kFeatureThreads) void k_dc_feature(
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       corners[k] = cell_corner(L, c, k);
-      const float* q = cnormals + 24 * i + 3 * k;
+      const uint64_t slot = pindex ? pindex[corner_point(L, c, k)] : 8 * i + k;
+      const float* q = pnormals + 3 * slot;
       normals[k] = {q[0], q[1], q[2]};
     }
     const qef::Vec3f p = qef::place_feature(corners, normals, &stalled);
@@ -251,26 +329,50 @@ hipError_t launch_sample_normals(const DevOp* ops, int n_ops, const float* point
   return hipGetLastError();
 }
 
+hipError_t launch_dc_corner_points(const Lattice& L, const DcBuffers& B, uint32_t* plist,
+                                   uint32_t* pindex, scan::State st, hipStream_t s) {
+  const uint64_t n_points = static_cast<uint64_t>(L.nx * L.ny * L.nz);
+  if (n_points == 0 || n_points > 0xffffffffull || !plist || !pindex || !B.mixed_flag ||
+      !st.slot || !st.ctl)
+    return hipErrorInvalidValue;
+  st.err = B.counters + kDcScanErr;
+  const PointLoad load{L, B.mixed_flag};
+  const PointSink sink{n_points, plist, pindex, B.counters};
+  return scan::launch<scan::kItems>(n_points, load, sink, st, s);
+}
+
 hipError_t launch_dc_normals(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
-                             const DcBuffers& B, uint64_t n_mixed, float* normals,
+                             const uint32_t* plist, uint64_t n_points, float* normals,
                              hipStream_t s) {
+  if (!ops || !plist || !normals) return hipErrorInvalidValue;
+  if (n_points == 0) return hipSuccess;
+  const uint64_t blocks = (n_points + 255) / 256;
+  if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_dc_normals, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
+                     n_ops, L, epsilon, plist, n_points, normals);
+  return hipGetLastError();
+}
+
+hipError_t launch_dc_cell_normals(const DevOp* ops, int n_ops, const Lattice& L, float epsilon,
+                                  const DcBuffers& B, uint64_t n_mixed, float* normals,
+                                  hipStream_t s) {
   if (!ops || !normals) return hipErrorInvalidValue;
   if (n_mixed == 0) return hipSuccess;
   const uint64_t blocks = (8 * n_mixed + 255) / 256;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_dc_normals, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
+  hipLaunchKernelGGL(k_dc_cell_normals, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, ops,
                      n_ops, L, epsilon, B.mixed, n_mixed, normals);
   return hipGetLastError();
 }
 
 hipError_t launch_dc_feature(const Lattice& L, const DcBuffers& B, uint64_t n_mixed,
-                             const float* normals, hipStream_t s) {
+                             const uint32_t* pindex, const float* normals, hipStream_t s) {
   if (!normals) return hipErrorInvalidValue;
   if (n_mixed == 0) return hipSuccess;
   const uint64_t blocks = (n_mixed + kFeatureThreads - 1) / kFeatureThreads;
   if (blocks > 0x7fffffffull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_dc_feature, dim3(static_cast<unsigned>(blocks)), dim3(kFeatureThreads), 0,
-                     s, L, B.mixed, n_mixed, normals, B.dual_points, B.counters);
+                     s, L, B.mixed, n_mixed, pindex, normals, B.dual_points, B.counters);
   return hipGetLastError();
 }
 
