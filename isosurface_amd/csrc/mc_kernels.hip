@@ -1071,12 +1071,17 @@ EmitArgs emit_args(const McBuffers& B) {
 // Grid-stride over the active list (its length is only known on the device):
 // at most `per_cu` resident workgroups per CU (0: as many as fit).
 template <class K>
-unsigned emit_blocks(const McBuffers& B, int per_cu, K kernel) {
+unsigned emit_blocks(const McBuffers& B, int per_cu, K kernel, int64_t n_cells = -1) {
   (void)kernel;
   // One 256-cell chunk of the active list per workgroup (the list's length
   // is only known on the device: workgroups past it exit at once), or at
   // most per_cu resident workgroups per CU, each taking several chunks.
-  int64_t blocks = (static_cast<int64_t>(B.active_capacity) + kEmitThreads - 1) / kEmitThreads;
+  // The list holds at most one entry per cell of the lattice: a list grown
+  // by a larger lattice does not enlarge the grid (tens of thousands of idle
+  // workgroups cost tens of microseconds).
+  int64_t items = static_cast<int64_t>(B.active_capacity);
+  if (n_cells >= 0 && n_cells < items) items = n_cells > 0 ? n_cells : 1;
+  int64_t blocks = (items + kEmitThreads - 1) / kEmitThreads;
   if (per_cu > 0) {
     const int64_t cap = static_cast<int64_t>(per_cu) * device_cus();
     if (blocks > cap) blocks = cap;
@@ -1103,13 +1108,13 @@ hipError_t launch_emit(const float* grid, const DevOp* ops, int n_ops, const flo
   const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, grid, positions);
   if (grid && positions) {
     auto k = k_emit<kEmitPositions, SingleSpace>;
-    launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
+    launch_k(k, dim3(emit_blocks(B, 0, k, D.n_cells)), dim3(kEmitThreads), s, A, sp);
   } else if (grid) {
     auto k = k_emit<kEmitGrid, SingleSpace>;
-    launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
+    launch_k(k, dim3(emit_blocks(B, 0, k, D.n_cells)), dim3(kEmitThreads), s, A, sp);
   } else {
     auto k = k_emit<kEmitProgram, SingleSpace>;
-    launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
+    launch_k(k, dim3(emit_blocks(B, 0, k, D.n_cells)), dim3(kEmitThreads), s, A, sp);
   }
   return hipGetLastError();
 }
@@ -1131,7 +1136,7 @@ hipError_t launch_emit_directed(const float* values, const Lattice& L, const McD
   const EmitArgs A = emit_args(B);
   const SingleSpace sp = single_space(L, D, B.signs, nullptr, 0, values, nullptr);
   auto k = k_emit<kEmitDirected, SingleSpace>;
-  launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
+  launch_k(k, dim3(emit_blocks(B, 0, k, D.n_cells)), dim3(kEmitThreads), s, A, sp);
   return hipGetLastError();
 }
 
@@ -1160,7 +1165,7 @@ hipError_t launch_emit_band(const float* values, const uint8_t* brick_flags, int
   A.brick_shift = brick_shift;
   const SingleSpace sp = single_space(L, D, B.signs, ops, n_ops, values, nullptr);
   auto k = k_emit<kEmitBand, SingleSpace>;
-  launch_k(k, dim3(emit_blocks(B, 0, k)), dim3(kEmitThreads), s, A, sp);
+  launch_k(k, dim3(emit_blocks(B, 0, k, D.n_cells)), dim3(kEmitThreads), s, A, sp);
   return hipGetLastError();
 }
 
