@@ -110,12 +110,10 @@ struct Tile {
   float f;                     // output of this lane's sample
 };
 
-// Forward pass of 16 samples; lane l's sample is l % 16 at (px, py, pz).
-// sF: fragments F1 F2 F3 in LDS; sB: b1 b2 b3 w4 b4 in LDS.  Layer 1
-// consumes the features one K-block at a time (computed just in time, and
-// stored hidden-major to xdst[row * n + s] when xdst is given).  kDoubleK:
-// pairs of K blocks per double-K MFMA (no xdst).
-template <bool kDoubleK>
+// Training forward pass of 16 samples; lane l's sample is l % 16 at
+// (px, py, pz).  sF: fragments F1 F2 F3 in LDS; sB: b1 b2 b3 w4 b4 in LDS.
+// Layer 1 consumes the features one K-block at a time (computed just in
+// time, and stored hidden-major to xdst[row * n + s] when xdst is given).
 __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB, float px,
                                              float py, float pz, int lane, Tile& T,
                                              uint16_t* xdst, int64_t n, int64_t s) {
@@ -136,68 +134,23 @@ __device__ __forceinline__ void forward_tile(const uint16_t* sF, const float* sB
     uint16_t xb[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) xb[j] = to_bf16(feature(f0 + j, px, py, pz));
-    if (!kDoubleK && xdst) {
+    if (xdst) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) xdst[(f0 + j) * n + s] = xb[j];
     }
     return pack4(xb[0], xb[1], xb[2], xb[3]);
   };
-  // Inference feature order (F1I): block kb < 6 holds function kb at the
-  // lane's four frequencies, block 6 the coordinates (rows 0-2 of lanes
-  // 0-15) and padding.
-  const int g = lane >> 4;
-  const float cg = __builtin_amdgcn_ldexpf((g & 1) ? 0x1.6a09e6p-1f : 0x1p-1f, g >> 1);
-  auto features_i = [&](int kb) {
-    float xf[4];
-    if (kb < 6) {
-      const int d = kb % 3;
-      const float m = cg * (d == 0 ? px : (d == 1 ? py : pz));  // c_k p for j = 0
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        // c_g 4^j p: a power-of-two scaling of m, exact
-        const float rev = __builtin_amdgcn_fractf(m * static_cast<float>(1 << (2 * j)));
-        xf[j] = kb < 3 ? __builtin_amdgcn_sinf(rev) : __builtin_amdgcn_cosf(rev);
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        xf[j] = g == 0 && j < 3 ? (j == 0 ? px : (j == 1 ? py : pz)) : 0.0f;
-    }
-    return pack4f(xf[0], xf[1], xf[2], xf[3]);
-  };
-  if (kDoubleK) {
 #pragma unroll 1
-    for (int kb = 0; kb + 1 < kKB1; kb += 2) {
-      const bf16x4 x0 = features_i(kb), x1 = features_i(kb + 1);
+  for (int kb = 0; kb < kKB1; ++kb) {
+    const bf16x4 x = features(kb);
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-        acc1[mb] = mfma2(frag(F1, kKB1, mb, kb, lane), frag(F1, kKB1, mb, kb + 1, lane), x0, x1,
-                         acc1[mb]);
-    }
-    if (kKB1 & 1) {
-      const bf16x4 x = features_i(kKB1 - 1);
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb)
-        acc1[mb] = mfma(frag(F1, kKB1, mb, kKB1 - 1, lane), x, acc1[mb]);
-    }
-  } else {
-#pragma unroll 1
-    for (int kb = 0; kb < kKB1; ++kb) {
-      const bf16x4 x = features(kb);
-#pragma unroll
-      for (int mb = 0; mb < 4; ++mb) acc1[mb] = mfma(frag(F1, kKB1, mb, kb, lane), x, acc1[mb]);
-    }
+    for (int mb = 0; mb < 4; ++mb) acc1[mb] = mfma(frag(F1, kKB1, mb, kb, lane), x, acc1[mb]);
   }
-  // A 64-wide K product: two double-K MFMAs, or four.
+  // A 64-wide K product: four MFMAs.
   auto product64 = [&](const uint16_t* F, int mb, const bf16x4* h) {
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (kDoubleK) {
-      acc = mfma2(frag(F, 4, mb, 0, lane), frag(F, 4, mb, 1, lane), h[0], h[1], acc);
-      acc = mfma2(frag(F, 4, mb, 2, lane), frag(F, 4, mb, 3, lane), h[2], h[3], acc);
-    } else {
 #pragma unroll
-      for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F, 4, mb, kb, lane), h[kb], acc);
-    }
+    for (int kb = 0; kb < 4; ++kb) acc = mfma(frag(F, 4, mb, kb, lane), h[kb], acc);
     return acc;
   };
   // Bias, ReLU and the bf16 rounding of a layer's 16 x 16 output block.
@@ -239,6 +192,126 @@ __device__ __forceinline__ void load_model(const Model& M, uint16_t* sF, int n_f
   }
 }
 
+// Inference forward of kT 16-sample tiles at once: every weight fragment
+// read from LDS feeds all kT tiles' MFMAs, 1/kT of the LDS traffic per
+// sample of one tile at a time.  Pairs of K blocks go through double-K
+// MFMAs.  The features come in the inference order of F1I: block kb < 6
+// holds function kb (sin of x, y, z, then cos) at the lane's four
+// frequencies, block 6 the coordinates (rows 0-2 of lanes 0-15) and
+// padding.  p[t]: this lane's sample of tile t (lane l's sample is l % 16);
+// f[t]: its output.
+template <int kT>
+__device__ __forceinline__ void forward_tiles(const uint16_t* sF, const float* sB,
+                                              const float (*p)[3], int lane, float* f) {
+  const int r0 = 4 * (lane >> 4);
+  const int g = lane >> 4;
+  const float cg = __builtin_amdgcn_ldexpf((g & 1) ? 0x1.6a09e6p-1f : 0x1p-1f, g >> 1);
+  const uint16_t* F1 = sF + kFragF1;
+  const uint16_t* F2 = sF + kFragF2;
+  const uint16_t* F3 = sF + kFragF3;
+  const float* b1 = sB;
+  const float* b2 = sB + kHidden;
+  const float* b3 = sB + 2 * kHidden;
+  const float* w4 = sB + 3 * kHidden;
+  const float b4 = sB[4 * kHidden];
+  auto feats = [&](const float* q, int kb) {
+    float xf[4];
+    if (kb < 6) {
+      const float m = cg * q[kb % 3];  // c_k q for j = 0
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        // c_g 4^j q: a power-of-two scaling of m, exact
+        const float rev = __builtin_amdgcn_fractf(m * static_cast<float>(1 << (2 * j)));
+        xf[j] = kb < 3 ? __builtin_amdgcn_sinf(rev) : __builtin_amdgcn_cosf(rev);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xf[j] = g == 0 && j < 3 ? q[j] : 0.0f;
+    }
+    return pack4f(xf[0], xf[1], xf[2], xf[3]);
+  };
+  // Bias, ReLU and the bf16 rounding of a layer's 16 x 16 output block.
+  auto activate = [&](const f32x4& acc, const float* b) {
+    return pack4f(fmaxf(acc[0] + b[r0 + 0], 0.0f), fmaxf(acc[1] + b[r0 + 1], 0.0f),
+                  fmaxf(acc[2] + b[r0 + 2], 0.0f), fmaxf(acc[3] + b[r0 + 3], 0.0f));
+  };
+  f32x4 z[kT][4];
+#pragma unroll
+  for (int t = 0; t < kT; ++t)
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) z[t][mb] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+  for (int kb = 0; kb + 1 < kKB1; kb += 2) {
+    bf16x4 x0[kT], x1[kT];
+#pragma unroll
+    for (int t = 0; t < kT; ++t) {
+      x0[t] = feats(p[t], kb);
+      x1[t] = feats(p[t], kb + 1);
+    }
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const bf16x4 w0 = frag(F1, kKB1, mb, kb, lane), w1 = frag(F1, kKB1, mb, kb + 1, lane);
+#pragma unroll
+      for (int t = 0; t < kT; ++t) z[t][mb] = mfma2(w0, w1, x0[t], x1[t], z[t][mb]);
+    }
+  }
+  if (kKB1 & 1) {
+    bf16x4 x[kT];
+#pragma unroll
+    for (int t = 0; t < kT; ++t) x[t] = feats(p[t], kKB1 - 1);
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      const bf16x4 w = frag(F1, kKB1, mb, kKB1 - 1, lane);
+#pragma unroll
+      for (int t = 0; t < kT; ++t) z[t][mb] = mfma(w, x[t], z[t][mb]);
+    }
+  }
+  bf16x4 h[kT][4];
+#pragma unroll
+  for (int t = 0; t < kT; ++t)
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) h[t][mb] = activate(z[t][mb], b1 + 16 * mb);
+  // Hidden layers 2 and 3 (W2, W3): 64-wide K, two double-K MFMAs a block.
+  auto layer = [&](const uint16_t* F, const float* b) {
+    bf16x4 y[kT][4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+      f32x4 a[kT];
+#pragma unroll
+      for (int t = 0; t < kT; ++t) a[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kp = 0; kp < 4; kp += 2) {
+        const bf16x4 w0 = frag(F, 4, mb, kp, lane), w1 = frag(F, 4, mb, kp + 1, lane);
+#pragma unroll
+        for (int t = 0; t < kT; ++t) a[t] = mfma2(w0, w1, h[t][kp], h[t][kp + 1], a[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < kT; ++t) y[t][mb] = activate(a[t], b + 16 * mb);
+    }
+#pragma unroll
+    for (int t = 0; t < kT; ++t)
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) h[t][mb] = y[t][mb];
+  };
+  layer(F2, b2);
+  layer(F3, b3);
+#pragma unroll
+  for (int t = 0; t < kT; ++t) {
+    float q = 0.0f;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        q += w4[16 * mb + r0 + j] * from_bf16(static_cast<uint16_t>(h[t][mb][j]));
+    // The four lanes of a sample (l, l^16, l^32, l^48) hold its 64 terms.
+    q += __shfl_xor(q, 16, 64);
+    q += __shfl_xor(q, 32, 64);
+    f[t] = q + b4;
+  }
+}
+
+// Tiles of 16 samples per fragment read in the inference forward.
+constexpr int kFwdTiles = 4;
 // 512 threads: the fragments' 40 KB of LDS then allow as many waves per SIMD
 // as the registers do.
 constexpr int kFwdThreads = 512;
@@ -260,17 +333,27 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward(Model M, const float* _
   const int lane = __lane_id();
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kFwdThreads / 64) + (threadIdx.x >> 6);
   const int64_t n_waves = static_cast<int64_t>(gridDim.x) * (kFwdThreads / 64);
-  for (int64_t t0 = wave * 16; t0 < n; t0 += n_waves * 16) {
-    // The fragments are re-read from LDS for every tile rather than held in
-    // registers across tiles: fewer registers, more waves per SIMD.
+  constexpr int64_t kStep = 16 * kFwdTiles;
+  for (int64_t t0 = wave * kStep; t0 < n; t0 += n_waves * kStep) {
+    // The fragments are re-read from LDS for every kFwdTiles tiles rather
+    // than held in registers across them: fewer registers, more waves per
+    // SIMD.
     asm volatile("" ::: "memory");
-    const int64_t s = t0 + (lane & 15);
-    const bool ok = s < n;
-    const float px = ok ? points[3 * s] : 0.0f, py = ok ? points[3 * s + 1] : 0.0f,
-                pz = ok ? points[3 * s + 2] : 0.0f;
-    Tile T;
-    forward_tile<true>(sF, sB, px, py, pz, lane, T, nullptr, 0, 0);
-    if (ok && lane < 16) out[s] = T.f;
+    float p[kFwdTiles][3], f[kFwdTiles];
+#pragma unroll
+    for (int t = 0; t < kFwdTiles; ++t) {
+      const int64_t si = t0 + 16 * t + (lane & 15);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) p[t][a] = si < n ? points[3 * si + a] : 0.0f;
+    }
+    forward_tiles<kFwdTiles>(sF, sB, p, lane, f);
+    if (lane < 16) {
+#pragma unroll
+      for (int t = 0; t < kFwdTiles; ++t) {
+        const int64_t si = t0 + 16 * t + lane;
+        if (si < n) out[si] = f[t];
+      }
+    }
   }
 }
 
@@ -309,7 +392,7 @@ __global__ __launch_bounds__(kThreads) void k_train_fb(Model M, Batch B, Samples
     const float t = fminf(fmaxf(S.sdf[src], -S.clamp), S.clamp);
     const float wgt = S.weight ? S.weight[src] : 1.0f;
     Tile T;
-    forward_tile<false>(sF, sB, px, py, pz, lane, T, B.x, B.n, s);
+    forward_tile(sF, sB, px, py, pz, lane, T, B.x, B.n, s);
     // Least squares plus (overshoot - 1) e^2, e the part of the error that
     // lies outside the interval between 0 and the target.
     const float r = T.f - t;
@@ -542,7 +625,7 @@ hipError_t launch_pack(const Model& M, hipStream_t s) {
 hipError_t launch_forward(const Model& M, const float* points, int64_t n, float* out,
                           hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  constexpr int64_t per_block = 16 * (kFwdThreads / 64);  // a 16-sample tile per wavefront
+  constexpr int64_t per_block = 16 * kFwdTiles * (kFwdThreads / 64);  // kFwdTiles tiles per wavefront
   int64_t blocks = (n + per_block - 1) / per_block;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(k_forward, dim3(static_cast<unsigned>(blocks)), dim3(kFwdThreads), 0, s, M,
