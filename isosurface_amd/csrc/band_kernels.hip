@@ -253,11 +253,8 @@ struct ExactShape {
 // and gather sign bits per x row in LDS; last, one lane per row merges the
 // row into the sign words (at most two words, OR for inside, AND-NOT for
 // outside, over the background the bricks' flags wrote).
-// Eight wavefronts per SIMD (64 VGPRs, five of them spilled to scratch)
-// hide the latency of the program evaluation better than the seven
-// that 72 unconstrained VGPRs allow: 117 ms instead of 138 ms at 1024^3.
 template <int kShift>
-__global__ __launch_bounds__(kExactThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void
+__global__ __launch_bounds__(kExactThreads) void
 k_band_exact(
     const DevOp* __restrict__ ops, int n_ops, Lattice L, BrickGrid G,
     const uint8_t* __restrict__ flags, const uint32_t* __restrict__ band_list, uint64_t n_band,
@@ -355,8 +352,12 @@ k_band_exact(
     }
     __syncthreads();
     const uint32_t T = s_pre[kGroupBricks];
+    // Whole wavefronts step through the queue (the culled evaluator works on
+    // all 64 lanes); lanes past its end repeat its last entry and drop it.
 #pragma unroll 1
-    for (uint32_t t = threadIdx.x; t < T; t += kExactThreads) {
+    for (uint32_t t0 = threadIdx.x & ~63u; t0 < T; t0 += kExactThreads) {
+      const bool valid = t0 + (threadIdx.x & 63u) < T;
+      const uint32_t t = valid ? t0 + (threadIdx.x & 63u) : T - 1;
       int j = 0;  // the brick whose list holds queue entry t: s_pre[j] <= t < s_pre[j + 1]
 #pragma unroll
       for (int step = kGroupBricks / 2; step >= 1; step >>= 1)
@@ -368,9 +369,11 @@ k_band_exact(
       const int64_t x = static_cast<int64_t>(s_lo[j][0]) + lx;
       const int64_t y = static_cast<int64_t>(s_lo[j][1]) + ly;
       const int64_t z = static_cast<int64_t>(s_lo[j][2]) + lz;
-      const float v = sdf_eval(ops, n_ops, L.origin[0] + static_cast<float>(x) * L.spacing[0],
-                               L.origin[1] + static_cast<float>(y) * L.spacing[1],
-                               L.origin[2] + static_cast<float>(z) * L.spacing[2]);
+      const float v =
+          sdf_eval_culled(ops, n_ops, L.origin[0] + static_cast<float>(x) * L.spacing[0],
+                          L.origin[1] + static_cast<float>(y) * L.spacing[1],
+                          L.origin[2] + static_cast<float>(z) * L.spacing[2]);
+      if (!valid) continue;
       if (values) values[(z * L.ny + y) * L.nx + x] = v;
       // classify_corners, distance.rs:52-54
       if (!(v > 0.0f)) atomicOr(&s_in[j][ly + kSide * lz], 1u << lx);

@@ -96,18 +96,17 @@ __global__ __launch_bounds__(256) void k_signs_program(const DevOp* __restrict__
   const uint32_t nxy = static_cast<uint32_t>(L.nx * L.ny);
   for (int64_t w = wave; w < n_words; w += n_waves) {
     const int64_t f = w * 64 + lane;
-    bool inside = false;
-    if (f < n_points) {
-      const uint32_t ff = static_cast<uint32_t>(f);
-      const uint32_t z = ff / nxy;
-      const uint32_t r = ff - z * nxy;
-      const uint32_t y = r / nx;
-      const uint32_t x = r - y * nx;
-      const float v = sdf_eval(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
-                               lattice_coord(L, 2, z));
-      inside = !(v > 0.0f);
-      if (values) values[f] = v;
-    }
+    // Every lane evaluates (the last word's spare lanes at the last point):
+    // the culled evaluator works on the whole wavefront.
+    const uint32_t ff = static_cast<uint32_t>(f < n_points ? f : n_points - 1);
+    const uint32_t z = ff / nxy;
+    const uint32_t r = ff - z * nxy;
+    const uint32_t y = r / nx;
+    const uint32_t x = r - y * nx;
+    const float v = sdf_eval_culled(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
+                                    lattice_coord(L, 2, z));
+    const bool inside = f < n_points && !(v > 0.0f);
+    if (values && f < n_points) values[f] = v;
     const uint64_t m = __ballot(inside);
     if (lane == 0) S[w] = m;
   }

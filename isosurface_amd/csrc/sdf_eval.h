@@ -121,6 +121,192 @@ __device__ __forceinline__ float sdf_eval(const DevOp* __restrict__ ops, int n, 
 }
 
 // ---------------------------------------------------------------------------
+// sdf_eval for a whole wavefront of nearby points, skipping the union and
+// smooth-union operands that cannot change any lane's value.  Returns
+// sdf_eval's value bit for bit; every lane of the wavefront must call it
+// (all 64 active) with finite coordinates.
+//
+// A run of primitives that join the value by min (kMin) or smooth min
+// (kSmooth, k > 0) is taken 64 operands at a time.  Lane j bounds operand j
+// over the box of the wavefront's current points (the f32 points after the
+// enclosing translations, as the primitives see them):
+// lb_j <= its f32 value <= ub_j at every point, each from the box's nearest
+// and farthest points for a sphere (other primitives are not bounded and
+// always evaluated).  Before operand j the running value is at most
+// T_j = min(max over lanes of the value at the group's start, ub_i for
+// i < j), because min and smooth min never exceed either operand.  Operand j
+// is skipped when
+//   kMin:    lb_j > T_j           (fminf(top, v) = top exactly), or
+//   kSmooth: lb_j - T_j >= k      (with room for the rounding of |top - v|:
+//            the blend weight h is then exactly 0 and the result is top).
+// The bounds carry 128 ulps of every magnitude involved, far above the few
+// roundings of a sphere's f32 evaluation.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_reduce_min(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_reduce_max(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// A wave-uniform value moved to a scalar register.
+__device__ __forceinline__ float wave_uniform(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+__device__ __forceinline__ float sdf_eval_culled(const DevOp* __restrict__ ops, int n, float px,
+                                                 float py, float pz) {
+  constexpr float u = 5.9604645e-08f;  // 2^-24
+  const int lane = __lane_id();
+  // Box of the lanes' current (translated) points, found at the first run
+  // of a translation scope.
+  float blo[3], bhi[3], bmag = 0.0f;
+  bool have_box = false;
+  float top = 0.0f;
+  float st[kMaxValueStack];
+  int sp = 0;
+  float cx = px, cy = py, cz = pz;
+  float sx[kMaxPointStack], sy[kMaxPointStack], sz[kMaxPointStack];
+  int psp = 0;
+  bool have_top = false;
+  // One primitive operand joined to the value stack.
+  auto primitive = [&](const DevOp& op) {
+    float qx = cx, qy = cy, qz = cz;
+    if (op.translated) {
+      qx = cx - op.off[0];
+      qy = cy - op.off[1];
+      qz = cz - op.off[2];
+    }
+    const float v = prim_value(op.code, op.prm, qx, qy, qz);
+    const uint32_t mode = op.combine;
+    if (mode == kPush) {
+      if (have_top) st[sp++] = top;
+      top = v;
+      have_top = true;
+    } else if (mode == kMin) {
+      top = fminf(top, v);
+    } else if (mode == kMax) {
+      top = fmaxf(top, v);
+    } else if (mode == kDiff) {
+      top = fmaxf(v, -top);
+    } else if (mode == kDiffRev) {
+      top = fmaxf(top, -v);
+    } else {
+      top = smooth_min(top, v, op.k);
+    }
+  };
+  auto cullable = [](const DevOp& op) {
+    return op.code <= kCylinder &&
+           (op.combine == kMin || (op.combine == kSmooth && op.k > 0.0f && op.k < 3.0e38f));
+  };
+  int i = 0;
+  while (i < n) {
+    if (have_top && cullable(ops[i])) {
+      if (!have_box) {
+        blo[0] = wave_uniform(wave_reduce_min(cx));
+        blo[1] = wave_uniform(wave_reduce_min(cy));
+        blo[2] = wave_uniform(wave_reduce_min(cz));
+        bhi[0] = wave_uniform(wave_reduce_max(cx));
+        bhi[1] = wave_uniform(wave_reduce_max(cy));
+        bhi[2] = wave_uniform(wave_reduce_max(cz));
+        bmag = fmaxf(fmaxf(fmaxf(__builtin_fabsf(blo[0]), __builtin_fabsf(bhi[0])),
+                           fmaxf(__builtin_fabsf(blo[1]), __builtin_fabsf(bhi[1]))),
+                     fmaxf(__builtin_fabsf(blo[2]), __builtin_fabsf(bhi[2])));
+        have_box = true;
+      }
+      // Lane j looks at operand i + j.
+      const int j = i + lane;
+      bool in_run = j < n && cullable(ops[j]);
+      const uint64_t run = __ballot(in_run);
+      // operands i .. i+m-1 (m >= 1: lane 0's operand is ops[i])
+      const int m = ~run ? static_cast<int>(__builtin_ctzll(~run)) : 64;
+      in_run = lane < m;
+      float lb = -3.0e38f, ub = 3.0e38f, k = 0.0f;
+      bool smooth = false;
+      if (in_run && ops[j].code == kSphere) {
+        const DevOp& op = ops[j];
+        const float o[3] = {op.translated ? op.off[0] : 0.0f, op.translated ? op.off[1] : 0.0f,
+                            op.translated ? op.off[2] : 0.0f};
+        float near2 = 0.0f, far2 = 0.0f, omag = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float dn = fmaxf(fmaxf(blo[a] - o[a], o[a] - bhi[a]), 0.0f);
+          const float df = fmaxf(__builtin_fabsf(blo[a] - o[a]), __builtin_fabsf(bhi[a] - o[a]));
+          near2 += dn * dn;
+          far2 += df * df;
+          omag = fmaxf(omag, __builtin_fabsf(o[a]));
+        }
+        const float r = op.prm[0];
+        const float far = __builtin_sqrtf(far2);
+        const float margin = 128.0f * u * (far + __builtin_fabsf(r) + bmag + omag) + 1.0e-30f;
+        lb = __builtin_sqrtf(near2) - r - margin;
+        ub = far - r + margin;
+        smooth = op.combine == kSmooth;
+        k = op.k;
+      }
+      // T_j: the running value's bound before operand j.
+      float pre = ub;  // inclusive prefix minimum of ub over the run's lanes
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const float t = __shfl_up(pre, o, 64);
+        if (lane >= o) pre = fminf(pre, t);
+      }
+      float excl = __shfl_up(pre, 1, 64);
+      if (lane == 0) excl = 3.0e38f;
+      const float T = fminf(wave_uniform(wave_reduce_max(top)), excl);
+      const bool skip =
+          in_run && (smooth ? (lb - T) * (1.0f - 4.0f * u) >= k * (1.0f + 4.0f * u) : lb > T);
+      uint64_t keep = __ballot(in_run && !skip);
+      while (keep) {
+        const int b = static_cast<int>(__builtin_ctzll(keep));
+        keep &= keep - 1;
+        primitive(ops[i + b]);
+      }
+      i += m;
+      continue;
+    }
+    const uint32_t code = ops[i].code;
+    if (code <= kCylinder) {
+      primitive(ops[i]);
+    } else if (code == kPushTranslate) {
+      sx[psp] = cx;
+      sy[psp] = cy;
+      sz[psp] = cz;
+      ++psp;
+      cx = cx - ops[i].off[0];
+      cy = cy - ops[i].off[1];
+      cz = cz - ops[i].off[2];
+      have_box = false;
+    } else if (code == kPopTranslate) {
+      --psp;
+      cx = sx[psp];
+      cy = sy[psp];
+      cz = sz[psp];
+      have_box = false;
+    } else {
+      const float t = top;
+      const float w = st[--sp];
+      if (code == kUnion) {
+        top = fminf(w, t);
+      } else if (code == kIntersection) {
+        top = fmaxf(w, t);
+      } else if (code == kDifference) {
+        top = fmaxf(t, -w);
+      } else if (code == kDifferenceRev) {
+        top = fmaxf(w, -t);
+      } else {
+        top = smooth_min(w, t, ops[i].k);
+      }
+    }
+    ++i;
+  }
+  return top;
+}
+
+// ---------------------------------------------------------------------------
 // Evaluation with an error bound, for proofs.  Returns sdf_eval's value at
 // (px, py, pz) and sets *err to a bound E such that, for every point x within
 // Euclidean distance `widen` of (px, py, pz), the f32 evaluation at x differs
