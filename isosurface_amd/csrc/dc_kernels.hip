@@ -36,15 +36,20 @@ __global__ __launch_bounds__(256) void k_dc_sample(const DevOp* __restrict__ ops
   const uint32_t nx = static_cast<uint32_t>(L.nx);
   const uint32_t nxy = static_cast<uint32_t>(L.nx * L.ny);
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < n_points;
-       f += stride) {
-    const uint32_t ff = static_cast<uint32_t>(f);
+  // Whole wavefronts step together (the culled evaluator works on all 64
+  // lanes); lanes past the end evaluate the last point and store nothing.
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  for (int64_t f0 = static_cast<int64_t>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u);
+       f0 < n_points; f0 += stride) {
+    const int64_t f = f0 + lane;
+    const uint32_t ff = static_cast<uint32_t>(f < n_points ? f : n_points - 1);
     const uint32_t z = ff / nxy;
     const uint32_t r = ff - z * nxy;
     const uint32_t y = r / nx;
     const uint32_t x = r - y * nx;
-    out[f] = sdf_eval(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
-                      lattice_coord(L, 2, z));
+    const float v = sdf_eval_culled(ops, n_ops, lattice_coord(L, 0, x), lattice_coord(L, 1, y),
+                                    lattice_coord(L, 2, z));
+    if (f < n_points) out[f] = v;
   }
 }
 

@@ -130,8 +130,9 @@ __device__ __forceinline__ float sdf_eval(const DevOp* __restrict__ ops, int n, 
 // (kSmooth, k > 0) is taken 64 operands at a time.  Lane j bounds operand j
 // over the box of the wavefront's current points (the f32 points after the
 // enclosing translations, as the primitives see them):
-// lb_j <= its f32 value <= ub_j at every point, each from the box's nearest
-// and farthest points for a sphere (other primitives are not bounded and
+// lb_j <= its f32 value <= ub_j at every point.  A sphere's and a box's
+// values are nondecreasing in each |q_c|, so the least and greatest |q_c|
+// over the box give the bounds (other primitives are not bounded and are
 // always evaluated).  Before operand j the running value is at most
 // T_j = min(max over lanes of the value at the group's start, ub_i for
 // i < j), because min and smooth min never exceed either operand.  Operand j
@@ -226,24 +227,37 @@ __device__ __forceinline__ float sdf_eval_culled(const DevOp* __restrict__ ops, 
       in_run = lane < m;
       float lb = -3.0e38f, ub = 3.0e38f, k = 0.0f;
       bool smooth = false;
-      if (in_run && ops[j].code == kSphere) {
+      if (in_run && (ops[j].code == kSphere || ops[j].code == kBox)) {
         const DevOp& op = ops[j];
         const float o[3] = {op.translated ? op.off[0] : 0.0f, op.translated ? op.off[1] : 0.0f,
                             op.translated ? op.off[2] : 0.0f};
-        float near2 = 0.0f, far2 = 0.0f, omag = 0.0f;
+        // Per axis, the least and greatest |q| over the box.
+        float an[3], af[3], omag = 0.0f;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const float dn = fmaxf(fmaxf(blo[a] - o[a], o[a] - bhi[a]), 0.0f);
-          const float df = fmaxf(__builtin_fabsf(blo[a] - o[a]), __builtin_fabsf(bhi[a] - o[a]));
-          near2 += dn * dn;
-          far2 += df * df;
+          an[a] = fmaxf(fmaxf(blo[a] - o[a], o[a] - bhi[a]), 0.0f);
+          af[a] = fmaxf(__builtin_fabsf(blo[a] - o[a]), __builtin_fabsf(bhi[a] - o[a]));
           omag = fmaxf(omag, __builtin_fabsf(o[a]));
         }
-        const float r = op.prm[0];
-        const float far = __builtin_sqrtf(far2);
-        const float margin = 128.0f * u * (far + __builtin_fabsf(r) + bmag + omag) + 1.0e-30f;
-        lb = __builtin_sqrtf(near2) - r - margin;
-        ub = far - r + margin;
+        const float far = __builtin_sqrtf((af[0] * af[0] + af[1] * af[1]) + af[2] * af[2]);
+        const float pm = __builtin_fabsf(op.prm[0]) + __builtin_fabsf(op.prm[1]) +
+                         __builtin_fabsf(op.prm[2]);
+        const float margin = 128.0f * u * (far + pm + bmag + omag) + 1.0e-30f;
+        // Both values are nondecreasing in each |q_c|, so the box's least
+        // and greatest |q| per axis bound them.
+        lb = __builtin_sqrtf((an[0] * an[0] + an[1] * an[1]) + an[2] * an[2]) - op.prm[0];
+        ub = far - op.prm[0];
+        // Boxes only when the group holds one: a wave-uniform branch, so
+        // sphere-only programs do not pay for the box bounds.
+        const bool is_box = op.code == kBox;
+        if (__ballot(is_box)) {
+          if (is_box) {
+            lb = prim_value(kBox, op.prm, an[0], an[1], an[2]);
+            ub = prim_value(kBox, op.prm, af[0], af[1], af[2]);
+          }
+        }
+        lb -= margin;
+        ub += margin;
         smooth = op.combine == kSmooth;
         k = op.k;
       }
